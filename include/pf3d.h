@@ -1,0 +1,185 @@
+/*
+ * pf3d.h - C ABI of the MI355X (gfx950) stiffness-assembly-and-solve library.
+ *
+ * All array arguments are DEVICE pointers (HIP global memory) unless noted;
+ * `stream` is a hipStream_t (NULL = default stream).  Every entry point
+ * returns 0 on success and a negative PF3D_E* code otherwise; the message
+ * of the last failure on the calling thread is in pf3d_last_error().
+ * Nothing allocates device memory except where stated.
+ *
+ * Element conventions follow pyfe3d v0.7.0 (references are to its sources):
+ *   - conn[e*nn + i]  = c_I, the global DOF offset of node I (6*node position);
+ *                       node coordinates are read at x[c_I/2 + 0..2]
+ *                       (quad4.pyx:526-537);
+ *   - init_k[e]       = first slot of element e in the r/c/v arrays
+ *                       (int64; NULL means size*e);
+ *   - values accumulate (v += ...) like the reference unless
+ *     PF3D_OVERWRITE; rows/cols are written unless PF3D_VALUES_ONLY.
+ *   - shell property record (32 doubles): A11 A12 A16 A22 A26 A66,
+ *     B11..B66, D11..D66, E44 E45 E55, scf_k13, scf_k23, h, offset,
+ *     intrho, intrhoz, intrhoz2, 4 unused  (shellprop.pxd:38-60)
+ *   - beam property record (16 doubles): A E G Iyy Izz Iyz J Ay Az intrho
+ *     intrhoy intrhoz intrhoy2 intrhoz2 intrhoyz, 1 unused (beamprop.pyx:15-93)
+ */
+#ifndef PF3D_H
+#define PF3D_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PF3D_OK 0
+#define PF3D_E_ARG -1     /* invalid argument (null pointer, bad size/type) */
+#define PF3D_E_HIP -2     /* HIP runtime error */
+#define PF3D_E_NOTIMPL -3 /* combination not provided */
+#define PF3D_E_SOLVE -4   /* solver did not reach the tolerance */
+
+#define PF3D_QUAD4 0
+#define PF3D_QUAD4R 1
+#define PF3D_TRIA3R 2
+#define PF3D_BEAMC 3
+
+#define PF3D_KC0 0
+#define PF3D_KG 1
+#define PF3D_KG_STRESS 2
+#define PF3D_M 3
+
+#define PF3D_VALUES_ONLY 1 /* update_KC0v_only != 0 (quad4.pyx:1278) */
+#define PF3D_OVERWRITE 2   /* store instead of accumulate */
+
+int pf3d_version(void);
+const char *pf3d_last_error(void);
+
+/* Slots written per element: Quad4Data/Quad4RData/Tria3RData/BeamCData
+ * KC0_SPARSE_SIZE, KG_SPARSE_SIZE, M_SPARSE_SIZE (quad4.pyx:174-180,
+ * quad4r.pyx:140-146, tria3r.pyx:148-151, beamc.pyx:43-46); for lumped
+ * masses the used prefix of M_SPARSE_SIZE. */
+int pf3d_slot_count(int etype, int matrix, int mtype);
+
+/* Element axes, material direction and local coordinates of a batch.
+ * geom[e*14 + 0..13] = r11 r12 r13 r21 r22 r23 r31 r32 r33 m11 m12 m21 m22,
+ * area (shells) or length (beams); xe[e*3nn ..] = local x, y, z per node.
+ * Replaces update_rotation_matrix + update_probe_xe + update_area/length:
+ * quad4.pyx:491-752, quad4r.pyx:286-548, tria3r.pyx:294-548,
+ * beamc.pyx:158-350.  evec = xmat (shells, may be NULL) or vxy (beams). */
+int pf3d_geometry(int etype, int64_t n_elem, const double *x, const int64_t *conn,
+                  const double *evec, double *geom, double *xe, void *stream);
+
+/* Shell KC0 of a batch (Quad4, Quad4R, Tria3R), geometry computed from x.
+ * Replaces, per element, update_rotation_matrix(x, xmat) ->
+ * update_probe_xe(x) -> update_KC0(KC0r, KC0c, KC0v, prop,
+ * update_KC0v_only[, hgfactor_*]): quad4.pyx:1204-1313,
+ * quad4r.pyx:1145-4619, tria3r.pyx:950-2974.
+ * k6rot (Quad4R/Tria3R), alpha (Tria3R alpha_shear_locking) and hg
+ * (Quad4R hgfactor_u,v,w,rx,ry; n_elem*5) may be NULL for the defaults. */
+int pf3d_shell_kc0(int etype, int64_t n_elem, const double *x, const int64_t *conn,
+                   const int64_t *init_k, const double *xmat, const int32_t *pid,
+                   const double *props, const double *k6rot, const double *alpha,
+                   const double *hg, int64_t *r, int64_t *c, double *v, int flags,
+                   void *stream);
+
+/* Shell geometric stiffness from global displacements u (6 per node).
+ * Replaces update_probe_ue(u) -> update_probe_xe(x) -> update_KG(...):
+ * quad4.pyx:1365-2257, quad4r.pyx:4681-5573, tria3r.pyx:3020-3575. */
+int pf3d_shell_kg(int etype, int64_t n_elem, const double *x, const int64_t *conn,
+                  const int64_t *init_k, const double *xmat, const int32_t *pid,
+                  const double *props, const double *u, int64_t *r, int64_t *c, double *v,
+                  int flags, void *stream);
+
+/* Shell geometric stiffness for a given membrane stress state (Nxx, Nyy,
+ * Nxy): one triple for all elements (stress_per_elem = 0) or one per
+ * element.  Replaces update_KG_given_stress: quad4.pyx:2259-3081,
+ * quad4r.pyx:5574-6397, tria3r.pyx:3576-4062. */
+int pf3d_shell_kg_stress(int etype, int64_t n_elem, const double *x, const int64_t *conn,
+                         const int64_t *init_k, const double *xmat, const double *stress,
+                         int stress_per_elem, int64_t *r, int64_t *c, double *v, int flags,
+                         void *stream);
+
+/* Shell mass matrix, mtype 0 consistent, 1 reduced, 2 lumped.
+ * Replaces update_M: quad4.pyx:3083-9489, quad4r.pyx:6398-12788,
+ * tria3r.pyx:4063-7723. */
+int pf3d_shell_m(int etype, int64_t n_elem, const double *x, const int64_t *conn,
+                 const int64_t *init_k, const double *xmat, const int32_t *pid,
+                 const double *props, int mtype, int64_t *r, int64_t *c, double *v, int flags,
+                 void *stream);
+
+/* BeamC KC0 (vxy: n_elem*3 orientation vectors).  Replaces
+ * update_rotation_matrix(vxyi, vxyj, vxyk, x) -> update_probe_xe(x) ->
+ * update_KC0(...): beamc.pyx:158-350, 489-1353. */
+int pf3d_beam_kc0(int64_t n_elem, const double *x, const int64_t *conn, const int64_t *init_k,
+                  const double *vxy, const int32_t *pid, const double *props, int64_t *r,
+                  int64_t *c, double *v, int flags, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* COO -> 6x6 BSR (replaces coo_matrix((v, (r, c))).tocsc() on the     */
+/* reference's caller side, e.g. tests/test_quad4_static_point_load.py:80) */
+/* ------------------------------------------------------------------ */
+
+/* Workspace bytes of the plan for n_slots COO slots and n_nodes nodes
+ * (matrix order 6*n_nodes).  The only call that sizes device memory. */
+int pf3d_bsr_plan_bytes(int64_t n_slots, int64_t n_nodes, int64_t *bytes);
+
+/* Sort the slots by (node row, node col, dof row, dof col); counts (host)
+ * receives {unique (row, col) entries, 6x6 blocks}. */
+int pf3d_bsr_plan_sort(int64_t n_slots, const int64_t *r, const int64_t *c, int64_t n_nodes,
+                       void *work, int64_t work_bytes, int64_t *counts, void *stream);
+
+/* Emit the plan: perm[n_slots], seg[n_unique+1], dest[n_unique] and the
+ * BSR structure row_ptr[n_nodes+1], col_idx[nnzb] (int32). */
+int pf3d_bsr_plan_emit(int64_t n_slots, int64_t n_nodes, void *work, int64_t work_bytes,
+                       int64_t n_unique, int64_t nnzb, int64_t *perm, int64_t *seg,
+                       int64_t *dest, int32_t *row_ptr, int32_t *col_idx, void *stream);
+
+/* BSR values (nnzb*36, row-major 6x6 blocks) = per-entry sums of v in slot
+ * order (deterministic); entries absent from the COO are zero. */
+int pf3d_bsr_apply(int64_t n_unique, const int64_t *perm, const int64_t *seg,
+                   const int64_t *dest, const double *v, int64_t nnzb, double *bsr_val,
+                   void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Static solve (replaces KC0[bu, :][:, bu] + scipy spsolve/cg,          */
+/* tests/test_quad4_static_point_load.py:84-102)                         */
+/* ------------------------------------------------------------------ */
+
+/* y = K x for a 6x6 BSR matrix with nb block rows (FP64 / FP32). */
+int pf3d_bsr_spmv(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx, const double *val,
+                  const double *x, double *y, void *stream);
+int pf3d_bsr_spmv_f32(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx,
+                      const float *val, const float *x, float *y, void *stream);
+
+/* In place K <- P K P + (I - P), P = diag(fixed == 0): the free-DOF block
+ * is the reference's KC0[bu, :][:, bu]; fixed DOFs solve to zero. */
+int pf3d_bsr_dirichlet(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx, double *val,
+                       const uint8_t *fixed, void *stream);
+
+/* b[i] = (float)a[i] */
+int pf3d_to_f32(int64_t n, const double *a, float *b, void *stream);
+
+/* Inverses of the 6x6 diagonal blocks (FP64 and, if dinv32, FP32);
+ * *singular (device int) counts blocks without a pivot. */
+int pf3d_bjacobi(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx, const double *val,
+                 double *dinv, float *dinv32, int *singular, void *stream);
+
+/* FP64 block-Jacobi PCG to ||r|| <= rtol ||b||; x is the initial guess when
+ * x0 != 0.  work: pf3d_pcg_bytes(nb, 0) bytes. */
+int pf3d_pcg_bytes(int64_t nb, int fp32, int64_t *bytes);
+int pf3d_pcg(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx, const double *val,
+             const double *dinv, const double *b, double *x, double rtol, int maxiter, int x0,
+             void *work, int64_t work_bytes, int64_t *iters, double *relres, void *stream);
+
+/* FP64 iterative refinement around an FP32 block-Jacobi PCG: the outer
+ * residual b - K64 x and the update x += d are FP64; stops at
+ * ||b - K64 x|| <= rtol ||b||.  stats = {outer steps, inner iterations}.
+ * work: pf3d_ir_bytes(nb) bytes. */
+int pf3d_ir_bytes(int64_t nb, int64_t *bytes);
+int pf3d_ir_solve(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx,
+                  const double *val64, const float *val32, const float *dinv32,
+                  const double *b, double *x, double rtol, double inner_rtol, int max_outer,
+                  int max_inner, int x0, void *work, int64_t work_bytes, int64_t *stats,
+                  double *relres, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,73 @@
+"""ctypes binding of ``libpf3d.so`` (the C ABI in ``include/pf3d.h``).
+
+The library is built in-tree by ``pyfe3d_amd.build``.  There is no CPU
+fallback: if the library is missing, every compute call raises.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'libpf3d.so')
+
+P = ctypes.c_void_p
+I32 = ctypes.c_int
+I64 = ctypes.c_int64
+F64 = ctypes.c_double
+
+# name -> argtypes (all return int status except where noted)
+SIGNATURES = {
+    'pf3d_version': [],
+    'pf3d_slot_count': [I32, I32, I32],
+    'pf3d_geometry': [I32, I64, P, P, P, P, P, P],
+    'pf3d_shell_kc0': [I32, I64, P, P, P, P, P, P, P, P, P, P, P, P, I32, P],
+    'pf3d_shell_kg': [I32, I64, P, P, P, P, P, P, P, P, P, P, I32, P],
+    'pf3d_shell_kg_stress': [I32, I64, P, P, P, P, P, I32, P, P, P, I32, P],
+    'pf3d_shell_m': [I32, I64, P, P, P, P, P, P, I32, P, P, P, I32, P],
+    'pf3d_beam_kc0': [I64, P, P, P, P, P, P, P, P, P, I32, P],
+    'pf3d_bsr_plan_bytes': [I64, I64, P],
+    'pf3d_bsr_plan_sort': [I64, P, P, I64, P, I64, P, P],
+    'pf3d_bsr_plan_emit': [I64, I64, P, I64, I64, I64, P, P, P, P, P, P],
+    'pf3d_bsr_apply': [I64, P, P, P, P, I64, P, P],
+    'pf3d_bsr_spmv': [I64, P, P, P, P, P, P],
+    'pf3d_bsr_spmv_f32': [I64, P, P, P, P, P, P],
+    'pf3d_bsr_dirichlet': [I64, P, P, P, P, P],
+    'pf3d_to_f32': [I64, P, P, P],
+    'pf3d_bjacobi': [I64, P, P, P, P, P, P, P],
+    'pf3d_pcg_bytes': [I64, I32, P],
+    'pf3d_pcg': [I64, P, P, P, P, P, P, F64, I32, I32, P, I64, P, P, P],
+    'pf3d_ir_bytes': [I64, P],
+    'pf3d_ir_solve': [I64, P, P, P, P, P, P, P, F64, F64, I32, I32, I32, P, I64, P, P, P],
+}
+
+_lib = None
+
+
+class Pf3dError(RuntimeError):
+    pass
+
+
+def load():
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError('libpf3d.so not found at %s: run `python -m pyfe3d_amd.build` '
+                          '(there is no CPU fallback)' % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    lib.pf3d_last_error.argtypes = []
+    lib.pf3d_last_error.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def call(name, *args):
+    lib = load()
+    st = getattr(lib, name)(*args)
+    if st < 0:
+        raise Pf3dError('%s failed (%d): %s' % (name, st, lib.pf3d_last_error().decode()))
+    return st

@@ -1,0 +1,767 @@
+// Element assembly kernels: KC0, KG (from displacements), KG (given stress)
+// and M for the Quad4 / Quad4R / Tria3R shells, KC0 for BeamC.
+//
+// Work mapping (gfx950, wave64): one lane owns one 6x6 node-pair block
+// (I, J) of one element, so a wave holds 4 quads (16 lanes each) or 7
+// triangles (9 lanes each).  Each lane integrates its block in registers,
+// rotates it to global axes (R6 K R6^T with R6 = blockdiag(R, R)) and parks
+// it in an LDS tile; the wave then streams the element's slots out in the
+// reference's slot order with unit-stride stores, so every store
+// instruction writes 64 consecutive doubles of the COO value array.
+#include "pf3d_device.hpp"
+
+namespace pf3d {
+
+
+
+// ------------------------------------------------------------------------
+// slot layouts: slot k of an element -> (local row, local col)
+// ------------------------------------------------------------------------
+template <int NN>
+__device__ __forceinline__ void slot_layout(int mat, int mtype, int k, int &rl, int &cl)
+{
+    constexpr int NDOF = 6 * NN;
+    if (mat == KC0) {  // quad4.pyx:1287-1293 (I, m, J, n)
+        rl = k / NDOF;
+        cl = k - rl * NDOF;
+    } else if (mat == KG || mat == KGS) {  // quad4.pyx:1500-1932 (I, a, J, b), translations
+        int row = k / (3 * NN), col = k - row * 3 * NN;
+        rl = 6 * (row / 3) + row % 3;
+        cl = 6 * (col / 3) + col % 3;
+    } else if (mtype == 2) {  // quad4.pyx:8005-8872 lumped
+        int I = k / (18 * NN), rem = k - I * 18 * NN;
+        int g = rem / (9 * NN), rem2 = rem - g * 9 * NN;
+        int d = rem2 / (3 * NN), col = rem2 - d * 3 * NN;
+        rl = 6 * I + 3 * g + d;
+        cl = 6 * (col / 3) + 3 * g + col % 3;
+    } else {  // quad4.pyx:3157-4598: row d skips column (d+3) mod 6
+        int row = k / (5 * NN), col = k - row * 5 * NN;
+        int d = row % 6, t = col % 5, skip = (d + 3) % 6;
+        rl = row;
+        cl = 6 * (col / 5) + (t < skip ? t : t + 1);
+    }
+}
+
+template <int NN>
+__host__ __device__ constexpr int slot_count(int mat, int mtype)
+{
+    return mat == KC0 ? 36 * NN * NN
+         : (mat == KG || mat == KGS) ? 9 * NN * NN
+         : (mtype == 2 ? 18 * NN * NN : 30 * NN * NN);
+}
+
+// Select a[i] for a lane-varying i.  Written as a sum of products with 0/1
+// weights (exact in IEEE arithmetic for finite values) so that the compiler
+// keeps the operands in registers instead of spilling them to an indexed
+// private array.
+template <int N>
+__device__ __forceinline__ double pick(const double (&a)[N], int i)
+{
+    double r = 0.;
+#pragma unroll
+    for (int k = 0; k < N; k++) r += a[k] * (double)(i == k);
+    return r;
+}
+
+// bilinear shape values of the lane's node pair at (xi, eta); the Jacobian
+// arithmetic is that of quad_shape (quad4.pyx:937-962)
+struct Pair {
+    double Ni, nxi, nyi, Nj, nxj, nyj, det;
+};
+
+__device__ __forceinline__ Pair quad_pair(const double *xe, double xi, double eta, double sxi,
+                                          double syi, double sxj, double syj)
+{
+    const double sx[4] = {-1, 1, 1, -1}, sy[4] = {-1, -1, 1, 1};
+    double J11 = 0, J12 = 0, J21 = 0, J22 = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const double dxi = 0.25 * sx[i] * (1 + sy[i] * eta);
+        const double deta = 0.25 * sy[i] * (1 + sx[i] * xi);
+        J11 += dxi * xe[2 * i];
+        J12 += dxi * xe[2 * i + 1];
+        J21 += deta * xe[2 * i];
+        J22 += deta * xe[2 * i + 1];
+    }
+    Pair p;
+    p.det = J11 * J22 - J12 * J21;
+    const double j11 = J22 / p.det, j12 = -J12 / p.det, j21 = -J21 / p.det, j22 = J11 / p.det;
+    const double dxi_i = 0.25 * sxi * (1 + syi * eta), deta_i = 0.25 * syi * (1 + sxi * xi);
+    const double dxi_j = 0.25 * sxj * (1 + syj * eta), deta_j = 0.25 * syj * (1 + sxj * xi);
+    p.Ni = 0.25 * (1 + sxi * xi) * (1 + syi * eta);
+    p.Nj = 0.25 * (1 + sxj * xi) * (1 + syj * eta);
+    p.nxi = j11 * dxi_i + j12 * deta_i;
+    p.nyi = j21 * dxi_i + j22 * deta_i;
+    p.nxj = j11 * dxi_j + j12 * deta_j;
+    p.nyj = j21 * dxi_j + j22 * deta_j;
+    return p;
+}
+
+__device__ __forceinline__ double node_sx(int I) { return (I == 1 || I == 2) ? 1. : -1.; }
+__device__ __forceinline__ double node_sy(int I) { return (I >= 2) ? 1. : -1.; }
+
+// ------------------------------------------------------------------------
+// block integrands (lane owns block I, J; K is 6x6 in local axes)
+// ------------------------------------------------------------------------
+
+// membrane/bending B^T [[A,B],[B,D]] B over (u, v, rx, ry); quad4.pyx:1002-1030
+__device__ __forceinline__ void add_abd(double K[6][6], const double C[6][6], double nxi,
+                                        double nyi, double nxj, double nyj, double w)
+{
+    double CB[4][6];  // columns u, v, rx, ry of C*B_J
+#pragma unroll
+    for (int a = 0; a < 6; a++) {
+        CB[0][a] = C[a][0] * nxj + C[a][2] * nyj;
+        CB[1][a] = C[a][1] * nyj + C[a][2] * nxj;
+        CB[2][a] = -(C[a][4] * nyj + C[a][5] * nxj);
+        CB[3][a] = C[a][3] * nxj + C[a][5] * nyj;
+    }
+    nxi *= w;
+    nyi *= w;
+    const int col[4] = {0, 1, 3, 4};
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const double *cb = CB[t];
+        K[0][col[t]] += nxi * cb[0] + nyi * cb[2];
+        K[1][col[t]] += nyi * cb[1] + nxi * cb[2];
+        K[3][col[t]] -= nyi * cb[4] + nxi * cb[5];
+        K[4][col[t]] += nxi * cb[3] + nyi * cb[5];
+    }
+}
+
+// transverse shear gyz = w,y - rx, gxz = w,x + ry split into gradient (w)
+// and rotation (rx, ry) parts; quad4.pyx:1032-1051, 1107-1171
+__device__ __forceinline__ void add_shear(double K[6][6], const double E[2][2], double Ni,
+                                          double nxi, double nyi, double Nj, double nxj,
+                                          double nyj, double w, bool gg, bool gr, bool rr)
+{
+    // E times the J columns: gradient (w) and rotations (rx, ry)
+    const double egw0 = E[0][0] * nyj + E[0][1] * nxj, egw1 = E[1][0] * nyj + E[1][1] * nxj;
+    const double erx0 = -E[0][0] * Nj, erx1 = -E[1][0] * Nj;
+    const double ery0 = E[0][1] * Nj, ery1 = E[1][1] * Nj;
+    const double gi0 = w * nyi, gi1 = w * nxi;  // row w of node I
+    const double rxi = -w * Ni;                 // row rx: gyz component only
+    const double ryi = w * Ni;                  // row ry: gxz component only
+    if (gg) K[2][2] += gi0 * egw0 + gi1 * egw1;
+    if (gr) {
+        K[2][3] += gi0 * erx0 + gi1 * erx1;
+        K[2][4] += gi0 * ery0 + gi1 * ery1;
+        K[3][2] += rxi * egw0;
+        K[4][2] += ryi * egw1;
+    }
+    if (rr) {
+        K[3][3] += rxi * erx0;
+        K[3][4] += rxi * ery0;
+        K[4][3] += ryi * erx1;
+        K[4][4] += ryi * ery1;
+    }
+}
+
+// drilling rz - (v,x - u,y)/2 penalty; quad4.pyx:1053-1073
+__device__ __forceinline__ void add_drill(double K[6][6], double Ni, double nxi, double nyi,
+                                          double Nj, double nxj, double nyj, double w,
+                                          bool rz_couple, bool rz_diag)
+{
+    const double bi[3] = {nyi / 2., -nxi / 2., Ni};
+    const double bj[3] = {nyj / 2., -nxj / 2., Nj};
+    const int dof[3] = {0, 1, 5};
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            bool rz = (a == 2) || (b == 2);
+            if (rz && !(rz_couple || (rz_diag && a == 2 && b == 2))) continue;
+            K[dof[a]][dof[b]] += w * bi[a] * bj[b];
+        }
+}
+
+// R6 K R6^T for one 6x6 node block (block-diagonal R6 = diag(R, R))
+__device__ __forceinline__ void rotate_block(double K[6][6], const double *r)
+{
+    double T[6][6];
+#pragma unroll
+    for (int p = 0; p < 2; p++)
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            double S[3][3];  // R * K_pq
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int j = 0; j < 3; j++)
+                    S[a][j] = r[a * 3 + 0] * K[3 * p + 0][3 * q + j] +
+                              r[a * 3 + 1] * K[3 * p + 1][3 * q + j] +
+                              r[a * 3 + 2] * K[3 * p + 2][3 * q + j];
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int b = 0; b < 3; b++)
+                    T[3 * p + a][3 * q + b] =
+                        S[a][0] * r[b * 3 + 0] + S[a][1] * r[b * 3 + 1] + S[a][2] * r[b * 3 + 2];
+        }
+#pragma unroll
+    for (int a = 0; a < 6; a++)
+#pragma unroll
+        for (int b = 0; b < 6; b++) K[a][b] = T[a][b];
+}
+
+// ------------------------------------------------------------------------
+// per-type local blocks
+// ------------------------------------------------------------------------
+
+// Quad4 KC0: quad4.pyx:755-1172
+__device__ __forceinline__ void quad4_kc0(double K[6][6], const Geom &g, const Shell &s, int I,
+                                          int J)
+{
+    const bool thick = (s.h / sqrt(g.area)) >= 1.;
+    const double sxi = node_sx(I), syi = node_sy(I), sxj = node_sx(J), syj = node_sy(J);
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        const Pair n = quad_pair(g.xe, (p >> 1) ? GP2 : -GP2, (p & 1) ? GP2 : -GP2, sxi, syi,
+                                 sxj, syj);
+        add_abd(K, s.C, n.nxi, n.nyi, n.nxj, n.nyj, n.det);
+        if (thick)
+            add_shear(K, s.Es, n.Ni, n.nxi, n.nyi, n.Nj, n.nxj, n.nyj, n.det, true, false, false);
+        add_drill(K, n.Ni, n.nxi, n.nyi, n.Nj, n.nxj, n.nyj, n.det, true, true);
+    }
+    const Pair n = quad_pair(g.xe, 0., 0., sxi, syi, sxj, syj);
+    add_shear(K, s.Es, n.Ni, n.nxi, n.nyi, n.Nj, n.nxj, n.nyj, 4. * n.det, !thick, true, true);
+}
+
+// Quad4R KC0: quad4r.pyx:1300-1312 (E1eq, E2eq), 3081-3462
+__device__ __forceinline__ void quad4r_kc0(double K[6][6], const Geom &g, const Shell &s, int I,
+                                           int J, double k6rot, const double *hgf)
+{
+    const double A11 = s.C[0][0], A12 = s.C[0][1], A16 = s.C[0][2];
+    const double A22 = s.C[1][1], A26 = s.C[1][2], A66 = s.C[2][2];
+    const double det3 = A11 * (A22 * A66 - A26 * A26) - A12 * (A12 * A66 - A26 * A16) +
+                        A16 * (A12 * A26 - A22 * A16);
+    const double a11 = (A22 * A66 - A26 * A26) / det3, a22 = (A11 * A66 - A16 * A16) / det3;
+    const double E1eq = 1. / (s.h * a11), E2eq = 1. / (s.h * a22);
+    const double damp = 1.0 + 1.0 / g.area, h3 = s.h * s.h * s.h;
+    double Ehg[5];
+    Ehg[0] = hgf[0] * 0.1 * E1eq * s.h / damp;
+    Ehg[1] = hgf[1] * 0.1 * E2eq * s.h / damp;
+    Ehg[3] = hgf[3] * 0.1 * E2eq * h3 / damp;
+    Ehg[4] = hgf[4] * 0.1 * E1eq * h3 / damp;
+    Ehg[2] = hgf[2] * 0.5 * (Ehg[3] + Ehg[4]);
+
+    const double sxi = node_sx(I), syi = node_sy(I), sxj = node_sx(J), syj = node_sy(J);
+    const Pair n = quad_pair(g.xe, 0., 0., sxi, syi, sxj, syj);
+    const double w = 4. * n.det;
+    add_abd(K, s.C, n.nxi, n.nyi, n.nxj, n.nyj, w);
+    add_shear(K, s.Es, n.Ni, n.nxi, n.nyi, n.Nj, n.nxj, n.nyj, w, true, true, true);
+    // hourglass modes gamma_I = d2N_I/dxdy at the centroid
+    const double *xe = g.xe;
+    const double J11 = 0.25 * (-xe[0] + xe[2] + xe[4] - xe[6]);
+    const double J12 = 0.25 * (-xe[1] + xe[3] + xe[5] - xe[7]);
+    const double J21 = 0.25 * (-xe[0] - xe[2] + xe[4] + xe[6]);
+    const double J22 = 0.25 * (-xe[1] - xe[3] + xe[5] + xe[7]);
+    const double dj = J11 * J22 - J12 * J21;
+    const double j11 = J22 / dj, j12 = -J12 / dj, j21 = -J21 / dj, j22 = J11 / dj;
+    const double gxy = 0.25 * (j11 * j22 + j12 * j21);
+    const double gI = (I & 1) ? -gxy : gxy, gJ = (J & 1) ? -gxy : gxy;
+#pragma unroll
+    for (int d = 0; d < 5; d++) K[d][d] += w * Ehg[d] * gI * gJ;
+    const double kd = 1e-6 * k6rot * A66;
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        const Pair m = quad_pair(g.xe, (p >> 1) ? GP2 : -GP2, (p & 1) ? GP2 : -GP2, sxi, syi,
+                                 sxj, syj);
+        add_drill(K, m.Ni, m.nxi, m.nyi, m.Nj, m.nxj, m.nyj, m.det * kd, true, true);
+    }
+}
+
+// Tria3R shape-function gradients (constant): tria3r.pyx:2116-2121
+__device__ __forceinline__ void tria_grad(const Geom &g, double Nx[3], double Ny[3])
+{
+    const double *p = g.xe;
+    const double a2 = 2 * g.area;
+    Nx[0] = (p[3] - p[5]) / a2; Nx[1] = (p[5] - p[1]) / a2; Nx[2] = (p[1] - p[3]) / a2;
+    Ny[0] = (p[4] - p[2]) / a2; Ny[1] = (p[0] - p[4]) / a2; Ny[2] = (p[2] - p[0]) / a2;
+}
+
+// Tria3R KC0: tria3r.pyx:1105-1122 (shear factor), 2116-2324 (terms).
+// Only the rz_I-rz_I drilling diagonal reaches the global matrix: the other
+// rz couplings computed at :2230-2324 are not used by the scatter at :2325-2974.
+__device__ __forceinline__ void tria3r_kc0(double K[6][6], const Geom &g, const Shell &s, int I,
+                                           int J, double k6rot, double alpha)
+{
+    const double *p = g.xe;
+    const double l12 = sqrt((p[0] - p[2]) * (p[0] - p[2]) + (p[1] - p[3]) * (p[1] - p[3]));
+    const double l23 = sqrt((p[2] - p[4]) * (p[2] - p[4]) + (p[3] - p[5]) * (p[3] - p[5]));
+    const double l31 = sqrt((p[4] - p[0]) * (p[4] - p[0]) + (p[5] - p[1]) * (p[5] - p[1]));
+    double maxl = l12;
+    if (l23 > maxl) maxl = l23;
+    if (l31 > maxl) maxl = l31;
+    const double f = 1 / (1 + alpha * maxl * maxl / (s.h * s.h));
+    double E[2][2] = {{s.Es[0][0] * f, s.Es[0][1] * f}, {s.Es[1][0] * f, s.Es[1][1] * f}};
+    double Nx[3], Ny[3];
+    tria_grad(g, Nx, Ny);
+    const double nxi = pick(Nx, I), nyi = pick(Ny, I), nxj = pick(Nx, J), nyj = pick(Ny, J);
+    const double detJ = 2 * g.area, w = 0.5 * detJ, third = 1. / 3.;
+    add_abd(K, s.C, nxi, nyi, nxj, nyj, w);
+    add_shear(K, E, third, nxi, nyi, third, nxj, nyj, w, true, true, true);
+    const double kd = 1e-6 * k6rot * s.C[2][2];
+    const double P[3][3] = {{2. / 3, 1. / 6, 1. / 6}, {1. / 6, 1. / 6, 2. / 3}, {1. / 6, 2. / 3, 1. / 6}};
+#pragma unroll
+    for (int t = 0; t < 3; t++)
+        add_drill(K, pick(P[t], I), nxi, nyi, pick(P[t], J), nxj, nyj, detJ * (0.5 / 3.) * kd,
+                  false, I == J);
+}
+
+// membrane stress resultants from local displacements (quad4.pyx:1965-1967)
+template <int NN>
+__device__ __forceinline__ void membrane_forces(const Shell &s, const double *Nx,
+                                                const double *Ny, const double *ue,
+                                                double Nf[3])
+{
+    double eps[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < NN; i++) {
+        const double *u = ue + 6 * i;
+        eps[0] += Nx[i] * u[0];
+        eps[1] += Ny[i] * u[1];
+        eps[2] += Ny[i] * u[0] + Nx[i] * u[1];
+        eps[3] += Nx[i] * u[4];
+        eps[4] -= Ny[i] * u[3];
+        eps[5] += Ny[i] * u[4] - Nx[i] * u[3];
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        double t = 0;
+#pragma unroll
+        for (int b = 0; b < 6; b++) t += s.C[a][b] * eps[b];
+        Nf[a] = t;
+    }
+}
+
+// ------------------------------------------------------------------------
+// kernel
+// ------------------------------------------------------------------------
+constexpr int WAVES_PER_BLOCK = 2;
+
+template <int ET, int MAT>
+__global__ void __launch_bounds__(64 * WAVES_PER_BLOCK)
+shell_assemble(AsmArgs a)
+{
+    constexpr int NN = (ET == TRIA3R) ? 3 : 4;
+    constexpr int NB = NN * NN;
+    constexpr int EPW = 64 / NB;
+    constexpr int NDOF = 6 * NN;
+    constexpr int TILE = NDOF * NDOF;
+    __shared__ double tile[WAVES_PER_BLOCK][EPW][TILE];
+    __shared__ int64_t sconn[WAVES_PER_BLOCK][EPW][NN];
+    __shared__ int64_t sk0[WAVES_PER_BLOCK][EPW];
+
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int slot = lane / NB, sub = lane - slot * NB;
+    const int I = sub / NN, J = sub - (sub / NN) * NN;
+    const int64_t e0 = ((int64_t)blockIdx.x * WAVES_PER_BLOCK + wave) * EPW;
+    const int64_t e = e0 + slot;
+    const bool active = slot < EPW && e < a.n_elem;
+    const int nslot = slot_count<NN>(MAT, a.mtype);
+
+    if (active) {
+        V3 X[NN];
+        int64_t cn[NN];
+#pragma unroll
+        for (int i = 0; i < NN; i++) {
+            cn[i] = a.conn[e * NN + i];
+            const double *px = a.x + cn[i] / 2;
+            X[i] = {px[0], px[1], px[2]};
+        }
+        if (sub == 0) {
+#pragma unroll
+            for (int i = 0; i < NN; i++) sconn[wave][slot][i] = cn[i];
+            sk0[wave][slot] = a.init_k ? a.init_k[e] : (int64_t)nslot * e;
+        }
+        V3 xm = {0., 0., 0.};
+        if (a.evec) xm = {a.evec[3 * e], a.evec[3 * e + 1], a.evec[3 * e + 2]};
+        Geom g;
+        if (ET == TRIA3R) tria_geometry(X, xm, g);
+        else quad_geometry(X, xm, g);
+        Shell s;
+        if (MAT != KGS)  // the given-stress path reads no property
+            load_shell(a.props + (a.pid ? (int64_t)a.pid[e] : 0) * SHELL_STRIDE, g.m, s);
+
+        double K[6][6];
+#pragma unroll
+        for (int i = 0; i < 6; i++)
+#pragma unroll
+            for (int j = 0; j < 6; j++) K[i][j] = 0.;
+
+        if (MAT == KC0) {
+            const double k6 = a.k6rot ? a.k6rot[e] : 100.;
+            if (ET == QUAD4) quad4_kc0(K, g, s, I, J);
+            else if (ET == QUAD4R) {
+                double hgf[5] = {1., 1., 1., 1., 1.};
+                if (a.hg)
+#pragma unroll
+                    for (int d = 0; d < 5; d++) hgf[d] = a.hg[5 * e + d];
+                quad4r_kc0(K, g, s, I, J, k6, hgf);
+            } else {
+                tria3r_kc0(K, g, s, I, J, k6, a.alpha ? a.alpha[e] : 0.7);
+            }
+        } else if (MAT == KG || MAT == KGS) {
+            double ue[6 * NN];
+            double Nf[3] = {0., 0., 0.};
+            if (MAT == KG) {
+                // ue = blockdiag(R^T) u (quad4.pyx:627-680)
+#pragma unroll
+                for (int i = 0; i < NN; i++)
+#pragma unroll
+                    for (int b = 0; b < 2; b++) {
+                        const double *pu = a.u + cn[i] + 3 * b;
+                        const double u0 = pu[0], u1 = pu[1], u2 = pu[2];
+#pragma unroll
+                        for (int t = 0; t < 3; t++)
+                            ue[6 * i + 3 * b + t] =
+                                g.r[t] * u0 + g.r[3 + t] * u1 + g.r[6 + t] * u2;
+                    }
+            } else {
+                const double *ps = a.stress + (a.stress_per_elem ? 3 * e : 0);
+                Nf[0] = ps[0]; Nf[1] = ps[1]; Nf[2] = ps[2];
+            }
+            double G = 0.;
+            if (ET == TRIA3R) {  // 1 point (tria3r.pyx:3020-3575)
+                double Nx[3], Ny[3];
+                tria_grad(g, Nx, Ny);
+                if (MAT == KG) membrane_forces<3>(s, Nx, Ny, ue, Nf);
+                const double w = 0.5 * 2 * g.area;
+                const double nxi = pick(Nx, I), nyi = pick(Ny, I);
+                const double nxj = pick(Nx, J), nyj = pick(Ny, J);
+                G = w * (nxj * (nxi * Nf[0] + nyi * Nf[2]) + nyj * (nxi * Nf[2] + nyi * Nf[1]));
+            } else {  // 2x2 Gauss (quad4.pyx:1933-2257)
+#pragma unroll
+                for (int p = 0; p < 4; p++) {
+                    QShape q;
+                    quad_shape(g.xe, (p >> 1) ? GP2 : -GP2, (p & 1) ? GP2 : -GP2, q);
+                    if (MAT == KG) membrane_forces<4>(s, q.Nx, q.Ny, ue, Nf);
+                    const Pair n = quad_pair(g.xe, (p >> 1) ? GP2 : -GP2, (p & 1) ? GP2 : -GP2,
+                                             node_sx(I), node_sy(I), node_sx(J), node_sy(J));
+                    G += q.det * (n.nxj * (n.nxi * Nf[0] + n.nyi * Nf[2]) +
+                                  n.nyj * (n.nxi * Nf[2] + n.nyi * Nf[1]));
+                }
+            }
+            K[2][2] = G;
+        } else {  // mass (quad4.pyx:3083-9489, tria3r.pyx:4063-7723)
+            double H = 0.;
+            const int mtype = a.mtype;
+            if (ET == TRIA3R) {
+                const double detJ = 2 * g.area;
+                if (mtype == 1) H = detJ / 18.;
+                else {
+                    const double pc[3][3] = {{2. / 3, 1. / 6, 1. / 6}, {1. / 6, 1. / 6, 2. / 3},
+                                             {1. / 6, 2. / 3, 1. / 6}};
+                    const double pl[3][3] = {{1, 0, 0}, {0, 0, 1}, {0, 1, 0}};
+#pragma unroll
+                    for (int t = 0; t < 3; t++) {
+                        const double ni = mtype == 0 ? pick(pc[t], I) : pick(pl[t], I);
+                        const double nj = mtype == 0 ? pick(pc[t], J) : pick(pl[t], J);
+                        H += ni * nj * detJ * (0.5 / 3.);
+                    }
+                }
+            } else {
+                if (mtype == 1) H = 0.0625 * g.area;
+                else {
+                    const double pt = mtype == 0 ? GP2 : 1.0;
+#pragma unroll
+                    for (int p = 0; p < 4; p++) {
+                        const Pair n = quad_pair(g.xe, (p >> 1) ? pt : -pt, (p & 1) ? pt : -pt,
+                                                 node_sx(I), node_sy(I), node_sx(J), node_sy(J));
+                        H += n.Ni * n.Nj * n.det;
+                    }
+                }
+            }
+            K[0][0] = K[1][1] = K[2][2] = H * s.intrho;
+            K[3][3] = K[4][4] = H * s.intrhoz2;
+            if (mtype != 2) {
+                K[0][4] = K[4][0] = H * s.intrhoz;
+                K[1][3] = K[3][1] = -H * s.intrhoz;
+            }
+        }
+        rotate_block(K, g.r);
+        double *t = &tile[wave][slot][0];
+#pragma unroll
+        for (int i = 0; i < 6; i++)
+#pragma unroll
+            for (int j = 0; j < 6; j++) t[(6 * I + i) * NDOF + 6 * J + j] = K[i][j];
+    }
+    __syncthreads();
+
+    // stream the wave's elements out in slot order
+    const int nel = (int)((a.n_elem - e0) < EPW ? (a.n_elem - e0) : EPW);
+    if (nel <= 0) return;
+    const bool write_idx = !(a.flags & F_VALUES_ONLY);
+    const bool overwrite = (a.flags & F_OVERWRITE);
+    for (int el = 0; el < nel; el++) {
+        const int64_t k0 = sk0[wave][el];
+        const double *t = &tile[wave][el][0];
+        for (int k = lane; k < nslot; k += 64) {
+            int rl, cl;
+            slot_layout<NN>(MAT, a.mtype, k, rl, cl);
+            const double val = t[rl * NDOF + cl];
+            if (overwrite) a.v[k0 + k] = val;
+            else a.v[k0 + k] += val;
+            if (write_idx) {
+                a.r[k0 + k] = sconn[wave][el][rl / 6] + rl % 6;
+                a.c[k0 + k] = sconn[wave][el][cl / 6] + cl % 6;
+            }
+        }
+    }
+}
+
+// Geometry of each element: r (9), m (4), area/length, local x/y (2*nn).
+template <int ET>
+__global__ void element_geometry(int64_t n, const double *x, const int64_t *conn,
+                                 const double *evec, double *geom, double *xe)
+{
+    constexpr int NN = ET == TRIA3R ? 3 : (ET == BEAMC ? 2 : 4);
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    V3 X[NN];
+#pragma unroll
+    for (int i = 0; i < NN; i++) {
+        const double *px = x + conn[e * NN + i] / 2;
+        X[i] = {px[0], px[1], px[2]};
+    }
+    V3 ev = {0., 0., 0.};
+    if (evec) ev = {evec[3 * e], evec[3 * e + 1], evec[3 * e + 2]};
+    Geom g;
+    if (ET == TRIA3R) tria_geometry(X, ev, g);
+    else if (ET == BEAMC) beam_geometry(X, ev, g);
+    else quad_geometry(X, ev, g);
+    double *o = geom + 14 * e;
+    for (int i = 0; i < 9; i++) o[i] = g.r[i];
+    for (int i = 0; i < 4; i++) o[9 + i] = g.m[i];
+    o[13] = g.area;
+    if (xe) {
+#pragma clang fp contract(off)
+        for (int i = 0; i < NN; i++) {
+            double xl = 0, yl = 0, zl = 0;
+            xl = g.r[0] * X[i].x + g.r[3] * X[i].y + g.r[6] * X[i].z;
+            yl = g.r[1] * X[i].x + g.r[4] * X[i].y + g.r[7] * X[i].z;
+            zl = g.r[2] * X[i].x + g.r[5] * X[i].y + g.r[8] * X[i].z;
+            xe[(3 * NN) * e + 3 * i] = xl;
+            xe[(3 * NN) * e + 3 * i + 1] = yl;
+            xe[(3 * NN) * e + 3 * i + 2] = zl;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// BeamC KC0: closed-form Timoshenko stiffness (beamc.pyx:543-624), one
+// element per 4 lanes (lane owns a 6x6 node block), 16 beams per wave.
+// ------------------------------------------------------------------------
+__device__ void beamc_local(double L, const double *bp, int I, int J, double K[6][6])
+{
+    const double A = bp[0], E = bp[1], G = bp[2], Iyy = bp[3], Izz = bp[4], Iyz = bp[5];
+    const double Jt = bp[6], Ay = bp[7], Az = bp[8];
+    const double L2 = L * L, L3 = L2 * L;
+    const double ay = 12 * E * Izz / (G * A * L2), az = 12 * E * Iyy / (G * A * L2);
+    const double by = 1 / (1. - ay), bz = 1 / (1. - az);
+    const double kyy = by * by * (A * G * L2 * ay * ay + 12 * E * Izz);
+    const double kzz = bz * bz * (A * G * L2 * az * az + 12 * E * Iyy);
+    const double cyz = E * Iyz * by * bz;
+    const double ryy = bz * bz * (A * G * L2 * az * az / 4 + E * Iyy * az * az - 2 * E * Iyy * az + 4 * E * Iyy) / L;
+    const double ryy2 = bz * bz * (A * G * L2 * az * az / 4 - E * Iyy * az * az + 2 * E * Iyy * az + 2 * E * Iyy) / L;
+    const double rzz = by * by * (A * G * L2 * ay * ay / 4 + E * Izz * ay * ay - 2 * E * Izz * ay + 4 * E * Izz) / L;
+    const double rzz2 = by * by * (A * G * L2 * ay * ay / 4 - E * Izz * ay * ay + 2 * E * Izz * ay + 2 * E * Izz) / L;
+    const double c1 = cyz * (-ay * az + ay + az - 4) / L, c2 = cyz * (ay * az - ay - az - 2) / L;
+    const double ta = Az * G * ay * by, tb = Ay * G * az * bz;
+    // full 12x12 upper triangle, entry (i, j) i <= j
+    auto up = [&](int i, int j) -> double {
+        switch (i * 12 + j) {
+        case 0: return A * E / L;
+        case 4: return Az * E * bz * (1 - az) / L;
+        case 5: return Ay * E * by * (ay - 1) / L;
+        case 6: return -A * E / L;
+        case 10: return Az * E * bz * (az - 1) / L;
+        case 11: return Ay * E * by * (1 - ay) / L;
+        case 13: return kyy / L3;
+        case 14: return 12 * cyz / L3;
+        case 15: return ta / L;
+        case 16: return -6 * cyz / L2;
+        case 17: return kyy / (2 * L2);
+        case 19: return -kyy / L3;
+        case 20: return -12 * cyz / L3;
+        case 21: return -ta / L;
+        case 22: return -6 * cyz / L2;
+        case 23: return kyy / (2 * L2);
+        case 26: return kzz / L3;
+        case 27: return -tb / L;
+        case 28: return -kzz / (2 * L2);
+        case 29: return 6 * cyz / L2;
+        case 31: return -12 * cyz / L3;
+        case 32: return -kzz / L3;
+        case 33: return tb / L;
+        case 34: return -kzz / (2 * L2);
+        case 35: return 6 * cyz / L2;
+        case 39: return G * Jt / L;
+        case 40: return tb / 2;
+        case 41: return ta / 2;
+        case 43: return -ta / L;
+        case 44: return tb / L;
+        case 45: return -G * Jt / L;
+        case 46: return tb / 2;
+        case 47: return ta / 2;
+        case 52: return ryy;
+        case 53: return c1;
+        case 54: return Az * E * bz * (az - 1) / L;
+        case 55: return 6 * cyz / L2;
+        case 56: return kzz / (2 * L2);
+        case 57: return -tb / 2;
+        case 58: return ryy2;
+        case 59: return c2;
+        case 65: return rzz;
+        case 66: return Ay * E * by * (1 - ay) / L;
+        case 67: return -kyy / (2 * L2);
+        case 68: return -6 * cyz / L2;
+        case 69: return -ta / 2;
+        case 70: return c2;
+        case 71: return rzz2;
+        case 78: return A * E / L;
+        case 82: return Az * E * bz * (1 - az) / L;
+        case 83: return Ay * E * by * (ay - 1) / L;
+        case 91: return kyy / L3;
+        case 92: return 12 * cyz / L3;
+        case 93: return ta / L;
+        case 94: return 6 * cyz / L2;
+        case 95: return -kyy / (2 * L2);
+        case 104: return kzz / L3;
+        case 105: return -tb / L;
+        case 106: return kzz / (2 * L2);
+        case 107: return -6 * cyz / L2;
+        case 117: return G * Jt / L;
+        case 118: return -tb / 2;
+        case 119: return -ta / 2;
+        case 130: return ryy;
+        case 131: return c1;
+        case 143: return rzz;
+        default: return 0.;
+        }
+    };
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) {
+            const int gi = 6 * I + i, gj = 6 * J + j;
+            K[i][j] = gi <= gj ? up(gi, gj) : up(gj, gi);
+        }
+}
+
+__global__ void __launch_bounds__(64) beamc_kc0(AsmArgs a)
+{
+    __shared__ double tile[16][144];
+    __shared__ int64_t sconn[16][2];
+    __shared__ int64_t sk0[16];
+    const int lane = threadIdx.x, slot = lane >> 2, I = (lane >> 1) & 1, J = lane & 1;
+    const int64_t e0 = (int64_t)blockIdx.x * 16;
+    const int64_t e = e0 + slot;
+    if (e < a.n_elem) {
+        V3 X[2];
+        int64_t cn[2];
+        for (int i = 0; i < 2; i++) {
+            cn[i] = a.conn[2 * e + i];
+            const double *px = a.x + cn[i] / 2;
+            X[i] = {px[0], px[1], px[2]};
+        }
+        if ((lane & 3) == 0) {
+            sconn[slot][0] = cn[0];
+            sconn[slot][1] = cn[1];
+            sk0[slot] = a.init_k ? a.init_k[e] : 144 * e;
+        }
+        V3 vxy = {a.evec[3 * e], a.evec[3 * e + 1], a.evec[3 * e + 2]};
+        Geom g;
+        beam_geometry(X, vxy, g);
+        double K[6][6];
+        beamc_local(g.area, a.props + (a.pid ? (int64_t)a.pid[e] : 0) * BEAM_STRIDE, I, J, K);
+        rotate_block(K, g.r);
+        for (int i = 0; i < 6; i++)
+            for (int j = 0; j < 6; j++) tile[slot][(6 * I + i) * 12 + 6 * J + j] = K[i][j];
+    }
+    __syncthreads();
+    const int nel = (int)((a.n_elem - e0) < 16 ? (a.n_elem - e0) : 16);
+    const bool write_idx = !(a.flags & F_VALUES_ONLY);
+    const bool overwrite = (a.flags & F_OVERWRITE);
+    for (int el = 0; el < nel; el++) {
+        const int64_t k0 = sk0[el];
+        for (int k = lane; k < 144; k += 64) {
+            const double val = tile[el][k];
+            if (overwrite) a.v[k0 + k] = val;
+            else a.v[k0 + k] += val;
+            if (write_idx) {
+                const int rl = k / 12, cl = k % 12;
+                a.r[k0 + k] = sconn[el][rl / 6] + rl % 6;
+                a.c[k0 + k] = sconn[el][cl / 6] + cl % 6;
+            }
+        }
+    }
+}
+
+}  // namespace pf3d
+
+// ------------------------------------------------------------------------
+// launchers (called from the C-ABI layer)
+// ------------------------------------------------------------------------
+using namespace pf3d;
+
+template <int ET, int MAT>
+static hipError_t launch_shell(const AsmArgs &a, hipStream_t s)
+{
+    constexpr int NN = (ET == TRIA3R) ? 3 : 4;
+    constexpr int EPB = (64 / (NN * NN)) * WAVES_PER_BLOCK;
+    const int64_t blocks = (a.n_elem + EPB - 1) / EPB;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL((shell_assemble<ET, MAT>), dim3((unsigned)blocks),
+                       dim3(64 * WAVES_PER_BLOCK), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int ET>
+static hipError_t launch_shell_mat(int mat, const AsmArgs &a, hipStream_t s)
+{
+    switch (mat) {
+    case KC0: return launch_shell<ET, KC0>(a, s);
+    case KG: return launch_shell<ET, KG>(a, s);
+    case KGS: return launch_shell<ET, KGS>(a, s);
+    default: return launch_shell<ET, MASS>(a, s);
+    }
+}
+
+hipError_t pf3d_launch_shell(int etype, int mat, const AsmArgs &a, hipStream_t s)
+{
+    switch (etype) {
+    case QUAD4: return launch_shell_mat<QUAD4>(mat, a, s);
+    case QUAD4R: return launch_shell_mat<QUAD4R>(mat, a, s);
+    default: return launch_shell_mat<TRIA3R>(mat, a, s);
+    }
+}
+
+hipError_t pf3d_launch_beam_kc0(const AsmArgs &a, hipStream_t s)
+{
+    const int64_t blocks = (a.n_elem + 15) / 16;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(beamc_kc0, dim3((unsigned)blocks), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t pf3d_launch_geometry(int etype, int64_t n, const double *x, const int64_t *conn,
+                                const double *evec, double *geom, double *xe, hipStream_t s)
+{
+    const unsigned blocks = (unsigned)((n + 127) / 128);
+    if (blocks == 0) return hipSuccess;
+    switch (etype) {
+    case QUAD4:
+    case QUAD4R:
+        hipLaunchKernelGGL(element_geometry<QUAD4>, dim3(blocks), dim3(128), 0, s, n, x, conn,
+                           evec, geom, xe);
+        break;
+    case TRIA3R:
+        hipLaunchKernelGGL(element_geometry<TRIA3R>, dim3(blocks), dim3(128), 0, s, n, x, conn,
+                           evec, geom, xe);
+        break;
+    default:
+        hipLaunchKernelGGL(element_geometry<BEAMC>, dim3(blocks), dim3(128), 0, s, n, x, conn,
+                           evec, geom, xe);
+    }
+    return hipGetLastError();
+}

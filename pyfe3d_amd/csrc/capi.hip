@@ -1,0 +1,167 @@
+// C-ABI entry points: argument checks, error reporting and kernel launch.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string>
+
+#include "../../include/pf3d.h"
+#include "pf3d_device.hpp"
+
+// launchers defined next to the kernels
+hipError_t pf3d_launch_shell(int etype, int mat, const pf3d::AsmArgs &a, hipStream_t s);
+hipError_t pf3d_launch_beam_kc0(const pf3d::AsmArgs &a, hipStream_t s);
+hipError_t pf3d_launch_geometry(int etype, int64_t n, const double *x, const int64_t *conn,
+                                const double *evec, double *geom, double *xe, hipStream_t s);
+
+static thread_local std::string g_err;
+
+int pf3d_fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int pf3d_hip_check(hipError_t e, const char *where)
+{
+    if (e == hipSuccess) return PF3D_OK;
+    return pf3d_fail(PF3D_E_HIP, "%s: %s", where, hipGetErrorString(e));
+}
+
+extern "C" {
+
+int pf3d_version(void) { return 10000; }
+
+const char *pf3d_last_error(void) { return g_err.c_str(); }
+
+int pf3d_slot_count(int etype, int matrix, int mtype)
+{
+    if (etype == PF3D_BEAMC) {
+        if (matrix == PF3D_KC0 || matrix == PF3D_KG) return 144;
+        if (matrix == PF3D_M) return mtype == 0 ? 144 : 36;
+        return pf3d_fail(PF3D_E_ARG, "BeamC has no KG_given_stress");
+    }
+    if (etype < 0 || etype > PF3D_TRIA3R) return pf3d_fail(PF3D_E_ARG, "bad etype %d", etype);
+    const int nn = etype == PF3D_TRIA3R ? 3 : 4;
+    switch (matrix) {
+    case PF3D_KC0: return 36 * nn * nn;
+    case PF3D_KG:
+    case PF3D_KG_STRESS: return 9 * nn * nn;
+    case PF3D_M:
+        if (mtype < 0 || mtype > 2) return pf3d_fail(PF3D_E_ARG, "bad mtype %d", mtype);
+        return mtype == 2 ? 18 * nn * nn : 30 * nn * nn;
+    }
+    return pf3d_fail(PF3D_E_ARG, "bad matrix %d", matrix);
+}
+
+int pf3d_geometry(int etype, int64_t n_elem, const double *x, const int64_t *conn,
+                  const double *evec, double *geom, double *xe, void *stream)
+{
+    if (etype < 0 || etype > PF3D_BEAMC) return pf3d_fail(PF3D_E_ARG, "bad etype %d", etype);
+    if (n_elem < 0) return pf3d_fail(PF3D_E_ARG, "negative n_elem");
+    if (n_elem == 0) return PF3D_OK;
+    if (!x || !conn || !geom) return pf3d_fail(PF3D_E_ARG, "null x/conn/geom");
+    if (etype == PF3D_BEAMC && !evec) return pf3d_fail(PF3D_E_ARG, "BeamC needs vxy");
+    return pf3d_hip_check(pf3d_launch_geometry(etype, n_elem, x, conn, evec, geom, xe,
+                                               (hipStream_t)stream),
+                          "pf3d_geometry");
+}
+
+static int check_common(int etype, int64_t n_elem, const double *x, const int64_t *conn,
+                        int64_t *r, int64_t *c, double *v, int flags)
+{
+    if (etype < 0 || etype > PF3D_TRIA3R) return pf3d_fail(PF3D_E_ARG, "bad shell etype %d", etype);
+    if (n_elem < 0) return pf3d_fail(PF3D_E_ARG, "negative n_elem");
+    if (n_elem > (int64_t)2147483647 * 16) return pf3d_fail(PF3D_E_ARG, "n_elem too large");
+    if (n_elem == 0) return 1;
+    if (!x || !conn || !v) return pf3d_fail(PF3D_E_ARG, "null x/conn/v");
+    if (!(flags & PF3D_VALUES_ONLY) && (!r || !c))
+        return pf3d_fail(PF3D_E_ARG, "null r/c without PF3D_VALUES_ONLY");
+    if (flags & ~(PF3D_VALUES_ONLY | PF3D_OVERWRITE))
+        return pf3d_fail(PF3D_E_ARG, "unknown flags 0x%x", flags);
+    return 0;
+}
+
+int pf3d_shell_kc0(int etype, int64_t n_elem, const double *x, const int64_t *conn,
+                   const int64_t *init_k, const double *xmat, const int32_t *pid,
+                   const double *props, const double *k6rot, const double *alpha,
+                   const double *hg, int64_t *r, int64_t *c, double *v, int flags,
+                   void *stream)
+{
+    int st = check_common(etype, n_elem, x, conn, r, c, v, flags);
+    if (st) return st > 0 ? PF3D_OK : st;
+    if (!props) return pf3d_fail(PF3D_E_ARG, "null props");
+    pf3d::AsmArgs a{};
+    a.n_elem = n_elem; a.x = x; a.conn = conn; a.init_k = init_k; a.evec = xmat; a.pid = pid;
+    a.props = props; a.k6rot = k6rot; a.alpha = alpha; a.hg = hg; a.flags = flags;
+    a.r = r; a.c = c; a.v = v;
+    return pf3d_hip_check(pf3d_launch_shell(etype, PF3D_KC0, a, (hipStream_t)stream),
+                          "pf3d_shell_kc0");
+}
+
+int pf3d_shell_kg(int etype, int64_t n_elem, const double *x, const int64_t *conn,
+                  const int64_t *init_k, const double *xmat, const int32_t *pid,
+                  const double *props, const double *u, int64_t *r, int64_t *c, double *v,
+                  int flags, void *stream)
+{
+    int st = check_common(etype, n_elem, x, conn, r, c, v, flags);
+    if (st) return st > 0 ? PF3D_OK : st;
+    if (!props || !u) return pf3d_fail(PF3D_E_ARG, "null props/u");
+    pf3d::AsmArgs a{};
+    a.n_elem = n_elem; a.x = x; a.conn = conn; a.init_k = init_k; a.evec = xmat; a.pid = pid;
+    a.props = props; a.u = u; a.flags = flags; a.r = r; a.c = c; a.v = v;
+    return pf3d_hip_check(pf3d_launch_shell(etype, PF3D_KG, a, (hipStream_t)stream),
+                          "pf3d_shell_kg");
+}
+
+int pf3d_shell_kg_stress(int etype, int64_t n_elem, const double *x, const int64_t *conn,
+                         const int64_t *init_k, const double *xmat, const double *stress,
+                         int stress_per_elem, int64_t *r, int64_t *c, double *v, int flags,
+                         void *stream)
+{
+    int st = check_common(etype, n_elem, x, conn, r, c, v, flags);
+    if (st) return st > 0 ? PF3D_OK : st;
+    if (!stress) return pf3d_fail(PF3D_E_ARG, "null stress");
+    pf3d::AsmArgs a{};
+    a.n_elem = n_elem; a.x = x; a.conn = conn; a.init_k = init_k; a.evec = xmat;
+    a.pid = nullptr; a.props = nullptr; a.stress = stress;
+    a.stress_per_elem = stress_per_elem; a.flags = flags; a.r = r; a.c = c; a.v = v;
+    return pf3d_hip_check(pf3d_launch_shell(etype, PF3D_KG_STRESS, a, (hipStream_t)stream),
+                          "pf3d_shell_kg_stress");
+}
+
+int pf3d_shell_m(int etype, int64_t n_elem, const double *x, const int64_t *conn,
+                 const int64_t *init_k, const double *xmat, const int32_t *pid,
+                 const double *props, int mtype, int64_t *r, int64_t *c, double *v, int flags,
+                 void *stream)
+{
+    int st = check_common(etype, n_elem, x, conn, r, c, v, flags);
+    if (st) return st > 0 ? PF3D_OK : st;
+    if (!props) return pf3d_fail(PF3D_E_ARG, "null props");
+    if (mtype < 0 || mtype > 2) return pf3d_fail(PF3D_E_ARG, "bad mtype %d", mtype);
+    pf3d::AsmArgs a{};
+    a.n_elem = n_elem; a.x = x; a.conn = conn; a.init_k = init_k; a.evec = xmat; a.pid = pid;
+    a.props = props; a.mtype = mtype; a.flags = flags; a.r = r; a.c = c; a.v = v;
+    return pf3d_hip_check(pf3d_launch_shell(etype, PF3D_M, a, (hipStream_t)stream),
+                          "pf3d_shell_m");
+}
+
+int pf3d_beam_kc0(int64_t n_elem, const double *x, const int64_t *conn, const int64_t *init_k,
+                  const double *vxy, const int32_t *pid, const double *props, int64_t *r,
+                  int64_t *c, double *v, int flags, void *stream)
+{
+    if (n_elem < 0) return pf3d_fail(PF3D_E_ARG, "negative n_elem");
+    if (n_elem == 0) return PF3D_OK;
+    if (!x || !conn || !v || !vxy || !props) return pf3d_fail(PF3D_E_ARG, "null argument");
+    if (!(flags & PF3D_VALUES_ONLY) && (!r || !c))
+        return pf3d_fail(PF3D_E_ARG, "null r/c without PF3D_VALUES_ONLY");
+    pf3d::AsmArgs a{};
+    a.n_elem = n_elem; a.x = x; a.conn = conn; a.init_k = init_k; a.evec = vxy; a.pid = pid;
+    a.props = props; a.flags = flags; a.r = r; a.c = c; a.v = v;
+    return pf3d_hip_check(pf3d_launch_beam_kc0(a, (hipStream_t)stream), "pf3d_beam_kc0");
+}
+
+}  // extern "C"

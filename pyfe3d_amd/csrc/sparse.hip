@@ -1,0 +1,243 @@
+// COO -> 6x6 block-sparse-row (BSR) conversion with a deterministic
+// segmented sum, replacing scipy's coo_matrix((v, (r, c))).tocsc() on the
+// reference's caller side (e.g. tests/test_quad4_static_point_load.py:80).
+//
+// Plan (once per mesh): sort the slots by (block row, block col, row%6,
+// col%6), keep the slot permutation, the start of each run of equal
+// (row, col) and the BSR position of each run.  Apply (every re-assembly):
+// each output entry sums its run of slots in slot order, so results are
+// bit-identical from run to run and independent of the launch shape.
+#include <hipcub/hipcub.hpp>
+
+#include "../../include/pf3d.h"
+#include "pf3d_device.hpp"
+
+int pf3d_fail(int code, const char *fmt, ...);
+int pf3d_hip_check(hipError_t e, const char *where);
+
+namespace {
+
+struct PlanWork {
+    // offsets (bytes) into the caller's workspace
+    size_t keys_in, keys_out, ids_in, ids_out, uhead, bhead, temp, total;
+    size_t temp_bytes;
+};
+
+inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+int end_bit_for(int64_t n_nodes)
+{
+    // keys < n_nodes^2 * 36
+    double maxkey = (double)n_nodes * (double)n_nodes * 36.0;
+    int b = 1;
+    while (b < 63 && (double)(1ULL << b) < maxkey) b++;
+    return b;
+}
+
+hipError_t layout(int64_t n, int64_t n_nodes, PlanWork &w)
+{
+    size_t a8 = align256(sizeof(int64_t) * (size_t)n);
+    size_t sort_bytes = 0, scan_bytes = 0;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(
+        nullptr, sort_bytes, (const int64_t *)nullptr, (int64_t *)nullptr,
+        (const int64_t *)nullptr, (int64_t *)nullptr, (int)0, 0, end_bit_for(n_nodes));
+    if (e != hipSuccess) return e;
+    e = hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (int64_t *)nullptr,
+                                         (int64_t *)nullptr, (int)0);
+    if (e != hipSuccess) return e;
+    // the size queries above do not depend on n for these primitives'
+    // upper bounds at n = 0; query again with the real size
+    sort_bytes = 0;
+    scan_bytes = 0;
+    e = hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (const int64_t *)nullptr,
+                                           (int64_t *)nullptr, (const int64_t *)nullptr,
+                                           (int64_t *)nullptr, n, 0, end_bit_for(n_nodes));
+    if (e != hipSuccess) return e;
+    e = hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (int64_t *)nullptr,
+                                         (int64_t *)nullptr, n);
+    if (e != hipSuccess) return e;
+    w.temp_bytes = align256(sort_bytes > scan_bytes ? sort_bytes : scan_bytes);
+    size_t o = 0;
+    w.keys_in = o; o += a8;
+    w.keys_out = o; o += a8;
+    w.ids_in = o; o += a8;
+    w.ids_out = o; o += a8;
+    w.uhead = o; o += a8;
+    w.bhead = o; o += a8;
+    w.temp = o; o += w.temp_bytes;
+    w.total = o;
+    return hipSuccess;
+}
+
+__global__ void make_keys(int64_t n, const int64_t *__restrict__ r, const int64_t *__restrict__ c,
+                          int64_t n_nodes, int64_t *__restrict__ keys, int64_t *__restrict__ ids)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int64_t ri = r[t], ci = c[t];
+    keys[t] = ((ri / 6) * n_nodes + ci / 6) * 36 + (ri % 6) * 6 + ci % 6;
+    ids[t] = t;
+}
+
+__global__ void mark_heads(int64_t n, const int64_t *__restrict__ keys, int64_t *__restrict__ uh,
+                           int64_t *__restrict__ bh)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int64_t k = keys[t];
+    const int64_t kp = t ? keys[t - 1] : -1;
+    uh[t] = (t == 0 || k != kp) ? 1 : 0;
+    bh[t] = (t == 0 || k / 36 != kp / 36) ? 1 : 0;
+}
+
+// after the exclusive scans uh/bh hold run and block ids; heads are
+// recognised from the keys again
+__global__ void emit(int64_t n, int64_t n_nodes, const int64_t *__restrict__ keys,
+                     const int64_t *__restrict__ uid, const int64_t *__restrict__ bid,
+                     int64_t *__restrict__ seg, int64_t *__restrict__ dest,
+                     int32_t *__restrict__ col_idx, int32_t *__restrict__ row_ptr,
+                     int64_t n_unique, int64_t nnzb)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int64_t k = keys[t];
+    const int64_t kp = t ? keys[t - 1] : -1;
+    if (t == 0 || k != kp) {
+        seg[uid[t]] = t;
+        dest[uid[t]] = bid[t] * 36 + k % 36;
+    }
+    if (t == 0 || k / 36 != kp / 36) {
+        const int64_t b = bid[t], bk = k / 36;
+        const int64_t row = bk / n_nodes;
+        col_idx[b] = (int32_t)(bk % n_nodes);
+        const int64_t prow = t ? (kp / 36) / n_nodes : -1;
+        for (int64_t rr = prow + 1; rr <= row; rr++) row_ptr[rr] = (int32_t)b;
+        if (b == nnzb - 1)
+            for (int64_t rr = row + 1; rr <= n_nodes; rr++) row_ptr[rr] = (int32_t)nnzb;
+    }
+    if (t == n - 1) seg[n_unique] = n;
+}
+
+__global__ void seg_sum(int64_t n_unique, const int64_t *__restrict__ perm,
+                        const int64_t *__restrict__ seg, const int64_t *__restrict__ dest,
+                        const double *__restrict__ v, double *__restrict__ out)
+{
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_unique) return;
+    const int64_t a = seg[j], b = seg[j + 1];
+    double s = 0.;
+    for (int64_t t = a; t < b; t++) s += v[perm[t]];
+    out[dest[j]] = s;
+}
+
+__global__ void fill_zero(int64_t n, double *__restrict__ p)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = 0.;
+}
+
+inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+extern "C" {
+
+int pf3d_bsr_plan_bytes(int64_t n_slots, int64_t n_nodes, int64_t *bytes)
+{
+    if (n_slots <= 0 || n_nodes <= 0 || !bytes)
+        return pf3d_fail(PF3D_E_ARG, "pf3d_bsr_plan_bytes: bad arguments");
+    if (n_nodes > 2147483647) return pf3d_fail(PF3D_E_ARG, "too many nodes for int32 BSR");
+    PlanWork w;
+    hipError_t e = layout(n_slots, n_nodes, w);
+    if (e != hipSuccess) return pf3d_hip_check(e, "pf3d_bsr_plan_bytes");
+    *bytes = (int64_t)w.total;
+    return PF3D_OK;
+}
+
+int pf3d_bsr_plan_sort(int64_t n_slots, const int64_t *r, const int64_t *c, int64_t n_nodes,
+                       void *work, int64_t work_bytes, int64_t *counts, void *stream)
+{
+    if (n_slots <= 0 || !r || !c || !work || !counts)
+        return pf3d_fail(PF3D_E_ARG, "pf3d_bsr_plan_sort: bad arguments");
+    PlanWork w;
+    hipError_t e = layout(n_slots, n_nodes, w);
+    if (e != hipSuccess) return pf3d_hip_check(e, "pf3d_bsr_plan_sort");
+    if ((int64_t)w.total > work_bytes)
+        return pf3d_fail(PF3D_E_ARG, "workspace too small: %lld < %lld", (long long)work_bytes,
+                         (long long)w.total);
+    hipStream_t s = (hipStream_t)stream;
+    char *base = (char *)work;
+    int64_t *kin = (int64_t *)(base + w.keys_in), *kout = (int64_t *)(base + w.keys_out);
+    int64_t *iin = (int64_t *)(base + w.ids_in), *iout = (int64_t *)(base + w.ids_out);
+    int64_t *uh = (int64_t *)(base + w.uhead), *bh = (int64_t *)(base + w.bhead);
+    void *tmp = base + w.temp;
+    hipLaunchKernelGGL(make_keys, dim3(nblk(n_slots, 256)), dim3(256), 0, s, n_slots, r, c,
+                       n_nodes, kin, iin);
+    size_t tb = w.temp_bytes;
+    e = hipcub::DeviceRadixSort::SortPairs(tmp, tb, kin, kout, iin, iout, n_slots, 0,
+                                           end_bit_for(n_nodes), s);
+    if (e != hipSuccess) return pf3d_hip_check(e, "radix sort");
+    hipLaunchKernelGGL(mark_heads, dim3(nblk(n_slots, 256)), dim3(256), 0, s, n_slots, kout, uh,
+                       bh);
+    // last flags are needed for the totals: keep copies before the scans
+    int64_t last[2];
+    e = hipMemcpyAsync(&last[0], uh + n_slots - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(&last[1], bh + n_slots - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return pf3d_hip_check(e, "copy flags");
+    // run and block ids (exclusive scans, out of place into the input
+    // buffers of the sort, which are free now)
+    tb = w.temp_bytes;
+    e = hipcub::DeviceScan::ExclusiveSum(tmp, tb, uh, kin, n_slots, s);
+    if (e != hipSuccess) return pf3d_hip_check(e, "scan");
+    tb = w.temp_bytes;
+    e = hipcub::DeviceScan::ExclusiveSum(tmp, tb, bh, iin, n_slots, s);
+    if (e != hipSuccess) return pf3d_hip_check(e, "scan");
+    int64_t tot[2];
+    e = hipMemcpyAsync(&tot[0], kin + n_slots - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(&tot[1], iin + n_slots - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return pf3d_hip_check(e, "plan totals");
+    counts[0] = tot[0] + last[0];  // unique (row, col) entries
+    counts[1] = tot[1] + last[1];  // 6x6 blocks
+    if (counts[1] > 2147483647) return pf3d_fail(PF3D_E_ARG, "too many blocks for int32 BSR");
+    return PF3D_OK;
+}
+
+int pf3d_bsr_plan_emit(int64_t n_slots, int64_t n_nodes, void *work, int64_t work_bytes,
+                       int64_t n_unique, int64_t nnzb, int64_t *perm, int64_t *seg,
+                       int64_t *dest, int32_t *row_ptr, int32_t *col_idx, void *stream)
+{
+    if (n_slots <= 0 || !work || !perm || !seg || !dest || !row_ptr || !col_idx)
+        return pf3d_fail(PF3D_E_ARG, "pf3d_bsr_plan_emit: bad arguments");
+    PlanWork w;
+    hipError_t e = layout(n_slots, n_nodes, w);
+    if (e != hipSuccess) return pf3d_hip_check(e, "pf3d_bsr_plan_emit");
+    if ((int64_t)w.total > work_bytes) return pf3d_fail(PF3D_E_ARG, "workspace too small");
+    hipStream_t s = (hipStream_t)stream;
+    char *base = (char *)work;
+    const int64_t *kout = (const int64_t *)(base + w.keys_out);
+    const int64_t *iout = (const int64_t *)(base + w.ids_out);
+    const int64_t *uid = (const int64_t *)(base + w.keys_in), *bid = (const int64_t *)(base + w.ids_in);
+    e = hipMemcpyAsync(perm, iout, sizeof(int64_t) * n_slots, hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return pf3d_hip_check(e, "copy perm");
+    hipLaunchKernelGGL(emit, dim3(nblk(n_slots, 256)), dim3(256), 0, s, n_slots, n_nodes, kout,
+                       uid, bid, seg, dest, col_idx, row_ptr, n_unique, nnzb);
+    return pf3d_hip_check(hipGetLastError(), "pf3d_bsr_plan_emit");
+}
+
+int pf3d_bsr_apply(int64_t n_unique, const int64_t *perm, const int64_t *seg, const int64_t *dest,
+                   const double *v, int64_t nnzb, double *bsr_val, void *stream)
+{
+    if (n_unique <= 0 || !perm || !seg || !dest || !v || !bsr_val)
+        return pf3d_fail(PF3D_E_ARG, "pf3d_bsr_apply: bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(fill_zero, dim3(nblk(nnzb * 36, 256)), dim3(256), 0, s, nnzb * 36,
+                       bsr_val);
+    hipLaunchKernelGGL(seg_sum, dim3(nblk(n_unique, 256)), dim3(256), 0, s, n_unique, perm, seg,
+                       dest, v, bsr_val);
+    return pf3d_hip_check(hipGetLastError(), "pf3d_bsr_apply");
+}
+
+}  // extern "C"

@@ -1,0 +1,96 @@
+"""Static solve K u = f on the GPU: 6x6 block-Jacobi PCG in FP64, or FP32
+PCG inside an FP64 iterative-refinement loop.
+
+Replaces the reference callers' free-DOF slicing and scipy solve
+(``KC0[bu, :][:, bu]`` then ``spsolve``/``cg``, e.g.
+tests/test_quad4_static_point_load.py:84-102).  Fixed DOFs are removed by
+P K P + (I - P) on the assembled BSR matrix, so u[fixed] = 0 and u[free]
+is the solution of the sliced system.  Convergence is judged on the FP64
+residual ||f - K u|| / ||f||.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+def _s(stream):
+    return (stream if stream is not None else torch.cuda.current_stream()).cuda_stream
+
+
+class StaticSystem:
+    def __init__(self, plan, val, fixed, stream=None):
+        """plan: BSRPlan; val: its BSR values (copied); fixed: bool per DOF."""
+        self.plan = plan
+        nb = plan.n_nodes
+        self.nb = nb
+        dev = val.device
+        fixed = torch.as_tensor(np.asarray(fixed, dtype=np.uint8) if not torch.is_tensor(fixed)
+                                else fixed.to(torch.uint8), device=dev).contiguous()
+        if fixed.numel() != 6 * nb:
+            raise ValueError('fixed must have one entry per DOF')
+        self.fixed = fixed
+        self.val = val.clone()
+        s = _s(stream)
+        _lib.call('pf3d_bsr_dirichlet', nb, plan.row_ptr.data_ptr(), plan.col_idx.data_ptr(),
+                  self.val.data_ptr(), fixed.data_ptr(), s)
+        self.dinv = torch.empty(nb * 36, dtype=torch.float64, device=dev)
+        self.dinv32 = torch.empty(nb * 36, dtype=torch.float32, device=dev)
+        bad = torch.zeros(1, dtype=torch.int32, device=dev)
+        _lib.call('pf3d_bjacobi', nb, plan.row_ptr.data_ptr(), plan.col_idx.data_ptr(),
+                  self.val.data_ptr(), self.dinv.data_ptr(), self.dinv32.data_ptr(),
+                  bad.data_ptr(), s)
+        self.val32 = torch.empty(self.val.numel(), dtype=torch.float32, device=dev)
+        _lib.call('pf3d_to_f32', self.val.numel(), self.val.data_ptr(), self.val32.data_ptr(), s)
+        if int(bad.item()):
+            raise _lib.Pf3dError('%d singular 6x6 diagonal blocks' % int(bad.item()))
+        nbytes = ctypes.c_int64()
+        _lib.call('pf3d_pcg_bytes', nb, 0, ctypes.byref(nbytes))
+        self._w64 = torch.empty(nbytes.value, dtype=torch.uint8, device=dev)
+        _lib.call('pf3d_ir_bytes', nb, ctypes.byref(nbytes))
+        self._wir = torch.empty(nbytes.value, dtype=torch.uint8, device=dev)
+
+    def matvec(self, x, y=None, stream=None):
+        if y is None:
+            y = torch.empty_like(x)
+        _lib.call('pf3d_bsr_spmv', self.nb, self.plan.row_ptr.data_ptr(),
+                  self.plan.col_idx.data_ptr(), self.val.data_ptr(), x.data_ptr(), y.data_ptr(),
+                  _s(stream))
+        return y
+
+    def rhs(self, f):
+        """P f on the device (fixed DOFs zeroed)."""
+        f = torch.as_tensor(f, dtype=torch.float64, device=self.val.device).clone()
+        f[self.fixed.bool()] = 0.
+        return f
+
+    def solve(self, f, method='fp64', rtol=1e-8, x0=None, maxiter=200000, inner_rtol=1e-4,
+              max_outer=60, stream=None):
+        """Returns (u, info) with info = dict(iters, outer, relres)."""
+        b = self.rhs(f)
+        x = torch.zeros_like(b) if x0 is None else x0.clone()
+        pr, pc = self.plan.row_ptr.data_ptr(), self.plan.col_idx.data_ptr()
+        relres = ctypes.c_double()
+        if method == 'fp64':
+            iters = ctypes.c_int64()
+            _lib.call('pf3d_pcg', self.nb, pr, pc, self.val.data_ptr(), self.dinv.data_ptr(),
+                      b.data_ptr(), x.data_ptr(), float(rtol), int(maxiter),
+                      int(x0 is not None), self._w64.data_ptr(), self._w64.numel(),
+                      ctypes.byref(iters), ctypes.byref(relres), _s(stream))
+            info = dict(iters=iters.value, outer=0)
+        elif method == 'mixed':
+            stats = (ctypes.c_int64 * 2)()
+            _lib.call('pf3d_ir_solve', self.nb, pr, pc, self.val.data_ptr(),
+                      self.val32.data_ptr(), self.dinv32.data_ptr(), b.data_ptr(),
+                      x.data_ptr(), float(rtol), float(inner_rtol), int(max_outer),
+                      int(maxiter), int(x0 is not None), self._wir.data_ptr(),
+                      self._wir.numel(), stats, ctypes.byref(relres), _s(stream))
+            info = dict(iters=int(stats[1]), outer=int(stats[0]))
+        else:
+            raise ValueError(method)
+        # true FP64 residual on the FP64 matrix
+        r = b - self.matvec(x, stream=stream)
+        info['relres'] = float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(b))
+        return x, info

@@ -1,0 +1,55 @@
+"""The C-ABI library loads and exports every entry point that
+include/pf3d.h declares (no compute calls; runs without a GPU)."""
+import ctypes
+import os
+import re
+
+from pyfe3d_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, 'include')):
+        if fn.endswith('.h'):
+            text = open(os.path.join(ROOT, 'include', fn)).read()
+            text = re.sub(r'/\*.*?\*/', '', text, flags=re.S)
+            names |= set(re.findall(r'\b(pf3d_\w+)\s*\(', text))
+    return names
+
+
+def test_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = declared()
+    assert len(names) >= 8
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_bound_signatures_are_declared():
+    assert set(_lib.SIGNATURES) <= declared()
+
+
+def test_host_only_queries():
+    lib = _lib.load()
+    assert lib.pf3d_version() > 0
+    assert lib.pf3d_slot_count(0, 0, 0) == 576
+    assert lib.pf3d_slot_count(2, 0, 0) == 324
+    assert lib.pf3d_slot_count(3, 0, 0) == 144
+    assert lib.pf3d_slot_count(0, 1, 0) == 144
+    assert lib.pf3d_slot_count(0, 3, 2) == 288
+    assert lib.pf3d_slot_count(2, 3, 0) == 270
+    assert lib.pf3d_slot_count(9, 0, 0) < 0
+    assert b'etype' in lib.pf3d_last_error()
+
+
+def test_argument_errors_need_no_device():
+    lib = _lib.load()
+    # negative size and null pointers are rejected before any HIP call
+    st = lib.pf3d_shell_kc0(0, -1, None, None, None, None, None, None, None, None, None,
+                            None, None, None, 0, None)
+    assert st == -1
+    st = lib.pf3d_shell_kc0(0, 10, None, None, None, None, None, None, None, None, None,
+                            None, None, None, 0, None)
+    assert st == -1 and b'null' in lib.pf3d_last_error()

@@ -1,0 +1,166 @@
+"""GPU element assembly through the C ABI against the reference's golden
+vectors and against the CPU oracle (indices bit-exact; values within
+1e-12 of each element's largest entry)."""
+import numpy as np
+import pytest
+import torch
+
+from golden_cases import SHELL_MATS, SHELLS, TOL, TOL_CASE, cases, element_errors
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+DEV = 'cuda'
+
+
+def T(a, dtype=None):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=DEV)
+
+
+def gpu_shell(etype, d, key, mat, mtype, values_only=False, overwrite=False, v0=None):
+    from pyfe3d_amd import assembly as A
+    E = d['conn'].shape[0]
+    size = d[key + 'v'].size // E
+    r = torch.zeros(size * E, dtype=torch.int64, device=DEV)
+    c = torch.zeros_like(r)
+    v = torch.zeros(size * E, dtype=torch.float64, device=DEV) if v0 is None else T(v0)
+    x = T(d['x'], torch.float64)
+    conn = T(6 * d['conn'], torch.int64)
+    init_k = T(size * d['perm'], torch.int64)
+    xmat = T(d['xmat'], torch.float64)
+    props = T(O.props_table(d['props'], etype), torch.float64)
+    pid = T(d['pid'], torch.int32)
+    kw = dict(init_k=init_k, xmat=xmat, values_only=values_only, overwrite=overwrite)
+    if mat == 'KC0':
+        A.shell_kc0(etype, x, conn, r, c, v, props, pid=pid, k6rot=T(d['k6rot'], torch.float64),
+                    alpha=T(d['alpha'], torch.float64), hg=T(d['hg'], torch.float64), **kw)
+    elif mat == 'KG':
+        A.shell_kg(etype, x, conn, T(d['u'], torch.float64), r, c, v, props, pid=pid, **kw)
+    elif mat == 'KGs':
+        A.shell_kg_stress(etype, x, conn, T(d['stress'], torch.float64), r, c, v, **kw)
+    else:
+        A.shell_m(etype, x, conn, r, c, v, props, mtype=mtype, pid=pid, **kw)
+    torch.cuda.synchronize()
+    return r.cpu().numpy(), c.cpu().numpy(), v.cpu().numpy(), size
+
+
+@pytest.mark.parametrize('etype', SHELLS)
+def test_shell_golden_gpu(etype):
+    bad = []
+    for name, d in cases(etype):
+        for key, mat, mtype in SHELL_MATS:
+            r, c, v, size = gpu_shell(etype, d, key, mat, mtype)
+            if not ((r == d[key + 'r']).all() and (c == d[key + 'c']).all()):
+                bad.append((name, key, 'indices'))
+            err = element_errors(v, d[key + 'v'], size, d['perm'])
+            print(etype, name, key, 'max rel err %.2e' % err)
+            if not err <= TOL_CASE.get((etype, name), TOL):
+                bad.append((name, key, err))
+    assert not bad, bad
+
+
+@pytest.mark.parametrize('etype', SHELLS)
+def test_geometry_gpu(etype):
+    from pyfe3d_amd import assembly as A
+    for name, d in cases(etype):
+        g, xe = A.geometry(etype, T(d['x'], torch.float64), T(6 * d['conn'], torch.int64),
+                           T(d['xmat'], torch.float64))
+        np.testing.assert_allclose(g.cpu().numpy(), d['geom'], rtol=0, atol=1e-14)
+        np.testing.assert_allclose(xe.cpu().numpy(), d['xe'], rtol=0, atol=1e-14)
+
+
+def test_beam_golden_gpu():
+    from pyfe3d_amd import assembly as A
+    for name, d in cases('beamc'):
+        E = d['conn'].shape[0]
+        r = torch.zeros(144 * E, dtype=torch.int64, device=DEV)
+        c = torch.zeros_like(r)
+        v = torch.zeros(144 * E, dtype=torch.float64, device=DEV)
+        A.beam_kc0(T(d['x'], torch.float64), T(6 * d['conn'], torch.int64),
+                   T(d['vxy'], torch.float64), r, c, v,
+                   T(O.props_table(d['props'], 'beamc'), torch.float64),
+                   pid=T(np.arange(E), torch.int32), init_k=T(144 * d['perm'], torch.int64))
+        assert (r.cpu().numpy() == d['KC0r']).all() and (c.cpu().numpy() == d['KC0c']).all()
+        assert element_errors(v.cpu().numpy(), d['KC0v'], 144, d['perm']) <= TOL
+
+
+def test_accumulate_overwrite_values_only():
+    """v += (default), v = (overwrite), indices untouched with values_only
+    (quad4.pyx:1278, 1313)."""
+    name, d = next(cases('quad4'))
+    E = d['conn'].shape[0]
+    ref = d['KC0v']
+    base = np.full(ref.size, 3.25)
+    _, _, v, _ = gpu_shell('quad4', d, 'KC0', 'KC0', 0, v0=base)
+    assert element_errors(v - 3.25, ref, 576, d['perm']) <= TOL
+    r, c, v, _ = gpu_shell('quad4', d, 'KC0', 'KC0', 0, v0=base, overwrite=True,
+                           values_only=True)
+    assert element_errors(v, ref, 576, d['perm']) <= TOL
+    assert (r == 0).all() and (c == 0).all()
+
+
+def _random_mesh(nx, ny, seed):
+    rng = np.random.default_rng(seed)
+    xs, ys = np.meshgrid(np.linspace(0, 1.0, nx + 1), np.linspace(0, 0.7, ny + 1), indexing='ij')
+    X = xs.copy(); Y = ys.copy()
+    X[1:-1, 1:-1] += (1.0 / nx) * rng.uniform(-0.2, 0.2, (nx - 1, ny - 1))
+    Y[1:-1, 1:-1] += (0.7 / ny) * rng.uniform(-0.2, 0.2, (nx - 1, ny - 1))
+    Z = 0.01 * np.sin(3 * X) * np.cos(2 * Y)
+    x = np.vstack([X.ravel(), Y.ravel(), Z.ravel()]).T.ravel()
+    nid = np.arange((nx + 1) * (ny + 1)).reshape(nx + 1, ny + 1)
+    conn = np.stack([nid[:-1, :-1].ravel(), nid[1:, :-1].ravel(), nid[1:, 1:].ravel(),
+                     nid[:-1, 1:].ravel()], axis=1)
+    return x, conn, rng
+
+
+@pytest.mark.parametrize('etype', SHELLS)
+def test_random_mesh_vs_oracle(etype):
+    """A few thousand jittered, warped elements with per-element laminates."""
+    from pyfe3d_amd import assembly as A
+    x, quads, rng = _random_mesh(40, 30, 7)
+    conn = quads if etype != 'tria3r' else np.concatenate(
+        [quads[:, [0, 1, 2]], quads[:, [0, 2, 3]]])
+    E = conn.shape[0]
+    props = np.zeros((17, 32))
+    for p in range(17):
+        A_ = rng.uniform(0.5, 1.5, 6) * 1e8; A_[2] *= 0.1; A_[4] *= 0.1
+        props[p, 0:6] = A_
+        props[p, 6:12] = rng.uniform(-1, 1, 6) * 1e3
+        props[p, 12:18] = A_ * 1e-6
+        props[p, 18:21] = [3e7, 2e6, 4e7]
+        props[p, 21:24] = [0.83, 0.81, 0.002 * (1 + p % 3)]
+        props[p, 25:28] = [5., 1e-4, 1e-6]
+    pid = rng.integers(0, 17, E)
+    xmat = rng.normal(size=(E, 3))
+    k6rot = rng.uniform(1, 1000, E)
+    alpha = rng.uniform(0.2, 1, E)
+    hg = rng.uniform(0.5, 2, (E, 5))
+    u = rng.normal(size=x.size * 2) * 1e-3
+    for mat, mtype in [('KC0', 0), ('KG', 0), ('M', 0)]:
+        size = O.slot_count(etype, mat, mtype)
+        rr = np.zeros(size * E, np.int64); cc = np.zeros_like(rr); vv = np.zeros(size * E)
+        O.assemble(etype, mat, x, conn, size * np.arange(E), xmat, pid, props, rr, cc, vv,
+                   mtype=mtype, k6rot=k6rot, alpha=alpha, hg=hg, u=u)
+        r = torch.zeros(size * E, dtype=torch.int64, device=DEV)
+        c = torch.zeros_like(r)
+        v = torch.zeros(size * E, dtype=torch.float64, device=DEV)
+        args = (T(x, torch.float64), T(6 * conn, torch.int64))
+        kw = dict(pid=T(pid, torch.int32), xmat=T(xmat, torch.float64))
+        if mat == 'KC0':
+            A.shell_kc0(etype, *args, r, c, v, T(props, torch.float64), k6rot=T(k6rot, torch.float64),
+                        alpha=T(alpha, torch.float64), hg=T(hg, torch.float64), **kw)
+        elif mat == 'KG':
+            A.shell_kg(etype, *args, T(u, torch.float64), r, c, v, T(props, torch.float64), **kw)
+        else:
+            A.shell_m(etype, *args, r, c, v, T(props, torch.float64), mtype=mtype, **kw)
+        assert (r.cpu().numpy() == rr).all() and (c.cpu().numpy() == cc).all()
+        err = element_errors(v.cpu().numpy(), vv, size, np.arange(E))
+        assert err <= TOL, (etype, mat, err)
+
+
+def test_empty_batch_is_a_no_op():
+    from pyfe3d_amd import assembly as A
+    x = torch.zeros(12, dtype=torch.float64, device=DEV)
+    conn = torch.zeros((0, 4), dtype=torch.int64, device=DEV)
+    v = torch.zeros(0, dtype=torch.float64, device=DEV)
+    A.shell_kc0('quad4', x, conn, v.long(), v.long(), v, torch.zeros((1, 32), dtype=torch.float64,
+                                                                          device=DEV))
