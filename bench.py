@@ -1,0 +1,236 @@
+"""Benchmark of the stiffness-assembly-and-solve hot path on one MI355X.
+
+Headline (``value``): KC0 assembly throughput, elements/s, on the
+1000x1000 Quad4 laminate plate (1,002,001 nodes, 6,012,006 DOF; interior
+nodes jittered +-0.2 element size, seeded; 8 plies of 0.125 mm with seeded
+angles from {0, 45, -45, 90}; E1=140e9, E2=10e9, G12=G13=G23=5e9,
+nu12=0.3; a x b = 1.0 x 0.7 m).  One step = one KC0 assembly of all 1e6
+elements into the 576e6-slot value array (values only, overwrite: the
+design-sweep re-assembly of the reference's tutorials), inputs resident in
+HBM.  Alongside: the same assembly with int64 row/col indices, KG (given
+stress) and M, COO->BSR apply, and static solves per second at 1e-8
+relative residual (plain FP64 PCG vs FP32/FP64 iterative refinement).
+
+Prints one JSON line.  ``--gpus N`` with N > 1 exits: this build measures
+one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes of one Quad4 KC0 element, values only: 576 doubles out,
+# 4 int64 DOF offsets and 12 coordinates in
+KC0_BYTES_VALUES = 576 * 8 + 4 * 8 + 12 * 8
+KC0_BYTES_INDICES = KC0_BYTES_VALUES + 576 * 16
+
+
+def laminate_record(seed=2026):
+    from pyfe3d_amd.shellprop import laminated_plate
+    rng = np.random.default_rng(seed)
+    stack = [float(a) for a in rng.choice([0., 45., -45., 90.], 8)]
+    lp = (140e9, 10e9, 0.3, 5e9, 5e9, 5e9)
+    prop = laminated_plate(stack, plyt=0.000125, laminaprop=lp, rho=1600., calc_scf=True)
+    return prop, stack
+
+
+def build_plate(n, seed=2026):
+    from pyfe3d_amd.mesh import PlateMesh
+    return PlateMesh(n + 1, n + 1, 1.0, 0.7, jitter=0.2, seed=seed)
+
+
+def timed(fn, steps, warmup, stream):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(steps):
+        fn()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    return ev0.elapsed_time(ev1) / steps, wall
+
+
+def cpu_baseline(mesh, prop, n_sample):
+    """The oracle (plain-C restatement of the reference's per-element loop,
+    with its index writes) on one host core over the first n_sample elements
+    of the same mesh."""
+    from oracle import oracle as O
+    q = mesh.quads[:n_sample]
+    E = q.shape[0]
+    r = np.zeros(576 * E, np.int64)
+    c = np.zeros_like(r)
+    v = np.zeros(576 * E)
+    t0 = time.perf_counter()
+    O.assemble('quad4', 'KC0', mesh.x, q, 576 * np.arange(E), np.zeros((E, 3)),
+               np.zeros(E, np.int64), prop.record()[None], r, c, v)
+    dt = time.perf_counter() - t0
+    return dict(value=E / dt, unit='elements/s', cores=1, kind='port',
+                sample='KC0 of the first %d elements of the 1000x1000 laminate plate '
+                       '(oracle, indices + values, %.1f s)' % (E, dt))
+
+
+def solve_rates(n, repeats, log):
+    """Static solves/s of an n x n clamped laminate plate under uniform
+    pressure, to ||f - K u|| <= 1e-8 ||f|| (FP64 residual on FP64 K)."""
+    import torch
+    from pyfe3d_amd import assembly as A
+    from pyfe3d_amd.sparse import BSRPlan
+    from pyfe3d_amd.solver import StaticSystem
+    mesh = build_plate(n)
+    prop, _ = laminate_record()
+    dev = 'cuda'
+    E = mesh.quads.shape[0]
+    r = torch.empty(576 * E, dtype=torch.int64, device=dev)
+    c = torch.empty_like(r)
+    v = torch.zeros(576 * E, dtype=torch.float64, device=dev)
+    A.shell_kc0('quad4', torch.as_tensor(mesh.x, device=dev),
+                torch.as_tensor(6 * mesh.quads, device=dev), r, c, v,
+                torch.as_tensor(prop.record()[None], device=dev))
+    plan = BSRPlan(r, c, mesh.n_nodes)
+    sysm = StaticSystem(plan, plan.apply(v), mesh.clamped())
+    f = mesh.pressure_load(1e3)
+    out = {}
+    for method in ('fp64', 'mixed'):
+        u, info = sysm.solve(f, method=method, rtol=1e-8)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(repeats):
+            u, info = sysm.solve(f, method=method, rtol=1e-8)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / repeats
+        out[method] = dict(solves_per_s=1. / dt, ms=dt * 1e3, iters=info['iters'],
+                           outer=info['outer'], relres=info['relres'])
+        log('solve %dx%d %s: %.2f ms, %s' % (n, n, method, dt * 1e3, info))
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--n', type=int, default=1000, help='elements per side')
+    ap.add_argument('--cpu-sample', type=int, default=200000)
+    ap.add_argument('--no-solve', action='store_true')
+    ap.add_argument('--quiet', action='store_true')
+    args = ap.parse_args()
+    log = (lambda *a: None) if args.quiet else (lambda *a: print(*a, file=sys.stderr, flush=True))
+    if args.gpus != 1:
+        print('bench.py measures one GPU: this build has no multi-GPU path '
+              '(run with --gpus 1)', file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from pyfe3d_amd import assembly as A
+    dev = 'cuda'
+    stream = torch.cuda.current_stream()
+
+    t0 = time.perf_counter()
+    mesh = build_plate(args.n)
+    prop, stack = laminate_record()
+    E = mesh.quads.shape[0]
+    log('mesh %d elements, %d nodes (%.1f s)' % (E, mesh.n_nodes, time.perf_counter() - t0))
+    x = torch.as_tensor(mesh.x, device=dev)
+    conn = torch.as_tensor(6 * mesh.quads, device=dev)
+    props = torch.as_tensor(prop.record()[None], device=dev)
+    v = torch.empty(576 * E, dtype=torch.float64, device=dev)
+
+    # validate the arrays once; the timed loops launch without host checks
+    A.shell_kc0('quad4', x, conn, None, None, v, props, values_only=True, overwrite=True)
+
+    def kc0_values():
+        A.shell_kc0('quad4', x, conn, None, None, v, props, values_only=True, overwrite=True,
+                    check=False)
+
+    ms, wall = timed(kc0_values, args.steps, args.warmup, stream)
+    log('KC0 values-only: %.3f ms/step (wall %.3f)' % (ms, wall))
+    achieved = KC0_BYTES_VALUES * E / (ms * 1e-3) / 1e9
+
+    extra = {}
+    r = torch.empty(576 * E, dtype=torch.int64, device=dev)
+    c = torch.empty_like(r)
+
+    def kc0_indices():
+        A.shell_kc0('quad4', x, conn, r, c, v, props, overwrite=True, check=False)
+
+    ms_i, _ = timed(kc0_indices, max(3, args.steps // 4), 1, stream)
+    extra['kc0_with_indices'] = dict(ms=ms_i, elements_per_s=E / (ms_i * 1e-3),
+                                     gbs=KC0_BYTES_INDICES * E / (ms_i * 1e-3) / 1e9)
+    log('KC0 with indices: %.3f ms' % ms_i)
+
+    def kc0_accumulate():
+        A.shell_kc0('quad4', x, conn, None, None, v, props, values_only=True, check=False)
+
+    ms_a, _ = timed(kc0_accumulate, max(3, args.steps // 4), 1, stream)
+    extra['kc0_accumulate'] = dict(ms=ms_a, elements_per_s=E / (ms_a * 1e-3),
+                                   gbs=(KC0_BYTES_VALUES + 576 * 8) * E / (ms_a * 1e-3) / 1e9)
+    del r, c
+    vg = torch.empty(144 * E, dtype=torch.float64, device=dev)
+    stress = torch.tensor([-1e3, 0., 0.], dtype=torch.float64, device=dev)
+    ms_g, _ = timed(lambda: A.shell_kg_stress('quad4', x, conn, stress, None, None, vg,
+                                              values_only=True, overwrite=True, check=False),
+                    max(3, args.steps // 4), 1, stream)
+    extra['kg_given_stress'] = dict(ms=ms_g, elements_per_s=E / (ms_g * 1e-3),
+                                    gbs=(144 * 8 + 128) * E / (ms_g * 1e-3) / 1e9)
+    del vg
+    vm = torch.empty(480 * E, dtype=torch.float64, device=dev)
+    ms_m, _ = timed(lambda: A.shell_m('quad4', x, conn, None, None, vm, props, mtype=0,
+                                      values_only=True, overwrite=True, check=False),
+                    max(3, args.steps // 4), 1, stream)
+    extra['m_consistent'] = dict(ms=ms_m, elements_per_s=E / (ms_m * 1e-3),
+                                 gbs=(480 * 8 + 128) * E / (ms_m * 1e-3) / 1e9)
+    del vm
+    log('KG %.3f ms, M %.3f ms' % (ms_g, ms_m))
+
+    if not args.no_solve:
+        try:
+            extra['solve_64x64'] = solve_rates(64, 3, log)
+        except Exception as exc:  # reported, not hidden
+            extra['solve_64x64'] = dict(error=str(exc))
+            log('solve failed: %s' % exc)
+
+    n_sample = min(args.cpu_sample, E)
+    base = cpu_baseline(mesh, prop, n_sample)
+    log('cpu baseline %.0f elements/s' % base['value'])
+
+    line = {
+        'metric': 'KC0 elements/s (%HBM roofline); solves/s at 1e-8 res, learned+mixed vs plain',
+        'value': E / (ms * 1e-3),
+        'unit': 'elements/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': ms,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f64',
+        'data': 'synthetic (seeded jittered plate and laminate, no external data)',
+        'config': {'workload': 'Quad4 KC0 assembly, values only (overwrite), %dx%d jittered '
+                               'plate, 8-ply laminate %s' % (args.n, args.n, stack),
+                   'elements': E, 'nodes': mesh.n_nodes, 'dof': mesh.n_dof,
+                   'slots': 576 * E},
+        'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
+                     'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
+                     'bytes_per_element': KC0_BYTES_VALUES},
+        'cpu_baseline': base,
+        'extra': extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == '__main__':
+    main()

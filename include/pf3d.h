@@ -168,7 +168,8 @@ int pf3d_pcg(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx, const d
              const double *dinv, const double *b, double *x, double rtol, int maxiter, int x0,
              void *work, int64_t work_bytes, int64_t *iters, double *relres, void *stream);
 
-/* FP64 iterative refinement around an FP32 block-Jacobi PCG: the outer
+/* FP64 iterative refinement around a block-Jacobi PCG on the FP32-stored
+ * matrix and block inverses (FP64 vectors and arithmetic): the outer
  * residual b - K64 x and the update x += d are FP64; stops at
  * ||b - K64 x|| <= rtol ||b||.  stats = {outer steps, inner iterations}.
  * work: pf3d_ir_bytes(nb) bytes. */

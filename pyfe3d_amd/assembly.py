@@ -9,7 +9,10 @@ These functions are the drop-in for the reference's per-element loops, e.g.
 becomes one ``shell_kc0('quad4', x, conn, KC0r, KC0c, KC0v, props)`` call.
 All array arguments are torch tensors on the GPU (torch is used only as the
 device allocator); slot positions, index values and accumulation semantics
-are those of the reference.
+are those of the reference.  With ``check=True`` (default) the node offsets,
+slot offsets and property indices are range-checked on the device before the
+launch (a few small reductions and one synchronisation); callers that
+re-assemble the same validated arrays in a loop pass ``check=False``.
 """
 import torch
 
@@ -55,12 +58,13 @@ def _flags(values_only, overwrite):
     return (VALUES_ONLY if values_only else 0) | (OVERWRITE if overwrite else 0)
 
 
-def _common(etype, x, conn, init_k, evec):
+def _common(etype, x, conn, init_k, evec, check=True):
     nn = NNODES[etype]
     if conn.dim() != 2 or conn.shape[1] != nn:
         raise ValueError('conn must have shape (n_elem, %d)' % nn)
     E = conn.shape[0]
-    _check_nodes(conn, x)
+    if check:
+        _check_nodes(conn, x)
     return (E, _ptr(x, torch.float64, 'x'), _ptr(conn, torch.int64, 'conn'),
             _ptr(init_k, torch.int64, 'init_k', E, optional=True),
             _ptr(evec, torch.float64, 'xmat/vxy', 3 * E, optional=True))
@@ -73,8 +77,10 @@ def _outputs(r, c, v, need, values_only):
     return _ptr(r, torch.int64, 'r'), _ptr(c, torch.int64, 'c'), pv
 
 
-def _check_span(n_slots_needed, r, c, v, values_only, init_k, size=None):
+def _check_span(n_slots_needed, r, c, v, values_only, init_k, check=True):
     """Every slot an element will write must lie inside r, c and v."""
+    if not check:
+        return
     outs = (('v', v),) + (() if values_only else (('r', r), ('c', c)))
     if init_k is None:
         for name, t in outs:
@@ -108,12 +114,13 @@ def _check_pid(pid, props):
             raise ValueError('pid in [%d, %d] but %d property records' % (lo, hi, props.shape[0]))
 
 
-def _props(props, pid, E, stride):
+def _props(props, pid, E, stride, check=True):
     if props.dim() != 2 or props.shape[1] != stride:
         raise ValueError('props must have shape (n_props, %d)' % stride)
     pp = _ptr(props, torch.float64, 'props')
     ppid = _ptr(pid, torch.int32, 'pid', E, optional=True)
-    _check_pid(pid, props)
+    if check:
+        _check_pid(pid, props)
     return pp, ppid
 
 
@@ -131,12 +138,12 @@ def geometry(etype, x, conn, evec=None, stream=None):
 
 
 def shell_kc0(etype, x, conn, r, c, v, props, pid=None, init_k=None, xmat=None, k6rot=None,
-              alpha=None, hg=None, values_only=False, overwrite=False, stream=None):
+              alpha=None, hg=None, values_only=False, overwrite=False, stream=None, check=True):
     """KC0 of a batch of Quad4/Quad4R/Tria3R elements (update_KC0)."""
-    E, px, pc, pk, pe = _common(etype, x, conn, init_k, xmat)
-    _check_span(slot_count(etype, 'KC0') * E, r, c, v, values_only, init_k)
+    E, px, pc, pk, pe = _common(etype, x, conn, init_k, xmat, check)
+    _check_span(slot_count(etype, 'KC0') * E, r, c, v, values_only, init_k, check)
     pr, pcc, pv = _outputs(r, c, v, True, values_only)
-    pp, ppid = _props(props, pid, E, SHELL_STRIDE)
+    pp, ppid = _props(props, pid, E, SHELL_STRIDE, check)
     _lib.call('pf3d_shell_kc0', ETYPE[etype], E, px, pc, pk, pe, ppid, pp,
               _ptr(k6rot, torch.float64, 'k6rot', E, optional=True),
               _ptr(alpha, torch.float64, 'alpha', E, optional=True),
@@ -145,25 +152,26 @@ def shell_kc0(etype, x, conn, r, c, v, props, pid=None, init_k=None, xmat=None, 
 
 
 def shell_kg(etype, x, conn, u, r, c, v, props, pid=None, init_k=None, xmat=None,
-             values_only=False, overwrite=False, stream=None):
+             values_only=False, overwrite=False, stream=None, check=True):
     """KG from global displacements (update_probe_ue + update_KG)."""
-    E, px, pc, pk, pe = _common(etype, x, conn, init_k, xmat)
-    _check_span(slot_count(etype, 'KG') * E, r, c, v, values_only, init_k)
+    E, px, pc, pk, pe = _common(etype, x, conn, init_k, xmat, check)
+    _check_span(slot_count(etype, 'KG') * E, r, c, v, values_only, init_k, check)
     pr, pcc, pv = _outputs(r, c, v, True, values_only)
-    pp, ppid = _props(props, pid, E, SHELL_STRIDE)
-    _check_nodes(conn, x, u)
+    pp, ppid = _props(props, pid, E, SHELL_STRIDE, check)
+    if check:
+        _check_nodes(conn, x, u)
     _lib.call('pf3d_shell_kg', ETYPE[etype], E, px, pc, pk, pe, ppid, pp,
               _ptr(u, torch.float64, 'u'), pr, pcc, pv, _flags(values_only, overwrite),
               _stream(stream))
 
 
 def shell_kg_stress(etype, x, conn, stress, r, c, v, init_k=None, xmat=None,
-                    values_only=False, overwrite=False, stream=None):
+                    values_only=False, overwrite=False, stream=None, check=True):
     """KG for a given membrane stress state (update_KG_given_stress).
 
     ``stress`` has 3 entries (one state for all elements) or (E, 3)."""
-    E, px, pc, pk, pe = _common(etype, x, conn, init_k, xmat)
-    _check_span(slot_count(etype, 'KG_stress') * E, r, c, v, values_only, init_k)
+    E, px, pc, pk, pe = _common(etype, x, conn, init_k, xmat, check)
+    _check_span(slot_count(etype, 'KG_stress') * E, r, c, v, values_only, init_k, check)
     pr, pcc, pv = _outputs(r, c, v, True, values_only)
     if stress.numel() == 3:
         per = 0
@@ -177,24 +185,24 @@ def shell_kg_stress(etype, x, conn, stress, r, c, v, init_k=None, xmat=None,
 
 
 def shell_m(etype, x, conn, r, c, v, props, mtype=0, pid=None, init_k=None, xmat=None,
-            values_only=False, overwrite=False, stream=None):
+            values_only=False, overwrite=False, stream=None, check=True):
     """Mass matrix (update_M), mtype 0 consistent, 1 reduced, 2 lumped."""
-    E, px, pc, pk, pe = _common(etype, x, conn, init_k, xmat)
-    _check_span(slot_count(etype, 'M', mtype) * E, r, c, v, values_only, init_k)
+    E, px, pc, pk, pe = _common(etype, x, conn, init_k, xmat, check)
+    _check_span(slot_count(etype, 'M', mtype) * E, r, c, v, values_only, init_k, check)
     pr, pcc, pv = _outputs(r, c, v, True, values_only)
-    pp, ppid = _props(props, pid, E, SHELL_STRIDE)
+    pp, ppid = _props(props, pid, E, SHELL_STRIDE, check)
     _lib.call('pf3d_shell_m', ETYPE[etype], E, px, pc, pk, pe, ppid, pp, int(mtype),
               pr, pcc, pv, _flags(values_only, overwrite), _stream(stream))
 
 
 def beam_kc0(x, conn, vxy, r, c, v, props, pid=None, init_k=None, values_only=False,
-             overwrite=False, stream=None):
+             overwrite=False, stream=None, check=True):
     """BeamC KC0 of a batch (update_rotation_matrix(vxy, x) + update_KC0)."""
-    E, px, pc, pk, pe = _common('beamc', x, conn, init_k, vxy)
+    E, px, pc, pk, pe = _common('beamc', x, conn, init_k, vxy, check)
     if pe is None:
         raise ValueError('vxy is required')
-    _check_span(144 * E, r, c, v, values_only, init_k)
+    _check_span(144 * E, r, c, v, values_only, init_k, check)
     pr, pcc, pv = _outputs(r, c, v, True, values_only)
-    pp, ppid = _props(props, pid, E, BEAM_STRIDE)
+    pp, ppid = _props(props, pid, E, BEAM_STRIDE, check)
     _lib.call('pf3d_beam_kc0', E, px, pc, pk, pe, ppid, pp, pr, pcc, pv,
               _flags(values_only, overwrite), _stream(stream))

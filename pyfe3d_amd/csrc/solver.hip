@@ -30,12 +30,14 @@ inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 // ---------------------------------------------------------------- BSR SpMV
 // One wavefront per block row: lane (j, i) multiplies row i of the j-th
 // block of the row (10 blocks per pass) with its 6-vector, then the wave
-// sums the partial rows through LDS.
-template <typename T>
+// sums the partial rows through LDS.  MT is the storage type of the matrix
+// values (FP64, or FP32 to halve the dominant traffic), VT that of the
+// vectors; products and sums are FP64.
+template <typename MT, typename VT>
 __global__ void __launch_bounds__(256) bsr_spmv(int64_t nb, const int32_t *__restrict__ rp,
                                                 const int32_t *__restrict__ ci,
-                                                const T *__restrict__ val,
-                                                const T *__restrict__ x, T *__restrict__ y)
+                                                const MT *__restrict__ val,
+                                                const VT *__restrict__ x, VT *__restrict__ y)
 {
     __shared__ double part[4][64];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -45,8 +47,8 @@ __global__ void __launch_bounds__(256) bsr_spmv(int64_t nb, const int32_t *__res
     if (row < nb && j < 10) {
         const int b0 = rp[row], b1 = rp[row + 1];
         for (int b = b0 + j; b < b1; b += 10) {
-            const T *v = val + (int64_t)b * 36 + i * 6;
-            const T *xv = x + (int64_t)ci[b] * 6;
+            const MT *v = val + (int64_t)b * 36 + i * 6;
+            const VT *xv = x + (int64_t)ci[b] * 6;
 #pragma unroll
             for (int k = 0; k < 6; k++) acc += (double)v[k] * (double)xv[k];
         }
@@ -57,7 +59,7 @@ __global__ void __launch_bounds__(256) bsr_spmv(int64_t nb, const int32_t *__res
         double s = 0.;
 #pragma unroll
         for (int jj = 0; jj < 10; jj++) s += part[wave][lane + 6 * jj];
-        y[row * 6 + lane] = (T)s;
+        y[row * 6 + lane] = (VT)s;
     }
 }
 
@@ -190,31 +192,12 @@ __global__ void __launch_bounds__(VB) finish(int nparts, const double *__restric
     }
 }
 
-// z = Dinv r (per node 6x6)
-template <typename T>
-__global__ void __launch_bounds__(VB) precond(int64_t nb, const T *__restrict__ dinv,
-                                              const T *__restrict__ r, T *__restrict__ z)
-{
-    const int64_t row = (int64_t)blockIdx.x * VB + threadIdx.x;
-    if (row >= nb) return;
-    T rv[6];
-#pragma unroll
-    for (int k = 0; k < 6; k++) rv[k] = r[row * 6 + k];
-#pragma unroll
-    for (int i = 0; i < 6; i++) {
-        T s = 0;
-#pragma unroll
-        for (int k = 0; k < 6; k++) s += dinv[row * 36 + i * 6 + k] * rv[k];
-        z[row * 6 + i] = s;
-    }
-}
-
 // x += alpha p ; r -= alpha q ; z = Dinv r ; partial (r.z, r.r)
-template <typename T>
+template <typename MT, typename T>
 __global__ void __launch_bounds__(VB) update_xrz(int64_t nb, const PcgState *__restrict__ st,
                                                  T *__restrict__ x, T *__restrict__ r,
                                                  const T *__restrict__ p, const T *__restrict__ q,
-                                                 const T *__restrict__ dinv, T *__restrict__ z,
+                                                 const MT *__restrict__ dinv, T *__restrict__ z,
                                                  double *__restrict__ partial)
 {
     __shared__ double sh[VB];
@@ -232,11 +215,11 @@ __global__ void __launch_bounds__(VB) update_xrz(int64_t nb, const PcgState *__r
         }
 #pragma unroll
         for (int i = 0; i < 6; i++) {
-            T s = 0;
+            double s = 0;
 #pragma unroll
-            for (int k = 0; k < 6; k++) s += dinv[row * 36 + i * 6 + k] * rv[k];
-            z[row * 6 + i] = s;
-            s0 += (double)s * (double)rv[i];
+            for (int k = 0; k < 6; k++) s += (double)dinv[row * 36 + i * 6 + k] * (double)rv[k];
+            z[row * 6 + i] = (T)s;
+            s0 += s * (double)rv[i];
             s1 += (double)rv[i] * (double)rv[i];
         }
     }
@@ -277,10 +260,10 @@ __global__ void copy_cast(int64_t n, const double *__restrict__ a, T *__restrict
     if (t < n) b[t] = (T)a[t];
 }
 
-__global__ void add_to(int64_t n, const float *__restrict__ d, double *__restrict__ x)
+__global__ void add_to(int64_t n, const double *__restrict__ d, double *__restrict__ x)
 {
     const int64_t t = (int64_t)blockIdx.x * VB + threadIdx.x;
-    if (t < n) x[t] += (double)d[t];
+    if (t < n) x[t] += d[t];
 }
 
 template <typename T>
@@ -303,11 +286,11 @@ struct Mat {
     const void *val, *dinv;
 };
 
-template <typename T>
-hipError_t spmv(const Mat &A, const T *x, T *y, hipStream_t s)
+template <typename MT, typename VT>
+hipError_t spmv(const Mat &A, const VT *x, VT *y, hipStream_t s)
 {
-    hipLaunchKernelGGL(bsr_spmv<T>, dim3(nblk(A.nb, 4)), dim3(256), 0, s, A.nb, A.rp, A.ci,
-                       (const T *)A.val, x, y);
+    hipLaunchKernelGGL((bsr_spmv<MT, VT>), dim3(nblk(A.nb, 4)), dim3(256), 0, s, A.nb, A.rp,
+                       A.ci, (const MT *)A.val, x, y);
     return hipGetLastError();
 }
 
@@ -317,37 +300,33 @@ size_t pcg_bytes(int64_t n, size_t tsize)
     return ((4 * n * tsize + 255) / 256) * 256 + 2 * RED_BLOCKS * sizeof(double) + 256;
 }
 
-// Solve A x = b to ||r|| <= rtol ||b|| (recursive residual).  x holds the
-// initial guess when x0 != 0, else it is zeroed.  Returns iterations run.
-template <typename T>
-int pcg_run(const Mat &A, const T *b, T *x, double rtol, int maxiter, int x0, void *work,
+// Solve A x = b to ||r|| <= rtol ||b|| (recursive residual), starting from
+// the true residual of x (x0 != 0) or from x = 0.  Matrix and block-Jacobi
+// inverse are stored as MT, vectors as VT.
+template <typename MT, typename VT>
+int pcg_run(const Mat &A, const VT *b, VT *x, double rtol, int maxiter, int x0, void *work,
             hipStream_t s, int64_t *iters, double *relres, int check_every)
 {
     const int64_t n = A.nb * 6;
     char *w = (char *)work;
-    T *r = (T *)w, *z = r + n, *p = z + n, *q = p + n;
-    double *partial = (double *)(w + ((4 * n * sizeof(T) + 255) / 256) * 256);
+    VT *r = (VT *)w, *z = r + n, *p = z + n, *q = p + n;
+    double *partial = (double *)(w + ((4 * n * sizeof(VT) + 255) / 256) * 256);
     PcgState *st = (PcgState *)(partial + 2 * RED_BLOCKS);
     const unsigned nv = nblk(n, VB), nr = red_blocks(A.nb), nd = red_blocks(n);
     hipError_t e;
-    // bb
-    hipLaunchKernelGGL(dot2<T>, dim3(nd), dim3(VB), 0, s, n, b, b, (const T *)nullptr,
-                       (const T *)nullptr, partial);
+    hipLaunchKernelGGL(dot2<VT>, dim3(nd), dim3(VB), 0, s, n, b, b, (const VT *)nullptr,
+                       (const VT *)nullptr, partial);
     hipLaunchKernelGGL(finish, dim3(1), dim3(VB), 0, s, (int)nd, partial, st, 3);
-    // r = b - A x
-    if (!x0) hipLaunchKernelGGL(fill0<T>, dim3(nv), dim3(VB), 0, s, n, x);
-    if ((e = spmv<T>(A, x, q, s)) != hipSuccess) return pf3d_hip_check(e, "pcg spmv");
-    hipLaunchKernelGGL(fill0<T>, dim3(nv), dim3(VB), 0, s, n, p);
-    // r = b - q via update_xrz with alpha = -1 on (x unchanged? no): do it explicitly
-    {
-        // reuse update kernels: set r = b, then r -= q with alpha = 1 and p = 0
-        hipMemcpyAsync(r, b, sizeof(T) * n, hipMemcpyDeviceToDevice, s);
-        hipLaunchKernelGGL(set_alpha, dim3(1), dim3(1), 0, s, st, 1.);
-        hipLaunchKernelGGL(update_xrz<T>, dim3(nr), dim3(VB), 0, s, A.nb, st, x, r, p, q,
-                           (const T *)A.dinv, z, partial);
-        hipLaunchKernelGGL(finish, dim3(1), dim3(VB), 0, s, (int)nr, partial, st, 2);
-    }
-    hipMemcpyAsync(p, z, sizeof(T) * n, hipMemcpyDeviceToDevice, s);
+    if (!x0) hipLaunchKernelGGL(fill0<VT>, dim3(nv), dim3(VB), 0, s, n, x);
+    if ((e = spmv<MT, VT>(A, x, q, s)) != hipSuccess) return pf3d_hip_check(e, "pcg spmv");
+    // r = b - A x ; z = M r ; (r.z, r.r) through the update kernel with p = 0
+    hipLaunchKernelGGL(fill0<VT>, dim3(nv), dim3(VB), 0, s, n, p);
+    hipMemcpyAsync(r, b, sizeof(VT) * n, hipMemcpyDeviceToDevice, s);
+    hipLaunchKernelGGL(set_alpha, dim3(1), dim3(1), 0, s, st, 1.);
+    hipLaunchKernelGGL((update_xrz<MT, VT>), dim3(nr), dim3(VB), 0, s, A.nb, st, x, r, p, q,
+                       (const MT *)A.dinv, z, partial);
+    hipLaunchKernelGGL(finish, dim3(1), dim3(VB), 0, s, (int)nr, partial, st, 2);
+    hipMemcpyAsync(p, z, sizeof(VT) * n, hipMemcpyDeviceToDevice, s);
     PcgState h;
     e = hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -357,14 +336,14 @@ int pcg_run(const Mat &A, const T *b, T *x, double rtol, int maxiter, int x0, vo
     int it = 0;
     while (rel > rtol && it < maxiter) {
         for (int k = 0; k < check_every && it < maxiter; k++, it++) {
-            spmv<T>(A, p, q, s);
-            hipLaunchKernelGGL(dot2<T>, dim3(nd), dim3(VB), 0, s, n, p, q, (const T *)nullptr,
-                               (const T *)nullptr, partial);
+            spmv<MT, VT>(A, p, q, s);
+            hipLaunchKernelGGL(dot2<VT>, dim3(nd), dim3(VB), 0, s, n, p, q, (const VT *)nullptr,
+                               (const VT *)nullptr, partial);
             hipLaunchKernelGGL(finish, dim3(1), dim3(VB), 0, s, (int)nd, partial, st, 0);
-            hipLaunchKernelGGL(update_xrz<T>, dim3(nr), dim3(VB), 0, s, A.nb, st, x, r, p, q,
-                               (const T *)A.dinv, z, partial);
+            hipLaunchKernelGGL((update_xrz<MT, VT>), dim3(nr), dim3(VB), 0, s, A.nb, st, x, r,
+                               p, q, (const MT *)A.dinv, z, partial);
             hipLaunchKernelGGL(finish, dim3(1), dim3(VB), 0, s, (int)nr, partial, st, 1);
-            hipLaunchKernelGGL(update_p<T>, dim3(nv), dim3(VB), 0, s, n, st, z, p);
+            hipLaunchKernelGGL(update_p<VT>, dim3(nv), dim3(VB), 0, s, n, st, z, p);
         }
         e = hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -388,7 +367,7 @@ int pf3d_bsr_spmv(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx, co
         return pf3d_fail(PF3D_E_ARG, "pf3d_bsr_spmv: bad arguments");
     if (!nb) return PF3D_OK;
     Mat A{nb, row_ptr, col_idx, val, nullptr};
-    return pf3d_hip_check(spmv<double>(A, x, y, (hipStream_t)stream), "pf3d_bsr_spmv");
+    return pf3d_hip_check(spmv<double, double>(A, x, y, (hipStream_t)stream), "pf3d_bsr_spmv");
 }
 
 int pf3d_bsr_spmv_f32(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx,
@@ -398,7 +377,7 @@ int pf3d_bsr_spmv_f32(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx
         return pf3d_fail(PF3D_E_ARG, "pf3d_bsr_spmv_f32: bad arguments");
     if (!nb) return PF3D_OK;
     Mat A{nb, row_ptr, col_idx, val, nullptr};
-    return pf3d_hip_check(spmv<float>(A, x, y, (hipStream_t)stream), "pf3d_bsr_spmv_f32");
+    return pf3d_hip_check(spmv<float, float>(A, x, y, (hipStream_t)stream), "pf3d_bsr_spmv_f32");
 }
 
 int pf3d_bsr_dirichlet(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx, double *val,
@@ -447,18 +426,33 @@ int pf3d_pcg(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx, const d
     if ((int64_t)pcg_bytes(6 * nb, sizeof(double)) > work_bytes)
         return pf3d_fail(PF3D_E_ARG, "pf3d_pcg: workspace too small");
     Mat A{nb, row_ptr, col_idx, val, dinv};
-    return pcg_run<double>(A, b, x, rtol, maxiter, x0, work, (hipStream_t)stream, iters, relres,
-                           25);
+    // restart from the true residual until it meets the tolerance (the
+    // recursive residual of long runs drifts from b - A x)
+    int64_t total = 0;
+    double rel = 0.;
+    for (int restart = 0; restart < 8; restart++) {
+        int64_t it = 0;
+        int st = pcg_run<double, double>(A, b, x, rtol, (int)(maxiter - total), x0 || restart,
+                                         work, (hipStream_t)stream, &it, &rel, 25);
+        if (st) return st;
+        total += it;
+        if (it == 0 || total >= maxiter) break;  // true residual already below rtol
+    }
+    *iters = total;
+    *relres = rel;
+    return PF3D_OK;
 }
 
-/* FP64 iterative refinement with an FP32 PCG inner solve:
+/* FP64 iterative refinement around a PCG on the FP32-stored matrix:
  *   r = b - K64 x ; solve K32 d = r to inner_rtol ; x += d ; until
- *   ||b - K64 x|| <= rtol ||b||.  work: pf3d_ir_bytes(nb). */
+ *   ||b - K64 x|| <= rtol ||b||.  The inner PCG reads K and the block
+ *   inverses in FP32 (half the bytes of the dominant stream) and keeps its
+ *   vectors and arithmetic in FP64.  work: pf3d_ir_bytes(nb). */
 int pf3d_ir_bytes(int64_t nb, int64_t *bytes)
 {
     if (nb <= 0 || !bytes) return pf3d_fail(PF3D_E_ARG, "pf3d_ir_bytes: bad arguments");
     const int64_t n = 6 * nb;
-    *bytes = (int64_t)pcg_bytes(n, sizeof(float)) + ((2 * n * 8 + 2 * n * 4 + 255) / 256) * 256 +
+    *bytes = (int64_t)pcg_bytes(n, sizeof(double)) + ((3 * n * 8 + 255) / 256) * 256 +
              2 * RED_BLOCKS * 8 + 256;
     return PF3D_OK;
 }
@@ -479,10 +473,9 @@ int pf3d_ir_solve(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx,
     const int64_t n = 6 * nb;
     char *w = (char *)work;
     void *inner = w;
-    char *o = w + pcg_bytes(n, sizeof(float));
-    double *r64 = (double *)o, *y64 = r64 + n;
-    float *r32 = (float *)(y64 + n), *d32 = r32 + n;
-    double *partial = (double *)(o + ((2 * n * 8 + 2 * n * 4 + 255) / 256) * 256);
+    char *o = w + pcg_bytes(n, sizeof(double));
+    double *r64 = (double *)o, *y64 = r64 + n, *d64 = y64 + n;
+    double *partial = (double *)(o + ((3 * n * 8 + 255) / 256) * 256);
     PcgState *st = (PcgState *)(partial + 2 * RED_BLOCKS);
     Mat A64{nb, row_ptr, col_idx, val64, nullptr};
     Mat A32{nb, row_ptr, col_idx, val32, dinv32};
@@ -495,9 +488,10 @@ int pf3d_ir_solve(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx,
     int outer = 0;
     double rel = 1.;
     for (;;) {
-        hipError_t e = spmv<double>(A64, x, y64, s);
+        hipError_t e = spmv<double, double>(A64, x, y64, s);
         if (e != hipSuccess) return pf3d_hip_check(e, "ir spmv");
-        hipLaunchKernelGGL(residual<float>, dim3(nv), dim3(VB), 0, s, n, b, y64, r64, r32);
+        hipLaunchKernelGGL(residual<double>, dim3(nv), dim3(VB), 0, s, n, b, y64, r64,
+                           (double *)nullptr);
         hipLaunchKernelGGL(dot2<double>, dim3(nd), dim3(VB), 0, s, n, r64, r64,
                            (const double *)nullptr, (const double *)nullptr, partial);
         hipLaunchKernelGGL(finish, dim3(1), dim3(VB), 0, s, (int)nd, partial, st, 4);
@@ -509,11 +503,12 @@ int pf3d_ir_solve(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx,
         if (rel <= rtol || outer >= max_outer || !isfinite(rel)) break;
         int64_t it = 0;
         double inner_rel = 0.;
-        int st2 = pcg_run<float>(A32, r32, d32, inner_rtol, max_inner, 0, inner, s, &it,
-                                 &inner_rel, 25);
+        // inner solve on the FP32-stored matrix with FP64 vectors
+        int st2 = pcg_run<float, double>(A32, r64, d64, inner_rtol, max_inner, 0, inner, s, &it,
+                                         &inner_rel, 25);
         if (st2) return st2;
         inner_total += it;
-        hipLaunchKernelGGL(add_to, dim3(nv), dim3(VB), 0, s, n, d32, x);
+        hipLaunchKernelGGL(add_to, dim3(nv), dim3(VB), 0, s, n, d64, x);
         outer++;
     }
     stats[0] = outer;

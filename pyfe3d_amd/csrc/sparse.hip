@@ -94,7 +94,7 @@ __global__ void mark_heads(int64_t n, const int64_t *__restrict__ keys, int64_t 
 // recognised from the keys again
 __global__ void emit(int64_t n, int64_t n_nodes, const int64_t *__restrict__ keys,
                      const int64_t *__restrict__ uid, const int64_t *__restrict__ bid,
-                     int64_t *__restrict__ seg, int64_t *__restrict__ dest,
+                     const int64_t *__restrict__ bflag, int64_t *__restrict__ seg, int64_t *__restrict__ dest,
                      int32_t *__restrict__ col_idx, int32_t *__restrict__ row_ptr,
                      int64_t n_unique, int64_t nnzb)
 {
@@ -102,12 +102,15 @@ __global__ void emit(int64_t n, int64_t n_nodes, const int64_t *__restrict__ key
     if (t >= n) return;
     const int64_t k = keys[t];
     const int64_t kp = t ? keys[t - 1] : -1;
+    // block of slot t: exclusive count of block heads before t, plus one if
+    // t starts a block, minus one
+    const int64_t b = bid[t] + bflag[t] - 1;
     if (t == 0 || k != kp) {
         seg[uid[t]] = t;
-        dest[uid[t]] = bid[t] * 36 + k % 36;
+        dest[uid[t]] = b * 36 + k % 36;
     }
     if (t == 0 || k / 36 != kp / 36) {
-        const int64_t b = bid[t], bk = k / 36;
+        const int64_t bk = k / 36;
         const int64_t row = bk / n_nodes;
         col_idx[b] = (int32_t)(bk % n_nodes);
         const int64_t prow = t ? (kp / 36) / n_nodes : -1;
@@ -220,10 +223,11 @@ int pf3d_bsr_plan_emit(int64_t n_slots, int64_t n_nodes, void *work, int64_t wor
     const int64_t *kout = (const int64_t *)(base + w.keys_out);
     const int64_t *iout = (const int64_t *)(base + w.ids_out);
     const int64_t *uid = (const int64_t *)(base + w.keys_in), *bid = (const int64_t *)(base + w.ids_in);
+    const int64_t *bflag = (const int64_t *)(base + w.bhead);
     e = hipMemcpyAsync(perm, iout, sizeof(int64_t) * n_slots, hipMemcpyDeviceToDevice, s);
     if (e != hipSuccess) return pf3d_hip_check(e, "copy perm");
     hipLaunchKernelGGL(emit, dim3(nblk(n_slots, 256)), dim3(256), 0, s, n_slots, n_nodes, kout,
-                       uid, bid, seg, dest, col_idx, row_ptr, n_unique, nnzb);
+                       uid, bid, bflag, seg, dest, col_idx, row_ptr, n_unique, nnzb);
     return pf3d_hip_check(hipGetLastError(), "pf3d_bsr_plan_emit");
 }
 
