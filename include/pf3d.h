@@ -111,6 +111,19 @@ int pf3d_beam_kc0(int64_t n_elem, const double *x, const int64_t *conn, const in
                   const double *vxy, const int32_t *pid, const double *props, int64_t *r,
                   int64_t *c, double *v, int flags, void *stream);
 
+/* BeamC geometric stiffness from global displacements u (axial force
+ * N = A E (u2 - u1)/L in local axes).  Replaces update_probe_ue(u) ->
+ * update_probe_xe(x) -> update_KG(...): beamc.pyx:234-286, 1392-2180. */
+int pf3d_beam_kg(int64_t n_elem, const double *x, const int64_t *conn, const int64_t *init_k,
+                 const double *vxy, const int32_t *pid, const double *props, const double *u,
+                 int64_t *r, int64_t *c, double *v, int flags, void *stream);
+
+/* BeamC mass, mtype 0 consistent (144 slots) or 1 lumped (36 slots).
+ * Replaces update_M: beamc.pyx:2181-3152. */
+int pf3d_beam_m(int64_t n_elem, const double *x, const int64_t *conn, const int64_t *init_k,
+                const double *vxy, const int32_t *pid, const double *props, int mtype,
+                int64_t *r, int64_t *c, double *v, int flags, void *stream);
+
 /* ------------------------------------------------------------------ */
 /* COO -> 6x6 BSR (replaces coo_matrix((v, (r, c))).tocsc() on the     */
 /* reference's caller side, e.g. tests/test_quad4_static_point_load.py:80) */

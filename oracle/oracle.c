@@ -17,7 +17,6 @@
  *     (KC0/KG/KG-given-stress/M slots, geometry, all four element types);
  *   - invariants: symmetry, rigid-body null space of KC0, mass sums, closed
  *     forms (Euler-Bernoulli beam limit, plane-stress Q4 membrane block).
- * BeamC KG and M are not restated here (parity unpinned for them: no oracle).
  */
 #include <math.h>
 #include <stdint.h>
@@ -542,6 +541,113 @@ static void beamc_kc0_local(double L, const double *bp, double *K)
         for (int j = 0; j < 12; j++) K[i * 12 + j] = (j >= i) ? U[i][j] : U[j][i];
 }
 
+/* BeamC geometric stiffness for axial force N (beamc.pyx:1889-2180,
+ * N = A E (u2 - u1)/L at :1889).  Only v, w, ry, rz take part; each DOF
+ * belongs to a bending plane (v, rz: y; w, ry: z) with its own alpha/beta:
+ *   disp-disp  +-(5 a_i a_j - 5 a_i - 5 a_j + 6)/(5L)   (+ same node)
+ *   disp-rot   s_rot * (+1 node 1, -1 node 2 of the disp DOF) / 10
+ *   rot-rot    s_rot_i s_rot_j L (5 a_i a_j - 5 a_i - 5 a_j + 8)/60 (same node)
+ *              -s_rot_i s_rot_j L (5 a_i a_j - 5 a_i - 5 a_j + 2)/60 (other node)
+ * all times N beta_i beta_j, with s_rot(ry) = -1, s_rot(rz) = +1. */
+static void beamc_kg_local(double L, const double *bp, double N, double *K)
+{
+    double A = bp[0], E = bp[1], G = bp[2], Iyy = bp[3], Izz = bp[4];
+    double ay = 12 * E * Izz / (G * A * L * L), az = 12 * E * Iyy / (G * A * L * L);
+    double by = 1 / (1. - ay), bz = 1 / (1. - az);
+    memset(K, 0, sizeof(double) * 144);
+    for (int i = 0; i < 12; i++)
+        for (int j = 0; j < 12; j++) {
+            int di = i % 6, dj = j % 6, ni = i / 6, nj = j / 6;
+            if (!(di == 1 || di == 2 || di == 4 || di == 5)) continue;
+            if (!(dj == 1 || dj == 2 || dj == 4 || dj == 5)) continue;
+            int yi = (di == 1 || di == 5), yj = (dj == 1 || dj == 5);
+            double a_i = yi ? ay : az, a_j = yj ? ay : az;
+            double b = N * (yi ? by : bz) * (yj ? by : bz);
+            int ri = di >= 4, rj = dj >= 4;
+            double si = di == 4 ? -1. : 1., sj = dj == 4 ? -1. : 1.;
+            double q = 5 * a_i * a_j - 5 * a_i - 5 * a_j;
+            double v;
+            if (!ri && !rj) v = (ni == nj ? 1. : -1.) * (q + 6) / (5 * L);
+            else if (!ri && rj) v = sj * (ni == 0 ? 1. : -1.) / 10;
+            else if (ri && !rj) v = si * (nj == 0 ? 1. : -1.) / 10;
+            else v = si * sj * L * (ni == nj ? (q + 8) / 60 : -(q + 2) / 60);
+            K[i * 12 + j] = b * v;
+        }
+}
+
+/* BeamC mass, consistent (mtype 0, beamc.pyx:2246-2968) and lumped
+ * (mtype 1, beamc.pyx:2969-3152).  Local 12x12, symmetric; upper triangle
+ * listed per row. */
+static void beamc_m_local(double L, const double *bp, int mtype, double *K)
+{
+    double A = bp[0], E = bp[1], G = bp[2], Iyy = bp[3], Izz = bp[4];
+    double r = bp[9], ry = bp[10], rz = bp[11], ry2 = bp[12], rz2 = bp[13], ryz = bp[14];
+    double ay = 12 * E * Izz / (G * A * L * L), az = 12 * E * Iyy / (G * A * L * L);
+    double by = 1 / (1. - ay), bz = 1 / (1. - az);
+    double L2 = L * L;
+    double U[12][12];
+    memset(U, 0, sizeof(U));
+    if (mtype == 1) {
+        for (int n = 0; n < 2; n++) {
+            int o = 6 * n;
+            U[o][o] = L * r / 2;
+            U[o + 1][o + 1] = L * by * by * r * (ay * ay - 2 * ay + 1) / 2;
+            U[o + 2][o + 2] = L * bz * bz * r * (az * az - 2 * az + 1) / 2;
+            U[o + 3][o + 3] = L * (ry2 + rz2) / 2;
+            U[o + 4][o + 4] = L * bz * bz * rz2 * (az * az - 2 * az + 1) / 2;
+            U[o + 5][o + 5] = L * by * by * ry2 * (ay * ay - 2 * ay + 1) / 2;
+        }
+    } else {
+        /* translation-translation bending terms */
+        double tyy = by * by * (70 * L2 * ay * ay * r - 147 * L2 * ay * r + 78 * L2 * r + 252 * ry2) / (210 * L);
+        double tzz = bz * bz * (70 * L2 * az * az * r - 147 * L2 * az * r + 78 * L2 * r + 252 * rz2) / (210 * L);
+        double tyy2 = by * by * (35 * L2 * ay * ay * r - 63 * L2 * ay * r + 27 * L2 * r - 252 * ry2) / (210 * L);
+        double tzz2 = bz * bz * (35 * L2 * az * az * r - 63 * L2 * az * r + 27 * L2 * r - 252 * rz2) / (210 * L);
+        /* translation-rotation bending terms */
+        double cy1 = by * by * (35 * L2 * ay * ay * r - 77 * L2 * ay * r + 44 * L2 * r + 420 * ay * ry2 + 84 * ry2) / 840;
+        double cy2 = by * by * (-35 * L2 * ay * ay * r + 63 * L2 * ay * r - 26 * L2 * r + 420 * ay * ry2 + 84 * ry2) / 840;
+        double cz1 = bz * bz * (-35 * L2 * az * az * r + 77 * L2 * az * r - 44 * L2 * r - 420 * az * rz2 - 84 * rz2) / 840;
+        double cz2 = bz * bz * (35 * L2 * az * az * r - 63 * L2 * az * r + 26 * L2 * r - 420 * az * rz2 - 84 * rz2) / 840;
+        /* rotation-rotation bending terms */
+        double qzz = L * bz * bz * (7 * L2 * az * az * r - 14 * L2 * az * r + 8 * L2 * r + 280 * az * az * rz2 - 140 * az * rz2 + 112 * rz2) / 840;
+        double qyy = L * by * by * (7 * L2 * ay * ay * r - 14 * L2 * ay * r + 8 * L2 * r + 280 * ay * ay * ry2 - 140 * ay * ry2 + 112 * ry2) / 840;
+        double qzz2 = L * bz * bz * (-7 * L2 * az * az * r + 14 * L2 * az * r - 6 * L2 * r + 140 * az * az * rz2 + 140 * az * rz2 - 28 * rz2) / 840;
+        double qyy2 = L * by * by * (-7 * L2 * ay * ay * r + 14 * L2 * ay * r - 6 * L2 * r + 140 * ay * ay * ry2 + 140 * ay * ry2 - 28 * ry2) / 840;
+        double pyz = 6 * by * bz * ryz / (5 * L);
+        double sz1 = by * bz * ryz * (5 * az + 1) / 10, sy1 = by * bz * ryz * (5 * ay + 1) / 10;
+        double qyz = L * by * bz * ryz * (-20 * ay * az + 5 * ay + 5 * az - 8) / 60;
+        double qyz2 = L * by * bz * ryz * (-10 * ay * az - 5 * ay - 5 * az + 2) / 60;
+        U[0][0] = L * r / 3;  U[0][6] = L * r / 6;  U[6][6] = L * r / 3;
+        U[0][1] = by * ry / 2;  U[0][7] = -by * ry / 2;  U[1][6] = by * ry / 2;  U[6][7] = -by * ry / 2;
+        U[0][2] = bz * rz / 2;  U[0][8] = -bz * rz / 2;  U[2][6] = bz * rz / 2;  U[6][8] = -bz * rz / 2;
+        U[0][4] = L * bz * rz * (1 - 4 * az) / 12;   U[6][10] = U[0][4];
+        U[0][5] = L * by * ry * (4 * ay - 1) / 12;   U[6][11] = U[0][5];
+        U[0][10] = -L * bz * rz * (2 * az + 1) / 12; U[4][6] = U[0][10];
+        U[0][11] = L * by * ry * (2 * ay + 1) / 12;  U[5][6] = U[0][11];
+        U[1][1] = tyy;  U[7][7] = tyy;  U[1][7] = tyy2;
+        U[2][2] = tzz;  U[8][8] = tzz;  U[2][8] = tzz2;
+        U[1][2] = pyz;  U[7][8] = pyz;  U[1][8] = -pyz;  U[2][7] = -pyz;
+        U[1][5] = cy1;  U[7][11] = -cy1;  U[1][11] = cy2;  U[5][7] = -cy2;
+        U[2][4] = cz1;  U[8][10] = -cz1;  U[2][10] = cz2;  U[4][8] = -cz2;
+        U[1][4] = -sz1; U[1][10] = -sz1; U[4][7] = sz1;  U[7][10] = sz1;
+        U[2][5] = sy1;  U[2][11] = sy1;  U[5][8] = -sy1; U[8][11] = -sy1;
+        U[1][3] = L * by * rz * (20 * ay - 21) / 60;  U[7][9] = U[1][3];
+        U[1][9] = L * by * rz * (10 * ay - 9) / 60;   U[3][7] = U[1][9];
+        U[2][3] = L * bz * ry * (21 - 20 * az) / 60;  U[8][9] = U[2][3];
+        U[2][9] = L * bz * ry * (9 - 10 * az) / 60;   U[3][8] = U[2][9];
+        U[3][3] = L * (ry2 + rz2) / 3;  U[9][9] = U[3][3];  U[3][9] = L * (ry2 + rz2) / 6;
+        U[3][4] = L2 * bz * ry * (5 * az - 6) / 120;  U[9][10] = -U[3][4];
+        U[3][5] = L2 * by * rz * (5 * ay - 6) / 120;  U[9][11] = -U[3][5];
+        U[3][10] = L2 * bz * ry * (4 - 5 * az) / 120; U[4][9] = -U[3][10];
+        U[3][11] = L2 * by * rz * (4 - 5 * ay) / 120; U[5][9] = -U[3][11];
+        U[4][4] = qzz;  U[10][10] = qzz;  U[4][10] = qzz2;
+        U[5][5] = qyy;  U[11][11] = qyy;  U[5][11] = qyy2;
+        U[4][5] = qyz;  U[10][11] = qyz;  U[4][11] = qyz2;  U[5][10] = qyz2;
+    }
+    for (int i = 0; i < 12; i++)
+        for (int j = 0; j < 12; j++) K[i * 12 + j] = (j >= i) ? U[i][j] : U[j][i];
+}
+
 /* ------------------------------------------------------------------ */
 /* local KG (w-w block) and M                                            */
 /* ------------------------------------------------------------------ */
@@ -773,8 +879,11 @@ static int element_local(int etype, int mat, int mtype, const double *g, const d
     int nn = nnodes(etype), n = 6 * nn;
     double area = g[13];
     if (etype == ET_BEAMC) {
-        if (mat != MAT_KC0) return -1; /* BeamC KG/M: not restated */
-        beamc_kc0_local(area, prop, K);
+        if (mat == MAT_KC0) beamc_kc0_local(area, prop, K);
+        else if (mat == MAT_KG) /* N = A E (u2 - u1) / L, beamc.pyx:1889 */
+            beamc_kg_local(area, prop, prop[0] * prop[1] * (-ue[0] + ue[6]) / area, K);
+        else if (mat == MAT_M) beamc_m_local(area, prop, mtype, K);
+        else return -1;
         return 0;
     }
     shell_t s;

@@ -206,3 +206,31 @@ def beam_kc0(x, conn, vxy, r, c, v, props, pid=None, init_k=None, values_only=Fa
     pp, ppid = _props(props, pid, E, BEAM_STRIDE, check)
     _lib.call('pf3d_beam_kc0', E, px, pc, pk, pe, ppid, pp, pr, pcc, pv,
               _flags(values_only, overwrite), _stream(stream))
+
+
+def beam_kg(x, conn, vxy, u, r, c, v, props, pid=None, init_k=None, values_only=False,
+            overwrite=False, stream=None, check=True):
+    """BeamC KG from global displacements (update_probe_ue + update_KG)."""
+    E, px, pc, pk, pe = _common('beamc', x, conn, init_k, vxy, check)
+    if pe is None:
+        raise ValueError('vxy is required')
+    _check_span(144 * E, r, c, v, values_only, init_k, check)
+    pr, pcc, pv = _outputs(r, c, v, True, values_only)
+    pp, ppid = _props(props, pid, E, BEAM_STRIDE, check)
+    if check:
+        _check_nodes(conn, x, u)
+    _lib.call('pf3d_beam_kg', E, px, pc, pk, pe, ppid, pp, _ptr(u, torch.float64, 'u'),
+              pr, pcc, pv, _flags(values_only, overwrite), _stream(stream))
+
+
+def beam_m(x, conn, vxy, r, c, v, props, mtype=0, pid=None, init_k=None, values_only=False,
+           overwrite=False, stream=None, check=True):
+    """BeamC mass matrix (update_M), mtype 0 consistent, 1 lumped."""
+    E, px, pc, pk, pe = _common('beamc', x, conn, init_k, vxy, check)
+    if pe is None:
+        raise ValueError('vxy is required')
+    _check_span(slot_count('beamc', 'M', mtype) * E, r, c, v, values_only, init_k, check)
+    pr, pcc, pv = _outputs(r, c, v, True, values_only)
+    pp, ppid = _props(props, pid, E, BEAM_STRIDE, check)
+    _lib.call('pf3d_beam_m', E, px, pc, pk, pe, ppid, pp, int(mtype), pr, pcc, pv,
+              _flags(values_only, overwrite), _stream(stream))

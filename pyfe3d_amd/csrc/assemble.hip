@@ -85,7 +85,9 @@ __device__ __forceinline__ Pair quad_pair(const double *xe, double xi, double et
     }
     Pair p;
     p.det = J11 * J22 - J12 * J21;
-    const double j11 = J22 / p.det, j12 = -J12 / p.det, j21 = -J21 / p.det, j22 = J11 / p.det;
+    // one reciprocal instead of four divisions (within an ulp of J_ij / det)
+    const double inv = 1. / p.det;
+    const double j11 = J22 * inv, j12 = -J12 * inv, j21 = -J21 * inv, j22 = J11 * inv;
     const double dxi_i = 0.25 * sxi * (1 + syi * eta), deta_i = 0.25 * syi * (1 + sxi * xi);
     const double dxi_j = 0.25 * sxj * (1 + syj * eta), deta_j = 0.25 * syj * (1 + sxj * xi);
     p.Ni = 0.25 * (1 + sxi * xi) * (1 + syi * eta);
@@ -511,6 +513,125 @@ shell_assemble(AsmArgs a)
     }
 }
 
+
+// ------------------------------------------------------------------------
+// KC0 with block symmetry.  K_e is symmetric, so a lane owns one node-pair
+// block (I, J) with I <= J: 10 lanes per quad (6 quads per wave), 6 lanes
+// per triangle (10 per wave).  The element matrix leaves in four (three)
+// phases, one per node row P: the lanes owning (P, J) write their block and
+// the lanes owning (J, P), J < P, write its transpose into a 6 x 6nn LDS
+// strip, and the wave streams the strip, which is one contiguous run of
+// 6*6nn slots of the reference layout (quad4.pyx:1287-1293).
+// ------------------------------------------------------------------------
+template <int NN>
+__device__ __forceinline__ void upper_pair(int sub, int &I, int &J)
+{
+    // sub -> (I, J), I <= J, row-major over the upper triangle
+    I = 0;
+#pragma unroll
+    for (int t = 1; t < NN; t++) I += (sub >= t * NN - (t * (t - 1)) / 2) ? 1 : 0;
+    J = I + sub - (I * NN - (I * (I - 1)) / 2);
+}
+
+// waves per SIMD the register allocation of the KC0 kernel targets
+#ifndef PF3D_KC0_WAVES
+#define PF3D_KC0_WAVES 2
+#endif
+
+template <int ET>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PF3D_KC0_WAVES)))
+shell_kc0_sym(AsmArgs a)
+{
+    constexpr int NN = (ET == TRIA3R) ? 3 : 4;
+    constexpr int NUP = NN * (NN + 1) / 2;
+    constexpr int EPW = 64 / NUP;
+    constexpr int NDOF = 6 * NN;
+    constexpr int RW = 6 * NDOF;  // slots per node row
+    __shared__ double strip[EPW][RW];
+    __shared__ int64_t sconn[EPW][NN];
+    __shared__ int64_t sk0[EPW];
+
+    const int lane = threadIdx.x;
+    const int slot = lane / NUP, sub = lane - slot * NUP;
+    int I, J;
+    upper_pair<NN>(sub, I, J);
+    const int64_t e0 = (int64_t)blockIdx.x * EPW;
+    const int64_t e = e0 + slot;
+    const bool active = slot < EPW && e < a.n_elem;
+
+    double K[6][6];
+#pragma unroll
+    for (int i = 0; i < 6; i++)
+#pragma unroll
+        for (int j = 0; j < 6; j++) K[i][j] = 0.;
+    if (active) {
+        V3 X[NN];
+        int64_t cn[NN];
+#pragma unroll
+        for (int i = 0; i < NN; i++) {
+            cn[i] = a.conn[e * NN + i];
+            const double *px = a.x + cn[i] / 2;
+            X[i] = {px[0], px[1], px[2]};
+        }
+        if (sub == 0) {
+#pragma unroll
+            for (int i = 0; i < NN; i++) sconn[slot][i] = cn[i];
+            sk0[slot] = a.init_k ? a.init_k[e] : (int64_t)(36 * NN * NN) * e;
+        }
+        V3 xm = {0., 0., 0.};
+        if (a.evec) xm = {a.evec[3 * e], a.evec[3 * e + 1], a.evec[3 * e + 2]};
+        Geom g;
+        if (ET == TRIA3R) tria_geometry(X, xm, g);
+        else quad_geometry(X, xm, g);
+        Shell s;
+        load_shell(a.props + (a.pid ? (int64_t)a.pid[e] : 0) * SHELL_STRIDE, g.m, s);
+        const double k6 = a.k6rot ? a.k6rot[e] : 100.;
+        if (ET == QUAD4) quad4_kc0(K, g, s, I, J);
+        else if (ET == QUAD4R) {
+            double hgf[5] = {1., 1., 1., 1., 1.};
+            if (a.hg)
+#pragma unroll
+                for (int d = 0; d < 5; d++) hgf[d] = a.hg[5 * e + d];
+            quad4r_kc0(K, g, s, I, J, k6, hgf);
+        } else {
+            tria3r_kc0(K, g, s, I, J, k6, a.alpha ? a.alpha[e] : 0.7);
+        }
+        rotate_block(K, g.r);
+    }
+    const int nel = (int)((a.n_elem - e0) < EPW ? (a.n_elem - e0) : EPW);
+    const bool write_idx = !(a.flags & F_VALUES_ONLY);
+    const bool overwrite = (a.flags & F_OVERWRITE);
+    for (int P = 0; P < NN; P++) {
+        __syncthreads();  // the previous strip has been streamed out
+        if (active && I == P) {
+            double *t = &strip[slot][6 * J];
+#pragma unroll
+            for (int i = 0; i < 6; i++)
+#pragma unroll
+                for (int j = 0; j < 6; j++) t[i * NDOF + j] = K[i][j];
+        } else if (active && J == P) {  // block (P, I) = K_IJ^T
+            double *t = &strip[slot][6 * I];
+#pragma unroll
+            for (int i = 0; i < 6; i++)
+#pragma unroll
+                for (int j = 0; j < 6; j++) t[j * NDOF + i] = K[i][j];
+        }
+        __syncthreads();
+        for (int idx = lane; idx < nel * RW; idx += 64) {
+            const int el = idx / RW, t = idx - el * RW;
+            const int64_t k = sk0[el] + P * RW + t;
+            const double val = strip[el][t];
+            if (overwrite) a.v[k] = val;
+            else a.v[k] += val;
+            if (write_idx) {
+                const int rl = t / NDOF, cl = t - rl * NDOF;
+                a.r[k] = sconn[el][P] + rl;
+                a.c[k] = sconn[el][cl / 6] + cl % 6;
+            }
+        }
+    }
+}
+
 // Geometry of each element: r (9), m (4), area/length, local x/y (2*nn).
 template <int ET>
 __global__ void element_geometry(int64_t n, const double *x, const int64_t *conn,
@@ -550,113 +671,203 @@ __global__ void element_geometry(int64_t n, const double *x, const int64_t *conn
 }
 
 // ------------------------------------------------------------------------
-// BeamC KC0: closed-form Timoshenko stiffness (beamc.pyx:543-624), one
-// element per 4 lanes (lane owns a 6x6 node block), 16 beams per wave.
+// BeamC: 16 beams per wave, 4 lanes per beam.  The first lane of a beam
+// writes the closed-form 12x12 local matrix into LDS; each lane then rotates
+// one 6x6 node block (R6 K R6^T) and the wave streams the slots out.
 // ------------------------------------------------------------------------
-__device__ void beamc_local(double L, const double *bp, int I, int J, double K[6][6])
+struct BeamSec {
+    double A, E, G, Iyy, Izz, Iyz, J, Ay, Az, ay, az, by, bz;
+};
+
+__device__ __forceinline__ BeamSec beam_section(const double *bp, double L)
 {
-    const double A = bp[0], E = bp[1], G = bp[2], Iyy = bp[3], Izz = bp[4], Iyz = bp[5];
-    const double Jt = bp[6], Ay = bp[7], Az = bp[8];
+    BeamSec s;
+    s.A = bp[0]; s.E = bp[1]; s.G = bp[2]; s.Iyy = bp[3]; s.Izz = bp[4]; s.Iyz = bp[5];
+    s.J = bp[6]; s.Ay = bp[7]; s.Az = bp[8];
+    s.ay = 12 * s.E * s.Izz / (s.G * s.A * L * L);
+    s.az = 12 * s.E * s.Iyy / (s.G * s.A * L * L);
+    s.by = 1 / (1. - s.ay);
+    s.bz = 1 / (1. - s.az);
+    return s;
+}
+
+// fill the symmetric 12x12 T from its upper triangle U (row-major, U[i][j], j >= i)
+__device__ __forceinline__ void beam_sym(double *T)
+{
+    for (int i = 1; i < 12; i++)
+        for (int j = 0; j < i; j++) T[i * 12 + j] = T[j * 12 + i];
+}
+
+// KC0: beamc.pyx:543-624
+__device__ void beam_kc0_local(double L, const BeamSec &s, double *U)
+{
+    const double A = s.A, E = s.E, G = s.G, Iyy = s.Iyy, Izz = s.Izz, Jt = s.J;
+    const double Ay = s.Ay, Az = s.Az, ay = s.ay, az = s.az, by = s.by, bz = s.bz;
     const double L2 = L * L, L3 = L2 * L;
-    const double ay = 12 * E * Izz / (G * A * L2), az = 12 * E * Iyy / (G * A * L2);
-    const double by = 1 / (1. - ay), bz = 1 / (1. - az);
     const double kyy = by * by * (A * G * L2 * ay * ay + 12 * E * Izz);
     const double kzz = bz * bz * (A * G * L2 * az * az + 12 * E * Iyy);
-    const double cyz = E * Iyz * by * bz;
+    const double cyz = E * s.Iyz * by * bz;
+    const double ta = Az * G * ay * by, tb = Ay * G * az * bz;
+    for (int k = 0; k < 144; k++) U[k] = 0.;
+#define B(i, j) U[(i) * 12 + (j)]
+    B(0, 0) = A * E / L; B(0, 6) = -A * E / L; B(6, 6) = A * E / L;
+    B(0, 4) = Az * E * bz * (1 - az) / L; B(0, 10) = Az * E * bz * (az - 1) / L;
+    B(0, 5) = Ay * E * by * (ay - 1) / L; B(0, 11) = Ay * E * by * (1 - ay) / L;
+    B(4, 6) = Az * E * bz * (az - 1) / L; B(6, 10) = Az * E * bz * (1 - az) / L;
+    B(5, 6) = Ay * E * by * (1 - ay) / L; B(6, 11) = Ay * E * by * (ay - 1) / L;
+    B(1, 1) = kyy / L3; B(1, 7) = -kyy / L3; B(7, 7) = kyy / L3;
+    B(1, 5) = kyy / (2 * L2); B(1, 11) = kyy / (2 * L2);
+    B(5, 7) = -kyy / (2 * L2); B(7, 11) = -kyy / (2 * L2);
+    B(2, 2) = kzz / L3; B(2, 8) = -kzz / L3; B(8, 8) = kzz / L3;
+    B(2, 4) = -kzz / (2 * L2); B(2, 10) = -kzz / (2 * L2);
+    B(4, 8) = kzz / (2 * L2); B(8, 10) = kzz / (2 * L2);
+    B(1, 2) = 12 * cyz / L3; B(1, 8) = -12 * cyz / L3;
+    B(2, 7) = -12 * cyz / L3; B(7, 8) = 12 * cyz / L3;
+    B(1, 4) = -6 * cyz / L2; B(1, 10) = -6 * cyz / L2;
+    B(2, 5) = 6 * cyz / L2; B(2, 11) = 6 * cyz / L2;
+    B(4, 7) = 6 * cyz / L2; B(5, 8) = -6 * cyz / L2;
+    B(7, 10) = 6 * cyz / L2; B(8, 11) = -6 * cyz / L2;
+    B(3, 3) = G * Jt / L; B(3, 9) = -G * Jt / L; B(9, 9) = G * Jt / L;
+    B(1, 3) = ta / L; B(1, 9) = -ta / L; B(3, 7) = -ta / L; B(7, 9) = ta / L;
+    B(2, 3) = -tb / L; B(2, 9) = tb / L; B(3, 8) = tb / L; B(8, 9) = -tb / L;
+    B(3, 4) = tb / 2; B(3, 10) = tb / 2; B(4, 9) = -tb / 2; B(9, 10) = -tb / 2;
+    B(3, 5) = ta / 2; B(3, 11) = ta / 2; B(5, 9) = -ta / 2; B(9, 11) = -ta / 2;
     const double ryy = bz * bz * (A * G * L2 * az * az / 4 + E * Iyy * az * az - 2 * E * Iyy * az + 4 * E * Iyy) / L;
     const double ryy2 = bz * bz * (A * G * L2 * az * az / 4 - E * Iyy * az * az + 2 * E * Iyy * az + 2 * E * Iyy) / L;
     const double rzz = by * by * (A * G * L2 * ay * ay / 4 + E * Izz * ay * ay - 2 * E * Izz * ay + 4 * E * Izz) / L;
     const double rzz2 = by * by * (A * G * L2 * ay * ay / 4 - E * Izz * ay * ay + 2 * E * Izz * ay + 2 * E * Izz) / L;
+    B(4, 4) = ryy; B(10, 10) = ryy; B(4, 10) = ryy2;
+    B(5, 5) = rzz; B(11, 11) = rzz; B(5, 11) = rzz2;
     const double c1 = cyz * (-ay * az + ay + az - 4) / L, c2 = cyz * (ay * az - ay - az - 2) / L;
-    const double ta = Az * G * ay * by, tb = Ay * G * az * bz;
-    // full 12x12 upper triangle, entry (i, j) i <= j
-    auto up = [&](int i, int j) -> double {
-        switch (i * 12 + j) {
-        case 0: return A * E / L;
-        case 4: return Az * E * bz * (1 - az) / L;
-        case 5: return Ay * E * by * (ay - 1) / L;
-        case 6: return -A * E / L;
-        case 10: return Az * E * bz * (az - 1) / L;
-        case 11: return Ay * E * by * (1 - ay) / L;
-        case 13: return kyy / L3;
-        case 14: return 12 * cyz / L3;
-        case 15: return ta / L;
-        case 16: return -6 * cyz / L2;
-        case 17: return kyy / (2 * L2);
-        case 19: return -kyy / L3;
-        case 20: return -12 * cyz / L3;
-        case 21: return -ta / L;
-        case 22: return -6 * cyz / L2;
-        case 23: return kyy / (2 * L2);
-        case 26: return kzz / L3;
-        case 27: return -tb / L;
-        case 28: return -kzz / (2 * L2);
-        case 29: return 6 * cyz / L2;
-        case 31: return -12 * cyz / L3;
-        case 32: return -kzz / L3;
-        case 33: return tb / L;
-        case 34: return -kzz / (2 * L2);
-        case 35: return 6 * cyz / L2;
-        case 39: return G * Jt / L;
-        case 40: return tb / 2;
-        case 41: return ta / 2;
-        case 43: return -ta / L;
-        case 44: return tb / L;
-        case 45: return -G * Jt / L;
-        case 46: return tb / 2;
-        case 47: return ta / 2;
-        case 52: return ryy;
-        case 53: return c1;
-        case 54: return Az * E * bz * (az - 1) / L;
-        case 55: return 6 * cyz / L2;
-        case 56: return kzz / (2 * L2);
-        case 57: return -tb / 2;
-        case 58: return ryy2;
-        case 59: return c2;
-        case 65: return rzz;
-        case 66: return Ay * E * by * (1 - ay) / L;
-        case 67: return -kyy / (2 * L2);
-        case 68: return -6 * cyz / L2;
-        case 69: return -ta / 2;
-        case 70: return c2;
-        case 71: return rzz2;
-        case 78: return A * E / L;
-        case 82: return Az * E * bz * (1 - az) / L;
-        case 83: return Ay * E * by * (ay - 1) / L;
-        case 91: return kyy / L3;
-        case 92: return 12 * cyz / L3;
-        case 93: return ta / L;
-        case 94: return 6 * cyz / L2;
-        case 95: return -kyy / (2 * L2);
-        case 104: return kzz / L3;
-        case 105: return -tb / L;
-        case 106: return kzz / (2 * L2);
-        case 107: return -6 * cyz / L2;
-        case 117: return G * Jt / L;
-        case 118: return -tb / 2;
-        case 119: return -ta / 2;
-        case 130: return ryy;
-        case 131: return c1;
-        case 143: return rzz;
-        default: return 0.;
-        }
-    };
-    for (int i = 0; i < 6; i++)
-        for (int j = 0; j < 6; j++) {
-            const int gi = 6 * I + i, gj = 6 * J + j;
-            K[i][j] = gi <= gj ? up(gi, gj) : up(gj, gi);
+    B(4, 5) = c1; B(10, 11) = c1; B(4, 11) = c2; B(5, 10) = c2;
+#undef B
+    beam_sym(U);
+}
+
+// KG for axial force N: beamc.pyx:1889-2180.  v, rz bend in y and w, ry in
+// z; with s_rot(ry) = -1, s_rot(rz) = +1 and each DOF's own alpha/beta:
+//   disp-disp +-(5 a_i a_j - 5 a_i - 5 a_j + 6)/(5L) (+ on the same node)
+//   disp-rot  s_rot (+1 / -1 for the disp DOF on node 1 / 2) / 10
+//   rot-rot   s_rot_i s_rot_j L ((q + 8)/60 same node, -(q + 2)/60 else)
+// all times N beta_i beta_j.
+__device__ void beam_kg_local(double L, const BeamSec &s, double N, double *U)
+{
+    for (int i = 0; i < 12; i++)
+        for (int j = 0; j < 12; j++) {
+            const int di = i % 6, dj = j % 6, ni = i / 6, nj = j / 6;
+            const bool act_i = di == 1 || di == 2 || di == 4 || di == 5;
+            const bool act_j = dj == 1 || dj == 2 || dj == 4 || dj == 5;
+            double v = 0.;
+            if (act_i && act_j) {
+                const bool yi = (di == 1 || di == 5), yj = (dj == 1 || dj == 5);
+                const double a_i = yi ? s.ay : s.az, a_j = yj ? s.ay : s.az;
+                const double bb = N * (yi ? s.by : s.bz) * (yj ? s.by : s.bz);
+                const bool ri = di >= 4, rj = dj >= 4;
+                const double si = di == 4 ? -1. : 1., sj = dj == 4 ? -1. : 1.;
+                const double q = 5 * a_i * a_j - 5 * a_i - 5 * a_j;
+                double f;
+                if (!ri && !rj) f = (ni == nj ? 1. : -1.) * (q + 6) / (5 * L);
+                else if (!ri && rj) f = sj * (ni == 0 ? 1. : -1.) / 10;
+                else if (ri && !rj) f = si * (nj == 0 ? 1. : -1.) / 10;
+                else f = si * sj * L * (ni == nj ? (q + 8) / 60 : -(q + 2) / 60);
+                v = bb * f;
+            }
+            U[i * 12 + j] = v;
         }
 }
 
-__global__ void __launch_bounds__(64) beamc_kc0(AsmArgs a)
+// M: consistent (mtype 0, beamc.pyx:2246-2968), lumped (mtype 1, :2969-3152)
+__device__ void beam_m_local(double L, const BeamSec &s, const double *bp, int mtype, double *U)
+{
+    const double r = bp[9], ry = bp[10], rz = bp[11], ry2 = bp[12], rz2 = bp[13], ryz = bp[14];
+    const double ay = s.ay, az = s.az, by = s.by, bz = s.bz, L2 = L * L;
+    for (int k = 0; k < 144; k++) U[k] = 0.;
+#define B(i, j) U[(i) * 12 + (j)]
+    if (mtype == 1) {
+        for (int n = 0; n < 2; n++) {
+            const int o = 6 * n;
+            B(o, o) = L * r / 2;
+            B(o + 1, o + 1) = L * by * by * r * (ay * ay - 2 * ay + 1) / 2;
+            B(o + 2, o + 2) = L * bz * bz * r * (az * az - 2 * az + 1) / 2;
+            B(o + 3, o + 3) = L * (ry2 + rz2) / 2;
+            B(o + 4, o + 4) = L * bz * bz * rz2 * (az * az - 2 * az + 1) / 2;
+            B(o + 5, o + 5) = L * by * by * ry2 * (ay * ay - 2 * ay + 1) / 2;
+        }
+        return;
+    }
+    const double tyy = by * by * (70 * L2 * ay * ay * r - 147 * L2 * ay * r + 78 * L2 * r + 252 * ry2) / (210 * L);
+    const double tzz = bz * bz * (70 * L2 * az * az * r - 147 * L2 * az * r + 78 * L2 * r + 252 * rz2) / (210 * L);
+    const double tyy2 = by * by * (35 * L2 * ay * ay * r - 63 * L2 * ay * r + 27 * L2 * r - 252 * ry2) / (210 * L);
+    const double tzz2 = bz * bz * (35 * L2 * az * az * r - 63 * L2 * az * r + 27 * L2 * r - 252 * rz2) / (210 * L);
+    const double cy1 = by * by * (35 * L2 * ay * ay * r - 77 * L2 * ay * r + 44 * L2 * r + 420 * ay * ry2 + 84 * ry2) / 840;
+    const double cy2 = by * by * (-35 * L2 * ay * ay * r + 63 * L2 * ay * r - 26 * L2 * r + 420 * ay * ry2 + 84 * ry2) / 840;
+    const double cz1 = bz * bz * (-35 * L2 * az * az * r + 77 * L2 * az * r - 44 * L2 * r - 420 * az * rz2 - 84 * rz2) / 840;
+    const double cz2 = bz * bz * (35 * L2 * az * az * r - 63 * L2 * az * r + 26 * L2 * r - 420 * az * rz2 - 84 * rz2) / 840;
+    const double qzz = L * bz * bz * (7 * L2 * az * az * r - 14 * L2 * az * r + 8 * L2 * r + 280 * az * az * rz2 - 140 * az * rz2 + 112 * rz2) / 840;
+    const double qyy = L * by * by * (7 * L2 * ay * ay * r - 14 * L2 * ay * r + 8 * L2 * r + 280 * ay * ay * ry2 - 140 * ay * ry2 + 112 * ry2) / 840;
+    const double qzz2 = L * bz * bz * (-7 * L2 * az * az * r + 14 * L2 * az * r - 6 * L2 * r + 140 * az * az * rz2 + 140 * az * rz2 - 28 * rz2) / 840;
+    const double qyy2 = L * by * by * (-7 * L2 * ay * ay * r + 14 * L2 * ay * r - 6 * L2 * r + 140 * ay * ay * ry2 + 140 * ay * ry2 - 28 * ry2) / 840;
+    const double pyz = 6 * by * bz * ryz / (5 * L);
+    const double sz1 = by * bz * ryz * (5 * az + 1) / 10, sy1 = by * bz * ryz * (5 * ay + 1) / 10;
+    const double qyz = L * by * bz * ryz * (-20 * ay * az + 5 * ay + 5 * az - 8) / 60;
+    const double qyz2 = L * by * bz * ryz * (-10 * ay * az - 5 * ay - 5 * az + 2) / 60;
+    B(0, 0) = L * r / 3; B(0, 6) = L * r / 6; B(6, 6) = L * r / 3;
+    B(0, 1) = by * ry / 2; B(0, 7) = -by * ry / 2; B(1, 6) = by * ry / 2; B(6, 7) = -by * ry / 2;
+    B(0, 2) = bz * rz / 2; B(0, 8) = -bz * rz / 2; B(2, 6) = bz * rz / 2; B(6, 8) = -bz * rz / 2;
+    B(0, 4) = L * bz * rz * (1 - 4 * az) / 12; B(6, 10) = B(0, 4);
+    B(0, 5) = L * by * ry * (4 * ay - 1) / 12; B(6, 11) = B(0, 5);
+    B(0, 10) = -L * bz * rz * (2 * az + 1) / 12; B(4, 6) = B(0, 10);
+    B(0, 11) = L * by * ry * (2 * ay + 1) / 12; B(5, 6) = B(0, 11);
+    B(1, 1) = tyy; B(7, 7) = tyy; B(1, 7) = tyy2;
+    B(2, 2) = tzz; B(8, 8) = tzz; B(2, 8) = tzz2;
+    B(1, 2) = pyz; B(7, 8) = pyz; B(1, 8) = -pyz; B(2, 7) = -pyz;
+    B(1, 5) = cy1; B(7, 11) = -cy1; B(1, 11) = cy2; B(5, 7) = -cy2;
+    B(2, 4) = cz1; B(8, 10) = -cz1; B(2, 10) = cz2; B(4, 8) = -cz2;
+    B(1, 4) = -sz1; B(1, 10) = -sz1; B(4, 7) = sz1; B(7, 10) = sz1;
+    B(2, 5) = sy1; B(2, 11) = sy1; B(5, 8) = -sy1; B(8, 11) = -sy1;
+    B(1, 3) = L * by * rz * (20 * ay - 21) / 60; B(7, 9) = B(1, 3);
+    B(1, 9) = L * by * rz * (10 * ay - 9) / 60; B(3, 7) = B(1, 9);
+    B(2, 3) = L * bz * ry * (21 - 20 * az) / 60; B(8, 9) = B(2, 3);
+    B(2, 9) = L * bz * ry * (9 - 10 * az) / 60; B(3, 8) = B(2, 9);
+    B(3, 3) = L * (ry2 + rz2) / 3; B(9, 9) = B(3, 3); B(3, 9) = L * (ry2 + rz2) / 6;
+    B(3, 4) = L2 * bz * ry * (5 * az - 6) / 120; B(9, 10) = -B(3, 4);
+    B(3, 5) = L2 * by * rz * (5 * ay - 6) / 120; B(9, 11) = -B(3, 5);
+    B(3, 10) = L2 * bz * ry * (4 - 5 * az) / 120; B(4, 9) = -B(3, 10);
+    B(3, 11) = L2 * by * rz * (4 - 5 * ay) / 120; B(5, 9) = -B(3, 11);
+    B(4, 4) = qzz; B(10, 10) = qzz; B(4, 10) = qzz2;
+    B(5, 5) = qyy; B(11, 11) = qyy; B(5, 11) = qyy2;
+    B(4, 5) = qyz; B(10, 11) = qyz; B(4, 11) = qyz2; B(5, 10) = qyz2;
+#undef B
+    beam_sym(U);
+}
+
+// slot -> (local row, local col) for beams: 12x12 row-major, or the lumped
+// mass's node-diagonal 3x3 translation and rotation blocks (beamc.pyx:2969+)
+__device__ __forceinline__ void beam_slot(int lumped, int k, int &rl, int &cl)
+{
+    if (!lumped) {
+        rl = k / 12;
+        cl = k - rl * 12;
+    } else {
+        const int n = k / 18, g = (k % 18) / 9, t = k % 9;
+        rl = 6 * n + 3 * g + t / 3;
+        cl = 6 * n + 3 * g + t % 3;
+    }
+}
+
+template <int MAT>
+__global__ void __launch_bounds__(64) beam_assemble(AsmArgs a)
 {
     __shared__ double tile[16][144];
     __shared__ int64_t sconn[16][2];
     __shared__ int64_t sk0[16];
+    __shared__ double sr[16][9];
     const int lane = threadIdx.x, slot = lane >> 2, I = (lane >> 1) & 1, J = lane & 1;
     const int64_t e0 = (int64_t)blockIdx.x * 16;
     const int64_t e = e0 + slot;
-    if (e < a.n_elem) {
+    const bool lumped = (MAT == MASS) && a.mtype == 1;
+    const int nslot = lumped ? 36 : 144;
+    if (e < a.n_elem && (lane & 3) == 0) {
         V3 X[2];
         int64_t cn[2];
         for (int i = 0; i < 2; i++) {
@@ -664,32 +875,52 @@ __global__ void __launch_bounds__(64) beamc_kc0(AsmArgs a)
             const double *px = a.x + cn[i] / 2;
             X[i] = {px[0], px[1], px[2]};
         }
-        if ((lane & 3) == 0) {
-            sconn[slot][0] = cn[0];
-            sconn[slot][1] = cn[1];
-            sk0[slot] = a.init_k ? a.init_k[e] : 144 * e;
-        }
+        sconn[slot][0] = cn[0];
+        sconn[slot][1] = cn[1];
+        sk0[slot] = a.init_k ? a.init_k[e] : (int64_t)nslot * e;
         V3 vxy = {a.evec[3 * e], a.evec[3 * e + 1], a.evec[3 * e + 2]};
         Geom g;
         beam_geometry(X, vxy, g);
-        double K[6][6];
-        beamc_local(g.area, a.props + (a.pid ? (int64_t)a.pid[e] : 0) * BEAM_STRIDE, I, J, K);
-        rotate_block(K, g.r);
+        for (int i = 0; i < 9; i++) sr[slot][i] = g.r[i];
+        const double *bp = a.props + (a.pid ? (int64_t)a.pid[e] : 0) * BEAM_STRIDE;
+        const BeamSec sec = beam_section(bp, g.area);
+        if (MAT == KC0) beam_kc0_local(g.area, sec, tile[slot]);
+        else if (MAT == KG) {
+            // axial force from the local axial displacements (beamc.pyx:1889)
+            double u1 = 0., u2 = 0.;
+            for (int i = 0; i < 3; i++) {
+                u1 += g.r[3 * i] * a.u[cn[0] + i];
+                u2 += g.r[3 * i] * a.u[cn[1] + i];
+            }
+            beam_kg_local(g.area, sec, sec.A * sec.E * (-u1 + u2) / g.area, tile[slot]);
+        } else beam_m_local(g.area, sec, bp, a.mtype, tile[slot]);
+    }
+    __syncthreads();
+    double K[6][6];
+    if (e < a.n_elem) {
+        for (int i = 0; i < 6; i++)
+            for (int j = 0; j < 6; j++) K[i][j] = tile[slot][(6 * I + i) * 12 + 6 * J + j];
+        double r[9];
+        for (int i = 0; i < 9; i++) r[i] = sr[slot][i];
+        rotate_block(K, r);
+    }
+    __syncthreads();
+    if (e < a.n_elem)
         for (int i = 0; i < 6; i++)
             for (int j = 0; j < 6; j++) tile[slot][(6 * I + i) * 12 + 6 * J + j] = K[i][j];
-    }
     __syncthreads();
     const int nel = (int)((a.n_elem - e0) < 16 ? (a.n_elem - e0) : 16);
     const bool write_idx = !(a.flags & F_VALUES_ONLY);
     const bool overwrite = (a.flags & F_OVERWRITE);
     for (int el = 0; el < nel; el++) {
         const int64_t k0 = sk0[el];
-        for (int k = lane; k < 144; k += 64) {
-            const double val = tile[el][k];
+        for (int k = lane; k < nslot; k += 64) {
+            int rl, cl;
+            beam_slot(lumped, k, rl, cl);
+            const double val = tile[el][rl * 12 + cl];
             if (overwrite) a.v[k0 + k] = val;
             else a.v[k0 + k] += val;
             if (write_idx) {
-                const int rl = k / 12, cl = k % 12;
                 a.r[k0 + k] = sconn[el][rl / 6] + rl % 6;
                 a.c[k0 + k] = sconn[el][cl / 6] + cl % 6;
             }
@@ -708,6 +939,13 @@ template <int ET, int MAT>
 static hipError_t launch_shell(const AsmArgs &a, hipStream_t s)
 {
     constexpr int NN = (ET == TRIA3R) ? 3 : 4;
+    if (MAT == KC0) {
+        constexpr int EPW = 64 / (NN * (NN + 1) / 2);
+        const int64_t blocks = (a.n_elem + EPW - 1) / EPW;
+        if (blocks == 0) return hipSuccess;
+        hipLaunchKernelGGL(shell_kc0_sym<ET>, dim3((unsigned)blocks), dim3(64), 0, s, a);
+        return hipGetLastError();
+    }
     constexpr int EPB = (64 / (NN * NN)) * WAVES_PER_BLOCK;
     const int64_t blocks = (a.n_elem + EPB - 1) / EPB;
     if (blocks == 0) return hipSuccess;
@@ -736,11 +974,16 @@ hipError_t pf3d_launch_shell(int etype, int mat, const AsmArgs &a, hipStream_t s
     }
 }
 
-hipError_t pf3d_launch_beam_kc0(const AsmArgs &a, hipStream_t s)
+hipError_t pf3d_launch_beam(int mat, const AsmArgs &a, hipStream_t s)
 {
     const int64_t blocks = (a.n_elem + 15) / 16;
     if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(beamc_kc0, dim3((unsigned)blocks), dim3(64), 0, s, a);
+    if (mat == KC0)
+        hipLaunchKernelGGL(beam_assemble<KC0>, dim3((unsigned)blocks), dim3(64), 0, s, a);
+    else if (mat == KG)
+        hipLaunchKernelGGL(beam_assemble<KG>, dim3((unsigned)blocks), dim3(64), 0, s, a);
+    else
+        hipLaunchKernelGGL(beam_assemble<MASS>, dim3((unsigned)blocks), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 

@@ -8,7 +8,7 @@
 
 // launchers defined next to the kernels
 hipError_t pf3d_launch_shell(int etype, int mat, const pf3d::AsmArgs &a, hipStream_t s);
-hipError_t pf3d_launch_beam_kc0(const pf3d::AsmArgs &a, hipStream_t s);
+hipError_t pf3d_launch_beam(int mat, const pf3d::AsmArgs &a, hipStream_t s);
 hipError_t pf3d_launch_geometry(int etype, int64_t n, const double *x, const int64_t *conn,
                                 const double *evec, double *geom, double *xe, hipStream_t s);
 
@@ -41,7 +41,10 @@ int pf3d_slot_count(int etype, int matrix, int mtype)
 {
     if (etype == PF3D_BEAMC) {
         if (matrix == PF3D_KC0 || matrix == PF3D_KG) return 144;
-        if (matrix == PF3D_M) return mtype == 0 ? 144 : 36;
+        if (matrix == PF3D_M) {
+            if (mtype < 0 || mtype > 1) return pf3d_fail(PF3D_E_ARG, "BeamC mtype must be 0 or 1");
+            return mtype == 0 ? 144 : 36;
+        }
         return pf3d_fail(PF3D_E_ARG, "BeamC has no KG_given_stress");
     }
     if (etype < 0 || etype > PF3D_TRIA3R) return pf3d_fail(PF3D_E_ARG, "bad etype %d", etype);
@@ -161,7 +164,46 @@ int pf3d_beam_kc0(int64_t n_elem, const double *x, const int64_t *conn, const in
     pf3d::AsmArgs a{};
     a.n_elem = n_elem; a.x = x; a.conn = conn; a.init_k = init_k; a.evec = vxy; a.pid = pid;
     a.props = props; a.flags = flags; a.r = r; a.c = c; a.v = v;
-    return pf3d_hip_check(pf3d_launch_beam_kc0(a, (hipStream_t)stream), "pf3d_beam_kc0");
+    return pf3d_hip_check(pf3d_launch_beam(PF3D_KC0, a, (hipStream_t)stream), "pf3d_beam_kc0");
+}
+
+static int beam_check(int64_t n_elem, const double *x, const int64_t *conn, const double *vxy,
+                      const double *props, int64_t *r, int64_t *c, double *v, int flags)
+{
+    if (n_elem < 0) return pf3d_fail(PF3D_E_ARG, "negative n_elem");
+    if (n_elem == 0) return 1;
+    if (!x || !conn || !v || !vxy || !props) return pf3d_fail(PF3D_E_ARG, "null argument");
+    if (!(flags & PF3D_VALUES_ONLY) && (!r || !c))
+        return pf3d_fail(PF3D_E_ARG, "null r/c without PF3D_VALUES_ONLY");
+    if (flags & ~(PF3D_VALUES_ONLY | PF3D_OVERWRITE))
+        return pf3d_fail(PF3D_E_ARG, "unknown flags 0x%x", flags);
+    return 0;
+}
+
+int pf3d_beam_kg(int64_t n_elem, const double *x, const int64_t *conn, const int64_t *init_k,
+                 const double *vxy, const int32_t *pid, const double *props, const double *u,
+                 int64_t *r, int64_t *c, double *v, int flags, void *stream)
+{
+    int st = beam_check(n_elem, x, conn, vxy, props, r, c, v, flags);
+    if (st) return st > 0 ? PF3D_OK : st;
+    if (!u) return pf3d_fail(PF3D_E_ARG, "null u");
+    pf3d::AsmArgs a{};
+    a.n_elem = n_elem; a.x = x; a.conn = conn; a.init_k = init_k; a.evec = vxy; a.pid = pid;
+    a.props = props; a.u = u; a.flags = flags; a.r = r; a.c = c; a.v = v;
+    return pf3d_hip_check(pf3d_launch_beam(PF3D_KG, a, (hipStream_t)stream), "pf3d_beam_kg");
+}
+
+int pf3d_beam_m(int64_t n_elem, const double *x, const int64_t *conn, const int64_t *init_k,
+                const double *vxy, const int32_t *pid, const double *props, int mtype,
+                int64_t *r, int64_t *c, double *v, int flags, void *stream)
+{
+    int st = beam_check(n_elem, x, conn, vxy, props, r, c, v, flags);
+    if (st) return st > 0 ? PF3D_OK : st;
+    if (mtype < 0 || mtype > 1) return pf3d_fail(PF3D_E_ARG, "BeamC mtype must be 0 or 1");
+    pf3d::AsmArgs a{};
+    a.n_elem = n_elem; a.x = x; a.conn = conn; a.init_k = init_k; a.evec = vxy; a.pid = pid;
+    a.props = props; a.mtype = mtype; a.flags = flags; a.r = r; a.c = c; a.v = v;
+    return pf3d_hip_check(pf3d_launch_beam(PF3D_M, a, (hipStream_t)stream), "pf3d_beam_m");
 }
 
 }  // extern "C"

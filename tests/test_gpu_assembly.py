@@ -70,17 +70,32 @@ def test_geometry_gpu(etype):
 
 def test_beam_golden_gpu():
     from pyfe3d_amd import assembly as A
+    bad = []
     for name, d in cases('beamc'):
         E = d['conn'].shape[0]
-        r = torch.zeros(144 * E, dtype=torch.int64, device=DEV)
-        c = torch.zeros_like(r)
-        v = torch.zeros(144 * E, dtype=torch.float64, device=DEV)
-        A.beam_kc0(T(d['x'], torch.float64), T(6 * d['conn'], torch.int64),
-                   T(d['vxy'], torch.float64), r, c, v,
-                   T(O.props_table(d['props'], 'beamc'), torch.float64),
-                   pid=T(np.arange(E), torch.int32), init_k=T(144 * d['perm'], torch.int64))
-        assert (r.cpu().numpy() == d['KC0r']).all() and (c.cpu().numpy() == d['KC0c']).all()
-        assert element_errors(v.cpu().numpy(), d['KC0v'], 144, d['perm']) <= TOL
+        for key, mtype in (('KC0', 0), ('KG', 0), ('M0', 0), ('M1', 1)):
+            size = d[key + 'v'].size // E
+            r = torch.zeros(size * E, dtype=torch.int64, device=DEV)
+            c = torch.zeros_like(r)
+            v = torch.zeros(size * E, dtype=torch.float64, device=DEV)
+            args = (T(d['x'], torch.float64), T(6 * d['conn'], torch.int64),
+                    T(d['vxy'], torch.float64))
+            kw = dict(pid=T(np.arange(E), torch.int32), init_k=T(size * d['perm'], torch.int64))
+            props = T(O.props_table(d['props'], 'beamc'), torch.float64)
+            if key == 'KC0':
+                A.beam_kc0(*args, r, c, v, props, **kw)
+            elif key == 'KG':
+                A.beam_kg(*args, T(d['u'], torch.float64), r, c, v, props, **kw)
+            else:
+                A.beam_m(*args, r, c, v, props, mtype=mtype, **kw)
+            if not ((r.cpu().numpy() == d[key + 'r']).all() and
+                    (c.cpu().numpy() == d[key + 'c']).all()):
+                bad.append((name, key, 'indices'))
+            err = element_errors(v.cpu().numpy(), d[key + 'v'], size, d['perm'])
+            print('beamc', name, key, 'max rel err %.2e' % err)
+            if not err <= TOL:
+                bad.append((name, key, err))
+    assert not bad, bad
 
 
 def test_accumulate_overwrite_values_only():

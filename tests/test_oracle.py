@@ -39,9 +39,33 @@ def test_beam_golden():
         geom, _ = O.geometry('beamc', d['x'], d['conn'], d['vxy'])
         np.testing.assert_allclose(geom[:, :9], d['geom'][:, :9], rtol=0, atol=1e-14)
         np.testing.assert_allclose(geom[:, 13], d['geom'][:, 9], rtol=1e-14)
-        r, c, v, size = _run('beamc', d, 'KC0', 'KC0', 0)
-        assert (r == d['KC0r']).all() and (c == d['KC0c']).all()
-        assert element_errors(v, d['KC0v'], size, d['perm']) <= TOL
+        for key, mat, mtype in (('KC0', 'KC0', 0), ('KG', 'KG', 0), ('M0', 'M', 0), ('M1', 'M', 1)):
+            r, c, v, size = _run('beamc', d, key, mat, mtype)
+            assert (r == d[key + 'r']).all() and (c == d[key + 'c']).all(), (name, key)
+            assert element_errors(v, d[key + 'v'], size, d['perm']) <= TOL, (name, key)
+
+
+def test_beam_mass_and_kg_invariants():
+    """Lumped mass: each node carries half of intrho*L per translation;
+    consistent mass has the same total translational mass; KG vanishes for a
+    rigid translation (no axial force) and is linear in N."""
+    L, E, A = 0.8, 70e9, 8e-5
+    bp = np.zeros(16)
+    bp[:9] = [A, E, E / 2.66 * 5 / 6, 1e-10, 4e-9, 0., 4.1e-9, 0., 0.]
+    bp[9:15] = [0.216, 0., 0., 1e-6, 2e-7, 0.]
+    X = np.array([[0.1, 0.2, 0.], [0.1 + L, 0.2, 0.]])
+    vxy = np.array([0., 1., 0.])
+    M1 = _dense_element('beamc', 'M', X, vxy, bp, mtype=1)
+    M0 = _dense_element('beamc', 'M', X, vxy, bp, mtype=0)
+    t = np.zeros(12); t[0::6] = 1
+    assert np.isclose(t @ M1 @ t, bp[9] * L, rtol=1e-14)
+    assert np.isclose(t @ M0 @ t, bp[9] * L, rtol=1e-14)
+    u = np.zeros(12); u[6] = 1e-4  # stretch: N = E A 1e-4 / L
+    K1 = _dense_element('beamc', 'KG', X, vxy, bp, u=u)
+    K2 = _dense_element('beamc', 'KG', X, vxy, bp, u=2 * u)
+    assert np.allclose(K2, 2 * K1, rtol=1e-14, atol=0)
+    K0 = _dense_element('beamc', 'KG', X, vxy, bp, u=np.ones(12) * 1e-3 * t)
+    assert np.abs(K0).max() == 0
 
 
 def _dense_element(etype, mat, X, evec, prop, mtype=0, u=None, stress=None, k6rot=150.):
