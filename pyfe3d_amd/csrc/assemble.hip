@@ -10,9 +10,12 @@
 // instruction writes 64 consecutive doubles of the COO value array.
 #include "pf3d_device.hpp"
 
+// unroll factor of the 2x2 Gauss loop of the Quad4 KC0 block integral
+#ifndef PF3D_GP_UNROLL
+#define PF3D_GP_UNROLL 4
+#endif
+
 namespace pf3d {
-
-
 
 // ------------------------------------------------------------------------
 // slot layouts: slot k of an element -> (local row, local col)
@@ -110,20 +113,20 @@ __device__ __forceinline__ double node_sy(int I) { return (I >= 2) ? 1. : -1.; }
 __device__ __forceinline__ void add_abd(double K[6][6], const double C[6][6], double nxi,
                                         double nyi, double nxj, double nyj, double w)
 {
-    double CB[4][6];  // columns u, v, rx, ry of C*B_J
-#pragma unroll
-    for (int a = 0; a < 6; a++) {
-        CB[0][a] = C[a][0] * nxj + C[a][2] * nyj;
-        CB[1][a] = C[a][1] * nyj + C[a][2] * nxj;
-        CB[2][a] = -(C[a][4] * nyj + C[a][5] * nxj);
-        CB[3][a] = C[a][3] * nxj + C[a][5] * nyj;
-    }
     nxi *= w;
     nyi *= w;
     const int col[4] = {0, 1, 3, 4};
+    // one column of C*B_J at a time (u, v, rx, ry), folded into K at once
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-        const double *cb = CB[t];
+        double cb[6];
+#pragma unroll
+        for (int a = 0; a < 6; a++) {
+            if (t == 0) cb[a] = C[a][0] * nxj + C[a][2] * nyj;
+            else if (t == 1) cb[a] = C[a][1] * nyj + C[a][2] * nxj;
+            else if (t == 2) cb[a] = -(C[a][4] * nyj + C[a][5] * nxj);
+            else cb[a] = C[a][3] * nxj + C[a][5] * nyj;
+        }
         K[0][col[t]] += nxi * cb[0] + nyi * cb[2];
         K[1][col[t]] += nyi * cb[1] + nxi * cb[2];
         K[3][col[t]] -= nyi * cb[4] + nxi * cb[5];
@@ -216,7 +219,7 @@ __device__ __forceinline__ void quad4_kc0(double K[6][6], const Geom &g, const S
 {
     const bool thick = (s.h / sqrt(g.area)) >= 1.;
     const double sxi = node_sx(I), syi = node_sy(I), sxj = node_sx(J), syj = node_sy(J);
-#pragma unroll
+#pragma unroll PF3D_GP_UNROLL
     for (int p = 0; p < 4; p++) {
         const Pair n = quad_pair(g.xe, (p >> 1) ? GP2 : -GP2, (p & 1) ? GP2 : -GP2, sxi, syi,
                                  sxj, syj);
@@ -538,7 +541,7 @@ __device__ __forceinline__ void upper_pair(int sub, int &I, int &J)
 #define PF3D_KC0_WAVES 2
 #endif
 
-template <int ET>
+template <int ET, int FLAGS>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PF3D_KC0_WAVES)))
 shell_kc0_sym(AsmArgs a)
 {
@@ -599,8 +602,9 @@ shell_kc0_sym(AsmArgs a)
         rotate_block(K, g.r);
     }
     const int nel = (int)((a.n_elem - e0) < EPW ? (a.n_elem - e0) : EPW);
-    const bool write_idx = !(a.flags & F_VALUES_ONLY);
-    const bool overwrite = (a.flags & F_OVERWRITE);
+    constexpr bool write_idx = !(FLAGS & F_VALUES_ONLY);
+    constexpr bool overwrite = (FLAGS & F_OVERWRITE);
+    const bool vec_ok = (reinterpret_cast<uintptr_t>(a.v) & 15) == 0;
     for (int P = 0; P < NN; P++) {
         __syncthreads();  // the previous strip has been streamed out
         if (active && I == P) {
@@ -617,16 +621,35 @@ shell_kc0_sym(AsmArgs a)
                 for (int j = 0; j < 6; j++) t[j * NDOF + i] = K[i][j];
         }
         __syncthreads();
-        for (int idx = lane; idx < nel * RW; idx += 64) {
-            const int el = idx / RW, t = idx - el * RW;
+        // pairs of consecutive slots (same row, same node column block):
+        // one 16-byte LDS read and one 16-byte store per pair when the slot
+        // is 16-byte aligned; unrolled so the LDS reads issue back to back
+        constexpr int PR = RW / 2;                 // pairs per element strip
+        constexpr int NIT = (EPW * PR + 63) / 64;  // pair loads per lane
+#pragma unroll
+        for (int it = 0; it < NIT; it++) {
+            const int pi = lane + 64 * it;
+            if (pi >= nel * PR) break;
+            const int el = pi / PR, t = 2 * (pi - el * PR);
             const int64_t k = sk0[el] + P * RW + t;
-            const double val = strip[el][t];
-            if (overwrite) a.v[k] = val;
-            else a.v[k] += val;
+            const double2 val = *reinterpret_cast<const double2 *>(&strip[el][t]);
+            if (((k & 1) == 0) && vec_ok) {
+                double2 *dst = reinterpret_cast<double2 *>(a.v + k);
+                if (overwrite) *dst = val;
+                else {
+                    double2 old = *dst;
+                    *dst = make_double2(old.x + val.x, old.y + val.y);
+                }
+            } else {
+                if (overwrite) { a.v[k] = val.x; a.v[k + 1] = val.y; }
+                else { a.v[k] += val.x; a.v[k + 1] += val.y; }
+            }
             if (write_idx) {
                 const int rl = t / NDOF, cl = t - rl * NDOF;
-                a.r[k] = sconn[el][P] + rl;
-                a.c[k] = sconn[el][cl / 6] + cl % 6;
+                const int64_t rr = sconn[el][P] + rl;
+                const int64_t cc = sconn[el][cl / 6] + cl % 6;
+                a.r[k] = rr; a.r[k + 1] = rr;
+                a.c[k] = cc; a.c[k + 1] = cc + 1;
             }
         }
     }
@@ -943,7 +966,22 @@ static hipError_t launch_shell(const AsmArgs &a, hipStream_t s)
         constexpr int EPW = 64 / (NN * (NN + 1) / 2);
         const int64_t blocks = (a.n_elem + EPW - 1) / EPW;
         if (blocks == 0) return hipSuccess;
-        hipLaunchKernelGGL(shell_kc0_sym<ET>, dim3((unsigned)blocks), dim3(64), 0, s, a);
+        switch (a.flags & (F_VALUES_ONLY | F_OVERWRITE)) {
+        case 0:
+            hipLaunchKernelGGL((shell_kc0_sym<ET, 0>), dim3((unsigned)blocks), dim3(64), 0, s, a);
+            break;
+        case F_VALUES_ONLY:
+            hipLaunchKernelGGL((shell_kc0_sym<ET, F_VALUES_ONLY>), dim3((unsigned)blocks),
+                               dim3(64), 0, s, a);
+            break;
+        case F_OVERWRITE:
+            hipLaunchKernelGGL((shell_kc0_sym<ET, F_OVERWRITE>), dim3((unsigned)blocks), dim3(64),
+                               0, s, a);
+            break;
+        default:
+            hipLaunchKernelGGL((shell_kc0_sym<ET, F_VALUES_ONLY | F_OVERWRITE>),
+                               dim3((unsigned)blocks), dim3(64), 0, s, a);
+        }
         return hipGetLastError();
     }
     constexpr int EPB = (64 / (NN * NN)) * WAVES_PER_BLOCK;

@@ -184,11 +184,15 @@ __device__ __forceinline__ void rot3(double M[3][3], const double m[4])
 #pragma unroll
         for (int j = 0; j < 3; j++)
             TM[i][j] = T[i][0] * M[0][j] + T[i][1] * M[1][j] + T[i][2] * M[2][j];
+    // upper triangle only, mirrored: the rotated table stays exactly
+    // symmetric (the reference computes its six entries once)
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
-        for (int j = 0; j < 3; j++)
+        for (int j = i; j < 3; j++) {
             M[i][j] = TM[i][0] * T[j][0] + TM[i][1] * T[j][1] + TM[i][2] * T[j][2];
+            M[j][i] = M[i][j];
+        }
 }
 
 __device__ __forceinline__ void load_sym3(double M[3][3], const double *p)
