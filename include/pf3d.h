@@ -125,6 +125,45 @@ int pf3d_beam_m(int64_t n_elem, const double *x, const int64_t *conn, const int6
                 int64_t *r, int64_t *c, double *v, int flags, void *stream);
 
 /* ------------------------------------------------------------------ */
+/* Internal forces fint = sum_e R6 K_e ue (update_fint)                */
+/* ------------------------------------------------------------------ */
+
+/* Element internal forces in global axes, fe[(e*nn + i)*6 + d] for node i
+ * of element e (nn = 4 or 3), from the displacements u (6*n_nodes).
+ * Replaces update_probe_ue + update_probe_finte + the rotation in
+ * update_fint: quad4.pyx:627-680,1174-1202,1316-1363; quad4r.pyx:421,549,
+ * 4620-4680; tria3r.pyx:426,549-948,2975-3030 (the Tria3R internal forces
+ * keep all drilling couplings, unlike its KC0 scatter).  Optional arrays
+ * (xmat, pid, k6rot, alpha, hg) as in pf3d_shell_kc0. */
+int pf3d_shell_fe(int etype, int64_t n_elem, const double *x, const int64_t *conn,
+                  const double *xmat, const int32_t *pid, const double *props,
+                  const double *k6rot, const double *alpha, const double *hg, const double *u,
+                  double *fe, void *stream);
+
+/* BeamC element internal forces, fe[(e*2 + i)*6 + d].
+ * Replaces beamc.pyx:234-286 (ue), 352-488 (finte), 1354-1391 (rotation). */
+int pf3d_beam_fe(int64_t n_elem, const double *x, const int64_t *conn, const double *vxy,
+                 const int32_t *pid, const double *props, const double *u, double *fe,
+                 void *stream);
+
+/* Node incidence plan of n_entries (element, node) entries whose DOF
+ * offsets are conn[0..n_entries): ids (n_entries) lists the entries node by
+ * node, in ascending entry order within a node, and offsets (n_nodes + 1)
+ * delimits each node's run.  Entries whose offset is not 6*node with
+ * node < n_nodes are left out of every run.  The workspace size comes from
+ * pf3d_incidence_bytes.  (Built once per mesh; no reference counterpart:
+ * the reference scatters element by element, e.g. quad4.pyx:1338-1361.) */
+int pf3d_incidence_bytes(int64_t n_entries, int64_t n_nodes, int64_t *bytes);
+int pf3d_incidence_plan(int64_t n_entries, const int64_t *conn, int64_t n_nodes, void *work,
+                        int64_t work_bytes, int64_t *offsets, int64_t *ids, void *stream);
+
+/* fint[6p + d] += sum over node p's entries of fe[id*6 + d], in ascending
+ * element order: the order in which the reference's per-element loop of
+ * update_fint accumulates (quad4.pyx:1338-1361, beamc.pyx:1376-1391). */
+int pf3d_gather_fint(int64_t n_nodes, const int64_t *offsets, const int64_t *ids,
+                     const double *fe, double *fint, void *stream);
+
+/* ------------------------------------------------------------------ */
 /* COO -> 6x6 BSR (replaces coo_matrix((v, (r, c))).tocsc() on the     */
 /* reference's caller side, e.g. tests/test_quad4_static_point_load.py:80) */
 /* ------------------------------------------------------------------ */

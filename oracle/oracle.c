@@ -25,7 +25,7 @@
 #define DOF 6
 
 enum { ET_QUAD4 = 0, ET_QUAD4R = 1, ET_TRIA3R = 2, ET_BEAMC = 3 };
-enum { MAT_KC0 = 0, MAT_KG = 1, MAT_KGS = 2, MAT_M = 3 };
+enum { MAT_KC0 = 0, MAT_KG = 1, MAT_KGS = 2, MAT_M = 3, MAT_FINT = 4 };
 
 /* shell property record (doubles): A11 A12 A16 A22 A26 A66, B.., D.., E44 E45
  * E55, scf_k13, scf_k23, h, offset, intrho, intrhoz, intrhoz2 */
@@ -436,7 +436,7 @@ static void quad4r_kc0_local(const double *xe, double area, const shell_t *s, do
 
 /* Tria3R: tria3r.pyx:1105-1122, 2116-2324 */
 static void tria3r_kc0_local(const double *xe, double area, const shell_t *s0, double k6rot,
-                             double alpha, double *K)
+                             double alpha, int scatter_mask, double *K)
 {
     const int n = 18;
     shell_t s = *s0;
@@ -474,7 +474,9 @@ static void tria3r_kc0_local(const double *xe, double area, const shell_t *s0, d
     }
     /* Only the rz_I-rz_I diagonal of the drilling rz couplings reaches the
      * global matrix: the u/v-rz and rz_I-rz_J (I != J) terms computed at
-     * tria3r.pyx:2230-2324 are never used by the scatter at :2325-2974. */
+     * tria3r.pyx:2230-2324 are never used by the scatter at :2325-2974.
+     * The internal forces (tria3r.pyx:549-948) do use them. */
+    if (!scatter_mask) return;
     for (int i = 0; i < n; i++)
         for (int j = 0; j < n; j++)
             if ((i % 6 == 5 || j % 6 == 5) && i != j) K[i * n + j] = 0.;
@@ -879,7 +881,7 @@ static int element_local(int etype, int mat, int mtype, const double *g, const d
     int nn = nnodes(etype), n = 6 * nn;
     double area = g[13];
     if (etype == ET_BEAMC) {
-        if (mat == MAT_KC0) beamc_kc0_local(area, prop, K);
+        if (mat == MAT_KC0 || mat == MAT_FINT) beamc_kc0_local(area, prop, K);
         else if (mat == MAT_KG) /* N = A E (u2 - u1) / L, beamc.pyx:1889 */
             beamc_kg_local(area, prop, prop[0] * prop[1] * (-ue[0] + ue[6]) / area, K);
         else if (mat == MAT_M) beamc_m_local(area, prop, mtype, K);
@@ -888,10 +890,10 @@ static int element_local(int etype, int mat, int mtype, const double *g, const d
     }
     shell_t s;
     load_shell(prop, g + 9, &s);
-    if (mat == MAT_KC0) {
+    if (mat == MAT_KC0 || mat == MAT_FINT) {
         if (etype == ET_QUAD4) quad4_kc0_local(xe, area, &s, K);
         else if (etype == ET_QUAD4R) quad4r_kc0_local(xe, area, &s, k6rot, hg, K);
-        else tria3r_kc0_local(xe, area, &s, k6rot, alpha, K);
+        else tria3r_kc0_local(xe, area, &s, k6rot, alpha, mat == MAT_KC0, K);
         return 0;
     }
     memset(K, 0, sizeof(double) * n * n);
@@ -958,6 +960,44 @@ int orc_assemble(int etype, int mat, int mtype, int64_t E, const double *x,
             }
             v[k0 + k] += Kg[rl[k] * n + cl[k]];
         }
+    }
+    return 0;
+}
+
+/*
+ * Internal forces: fint[c_I + d] += (R6 K_e ue)_I for every element, in
+ * element order (update_probe_ue -> update_probe_xe -> update_fint:
+ * quad4.pyx:1174-1202, 1316-1363; quad4r.pyx:549, 4620; tria3r.pyx:549,
+ * 2975; beamc.pyx:352, 1354).
+ */
+int orc_fint(int etype, int64_t E, const double *x, const int64_t *conn, const double *evec,
+             const int64_t *pid, const double *props, const double *k6rot, const double *alpha,
+             const double *hg, const double *u, double *fint)
+{
+    int nn = nnodes(etype), n = 6 * nn;
+    int stride = etype == ET_BEAMC ? BP_STRIDE : SP_STRIDE;
+    static const double hg_default[5] = {1., 1., 1., 1., 1.};
+    for (int64_t e = 0; e < E; e++) {
+        const int64_t *cn = conn + e * nn;
+        double g[14], xe[12], ue[24], K[576], fe[24];
+        element_geometry(etype, x, cn, evec + 3 * e, g, xe);
+        local_displ(nn, u, cn, g, ue);
+        int st = element_local(etype, MAT_FINT, 0, g, xe, props + pid[e] * stride,
+                               k6rot ? k6rot[e] : 100., alpha ? alpha[e] : 0.7,
+                               hg ? hg + 5 * e : hg_default, ue, NULL, K);
+        if (st) return st;
+        for (int i = 0; i < n; i++) {
+            double s = 0;
+            for (int j = 0; j < n; j++) s += K[i * n + j] * ue[j];
+            fe[i] = s;
+        }
+        for (int I = 0; I < nn; I++)
+            for (int blk = 0; blk < 2; blk++)
+                for (int a = 0; a < 3; a++) {
+                    double s = 0;
+                    for (int i = 0; i < 3; i++) s += g[a * 3 + i] * fe[6 * I + 3 * blk + i];
+                    fint[6 * cn[I] + 3 * blk + a] += s;
+                }
     }
     return 0;
 }

@@ -84,3 +84,19 @@ def assemble(etype, mat, x, conn, init_k, evec, pid, props, r, c, v, mtype=0,
         _p(f(u)), _p(f(stress)), _p(r), _p(c), _p(v), int(bool(values_only)))
     if st:
         raise NotImplementedError('oracle has no restatement for %s %s' % (etype, mat))
+
+
+def fint(etype, x, conn, evec, pid, props, u, k6rot=None, alpha=None, hg=None, out=None):
+    """Accumulated internal forces of a batch (update_fint per element)."""
+    L = lib()
+    L.orc_fint.argtypes = [ctypes.c_int, ctypes.c_int64] + [ctypes.c_void_p] * 10
+    conn = np.ascontiguousarray(conn, dtype=np.int64)
+    f = lambda a, dt=np.float64: None if a is None else np.ascontiguousarray(a, dtype=dt)
+    if out is None:
+        out = np.zeros(np.asarray(u).size)
+    st = L.orc_fint(ETYPES[etype], conn.shape[0], _p(f(x)), _p(conn), _p(f(evec)),
+                    _p(f(pid, np.int64)), _p(props_table(props, etype)), _p(f(k6rot)),
+                    _p(f(alpha)), _p(f(hg)), _p(f(u)), _p(out))
+    if st:
+        raise NotImplementedError(etype)
+    return out

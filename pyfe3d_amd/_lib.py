@@ -26,6 +26,11 @@ SIGNATURES = {
     'pf3d_beam_kc0': [I64, P, P, P, P, P, P, P, P, P, I32, P],
     'pf3d_beam_kg': [I64, P, P, P, P, P, P, P, P, P, P, I32, P],
     'pf3d_beam_m': [I64, P, P, P, P, P, P, I32, P, P, P, I32, P],
+    'pf3d_shell_fe': [I32, I64, P, P, P, P, P, P, P, P, P, P, P],
+    'pf3d_beam_fe': [I64, P, P, P, P, P, P, P, P],
+    'pf3d_incidence_bytes': [I64, I64, P],
+    'pf3d_incidence_plan': [I64, P, I64, P, I64, P, P, P],
+    'pf3d_gather_fint': [I64, P, P, P, P, P],
     'pf3d_bsr_plan_bytes': [I64, I64, P],
     'pf3d_bsr_plan_sort': [I64, P, P, I64, P, I64, P, P],
     'pf3d_bsr_plan_emit': [I64, I64, P, I64, I64, I64, P, P, P, P, P, P],

@@ -14,6 +14,8 @@ slot offsets and property indices are range-checked on the device before the
 launch (a few small reductions and one synchronisation); callers that
 re-assemble the same validated arrays in a loop pass ``check=False``.
 """
+import ctypes
+
 import torch
 
 from . import _lib
@@ -234,3 +236,101 @@ def beam_m(x, conn, vxy, r, c, v, props, mtype=0, pid=None, init_k=None, values_
     pp, ppid = _props(props, pid, E, BEAM_STRIDE, check)
     _lib.call('pf3d_beam_m', E, px, pc, pk, pe, ppid, pp, int(mtype), pr, pcc, pv,
               _flags(values_only, overwrite), _stream(stream))
+
+
+# ----------------------------------------------------------------------
+# internal forces (update_fint)
+# ----------------------------------------------------------------------
+
+def shell_fe(etype, x, conn, u, props, pid=None, xmat=None, k6rot=None, alpha=None, hg=None,
+             fe=None, stream=None, check=True):
+    """Element internal forces in global axes, shape (E, nn, 6).
+
+    The per-element part of ``update_fint`` (update_probe_ue +
+    update_probe_finte + rotation; quad4.pyx:627-680, 1174-1202, 1316-1363;
+    quad4r.pyx:549, 4620; tria3r.pyx:549, 2975)."""
+    E, px, pc, _, pe = _common(etype, x, conn, None, xmat, check)
+    if check:
+        _check_nodes(conn, x, u)
+    pp, ppid = _props(props, pid, E, SHELL_STRIDE, check)
+    nn = NNODES[etype]
+    if fe is None:
+        fe = torch.empty((E, nn, 6), dtype=torch.float64, device=x.device)
+    _lib.call('pf3d_shell_fe', ETYPE[etype], E, px, pc, pe, ppid, pp,
+              _ptr(k6rot, torch.float64, 'k6rot', E, optional=True),
+              _ptr(alpha, torch.float64, 'alpha', E, optional=True),
+              _ptr(hg, torch.float64, 'hg', 5 * E, optional=True),
+              _ptr(u, torch.float64, 'u'), _ptr(fe, torch.float64, 'fe', 6 * nn * E),
+              _stream(stream))
+    return fe
+
+
+def beam_fe(x, conn, vxy, u, props, pid=None, fe=None, stream=None, check=True):
+    """BeamC element internal forces in global axes, shape (E, 2, 6)
+    (beamc.pyx:234-286, 352-488, 1354-1391)."""
+    E, px, pc, _, pe = _common('beamc', x, conn, None, vxy, check)
+    if pe is None:
+        raise ValueError('vxy is required')
+    if check:
+        _check_nodes(conn, x, u)
+    pp, ppid = _props(props, pid, E, BEAM_STRIDE, check)
+    if fe is None:
+        fe = torch.empty((E, 2, 6), dtype=torch.float64, device=x.device)
+    _lib.call('pf3d_beam_fe', E, px, pc, pe, ppid, pp, _ptr(u, torch.float64, 'u'),
+              _ptr(fe, torch.float64, 'fe', 12 * E), _stream(stream))
+    return fe
+
+
+class NodeGather:
+    """Deterministic element-to-node sum of element forces.
+
+    Built once per mesh from ``conn`` (E, nn): a radix sort of the entries
+    by node keeps, for every node, its (element, node) entries in ascending
+    element order, so ``fint[6p + d] += sum fe[e, i, d]`` is summed in the
+    order of the reference's element loop (quad4.pyx:1338-1361) and gives
+    the same bits on every run."""
+
+    def __init__(self, conn, n_nodes, stream=None):
+        if conn.dim() != 2:
+            raise ValueError('conn must have shape (n_elem, nn)')
+        self.n_nodes = int(n_nodes)
+        self.n_entries = conn.numel()
+        self.nn = conn.shape[1]
+        dev = conn.device
+        self.offsets = torch.zeros(self.n_nodes + 1, dtype=torch.int64, device=dev)
+        self.ids = torch.empty(max(1, self.n_entries), dtype=torch.int64, device=dev)
+        if self.n_entries == 0:
+            return
+        nbytes = ctypes.c_int64()
+        _lib.call('pf3d_incidence_bytes', self.n_entries, self.n_nodes, ctypes.byref(nbytes))
+        work = torch.empty(nbytes.value, dtype=torch.uint8, device=dev)
+        _lib.call('pf3d_incidence_plan', self.n_entries, _ptr(conn, torch.int64, 'conn'),
+                  self.n_nodes, work.data_ptr(), nbytes.value, self.offsets.data_ptr(),
+                  self.ids.data_ptr(), _stream(stream))
+        # the workspace is released when this call returns
+        (stream if stream is not None else torch.cuda.current_stream()).synchronize()
+
+    def __call__(self, fe, fint, stream=None):
+        """fint (6*n_nodes) += the node sums of fe (E, nn, 6)."""
+        if fe.numel() != 6 * self.n_entries:
+            raise ValueError('fe has %d entries, the plan %d' % (fe.numel(), 6 * self.n_entries))
+        if self.n_entries == 0:
+            return fint
+        _lib.call('pf3d_gather_fint', self.n_nodes, self.offsets.data_ptr(), self.ids.data_ptr(),
+                  _ptr(fe, torch.float64, 'fe'),
+                  _ptr(fint, torch.float64, 'fint', 6 * self.n_nodes), _stream(stream))
+        return fint
+
+
+def update_fint(etype, x, conn, u, fint, props, gather=None, vxy=None, stream=None, **kw):
+    """fint += internal forces of a batch of elements, the loop
+    ``for e in elems: e.update_probe_ue(u); e.update_fint(fint, prop)``.
+
+    ``gather`` is a :class:`NodeGather` for ``conn`` (built here if None)."""
+    if etype == 'beamc':
+        fe = beam_fe(x, conn, vxy, u, props, stream=stream, **kw)
+    else:
+        fe = shell_fe(etype, x, conn, u, props, stream=stream, **kw)
+    if gather is None:
+        gather = NodeGather(conn, fint.numel() // 6, stream=stream)
+    return gather(fe, fint, stream=stream)

@@ -288,8 +288,12 @@ __device__ __forceinline__ void tria_grad(const Geom &g, double Nx[3], double Ny
 // Tria3R KC0: tria3r.pyx:1105-1122 (shear factor), 2116-2324 (terms).
 // Only the rz_I-rz_I drilling diagonal reaches the global matrix: the other
 // rz couplings computed at :2230-2324 are not used by the scatter at :2325-2974.
+// full_drill keeps every drilling coupling: the internal forces use them
+// (tria3r.pyx:549-948) although the KC0 scatter drops all rz couplings but
+// the rz_I-rz_I diagonal.
 __device__ __forceinline__ void tria3r_kc0(double K[6][6], const Geom &g, const Shell &s, int I,
-                                           int J, double k6rot, double alpha)
+                                           int J, double k6rot, double alpha,
+                                           bool full_drill = false)
 {
     const double *p = g.xe;
     const double l12 = sqrt((p[0] - p[2]) * (p[0] - p[2]) + (p[1] - p[3]) * (p[1] - p[3]));
@@ -311,7 +315,7 @@ __device__ __forceinline__ void tria3r_kc0(double K[6][6], const Geom &g, const 
 #pragma unroll
     for (int t = 0; t < 3; t++)
         add_drill(K, pick(P[t], I), nxi, nyi, pick(P[t], J), nxj, nyj, detJ * (0.5 / 3.) * kd,
-                  false, I == J);
+                  full_drill, I == J);
 }
 
 // membrane stress resultants from local displacements (quad4.pyx:1965-1967)
@@ -655,6 +659,116 @@ shell_kc0_sym(AsmArgs a)
     }
 }
 
+// ------------------------------------------------------------------------
+// Element internal forces in global axes, fe[e][6nn] = R6 K_e ue with
+// ue = blockdiag(R^T) u (update_probe_ue -> update_probe_finte -> rotation
+// in update_fint: quad4.pyx:627-680, 1174-1202, 1316-1363; quad4r.pyx:549,
+// 4620; tria3r.pyx:549, 2975).  Same lane mapping as shell_kc0_sym: the lane
+// owning block (I, J), I <= J, adds K_IJ ue_J to node I and K_IJ^T ue_I to
+// node J; the node sums are formed in a fixed lane order.
+// ------------------------------------------------------------------------
+template <int ET>
+__global__ void __launch_bounds__(64) shell_fe(AsmArgs a, double *__restrict__ fe)
+{
+    constexpr int NN = (ET == TRIA3R) ? 3 : 4;
+    constexpr int NUP = NN * (NN + 1) / 2;
+    constexpr int EPW = 64 / NUP;
+    __shared__ double part[EPW][NUP][2][6];
+    __shared__ double sr[EPW][9];
+    const int lane = threadIdx.x;
+    const int slot = lane / NUP, sub = lane - slot * NUP;
+    int I, J;
+    upper_pair<NN>(sub, I, J);
+    const int64_t e = (int64_t)blockIdx.x * EPW + slot;
+    const bool active = slot < EPW && e < a.n_elem;
+    if (active) {
+        V3 X[NN];
+        int64_t cn[NN];
+#pragma unroll
+        for (int i = 0; i < NN; i++) {
+            cn[i] = a.conn[e * NN + i];
+            const double *px = a.x + cn[i] / 2;
+            X[i] = {px[0], px[1], px[2]};
+        }
+        V3 xm = {0., 0., 0.};
+        if (a.evec) xm = {a.evec[3 * e], a.evec[3 * e + 1], a.evec[3 * e + 2]};
+        Geom g;
+        if (ET == TRIA3R) tria_geometry(X, xm, g);
+        else quad_geometry(X, xm, g);
+        Shell s;
+        load_shell(a.props + (a.pid ? (int64_t)a.pid[e] : 0) * SHELL_STRIDE, g.m, s);
+        double K[6][6];
+#pragma unroll
+        for (int i = 0; i < 6; i++)
+#pragma unroll
+            for (int j = 0; j < 6; j++) K[i][j] = 0.;
+        const double k6 = a.k6rot ? a.k6rot[e] : 100.;
+        if (ET == QUAD4) quad4_kc0(K, g, s, I, J);
+        else if (ET == QUAD4R) {
+            double hgf[5] = {1., 1., 1., 1., 1.};
+            if (a.hg)
+#pragma unroll
+                for (int d = 0; d < 5; d++) hgf[d] = a.hg[5 * e + d];
+            quad4r_kc0(K, g, s, I, J, k6, hgf);
+        } else {
+            tria3r_kc0(K, g, s, I, J, k6, a.alpha ? a.alpha[e] : 0.7, true);
+        }
+        // local displacements of nodes I and J
+        int64_t ci = cn[0], cj = cn[0];
+#pragma unroll
+        for (int i = 1; i < NN; i++) {
+            ci = (I == i) ? cn[i] : ci;
+            cj = (J == i) ? cn[i] : cj;
+        }
+        double ui[6], uj[6];
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int t = 0; t < 3; t++) {
+                double si = 0., sj = 0.;
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    si += g.r[3 * q + t] * a.u[ci + 3 * b + q];
+                    sj += g.r[3 * q + t] * a.u[cj + 3 * b + q];
+                }
+                ui[3 * b + t] = si;
+                uj[3 * b + t] = sj;
+            }
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            double fi = 0., fj = 0.;
+#pragma unroll
+            for (int j = 0; j < 6; j++) {
+                fi += K[i][j] * uj[j];
+                fj += K[j][i] * ui[j];
+            }
+            part[slot][sub][0][i] = fi;
+            part[slot][sub][1][i] = (I != J) ? fj : 0.;
+        }
+        if (sub == 0)
+#pragma unroll
+            for (int i = 0; i < 9; i++) sr[slot][i] = g.r[i];
+    }
+    __syncthreads();
+    if (active && sub < NN) {
+        const int P = sub;
+        double f[6] = {0., 0., 0., 0., 0., 0.};
+        for (int t = 0; t < NUP; t++) {
+            int It, Jt;
+            upper_pair<NN>(t, It, Jt);
+            if (It == P)
+                for (int i = 0; i < 6; i++) f[i] += part[slot][t][0][i];
+            else if (Jt == P)
+                for (int i = 0; i < 6; i++) f[i] += part[slot][t][1][i];
+        }
+        double *o = fe + (e * NN + P) * 6;
+        for (int b = 0; b < 2; b++)
+            for (int q = 0; q < 3; q++)
+                o[3 * b + q] = sr[slot][3 * q] * f[3 * b] + sr[slot][3 * q + 1] * f[3 * b + 1] +
+                               sr[slot][3 * q + 2] * f[3 * b + 2];
+    }
+}
+
 // Geometry of each element: r (9), m (4), area/length, local x/y (2*nn).
 template <int ET>
 __global__ void element_geometry(int64_t n, const double *x, const int64_t *conn,
@@ -951,6 +1065,52 @@ __global__ void __launch_bounds__(64) beam_assemble(AsmArgs a)
     }
 }
 
+// BeamC element internal forces (beamc.pyx:352-488, 1354-1391)
+__global__ void __launch_bounds__(64) beam_fe(AsmArgs a, double *__restrict__ fe)
+{
+    __shared__ double tile[16][144];
+    __shared__ double sr[16][9];
+    __shared__ double sue[16][12];
+    const int lane = threadIdx.x, slot = lane >> 2, I = lane & 3;
+    const int64_t e = (int64_t)blockIdx.x * 16 + slot;
+    if (e < a.n_elem && I == 0) {
+        V3 X[2];
+        int64_t cn[2];
+        for (int i = 0; i < 2; i++) {
+            cn[i] = a.conn[2 * e + i];
+            const double *px = a.x + cn[i] / 2;
+            X[i] = {px[0], px[1], px[2]};
+        }
+        V3 vxy = {a.evec[3 * e], a.evec[3 * e + 1], a.evec[3 * e + 2]};
+        Geom g;
+        beam_geometry(X, vxy, g);
+        for (int i = 0; i < 9; i++) sr[slot][i] = g.r[i];
+        for (int n = 0; n < 2; n++)
+            for (int b = 0; b < 2; b++)
+                for (int t = 0; t < 3; t++) {
+                    double s = 0.;
+                    for (int q = 0; q < 3; q++) s += g.r[3 * q + t] * a.u[cn[n] + 3 * b + q];
+                    sue[slot][6 * n + 3 * b + t] = s;
+                }
+        const double *bp = a.props + (a.pid ? (int64_t)a.pid[e] : 0) * BEAM_STRIDE;
+        beam_kc0_local(g.area, beam_section(bp, g.area), tile[slot]);
+    }
+    __syncthreads();
+    if (e < a.n_elem && I < 2) {
+        double f[6];
+        for (int i = 0; i < 6; i++) {
+            double s = 0.;
+            for (int j = 0; j < 12; j++) s += tile[slot][(6 * I + i) * 12 + j] * sue[slot][j];
+            f[i] = s;
+        }
+        double *o = fe + (e * 2 + I) * 6;
+        for (int b = 0; b < 2; b++)
+            for (int q = 0; q < 3; q++)
+                o[3 * b + q] = sr[slot][3 * q] * f[3 * b] + sr[slot][3 * q + 1] * f[3 * b + 1] +
+                               sr[slot][3 * q + 2] * f[3 * b + 2];
+    }
+}
+
 }  // namespace pf3d
 
 // ------------------------------------------------------------------------
@@ -1044,5 +1204,31 @@ hipError_t pf3d_launch_geometry(int etype, int64_t n, const double *x, const int
         hipLaunchKernelGGL(element_geometry<BEAMC>, dim3(blocks), dim3(128), 0, s, n, x, conn,
                            evec, geom, xe);
     }
+    return hipGetLastError();
+}
+
+hipError_t pf3d_launch_shell_fe(int etype, const AsmArgs &a, double *fe, hipStream_t s)
+{
+    const int nup = etype == TRIA3R ? 6 : 10;
+    const int64_t blocks = (a.n_elem + 64 / nup - 1) / (64 / nup);
+    if (blocks == 0) return hipSuccess;
+    switch (etype) {
+    case QUAD4:
+        hipLaunchKernelGGL(shell_fe<QUAD4>, dim3((unsigned)blocks), dim3(64), 0, s, a, fe);
+        break;
+    case QUAD4R:
+        hipLaunchKernelGGL(shell_fe<QUAD4R>, dim3((unsigned)blocks), dim3(64), 0, s, a, fe);
+        break;
+    default:
+        hipLaunchKernelGGL(shell_fe<TRIA3R>, dim3((unsigned)blocks), dim3(64), 0, s, a, fe);
+    }
+    return hipGetLastError();
+}
+
+hipError_t pf3d_launch_beam_fe(const AsmArgs &a, double *fe, hipStream_t s)
+{
+    const int64_t blocks = (a.n_elem + 15) / 16;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(beam_fe, dim3((unsigned)blocks), dim3(64), 0, s, a, fe);
     return hipGetLastError();
 }

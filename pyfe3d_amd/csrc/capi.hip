@@ -9,6 +9,8 @@
 // launchers defined next to the kernels
 hipError_t pf3d_launch_shell(int etype, int mat, const pf3d::AsmArgs &a, hipStream_t s);
 hipError_t pf3d_launch_beam(int mat, const pf3d::AsmArgs &a, hipStream_t s);
+hipError_t pf3d_launch_shell_fe(int etype, const pf3d::AsmArgs &a, double *fe, hipStream_t s);
+hipError_t pf3d_launch_beam_fe(const pf3d::AsmArgs &a, double *fe, hipStream_t s);
 hipError_t pf3d_launch_geometry(int etype, int64_t n, const double *x, const int64_t *conn,
                                 const double *evec, double *geom, double *xe, hipStream_t s);
 
@@ -204,6 +206,37 @@ int pf3d_beam_m(int64_t n_elem, const double *x, const int64_t *conn, const int6
     a.n_elem = n_elem; a.x = x; a.conn = conn; a.init_k = init_k; a.evec = vxy; a.pid = pid;
     a.props = props; a.mtype = mtype; a.flags = flags; a.r = r; a.c = c; a.v = v;
     return pf3d_hip_check(pf3d_launch_beam(PF3D_M, a, (hipStream_t)stream), "pf3d_beam_m");
+}
+
+int pf3d_shell_fe(int etype, int64_t n_elem, const double *x, const int64_t *conn,
+                  const double *xmat, const int32_t *pid, const double *props,
+                  const double *k6rot, const double *alpha, const double *hg, const double *u,
+                  double *fe, void *stream)
+{
+    if (etype < 0 || etype > PF3D_TRIA3R) return pf3d_fail(PF3D_E_ARG, "bad shell etype %d", etype);
+    if (n_elem < 0) return pf3d_fail(PF3D_E_ARG, "negative n_elem");
+    if (n_elem > (int64_t)2147483647 * 6) return pf3d_fail(PF3D_E_ARG, "n_elem too large");
+    if (n_elem == 0) return PF3D_OK;
+    if (!x || !conn || !props || !u || !fe) return pf3d_fail(PF3D_E_ARG, "null x/conn/props/u/fe");
+    pf3d::AsmArgs a{};
+    a.n_elem = n_elem; a.x = x; a.conn = conn; a.evec = xmat; a.pid = pid; a.props = props;
+    a.k6rot = k6rot; a.alpha = alpha; a.hg = hg; a.u = u;
+    return pf3d_hip_check(pf3d_launch_shell_fe(etype, a, fe, (hipStream_t)stream),
+                          "pf3d_shell_fe");
+}
+
+int pf3d_beam_fe(int64_t n_elem, const double *x, const int64_t *conn, const double *vxy,
+                 const int32_t *pid, const double *props, const double *u, double *fe,
+                 void *stream)
+{
+    if (n_elem < 0) return pf3d_fail(PF3D_E_ARG, "negative n_elem");
+    if (n_elem > (int64_t)2147483647 * 16) return pf3d_fail(PF3D_E_ARG, "n_elem too large");
+    if (n_elem == 0) return PF3D_OK;
+    if (!x || !conn || !vxy || !props || !u || !fe) return pf3d_fail(PF3D_E_ARG, "null argument");
+    pf3d::AsmArgs a{};
+    a.n_elem = n_elem; a.x = x; a.conn = conn; a.evec = vxy; a.pid = pid; a.props = props;
+    a.u = u;
+    return pf3d_hip_check(pf3d_launch_beam_fe(a, fe, (hipStream_t)stream), "pf3d_beam_fe");
 }
 
 }  // extern "C"

@@ -139,6 +139,48 @@ __global__ void fill_zero(int64_t n, double *__restrict__ p)
     if (t < n) p[t] = 0.;
 }
 
+// ---- node incidence plan for the internal-force gather -----------------
+// keys: node position of each (element, node) entry; entries whose DOF
+// offset is not 6*node with node < n_nodes go to a bucket past the last node
+__global__ void incidence_keys(int64_t n, const int64_t *__restrict__ conn, int64_t n_nodes,
+                               int64_t *__restrict__ keys, int64_t *__restrict__ ids)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int64_t c = conn[t];
+    keys[t] = (c < 0 || c % 6 != 0 || c / 6 >= n_nodes) ? n_nodes : c / 6;
+    ids[t] = t;
+}
+
+// offsets[p] = first sorted entry whose node >= p (binary search)
+__global__ void incidence_offsets(int64_t n, const int64_t *__restrict__ keys, int64_t n_nodes,
+                                  int64_t *__restrict__ offsets)
+{
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > n_nodes) return;
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        if (keys[mid] < p) lo = mid + 1;
+        else hi = mid;
+    }
+    offsets[p] = lo;
+}
+
+// fint[6p + d] += sum of fe[id*6 + d] over the entries of node p, in
+// ascending entry (= element) order
+__global__ void gather_nodes(int64_t n_nodes, const int64_t *__restrict__ offsets,
+                             const int64_t *__restrict__ ids, const double *__restrict__ fe,
+                             double *__restrict__ fint)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 6 * n_nodes) return;
+    const int64_t p = t / 6, d = t - 6 * p;
+    double s = fint[t];
+    for (int64_t k = offsets[p]; k < offsets[p + 1]; k++) s += fe[ids[k] * 6 + d];
+    fint[t] = s;
+}
+
 inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 }  // namespace
@@ -242,6 +284,90 @@ int pf3d_bsr_apply(int64_t n_unique, const int64_t *perm, const int64_t *seg, co
     hipLaunchKernelGGL(seg_sum, dim3(nblk(n_unique, 256)), dim3(256), 0, s, n_unique, perm, seg,
                        dest, v, bsr_val);
     return pf3d_hip_check(hipGetLastError(), "pf3d_bsr_apply");
+}
+
+}  // extern "C"
+
+// ---- internal-force gather ---------------------------------------------
+namespace {
+struct IncWork {
+    size_t keys_in, keys_out, ids_in, temp, total, temp_bytes;
+};
+
+// radix bits for keys in [0, n_nodes]
+int inc_bits(int64_t n_nodes)
+{
+    int b = 1;
+    while (b < 63 && (int64_t(1) << b) <= n_nodes) b++;
+    return b;
+}
+
+hipError_t inc_layout(int64_t n, int64_t n_nodes, IncWork &w)
+{
+    size_t a8 = align256(sizeof(int64_t) * (size_t)n);
+    size_t sort_bytes = 0;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(
+        nullptr, sort_bytes, (const int64_t *)nullptr, (int64_t *)nullptr,
+        (const int64_t *)nullptr, (int64_t *)nullptr, n, 0, inc_bits(n_nodes));
+    if (e != hipSuccess) return e;
+    w.temp_bytes = align256(sort_bytes);
+    size_t o = 0;
+    w.keys_in = o; o += a8;
+    w.keys_out = o; o += a8;
+    w.ids_in = o; o += a8;
+    w.temp = o; o += w.temp_bytes;
+    w.total = o;
+    return hipSuccess;
+}
+}  // namespace
+
+extern "C" {
+
+int pf3d_incidence_bytes(int64_t n_entries, int64_t n_nodes, int64_t *bytes)
+{
+    if (n_entries <= 0 || n_nodes <= 0 || !bytes)
+        return pf3d_fail(PF3D_E_ARG, "pf3d_incidence_bytes: bad arguments");
+    IncWork w;
+    hipError_t e = inc_layout(n_entries, n_nodes, w);
+    if (e != hipSuccess) return pf3d_hip_check(e, "pf3d_incidence_bytes");
+    *bytes = (int64_t)w.total;
+    return PF3D_OK;
+}
+
+int pf3d_incidence_plan(int64_t n_entries, const int64_t *conn, int64_t n_nodes, void *work,
+                        int64_t work_bytes, int64_t *offsets, int64_t *ids, void *stream)
+{
+    if (n_entries <= 0 || n_nodes <= 0 || !conn || !work || !offsets || !ids)
+        return pf3d_fail(PF3D_E_ARG, "pf3d_incidence_plan: bad arguments");
+    IncWork w;
+    hipError_t e = inc_layout(n_entries, n_nodes, w);
+    if (e != hipSuccess) return pf3d_hip_check(e, "pf3d_incidence_plan");
+    if ((int64_t)w.total > work_bytes)
+        return pf3d_fail(PF3D_E_ARG, "workspace too small: %lld < %lld", (long long)work_bytes,
+                         (long long)w.total);
+    hipStream_t s = (hipStream_t)stream;
+    char *base = (char *)work;
+    int64_t *kin = (int64_t *)(base + w.keys_in), *kout = (int64_t *)(base + w.keys_out);
+    int64_t *iin = (int64_t *)(base + w.ids_in);
+    hipLaunchKernelGGL(incidence_keys, dim3(nblk(n_entries, 256)), dim3(256), 0, s, n_entries,
+                       conn, n_nodes, kin, iin);
+    size_t tb = w.temp_bytes;
+    e = hipcub::DeviceRadixSort::SortPairs(base + w.temp, tb, kin, kout, iin, ids, n_entries, 0,
+                                           inc_bits(n_nodes), s);
+    if (e != hipSuccess) return pf3d_hip_check(e, "radix sort");
+    hipLaunchKernelGGL(incidence_offsets, dim3(nblk(n_nodes + 1, 256)), dim3(256), 0, s,
+                       n_entries, kout, n_nodes, offsets);
+    return pf3d_hip_check(hipGetLastError(), "pf3d_incidence_plan");
+}
+
+int pf3d_gather_fint(int64_t n_nodes, const int64_t *offsets, const int64_t *ids,
+                     const double *fe, double *fint, void *stream)
+{
+    if (n_nodes <= 0 || !offsets || !ids || !fe || !fint)
+        return pf3d_fail(PF3D_E_ARG, "pf3d_gather_fint: bad arguments");
+    hipLaunchKernelGGL(gather_nodes, dim3(nblk(6 * n_nodes, 256)), dim3(256), 0,
+                       (hipStream_t)stream, n_nodes, offsets, ids, fe, fint);
+    return pf3d_hip_check(hipGetLastError(), "pf3d_gather_fint");
 }
 
 }  // extern "C"

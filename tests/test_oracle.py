@@ -196,3 +196,40 @@ def test_beam_euler_bernoulli_limit():
     assert np.isclose(K[5, 11], 2 * E * Izz / L, rtol=1e-12)
     assert np.isclose(K[4, 4], 4 * E * Iyy / L, rtol=1e-12)
     assert np.isclose(K[3, 3], 1e30 * J / L, rtol=1e-12)
+
+
+FINT_TOL = 1e-12
+
+
+def _fint_oracle(etype, d):
+    beam = etype == 'beamc'
+    E = d['conn'].shape[0]
+    return O.fint(etype, d['x'], d['conn'], d['vxy'] if beam else d['xmat'],
+                  np.arange(E) if beam else d['pid'], d['props'], d['u'],
+                  k6rot=None if beam else d['k6rot'], alpha=None if beam else d['alpha'],
+                  hg=None if beam else d['hg'])
+
+
+@pytest.mark.parametrize('etype', SHELLS + ['beamc'])
+def test_fint_golden(etype):
+    """update_fint summed over every element (quad4.pyx:1316, beamc.pyx:1354)."""
+    for name, d in cases(etype):
+        f = _fint_oracle(etype, d)
+        scale = np.abs(d['fint']).max()
+        tol = TOL_CASE.get((etype, name), FINT_TOL)
+        assert np.abs(f - d['fint']).max() <= tol * scale, (etype, name)
+
+
+@pytest.mark.parametrize('etype', ['quad4', 'quad4r', 'beamc'])
+def test_fint_is_kc0_times_u(etype):
+    """For these elements the internal forces are KC0 u (linear statics);
+    Tria3R is not: its KC0 scatter keeps only the rz-rz diagonal of the
+    drilling term while its internal forces keep every coupling."""
+    import scipy.sparse as ss
+    for name, d in cases(etype):
+        r, c, v, _ = _run(etype, d, 'KC0', 'KC0', 0)
+        n = d['u'].size
+        K = ss.coo_matrix((v, (r, c)), shape=(n, n)).tocsr()
+        ku = K @ d['u']
+        f = _fint_oracle(etype, d)
+        assert np.abs(f - ku).max() <= 1e-11 * np.abs(ku).max(), (etype, name)
