@@ -124,6 +124,15 @@ int pf3d_beam_m(int64_t n_elem, const double *x, const int64_t *conn, const int6
                 const double *vxy, const int32_t *pid, const double *props, int mtype,
                 int64_t *r, int64_t *c, double *v, int flags, void *stream);
 
+/* Shell property records (32 doubles each) of isotropic plates, one per
+ * element, from a thickness field h[n] (device) and the record of the same
+ * material at unit thickness (host, 32 doubles, h field = 1, no offset):
+ * A, E44..E55, intrho scale with h, D and intrhoz2 with h^3.  Replaces
+ * one isotropic_plate(h, E, nu, rho=...) per element on the caller side
+ * (shellprop_utils.py:182-206; shellprop.pyx:568-621). */
+int pf3d_isotropic_records(int64_t n, const double *h, const double *unit_record, double *out,
+                           void *stream);
+
 /* ------------------------------------------------------------------ */
 /* Internal forces fint = sum_e R6 K_e ue (update_fint)                */
 /* ------------------------------------------------------------------ */
@@ -189,6 +198,28 @@ int pf3d_bsr_apply(int64_t n_unique, const int64_t *perm, const int64_t *seg,
                    const int64_t *dest, const double *v, int64_t nnzb, double *bsr_val,
                    void *stream);
 
+/* Element-block plan: the same BSR from the element topology instead of
+ * the r/c arrays, for COO values written by the element entry points
+ * (block (I, J) of element e at init_k[e] + 36*nn*I + 6*J, row stride
+ * 6*nn: quad4.pyx:1287-1293).  One int64 per element block instead of one
+ * per slot; values are summed in ascending slot order, so the result is
+ * bit-identical to pf3d_bsr_apply on the same COO.  init_k NULL means
+ * 36*nn*nn*e. */
+int pf3d_block_plan_bytes(int64_t n_elem, int nn, int64_t n_nodes, int64_t *bytes);
+int pf3d_block_plan_sort(int64_t n_elem, int nn, const int64_t *conn, const int64_t *init_k,
+                         int64_t n_nodes, void *work, int64_t work_bytes, int64_t *nnzb,
+                         void *stream);
+/* src[n_elem*nn*nn] (first slot of each sorted element block),
+ * src_ptr[nnzb+1], row_ptr[n_nodes+1], col_idx[nnzb] */
+int pf3d_block_plan_emit(int64_t n_elem, int nn, const int64_t *init_k, int64_t n_nodes,
+                         void *work, int64_t work_bytes, int64_t nnzb, int64_t *src,
+                         int64_t *src_ptr, int32_t *row_ptr, int32_t *col_idx, void *stream);
+/* bsr_val (=, or += with accumulate) sums of the element blocks; a mesh of
+ * several element types applies one plan per type, the later ones with
+ * accumulate = 1 (their plans must share the BSR structure). */
+int pf3d_block_apply(int64_t nnzb, const int64_t *src_ptr, const int64_t *src, int nn,
+                     const double *v, double *bsr_val, int accumulate, void *stream);
+
 /* ------------------------------------------------------------------ */
 /* Static solve (replaces KC0[bu, :][:, bu] + scipy spsolve/cg,          */
 /* tests/test_quad4_static_point_load.py:84-102)                         */
@@ -231,6 +262,35 @@ int pf3d_ir_solve(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx,
                   const double *b, double *x, double rtol, double inner_rtol, int max_outer,
                   int max_inner, int x0, void *work, int64_t work_bytes, int64_t *stats,
                   double *relres, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Batched solves of a design sweep: one launch for many independent   */
+/* systems (the reference's tutorials re-assemble and spsolve per      */
+/* design, doc/source/tut-opt-lightweight-smeared-deflection-SLSQP.ipynb) */
+/* ------------------------------------------------------------------ */
+
+/* The systems are the diagonal blocks of one block-diagonal BSR matrix of
+ * nb block rows: system v owns rows [row0[v], row0[v+1]) (row0 on the
+ * device).  Each persistent workgroup takes the next system (in `order`,
+ * or 0..n_var-1 if NULL) and solves it alone:
+ *   val32 == NULL : FP64 block-Jacobi PCG (val64, dinv64), restarted from
+ *                   the true residual;
+ *   val32 != NULL : FP64 iterative refinement around PCG on the FP32
+ *                   matrix and inverses (val32, dinv32), inner tolerance
+ *                   inner_rtol * ||r||.
+ * Both stop at ||b - K64 x|| <= rtol ||b|| (per system), after max_iter
+ * PCG iterations or max_outer restarts/refinement steps.  x0 != 0 starts
+ * from the x given.  Per system: iters (PCG iterations), outer (steps),
+ * relres (final FP64 true relative residual).  n_groups workgroups
+ * (0 = 2048).  No host synchronisation: the call may be captured in a
+ * graph.  work: pf3d_batch_bytes(nb). */
+int pf3d_batch_bytes(int64_t nb, int64_t *bytes);
+int pf3d_batch_solve(int64_t n_var, int64_t nb, const int64_t *row0, const int32_t *order,
+                     const int32_t *row_ptr, const int32_t *col_idx, const double *val64,
+                     const double *dinv64, const float *val32, const float *dinv32,
+                     const double *b, double *x, int x0, double rtol, double inner_rtol,
+                     int max_iter, int max_outer, void *work, int64_t work_bytes,
+                     int32_t *iters, int32_t *outer, double *relres, int n_groups, void *stream);
 
 #ifdef __cplusplus
 }

@@ -26,6 +26,7 @@ SIGNATURES = {
     'pf3d_beam_kc0': [I64, P, P, P, P, P, P, P, P, P, I32, P],
     'pf3d_beam_kg': [I64, P, P, P, P, P, P, P, P, P, P, I32, P],
     'pf3d_beam_m': [I64, P, P, P, P, P, P, I32, P, P, P, I32, P],
+    'pf3d_isotropic_records': [I64, P, P, P, P],
     'pf3d_shell_fe': [I32, I64, P, P, P, P, P, P, P, P, P, P, P],
     'pf3d_beam_fe': [I64, P, P, P, P, P, P, P, P],
     'pf3d_incidence_bytes': [I64, I64, P],
@@ -35,6 +36,10 @@ SIGNATURES = {
     'pf3d_bsr_plan_sort': [I64, P, P, I64, P, I64, P, P],
     'pf3d_bsr_plan_emit': [I64, I64, P, I64, I64, I64, P, P, P, P, P, P],
     'pf3d_bsr_apply': [I64, P, P, P, P, I64, P, P],
+    'pf3d_block_plan_bytes': [I64, I32, I64, P],
+    'pf3d_block_plan_sort': [I64, I32, P, P, I64, P, I64, P, P],
+    'pf3d_block_plan_emit': [I64, I32, P, I64, P, I64, I64, P, P, P, P, P],
+    'pf3d_block_apply': [I64, P, P, I32, P, P, I32, P],
     'pf3d_bsr_spmv': [I64, P, P, P, P, P, P],
     'pf3d_bsr_spmv_f32': [I64, P, P, P, P, P, P],
     'pf3d_bsr_dirichlet': [I64, P, P, P, P, P],
@@ -44,6 +49,9 @@ SIGNATURES = {
     'pf3d_pcg': [I64, P, P, P, P, P, P, F64, I32, I32, P, I64, P, P, P],
     'pf3d_ir_bytes': [I64, P],
     'pf3d_ir_solve': [I64, P, P, P, P, P, P, P, F64, F64, I32, I32, I32, P, I64, P, P, P],
+    'pf3d_batch_bytes': [I64, P],
+    'pf3d_batch_solve': [I64, I64, P, P, P, P, P, P, P, P, P, P, I32, F64, F64, I32, I32, P, I64,
+                         P, P, P, I32, P],
 }
 
 _lib = None

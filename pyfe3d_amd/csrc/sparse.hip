@@ -371,3 +371,229 @@ int pf3d_gather_fint(int64_t n_nodes, const int64_t *offsets, const int64_t *ids
 }
 
 }  // extern "C"
+
+// ---- element-block plan --------------------------------------------------
+// For COO arrays filled by the element kernels the slots of element e come
+// in 6x6 node blocks: block (I, J) starts at init_k[e] + 36*nn*I + 6*J with
+// row stride 6*nn (quad4.pyx:1287-1293; tria3r.pyx and beamc.pyx use the
+// same row-major order).  A plan over these blocks needs one int64 per
+// element block instead of one per slot, and its apply reads each block
+// row as 6 contiguous values.  Block sums run over the sources in
+// ascending (e, I, J), i.e. ascending slot order, like pf3d_bsr_apply.
+namespace {
+
+struct BlkWork {
+    size_t keys_in, keys_out, ids_in, ids_out, head, temp, total, temp_bytes;
+};
+
+hipError_t blk_layout(int64_t n, int64_t n_nodes, BlkWork &w)
+{
+    const size_t a8 = align256(sizeof(int64_t) * (size_t)n);
+    size_t sort_bytes = 0, scan_bytes = 0;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(
+        nullptr, sort_bytes, (const int64_t *)nullptr, (int64_t *)nullptr,
+        (const int64_t *)nullptr, (int64_t *)nullptr, n, 0, 63);
+    if (e != hipSuccess) return e;
+    e = hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (int64_t *)nullptr,
+                                         (int64_t *)nullptr, n);
+    if (e != hipSuccess) return e;
+    w.temp_bytes = align256(sort_bytes > scan_bytes ? sort_bytes : scan_bytes);
+    size_t o = 0;
+    w.keys_in = o; o += a8;
+    w.keys_out = o; o += a8;
+    w.ids_in = o; o += a8;
+    w.ids_out = o; o += a8;
+    w.head = o; o += a8;
+    w.temp = o; o += w.temp_bytes;
+    w.total = o;
+    return hipSuccess;
+}
+
+// block key of entry id = e*nn*nn + I*nn + J (ids_in[t], or t)
+__global__ void blk_keys(int64_t n_elem, int nn, const int64_t *__restrict__ conn,
+                         int64_t n_nodes, const int64_t *__restrict__ ids_in,
+                         int64_t *__restrict__ keys, int64_t *__restrict__ ids)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nb = (int64_t)nn * nn;
+    if (t >= n_elem * nb) return;
+    const int64_t id = ids_in ? ids_in[t] : t;
+    const int64_t e = id / nb;
+    const int ij = (int)(id - e * nb), I = ij / nn, J = ij - (ij / nn) * nn;
+    const int64_t a = conn[e * nn + I] / 6, b = conn[e * nn + J] / 6;
+    keys[t] = a * n_nodes + b;
+    ids[t] = id;
+}
+
+// first slot of each element block (to order the sources by slot)
+__global__ void blk_base(int64_t n_elem, int nn, const int64_t *__restrict__ init_k,
+                         int64_t *__restrict__ keys, int64_t *__restrict__ ids)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nb = (int64_t)nn * nn;
+    if (t >= n_elem * nb) return;
+    const int64_t e = t / nb;
+    const int ij = (int)(t - e * nb), I = ij / nn, J = ij - (ij / nn) * nn;
+    keys[t] = init_k[e] + 36 * nn * I + 6 * J;
+    ids[t] = t;
+}
+
+__global__ void blk_heads(int64_t n, const int64_t *__restrict__ keys, int64_t *__restrict__ h)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    h[t] = (t == 0 || keys[t] != keys[t - 1]) ? 1 : 0;
+}
+
+// src[t] = first slot of the t-th sorted element block; heads give the
+// block ranges, the BSR column and the row pointers
+__global__ void blk_emit(int64_t n, int nn, int64_t n_nodes, const int64_t *__restrict__ keys,
+                         const int64_t *__restrict__ ids, const int64_t *__restrict__ bid,
+                         const int64_t *__restrict__ init_k, int64_t nnzb,
+                         int64_t *__restrict__ src, int64_t *__restrict__ src_ptr,
+                         int32_t *__restrict__ row_ptr, int32_t *__restrict__ col_idx)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int64_t nb = (int64_t)nn * nn;
+    const int64_t id = ids[t], e = id / nb;
+    const int ij = (int)(id - e * nb), I = ij / nn, J = ij - (ij / nn) * nn;
+    const int64_t k0 = init_k ? init_k[e] : 36 * nb * e;
+    src[t] = k0 + 36 * nn * I + 6 * J;
+    const int64_t k = keys[t];
+    const int64_t kp = t ? keys[t - 1] : -1;
+    if (t == 0 || k != kp) {
+        const int64_t b = bid[t];
+        src_ptr[b] = t;
+        const int64_t row = k / n_nodes;
+        col_idx[b] = (int32_t)(k - row * n_nodes);
+        const int64_t prow = t ? kp / n_nodes : -1;
+        for (int64_t rr = prow + 1; rr <= row; rr++) row_ptr[rr] = (int32_t)b;
+        if (b == nnzb - 1)
+            for (int64_t rr = row + 1; rr <= n_nodes; rr++) row_ptr[rr] = (int32_t)nnzb;
+    }
+    if (t == n - 1) src_ptr[nnzb] = n;
+}
+
+// one thread per (BSR block, row m): 6 contiguous values per source
+__global__ void blk_apply(int64_t nnzb, const int64_t *__restrict__ src_ptr,
+                          const int64_t *__restrict__ src, int stride,
+                          const double *__restrict__ v, double *__restrict__ out, int accumulate)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nnzb * 6) return;
+    const int64_t b = t / 6;
+    const int m = (int)(t - b * 6);
+    double s[6];
+    double *o = out + b * 36 + m * 6;
+#pragma unroll
+    for (int n = 0; n < 6; n++) s[n] = accumulate ? o[n] : 0.;
+    for (int64_t k = src_ptr[b]; k < src_ptr[b + 1]; k++) {
+        const double *p = v + src[k] + (int64_t)stride * m;
+#pragma unroll
+        for (int n = 0; n < 6; n++) s[n] += p[n];
+    }
+#pragma unroll
+    for (int n = 0; n < 6; n++) o[n] = s[n];
+}
+
+}  // namespace
+
+extern "C" {
+
+int pf3d_block_plan_bytes(int64_t n_elem, int nn, int64_t n_nodes, int64_t *bytes)
+{
+    if (n_elem <= 0 || nn < 2 || nn > 4 || n_nodes <= 0 || !bytes)
+        return pf3d_fail(PF3D_E_ARG, "pf3d_block_plan_bytes: bad arguments");
+    if (n_nodes > 2147483647 || n_nodes > 3037000499LL)
+        return pf3d_fail(PF3D_E_ARG, "too many nodes");
+    BlkWork w;
+    hipError_t e = blk_layout(n_elem * nn * nn, n_nodes, w);
+    if (e != hipSuccess) return pf3d_hip_check(e, "pf3d_block_plan_bytes");
+    *bytes = (int64_t)w.total;
+    return PF3D_OK;
+}
+
+int pf3d_block_plan_sort(int64_t n_elem, int nn, const int64_t *conn, const int64_t *init_k,
+                         int64_t n_nodes, void *work, int64_t work_bytes, int64_t *nnzb,
+                         void *stream)
+{
+    if (n_elem <= 0 || nn < 2 || nn > 4 || !conn || !work || !nnzb)
+        return pf3d_fail(PF3D_E_ARG, "pf3d_block_plan_sort: bad arguments");
+    const int64_t n = n_elem * nn * nn;
+    BlkWork w;
+    hipError_t e = blk_layout(n, n_nodes, w);
+    if (e != hipSuccess) return pf3d_hip_check(e, "pf3d_block_plan_sort");
+    if ((int64_t)w.total > work_bytes)
+        return pf3d_fail(PF3D_E_ARG, "workspace too small: %lld < %lld", (long long)work_bytes,
+                         (long long)w.total);
+    hipStream_t s = (hipStream_t)stream;
+    char *base = (char *)work;
+    int64_t *kin = (int64_t *)(base + w.keys_in), *kout = (int64_t *)(base + w.keys_out);
+    int64_t *iin = (int64_t *)(base + w.ids_in), *iout = (int64_t *)(base + w.ids_out);
+    int64_t *h = (int64_t *)(base + w.head);
+    size_t tb = w.temp_bytes;
+    const int64_t *first = nullptr;
+    if (init_k) {
+        // order the element blocks by slot first; the stable sort by block
+        // key below keeps that order within a block
+        hipLaunchKernelGGL(blk_base, dim3(nblk(n, 256)), dim3(256), 0, s, n_elem, nn, init_k,
+                           kin, iin);
+        e = hipcub::DeviceRadixSort::SortPairs(base + w.temp, tb, kin, kout, iin, iout, n, 0, 63,
+                                               s);
+        if (e != hipSuccess) return pf3d_hip_check(e, "radix sort");
+        first = iout;
+    }
+    hipLaunchKernelGGL(blk_keys, dim3(nblk(n, 256)), dim3(256), 0, s, n_elem, nn, conn, n_nodes,
+                       first, kin, iin);
+    tb = w.temp_bytes;
+    e = hipcub::DeviceRadixSort::SortPairs(base + w.temp, tb, kin, kout, iin, iout, n, 0,
+                                           inc_bits(n_nodes * n_nodes), s);
+    if (e != hipSuccess) return pf3d_hip_check(e, "radix sort");
+    hipLaunchKernelGGL(blk_heads, dim3(nblk(n, 256)), dim3(256), 0, s, n, kout, h);
+    int64_t last = 0, tot = 0;
+    e = hipMemcpyAsync(&last, h + n - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return pf3d_hip_check(e, "copy flag");
+    // block ids: exclusive scan of the heads, into the sort's free input keys
+    tb = w.temp_bytes;
+    e = hipcub::DeviceScan::ExclusiveSum(base + w.temp, tb, h, kin, n, s);
+    if (e != hipSuccess) return pf3d_hip_check(e, "scan");
+    e = hipMemcpyAsync(&tot, kin + n - 1, sizeof(int64_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return pf3d_hip_check(e, "plan totals");
+    *nnzb = tot + last;
+    if (*nnzb > 2147483647) return pf3d_fail(PF3D_E_ARG, "too many blocks for int32 BSR");
+    return PF3D_OK;
+}
+
+int pf3d_block_plan_emit(int64_t n_elem, int nn, const int64_t *init_k, int64_t n_nodes,
+                         void *work, int64_t work_bytes, int64_t nnzb, int64_t *src,
+                         int64_t *src_ptr, int32_t *row_ptr, int32_t *col_idx, void *stream)
+{
+    if (n_elem <= 0 || nn < 2 || nn > 4 || nnzb <= 0 || !work || !src || !src_ptr || !row_ptr ||
+        !col_idx)
+        return pf3d_fail(PF3D_E_ARG, "pf3d_block_plan_emit: bad arguments");
+    const int64_t n = n_elem * nn * nn;
+    BlkWork w;
+    hipError_t e = blk_layout(n, n_nodes, w);
+    if (e != hipSuccess) return pf3d_hip_check(e, "pf3d_block_plan_emit");
+    if ((int64_t)w.total > work_bytes) return pf3d_fail(PF3D_E_ARG, "workspace too small");
+    char *base = (char *)work;
+    hipLaunchKernelGGL(blk_emit, dim3(nblk(n, 256)), dim3(256), 0, (hipStream_t)stream, n, nn,
+                       n_nodes, (const int64_t *)(base + w.keys_out),
+                       (const int64_t *)(base + w.ids_out), (const int64_t *)(base + w.keys_in),
+                       init_k, nnzb, src, src_ptr, row_ptr, col_idx);
+    return pf3d_hip_check(hipGetLastError(), "pf3d_block_plan_emit");
+}
+
+int pf3d_block_apply(int64_t nnzb, const int64_t *src_ptr, const int64_t *src, int nn,
+                     const double *v, double *bsr_val, int accumulate, void *stream)
+{
+    if (nnzb <= 0 || !src_ptr || !src || nn < 2 || nn > 4 || !v || !bsr_val)
+        return pf3d_fail(PF3D_E_ARG, "pf3d_block_apply: bad arguments");
+    hipLaunchKernelGGL(blk_apply, dim3(nblk(nnzb * 6, 256)), dim3(256), 0, (hipStream_t)stream,
+                       nnzb, src_ptr, src, 6 * nn, v, bsr_val, accumulate);
+    return pf3d_hip_check(hipGetLastError(), "pf3d_block_apply");
+}
+
+}  // extern "C"

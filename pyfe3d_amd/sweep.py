@@ -1,0 +1,191 @@
+"""Batched design sweep: many plate variants assembled and solved together.
+
+The reference's design-optimisation tutorials re-assemble KC0 for every new
+design (``update_KC0v_only=1``), convert with ``coo_matrix(...).tocsc()``,
+slice the free DOFs and call ``spsolve``, one design at a time
+(doc/source/tut-opt-lightweight-smeared-deflection-SLSQP.ipynb; the
+static pattern of tests/test_quad4_static_point_load.py:40-102).  Here the
+variants (structured Quad4 plates of uneven sizes, clamped edges, one
+thickness per element) are laid out as one block-diagonal system:
+
+* one KC0 launch assembles every element of every variant (values only);
+* an element-block plan sums the COO into one BSR matrix (deterministic);
+* Dirichlet rows/columns, block-Jacobi inverses and the FP32 copies are
+  one launch each;
+* one persistent launch solves every variant (a workgroup per variant,
+  largest first), in FP64 PCG or FP32-inner / FP64 iterative refinement.
+
+Nothing in :meth:`PlateSweep.step` synchronises with the host, so the whole
+step can be captured in a graph and replayed (:meth:`PlateSweep.capture`).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import assembly as A
+from .mesh import PlateMesh
+from .shellprop import isotropic_plate
+from .sparse import BlockPlan
+
+
+def _s(stream):
+    return (stream if stream is not None else torch.cuda.current_stream()).cuda_stream
+
+
+def smooth_field(nxe, nye, a, b, corr, rng):
+    """Zero-mean, unit-variance smooth Gaussian random field on the element
+    centroids of an nxe x nye plate: white noise filtered with a Gaussian
+    kernel of standard deviation ``corr`` (length units)."""
+    from scipy.ndimage import gaussian_filter
+    w = rng.standard_normal((nxe, nye))
+    g = gaussian_filter(w, sigma=(corr * nxe / a, corr * nye / b), mode='reflect')
+    sd = g.std()
+    return (g - g.mean()) / (sd if sd > 0 else 1.)
+
+
+class PlateSweep:
+    """n_var clamped Quad4 plates a x b with (nxe, nye) elements each."""
+
+    def __init__(self, dims, a=1.0, b=0.7, E=70e9, nu=0.33, rho=2700., device='cuda',
+                 stream=None):
+        self.dims = [(int(p), int(q)) for p, q in dims]
+        self.n_var = len(self.dims)
+        self.a, self.b = a, b
+        xs, conns, fixed, nodes, elems = [], [], [], [0], [0]
+        for nxe, nye in self.dims:
+            m = PlateMesh(nxe + 1, nye + 1, a, b)
+            xs.append(m.x)
+            conns.append(m.quads + nodes[-1])
+            xx, yy = m.ncoords[:, 0], m.ncoords[:, 1]
+            edge = (np.isclose(xx, 0) | np.isclose(xx, a) | np.isclose(yy, 0) | np.isclose(yy, b))
+            fixed.append(np.repeat(edge, 6))
+            nodes.append(nodes[-1] + m.n_nodes)
+            elems.append(elems[-1] + m.quads.shape[0])
+        self.node0 = np.array(nodes, dtype=np.int64)
+        self.elem0 = np.array(elems, dtype=np.int64)
+        self.n_nodes = int(self.node0[-1])
+        self.n_elem = int(self.elem0[-1])
+        dev = torch.device(device)
+        self.device = dev
+        self.x = torch.as_tensor(np.concatenate(xs), device=dev)
+        self.conn = torch.as_tensor(6 * np.concatenate(conns), device=dev)
+        self.fixed = torch.as_tensor(np.concatenate(fixed).astype(np.uint8), device=dev)
+        self.row0 = torch.as_tensor(self.node0, device=dev)
+        sizes = np.diff(self.node0)
+        self.order = torch.as_tensor(np.argsort(-sizes, kind='stable').astype(np.int32),
+                                     device=dev)
+        self.plan = BlockPlan(self.conn, self.n_nodes, stream=stream)
+        nnzb = self.plan.nnzb
+        self.v = torch.empty(576 * self.n_elem, dtype=torch.float64, device=dev)
+        self.val = torch.empty(nnzb * 36, dtype=torch.float64, device=dev)
+        self.val32 = torch.empty(nnzb * 36, dtype=torch.float32, device=dev)
+        self.dinv = torch.empty(self.n_nodes * 36, dtype=torch.float64, device=dev)
+        self.dinv32 = torch.empty(self.n_nodes * 36, dtype=torch.float32, device=dev)
+        self.singular = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.props = torch.empty((self.n_elem, A.SHELL_STRIDE), dtype=torch.float64, device=dev)
+        unit = isotropic_plate(1.0, E, nu, rho=rho).record()
+        self._unit = np.ascontiguousarray(unit, dtype=np.float64)
+        self.pid = torch.arange(self.n_elem, dtype=torch.int32, device=dev)
+        nbytes = ctypes.c_int64()
+        _lib.call('pf3d_batch_bytes', self.n_nodes, ctypes.byref(nbytes))
+        self.work = torch.empty(nbytes.value, dtype=torch.uint8, device=dev)
+        self.b = torch.zeros(6 * self.n_nodes, dtype=torch.float64, device=dev)
+        self.u = torch.zeros(6 * self.n_nodes, dtype=torch.float64, device=dev)
+        self.iters = torch.zeros(self.n_var, dtype=torch.int32, device=dev)
+        self.outer = torch.zeros(self.n_var, dtype=torch.int32, device=dev)
+        self.relres = torch.zeros(self.n_var, dtype=torch.float64, device=dev)
+        self.h = torch.full((self.n_elem,), 0.002, dtype=torch.float64, device=dev)
+        self.graph = None
+
+    # ---------------------------------------------------------------- inputs
+    def random_designs(self, seed, h0=0.002, dh=0.001, hmin=0.0002, corr=0.2):
+        """Seeded thickness fields h = h0 + dh * g (g: smooth Gaussian field,
+        correlation length corr * a, clamped to h >= hmin) and one point
+        load Fz = -1000 N at a random node of the central 20% of each plate.
+        Returns the host arrays (h per element, f per DOF)."""
+        rng = np.random.default_rng(seed)
+        h = np.empty(self.n_elem)
+        f = np.zeros(6 * self.n_nodes)
+        for v, (nxe, nye) in enumerate(self.dims):
+            g = smooth_field(nxe, nye, self.a, self.b, corr * self.a, rng)
+            # element order of PlateMesh: i (x) major, j (y) minor
+            h[self.elem0[v]:self.elem0[v + 1]] = np.maximum(h0 + dh * g.ravel(), hmin)
+            i = rng.integers(int(np.ceil(0.4 * nxe)), int(np.floor(0.6 * nxe)) + 1)
+            j = rng.integers(int(np.ceil(0.4 * nye)), int(np.floor(0.6 * nye)) + 1)
+            node = self.node0[v] + i * (nye + 1) + j
+            f[6 * node + 2] = -1000.
+        return h, f
+
+    def set_thickness(self, h):
+        """Per-element thickness (device or host array); the property
+        records are rebuilt from it at the start of each step."""
+        self.h.copy_(torch.as_tensor(h, dtype=torch.float64))
+
+    def set_loads(self, f):
+        """Nodal forces per DOF; fixed DOFs are zeroed (P f)."""
+        self.b.copy_(torch.as_tensor(f, dtype=torch.float64))
+        self.b.mul_(1 - self.fixed.to(torch.float64))
+
+    # ------------------------------------------------------------- the step
+    def assemble(self, stream=None):
+        """Property records from h -> KC0 of every variant -> BSR ->
+        Dirichlet -> block Jacobi -> FP32 copies."""
+        s = _s(stream)
+        _lib.call('pf3d_isotropic_records', self.n_elem, self.h.data_ptr(),
+                  self._unit.ctypes.data, self.props.data_ptr(), s)
+        A.shell_kc0('quad4', self.x, self.conn, None, None, self.v, self.props, pid=self.pid,
+                    values_only=True, overwrite=True, stream=stream, check=False)
+        pr, pc = self.plan.row_ptr.data_ptr(), self.plan.col_idx.data_ptr()
+        self.plan.apply(self.v, out=self.val, stream=stream)
+        _lib.call('pf3d_bsr_dirichlet', self.n_nodes, pr, pc, self.val.data_ptr(),
+                  self.fixed.data_ptr(), s)
+        _lib.call('pf3d_bjacobi', self.n_nodes, pr, pc, self.val.data_ptr(),
+                  self.dinv.data_ptr(), self.dinv32.data_ptr(), self.singular.data_ptr(), s)
+        _lib.call('pf3d_to_f32', self.val.numel(), self.val.data_ptr(), self.val32.data_ptr(), s)
+
+    def solve(self, mode='mixed', rtol=1e-8, inner_rtol=1e-3, max_iter=100000, max_outer=50,
+              x0=False, n_groups=0, stream=None):
+        """Solve every variant into self.u (x0: start from the current u)."""
+        mixed = mode == 'mixed'
+        if mode not in ('mixed', 'fp64'):
+            raise ValueError(mode)
+        _lib.call('pf3d_batch_solve', self.n_var, self.n_nodes, self.row0.data_ptr(),
+                  self.order.data_ptr(), self.plan.row_ptr.data_ptr(),
+                  self.plan.col_idx.data_ptr(), self.val.data_ptr(), self.dinv.data_ptr(),
+                  self.val32.data_ptr() if mixed else None,
+                  self.dinv32.data_ptr() if mixed else None, self.b.data_ptr(),
+                  self.u.data_ptr(), 1 if x0 else 0, float(rtol), float(inner_rtol),
+                  int(max_iter), int(max_outer), self.work.data_ptr(), self.work.numel(),
+                  self.iters.data_ptr(), self.outer.data_ptr(), self.relres.data_ptr(),
+                  int(n_groups), _s(stream))
+
+    def step(self, mode='mixed', rtol=1e-8, stream=None, **kw):
+        self.assemble(stream=stream)
+        self.solve(mode=mode, rtol=rtol, stream=stream, **kw)
+
+    def capture(self, mode='mixed', rtol=1e-8, **kw):
+        """Capture :meth:`step` in a graph; replay with ``self.graph.replay()``
+        after new thicknesses (:meth:`set_thickness`) or loads (:meth:`set_loads`)."""
+        s = torch.cuda.Stream(device=self.device)
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            self.step(mode=mode, rtol=rtol, stream=s, **kw)  # warm-up outside the graph
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.step(mode=mode, rtol=rtol, **kw)
+        self.graph = g
+        return g
+
+    def check(self):
+        """Raise if any variant missed the tolerance or a block was singular."""
+        if int(self.singular.item()):
+            raise _lib.Pf3dError('%d singular diagonal blocks' % int(self.singular.item()))
+
+    def variant(self, v):
+        """(node range, element range) of variant v."""
+        return (slice(int(self.node0[v]), int(self.node0[v + 1])),
+                slice(int(self.elem0[v]), int(self.elem0[v + 1])))

@@ -1,0 +1,153 @@
+"""Element-block BSR plan, device property records and the batched design
+sweep (one persistent launch solving every variant) against the slot plan,
+the host property records and scipy's spsolve per variant."""
+import numpy as np
+import pytest
+import scipy.sparse as ss
+import scipy.sparse.linalg as sla
+import torch
+
+from oracle import oracle as O
+from test_gpu_assembly import T, _random_mesh
+
+pytestmark = pytest.mark.gpu
+DEV = 'cuda'
+
+
+@pytest.mark.parametrize('etype', ['quad4', 'tria3r'])
+def test_block_plan_bit_identical_to_slot_plan(etype):
+    """Same BSR structure and bit-identical values as the per-slot plan,
+    with a permuted init_k (elements' slots out of element order)."""
+    from pyfe3d_amd import assembly as A
+    from pyfe3d_amd.sparse import BlockPlan, BSRPlan
+    x, quads, rng = _random_mesh(17, 13, 5)
+    conn = quads if etype == 'quad4' else np.concatenate([quads[:, [0, 1, 2]], quads[:, [0, 2, 3]]])
+    E, nn = conn.shape
+    size = 36 * nn * nn
+    n_nodes = x.size // 3
+    perm = rng.permutation(E)
+    init_k = T(size * perm, torch.int64)
+    props = np.zeros((1, 32))
+    props[0, :6] = [1e8, 3e7, 0, 1e8, 0, 3.5e7]
+    props[0, 12:18] = [1e2, 3e1, 0, 1e2, 0, 3.5e1]
+    props[0, 18:24] = [3e7, 0, 3e7, 5 / 6., 5 / 6., 0.01]
+    r = torch.zeros(size * E, dtype=torch.int64, device=DEV)
+    c = torch.zeros_like(r)
+    v = torch.zeros(size * E, dtype=torch.float64, device=DEV)
+    tc = T(6 * conn, torch.int64)
+    A.shell_kc0(etype, T(x, torch.float64), tc, r, c, v, T(props, torch.float64), init_k=init_k)
+    sp = BSRPlan(r, c, n_nodes)
+    bp = BlockPlan(tc, n_nodes, init_k=init_k)
+    assert bp.nnzb == sp.nnzb
+    assert torch.equal(bp.row_ptr, sp.row_ptr) and torch.equal(bp.col_idx, sp.col_idx)
+    assert torch.equal(bp.apply(v), sp.apply(v))
+    # accumulate mode adds a second copy
+    out = bp.apply(v)
+    bp.apply(v, out=out, accumulate=True)
+    assert torch.equal(out, 2 * sp.apply(v))
+
+
+def test_isotropic_records_match_host():
+    from pyfe3d_amd import _lib
+    from pyfe3d_amd.shellprop import isotropic_plate, isotropic_records
+    h = np.linspace(0.0002, 0.01, 37)
+    ref = isotropic_records(h, 70e9, 0.33, rho=2700.)
+    out = torch.empty((h.size, 32), dtype=torch.float64, device=DEV)
+    unit = np.ascontiguousarray(isotropic_plate(1.0, 70e9, 0.33, rho=2700.).record())
+    _lib.call('pf3d_isotropic_records', h.size, T(h, torch.float64).data_ptr(), unit.ctypes.data,
+              out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    got = out.cpu().numpy()
+    np.testing.assert_allclose(got, ref, rtol=1e-14, atol=0)
+    # and the thickness-h plate computed directly (shellprop_utils.py:182-206)
+    direct = isotropic_plate(h[5], 70e9, 0.33, rho=2700.).record()
+    np.testing.assert_allclose(got[5, :28], direct[:28], rtol=1e-13, atol=1e-300)
+
+
+DIMS = [(4, 3), (6, 6), (9, 5), (3, 7), (12, 10), (5, 9), (8, 8)]
+
+
+def _sweep(seed=1):
+    from pyfe3d_amd.sweep import PlateSweep
+    sw = PlateSweep(DIMS)
+    h, f = sw.random_designs(seed)
+    sw.set_thickness(h)
+    sw.set_loads(f)
+    return sw, h, f
+
+
+def _variant_reference(sw, h, f, v):
+    """Oracle KC0 of variant v -> scipy CSR -> free-DOF spsolve."""
+    from pyfe3d_amd.shellprop import isotropic_records
+    nodes, elems = sw.variant(v)
+    x = sw.x.cpu().numpy()[3 * nodes.start:3 * nodes.stop]
+    conn = sw.conn.cpu().numpy()[elems] // 6 - nodes.start
+    E = conn.shape[0]
+    n = 6 * (nodes.stop - nodes.start)
+    props = isotropic_records(h[elems], 70e9, 0.33, rho=2700.)
+    r = np.zeros(576 * E, np.int64); c = np.zeros_like(r); val = np.zeros(576 * E)
+    O.assemble('quad4', 'KC0', x, conn, 576 * np.arange(E), np.zeros((E, 3)), np.arange(E),
+               props, r, c, val)
+    K = ss.coo_matrix((val, (r, c)), shape=(n, n)).tocsc()
+    fixed = sw.fixed.cpu().numpy()[6 * nodes.start:6 * nodes.stop].astype(bool)
+    bu = ~fixed
+    u = np.zeros(n)
+    u[bu] = sla.spsolve(K[bu, :][:, bu], f[6 * nodes.start:6 * nodes.stop][bu])
+    return u
+
+
+@pytest.mark.parametrize('mode', ['fp64', 'mixed'])
+def test_sweep_matches_spsolve_per_variant(mode):
+    sw, h, f = _sweep()
+    sw.assemble()
+    sw.solve(mode=mode, rtol=1e-12)
+    torch.cuda.synchronize()
+    sw.check()
+    relres = sw.relres.cpu().numpy()
+    iters = sw.iters.cpu().numpy()
+    print(mode, 'iters', iters, 'relres', relres)
+    assert (relres <= 1e-12).all()
+    assert (iters > 0).all()
+    u = sw.u.cpu().numpy()
+    for v in range(sw.n_var):
+        nodes, _ = sw.variant(v)
+        uref = _variant_reference(sw, h, f, v)
+        ug = u[6 * nodes.start:6 * nodes.stop]
+        err = np.linalg.norm(ug - uref) / np.linalg.norm(uref)
+        assert err <= 1e-8, (v, err)
+
+
+def test_sweep_rtol_1e8_and_warm_start():
+    """At the bench tolerance every variant meets 1e-8 in both modes; a
+    second solve started from the converged u needs no iteration."""
+    sw, h, f = _sweep(seed=4)
+    sw.assemble()
+    for mode in ('fp64', 'mixed'):
+        sw.solve(mode=mode, rtol=1e-8)
+        torch.cuda.synchronize()
+        assert (sw.relres.cpu().numpy() <= 1e-8).all(), mode
+        sw.solve(mode=mode, rtol=1e-8, x0=True)
+        torch.cuda.synchronize()
+        assert (sw.iters.cpu().numpy() == 0).all(), mode
+
+
+def test_sweep_graph_replays_match_eager():
+    """The captured step (records, KC0, BSR, Dirichlet, Jacobi, FP32 copies,
+    batched solve) replayed for new designs gives the eager results bit
+    for bit; eager work and synchronisations run between replays."""
+    sw, h, f = _sweep(seed=2)
+    sw.capture(mode='mixed', rtol=1e-8)
+    for seed in (5, 6, 7):
+        h2, f2 = sw.random_designs(seed)
+        sw.set_thickness(h2)
+        sw.set_loads(f2)
+        sw.u.zero_()
+        sw.graph.replay()
+        torch.cuda.synchronize()
+        ug = sw.u.clone()
+        itg = sw.iters.clone()
+        sw.u.zero_()
+        sw.step(mode='mixed', rtol=1e-8)
+        torch.cuda.synchronize()
+        assert torch.equal(ug, sw.u), seed
+        assert torch.equal(itg, sw.iters), seed
+        assert (sw.relres.cpu().numpy() <= 1e-8).all()
