@@ -117,6 +117,55 @@ def solve_rates(n, repeats, log):
     return out
 
 
+def sweep_dims(n_var, seed=2026):
+    """Plate sizes of the design sweep: nx, ny ~ U{8..64} elements (seeded)."""
+    rng = np.random.default_rng(seed)
+    return [tuple(int(t) for t in rng.integers(8, 65, 2)) for _ in range(n_var)]
+
+
+def sweep_rates(n_var, log, steps=2):
+    """Batched design sweep: n_var clamped isotropic plates with smooth
+    seeded thickness fields and a central point load, assembled and solved
+    to 1e-8 (every variant) per step, as one captured graph per mode."""
+    import torch
+    from pyfe3d_amd.sweep import PlateSweep
+    t0 = time.perf_counter()
+    sw = PlateSweep(sweep_dims(n_var))
+    h, f = sw.random_designs(7)
+    sw.set_thickness(h)
+    sw.set_loads(f)
+    log('sweep: %d variants, %d elements, %d DOF, %d blocks (setup %.1f s)'
+        % (n_var, sw.n_elem, 6 * sw.n_nodes, sw.plan.nnzb, time.perf_counter() - t0))
+    out = dict(variants=n_var, elements=sw.n_elem, dof=6 * sw.n_nodes)
+    for mode in ('fp64', 'mixed'):
+        t0 = time.perf_counter()
+        sw.step(mode=mode, rtol=1e-8)
+        torch.cuda.synchronize()
+        first = time.perf_counter() - t0
+        sw.check()
+        g = sw.capture(mode=mode, rtol=1e-8)
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        ev0.record()
+        for _ in range(steps):
+            g.replay()
+        ev1.record()
+        torch.cuda.synchronize()
+        ms = ev0.elapsed_time(ev1) / steps
+        rel = sw.relres.cpu().numpy()
+        its = sw.iters.cpu().numpy()
+        out[mode] = dict(ms_per_sweep=ms, solves_per_s=n_var / (ms * 1e-3),
+                         max_relres=float(rel.max()), converged=int((rel <= 1e-8).sum()),
+                         pcg_iters_total=int(its.sum()), pcg_iters_max=int(its.max()),
+                         outer_max=int(sw.outer.cpu().numpy().max()), first_step_s=first)
+        log('sweep %s: %.1f ms/sweep, %.1f solves/s, max relres %.2e, iters %d'
+            % (mode, ms, n_var / (ms * 1e-3), rel.max(), its.sum()))
+        del g
+        sw.graph = None
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -125,6 +174,7 @@ def main():
     ap.add_argument('--n', type=int, default=1000, help='elements per side')
     ap.add_argument('--cpu-sample', type=int, default=200000)
     ap.add_argument('--no-solve', action='store_true')
+    ap.add_argument('--sweep-variants', type=int, default=4096)
     ap.add_argument('--quiet', action='store_true')
     args = ap.parse_args()
     log = (lambda *a: None) if args.quiet else (lambda *a: print(*a, file=sys.stderr, flush=True))
@@ -201,6 +251,15 @@ def main():
         except Exception as exc:  # reported, not hidden
             extra['solve_64x64'] = dict(error=str(exc))
             log('solve failed: %s' % exc)
+
+    if not args.no_solve and args.sweep_variants > 0:
+        del v
+        torch.cuda.empty_cache()
+        try:
+            extra['design_sweep'] = sweep_rates(args.sweep_variants, log)
+        except Exception as exc:
+            extra['design_sweep'] = dict(error=str(exc))
+            log('sweep failed: %s' % exc)
 
     n_sample = min(args.cpu_sample, E)
     base = cpu_baseline(mesh, prop, n_sample)

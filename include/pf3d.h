@@ -292,6 +292,39 @@ int pf3d_batch_solve(int64_t n_var, int64_t nb, const int64_t *row0, const int32
                      int max_iter, int max_outer, void *work, int64_t work_bytes,
                      int32_t *iters, int32_t *outer, double *relres, int n_groups, void *stream);
 
+/* ------------------------------------------------------------------ */
+/* Warm-start network kernels (no reference counterpart: pyfe3d starts   */
+/* every solve from zero).  FP32 tensors, NHWC activations, weights      */
+/* [c_out][(ky, kx, c_in)]; products on the matrix cores in bfloat16     */
+/* with FP32 accumulation.                                               */
+/* ------------------------------------------------------------------ */
+
+/* C[m*ldc + n] (+)= sum_k A[m*sam + k*sak] B[n*sbn + k*sbk] (+ bias[n],
+ * ReLU).  splits > 1 divides K into slices summed in slice order through
+ * partial (splits*M*N floats). */
+int pf3d_nn_gemm(int M, int N, int K, const float *A, int64_t sam, int64_t sak, const float *B,
+                 int64_t sbn, int64_t sbk, float *C, int64_t ldc, int accumulate,
+                 const float *bias, int relu, int splits, float *partial, void *stream);
+/* 3x3 patches with zero padding 1 and stride 1 or 2, and the adjoint */
+int pf3d_nn_im2col(int B, int H, int W, int C, int stride, const float *x, float *col,
+                   void *stream);
+int pf3d_nn_col2im(int B, int H, int W, int C, int stride, const float *dcol, float *dx,
+                   int accumulate, void *stream);
+int pf3d_nn_relu_bwd(int64_t n, const float *y, float *dy, void *stream);
+int pf3d_nn_bias_grad(int64_t rows, int C, const float *dy, float *db, void *stream);
+/* y = nearest-upsample(lo) + skip on an H x W grid, and d(lo) */
+int pf3d_nn_up2_add(int B, int H, int W, int C, const float *lo, const float *skip, float *y,
+                    void *stream);
+int pf3d_nn_up2_bwd(int B, int H, int W, int C, const float *dy, float *dlo, void *stream);
+/* element-centre values (B, n, n, C) -> interior node means on the
+ * (n+1)^2 grid, and the gradient of 0.5*scale*sum_c wc[c] (u - t)^2 */
+int pf3d_nn_node_avg(int B, int n, int C, const float *e, float *u, void *stream);
+int pf3d_nn_node_loss_grad(int B, int n, int C, const float *u, const float *target,
+                           const float *wc, float scale, float *de, void *stream);
+/* Adam step `step` (1-based) with bias correction */
+int pf3d_nn_adam(int64_t n, float *p, const float *g, float *m, float *v, float lr, float b1,
+                 float b2, float eps, int step, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,6 +13,7 @@ P = ctypes.c_void_p
 I32 = ctypes.c_int
 I64 = ctypes.c_int64
 F64 = ctypes.c_double
+F32 = ctypes.c_float
 
 # name -> argtypes (all return int status except where noted)
 SIGNATURES = {
@@ -50,6 +51,16 @@ SIGNATURES = {
     'pf3d_ir_bytes': [I64, P],
     'pf3d_ir_solve': [I64, P, P, P, P, P, P, P, F64, F64, I32, I32, I32, P, I64, P, P, P],
     'pf3d_batch_bytes': [I64, P],
+    'pf3d_nn_gemm': [I32, I32, I32, P, I64, I64, P, I64, I64, P, I64, I32, P, I32, I32, P, P],
+    'pf3d_nn_im2col': [I32, I32, I32, I32, I32, P, P, P],
+    'pf3d_nn_col2im': [I32, I32, I32, I32, I32, P, P, I32, P],
+    'pf3d_nn_relu_bwd': [I64, P, P, P],
+    'pf3d_nn_bias_grad': [I64, I32, P, P, P],
+    'pf3d_nn_up2_add': [I32, I32, I32, I32, P, P, P, P],
+    'pf3d_nn_up2_bwd': [I32, I32, I32, I32, P, P, P],
+    'pf3d_nn_node_avg': [I32, I32, I32, P, P, P],
+    'pf3d_nn_node_loss_grad': [I32, I32, I32, P, P, P, F32, P, P],
+    'pf3d_nn_adam': [I64, P, P, P, P, F32, F32, F32, F32, I32, P],
     'pf3d_batch_solve': [I64, I64, P, P, P, P, P, P, P, P, P, P, I32, F64, F64, I32, I32, P, I64,
                          P, P, P, I32, P],
 }

@@ -213,21 +213,91 @@ __device__ __forceinline__ void rotate_block(double K[6][6], const double *r)
 // per-type local blocks
 // ------------------------------------------------------------------------
 
+// Gauss-point moments of the lane's node pair: with g = (N,x, N,y) and w
+// the Gauss weight times detJ,
+//   G[p][q] = sum w g_I[p] g_J[q],  NG[q] = sum w N_I g_J[q],
+//   GN[p] = sum w g_I[p] N_J,       NN = sum w N_I N_J.
+// Every 2x2-Gauss term of the Quad4 stiffness that has a constant material
+// table (membrane/bending ABD, thick-shell shear gradients, drilling) is a
+// fixed linear combination of these nine sums, so the per-point work is
+// nine FMAs instead of the B^T C B products.
+struct Moments {
+    double G[2][2], NG[2], GN[2], NN;
+};
+
+__device__ __forceinline__ void add_moments(Moments &m, const Pair &n, double w)
+{
+    const double gi[2] = {w * n.nxi, w * n.nyi}, gj[2] = {n.nxj, n.nyj};
+    const double wNi = w * n.Ni;
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+#pragma unroll
+        for (int q = 0; q < 2; q++) m.G[p][q] += gi[p] * gj[q];
+        m.NG[p] += wNi * gj[p];
+        m.GN[p] += gi[p] * n.Nj;
+    }
+    m.NN += wNi * n.Nj;
+}
+
+// membrane/bending part from G: the strains of DOF d of a node are
+// s * g[p] in strain row r for its two (p, r, s) pairs (quad4.pyx:1002-1030):
+//   u: (x, exx, +), (y, gxy, +)    v: (x, gxy, +), (y, eyy, +)
+//   rx: (x, kxy, -), (y, kyy, -)   ry: (x, kxx, +), (y, kxy, +)
+// so K[a][b] = sum_pq s_ap s_bq C[r_ap][r_bq] G[p][q].
+__device__ __forceinline__ void abd_from_moments(double K[6][6], const double C[6][6],
+                                                 const double G[2][2])
+{
+    constexpr int dof[4] = {0, 1, 3, 4};
+    constexpr int row[4][2] = {{0, 2}, {2, 1}, {5, 4}, {3, 5}};
+    constexpr double sg[4] = {1., 1., -1., 1.};
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            double t = 0.;
+#pragma unroll
+            for (int p = 0; p < 2; p++)
+#pragma unroll
+                for (int q = 0; q < 2; q++) t += C[row[a][p]][row[b][q]] * G[p][q];
+            K[dof[a]][dof[b]] += (sg[a] * sg[b]) * t;
+        }
+}
+
+// drilling (rz - (v,x - u,y)/2)^2 with b = (N,y/2, -N,x/2, N) on (u, v, rz)
+// (quad4.pyx:1053-1073), from the moments
+__device__ __forceinline__ void drill_from_moments(double K[6][6], const Moments &m, double f)
+{
+    K[0][0] += f * 0.25 * m.G[1][1];
+    K[0][1] -= f * 0.25 * m.G[1][0];
+    K[1][0] -= f * 0.25 * m.G[0][1];
+    K[1][1] += f * 0.25 * m.G[0][0];
+    K[0][5] += f * 0.5 * m.GN[1];
+    K[1][5] -= f * 0.5 * m.GN[0];
+    K[5][0] += f * 0.5 * m.NG[1];
+    K[5][1] -= f * 0.5 * m.NG[0];
+    K[5][5] += f * m.NN;
+}
+
 // Quad4 KC0: quad4.pyx:755-1172
 __device__ __forceinline__ void quad4_kc0(double K[6][6], const Geom &g, const Shell &s, int I,
                                           int J)
 {
     const bool thick = (s.h / sqrt(g.area)) >= 1.;
     const double sxi = node_sx(I), syi = node_sy(I), sxj = node_sx(J), syj = node_sy(J);
+    Moments m = {{{0., 0.}, {0., 0.}}, {0., 0.}, {0., 0.}, 0.};
 #pragma unroll PF3D_GP_UNROLL
     for (int p = 0; p < 4; p++) {
         const Pair n = quad_pair(g.xe, (p >> 1) ? GP2 : -GP2, (p & 1) ? GP2 : -GP2, sxi, syi,
                                  sxj, syj);
-        add_abd(K, s.C, n.nxi, n.nyi, n.nxj, n.nyj, n.det);
-        if (thick)
-            add_shear(K, s.Es, n.Ni, n.nxi, n.nyi, n.Nj, n.nxj, n.nyj, n.det, true, false, false);
-        add_drill(K, n.Ni, n.nxi, n.nyi, n.Nj, n.nxj, n.nyj, n.det, true, true);
+        add_moments(m, n, n.det);
     }
+    abd_from_moments(K, s.C, m.G);
+    // thick shells: shear gradient-gradient term at the 2x2 points
+    // (quad4.pyx:1032-1051), rows/cols (gyz, gxz) <- (N,y, N,x)
+    if (thick)
+        K[2][2] += s.Es[0][0] * m.G[1][1] + s.Es[0][1] * m.G[1][0] + s.Es[1][0] * m.G[0][1] +
+                   s.Es[1][1] * m.G[0][0];
+    drill_from_moments(K, m, 1.);
     const Pair n = quad_pair(g.xe, 0., 0., sxi, syi, sxj, syj);
     add_shear(K, s.Es, n.Ni, n.nxi, n.nyi, n.Nj, n.nxj, n.nyj, 4. * n.det, !thick, true, true);
 }
@@ -540,6 +610,24 @@ __device__ __forceinline__ void upper_pair(int sub, int &I, int &J)
     J = I + sub - (I * NN - (I * (I - 1)) / 2);
 }
 
+// 16-byte vectors for the slot streaming; F_VEC (internal launch flag):
+// every element's first slot is even and v, r, c are 16-byte aligned
+using dbl2 = double __attribute__((ext_vector_type(2)));
+using lng2 = long long __attribute__((ext_vector_type(2)));
+constexpr int F_VEC = 4;
+
+// the COO output is written once and not read again by this kernel:
+// optionally bypass the caches' allocate-on-write (PF3D_NT_STORE)
+template <typename T>
+__device__ __forceinline__ void store2(T *p, T v)
+{
+#if defined(PF3D_NT_STORE) && PF3D_NT_STORE
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+
 // waves per SIMD the register allocation of the KC0 kernel targets
 #ifndef PF3D_KC0_WAVES
 #define PF3D_KC0_WAVES 2
@@ -636,14 +724,16 @@ shell_kc0_sym(AsmArgs a)
             if (pi >= nel * PR) break;
             const int el = pi / PR, t = 2 * (pi - el * PR);
             const int64_t k = sk0[el] + P * RW + t;
-            const double2 val = *reinterpret_cast<const double2 *>(&strip[el][t]);
-            if (((k & 1) == 0) && vec_ok) {
-                double2 *dst = reinterpret_cast<double2 *>(a.v + k);
+            const dbl2 val = *reinterpret_cast<const dbl2 *>(&strip[el][t]);
+            if (FLAGS & F_VEC) {
+                // slots 16-byte aligned (init_k = size*e, aligned v)
+                dbl2 *dst = reinterpret_cast<dbl2 *>(a.v + k);
+                if (overwrite) store2(dst, val);
+                else store2(dst, *dst + val);
+            } else if (((k & 1) == 0) && vec_ok) {
+                dbl2 *dst = reinterpret_cast<dbl2 *>(a.v + k);
                 if (overwrite) *dst = val;
-                else {
-                    double2 old = *dst;
-                    *dst = make_double2(old.x + val.x, old.y + val.y);
-                }
+                else *dst = *dst + val;
             } else {
                 if (overwrite) { a.v[k] = val.x; a.v[k + 1] = val.y; }
                 else { a.v[k] += val.x; a.v[k + 1] += val.y; }
@@ -652,8 +742,13 @@ shell_kc0_sym(AsmArgs a)
                 const int rl = t / NDOF, cl = t - rl * NDOF;
                 const int64_t rr = sconn[el][P] + rl;
                 const int64_t cc = sconn[el][cl / 6] + cl % 6;
-                a.r[k] = rr; a.r[k + 1] = rr;
-                a.c[k] = cc; a.c[k + 1] = cc + 1;
+                if (FLAGS & F_VEC) {
+                    store2(reinterpret_cast<lng2 *>(a.r + k), lng2{rr, rr});
+                    store2(reinterpret_cast<lng2 *>(a.c + k), lng2{cc, cc + 1});
+                } else {
+                    a.r[k] = rr; a.r[k + 1] = rr;
+                    a.c[k] = cc; a.c[k + 1] = cc + 1;
+                }
             }
         }
     }
@@ -1126,21 +1221,20 @@ static hipError_t launch_shell(const AsmArgs &a, hipStream_t s)
         constexpr int EPW = 64 / (NN * (NN + 1) / 2);
         const int64_t blocks = (a.n_elem + EPW - 1) / EPW;
         if (blocks == 0) return hipSuccess;
-        switch (a.flags & (F_VALUES_ONLY | F_OVERWRITE)) {
-        case 0:
-            hipLaunchKernelGGL((shell_kc0_sym<ET, 0>), dim3((unsigned)blocks), dim3(64), 0, s, a);
-            break;
-        case F_VALUES_ONLY:
-            hipLaunchKernelGGL((shell_kc0_sym<ET, F_VALUES_ONLY>), dim3((unsigned)blocks),
-                               dim3(64), 0, s, a);
-            break;
-        case F_OVERWRITE:
-            hipLaunchKernelGGL((shell_kc0_sym<ET, F_OVERWRITE>), dim3((unsigned)blocks), dim3(64),
-                               0, s, a);
-            break;
-        default:
-            hipLaunchKernelGGL((shell_kc0_sym<ET, F_VALUES_ONLY | F_OVERWRITE>),
-                               dim3((unsigned)blocks), dim3(64), 0, s, a);
+        const bool vec = !a.init_k && (reinterpret_cast<uintptr_t>(a.v) & 15) == 0 &&
+                         ((a.flags & F_VALUES_ONLY) ||
+                          ((reinterpret_cast<uintptr_t>(a.r) & 15) == 0 &&
+                           (reinterpret_cast<uintptr_t>(a.c) & 15) == 0));
+        const dim3 grid((unsigned)blocks), blk(64);
+        switch ((a.flags & (F_VALUES_ONLY | F_OVERWRITE)) | (vec ? F_VEC : 0)) {
+        case 0: hipLaunchKernelGGL((shell_kc0_sym<ET, 0>), grid, blk, 0, s, a); break;
+        case 1: hipLaunchKernelGGL((shell_kc0_sym<ET, 1>), grid, blk, 0, s, a); break;
+        case 2: hipLaunchKernelGGL((shell_kc0_sym<ET, 2>), grid, blk, 0, s, a); break;
+        case 3: hipLaunchKernelGGL((shell_kc0_sym<ET, 3>), grid, blk, 0, s, a); break;
+        case 4: hipLaunchKernelGGL((shell_kc0_sym<ET, 4>), grid, blk, 0, s, a); break;
+        case 5: hipLaunchKernelGGL((shell_kc0_sym<ET, 5>), grid, blk, 0, s, a); break;
+        case 6: hipLaunchKernelGGL((shell_kc0_sym<ET, 6>), grid, blk, 0, s, a); break;
+        default: hipLaunchKernelGGL((shell_kc0_sym<ET, 7>), grid, blk, 0, s, a);
         }
         return hipGetLastError();
     }
