@@ -271,26 +271,27 @@ int pf3d_ir_solve(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx,
 
 /* The systems are the diagonal blocks of one block-diagonal BSR matrix of
  * nb block rows: system v owns rows [row0[v], row0[v+1]) (row0 on the
- * device).  Each persistent workgroup takes the next system (in `order`,
- * or 0..n_var-1 if NULL) and solves it alone:
- *   val32 == NULL : FP64 block-Jacobi PCG (val64, dinv64), restarted from
- *                   the true residual;
- *   val32 != NULL : FP64 iterative refinement around PCG on the FP32
- *                   matrix and inverses (val32, dinv32), inner tolerance
- *                   inner_rtol * ||r||.
- * Both stop at ||b - K64 x|| <= rtol ||b|| (per system), after max_iter
- * PCG iterations or max_outer restarts/refinement steps.  x0 != 0 starts
- * from the x given.  Per system: iters (PCG iterations), outer (steps),
- * relres (final FP64 true relative residual).  n_groups workgroups
- * (0 = 2048).  No host synchronisation: the call may be captured in a
+ * device).  Each persistent workgroup (`threads` = 256, 512 or 1024; 0 =
+ * 1024) takes the next system (in `order`, or 0..n_var-1 if NULL) and runs
+ * a 6x6 block-Jacobi PCG on it alone, FP64 vectors and arithmetic:
+ *   val32 == NULL : operator and inverses in FP64 (val64, dinv64);
+ *   val32 != NULL : operator and inverses read in FP32 (val32, dinv32) with
+ *                   reliable updates: r <- b - K64 x every `replace`
+ *                   iterations (0: only at convergence checks), keeping the
+ *                   search direction.
+ * Each system stops when ||b - K64 x|| <= rtol ||b|| (FP64 true residual)
+ * or after max_iter iterations.  x0 != 0 starts from the x given.  Per
+ * system: iters (PCG iterations), outer (true-residual evaluations),
+ * relres (final true relative residual).  n_groups workgroups (0 = one
+ * per CU-slot).  No host synchronisation: the call may be captured in a
  * graph.  work: pf3d_batch_bytes(nb). */
 int pf3d_batch_bytes(int64_t nb, int64_t *bytes);
 int pf3d_batch_solve(int64_t n_var, int64_t nb, const int64_t *row0, const int32_t *order,
                      const int32_t *row_ptr, const int32_t *col_idx, const double *val64,
                      const double *dinv64, const float *val32, const float *dinv32,
-                     const double *b, double *x, int x0, double rtol, double inner_rtol,
-                     int max_iter, int max_outer, void *work, int64_t work_bytes,
-                     int32_t *iters, int32_t *outer, double *relres, int n_groups, void *stream);
+                     const double *b, double *x, int x0, double rtol, int replace, int max_iter,
+                     void *work, int64_t work_bytes, int32_t *iters, int32_t *outer,
+                     double *relres, int n_groups, int threads, void *stream);
 
 /* ------------------------------------------------------------------ */
 /* Warm-start network kernels (no reference counterpart: pyfe3d starts   */

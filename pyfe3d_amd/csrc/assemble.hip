@@ -617,11 +617,15 @@ using lng2 = long long __attribute__((ext_vector_type(2)));
 constexpr int F_VEC = 4;
 
 // the COO output is written once and not read again by this kernel:
-// optionally bypass the caches' allocate-on-write (PF3D_NT_STORE)
+// non-temporal stores keep it from evicting the inputs from the caches
+// (measured on MI355X: 1.12 -> 0.96 ms for 1e6 Quad4 KC0)
+#ifndef PF3D_NT_STORE
+#define PF3D_NT_STORE 1
+#endif
 template <typename T>
 __device__ __forceinline__ void store2(T *p, T v)
 {
-#if defined(PF3D_NT_STORE) && PF3D_NT_STORE
+#if PF3D_NT_STORE
     __builtin_nontemporal_store(v, p);
 #else
     *p = v;
@@ -630,7 +634,7 @@ __device__ __forceinline__ void store2(T *p, T v)
 
 // waves per SIMD the register allocation of the KC0 kernel targets
 #ifndef PF3D_KC0_WAVES
-#define PF3D_KC0_WAVES 2
+#define PF3D_KC0_WAVES 3
 #endif
 
 template <int ET, int FLAGS>

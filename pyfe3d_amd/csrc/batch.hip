@@ -11,24 +11,29 @@
 //
 // Per system, the reference caller's pattern (assemble, slice the free
 // DOFs, spsolve; tests/test_quad4_static_point_load.py:80-102, and the
-// re-assembly loop of the design-sweep tutorials) becomes:
-//   plain   : PCG with 6x6 block-Jacobi on the FP64 matrix, restarted from
-//             the true residual b - K x until ||b - K x|| <= rtol ||b||;
-//   mixed   : FP64 iterative refinement, r = b - K64 x, inner PCG on the
-//             FP32-stored matrix and block inverses (FP64 vectors and
-//             arithmetic) to inner_rtol ||r||, x += d, until the FP64 true
-//             residual meets rtol.
+// re-assembly loop of the design-sweep tutorials) becomes a 6x6
+// block-Jacobi preconditioned CG on x with FP64 vectors and arithmetic:
+//   plain : operator and block inverses in FP64;
+//   mixed : operator and block inverses read in FP32 (about 0.6x the bytes
+//           of an iteration), with reliable updates: every `replace`
+//           iterations, and whenever the recursive residual claims
+//           convergence, r is replaced by the FP64 true residual b - K64 x
+//           and the iteration continues with the same search direction
+//           (no restart, so the Krylov space is kept).
+// Both stop when the FP64 true residual meets ||b - K64 x|| <= rtol ||b||.
 // x may hold an initial guess (x0 != 0).
 //
-// Work mapping inside a workgroup (256 threads, 4 waves): lanes 0..59 of
+// Work mapping inside a workgroup (BT threads, BT/64 waves): lanes 0..59 of
 // each wave own 10 block rows, lane = 6*row_in_wave + component, so a
 // block row's six threads sit in one wave and exchange values with
-// shuffles.  Every phase uses this mapping; only the sparse product reads
+// shuffles.  Every phase uses this mapping; only the sparse products read
 // other threads' entries (after a barrier).  Reductions use a fixed tree
-// (wave shuffles, then the 4 wave sums in order), so results do not depend
+// (wave shuffles, then the wave sums in order), so results do not depend
 // on timing.
 #include <math.h>
 #include <stdint.h>
+
+#include <type_traits>
 
 #include "../../include/pf3d.h"
 #include "pf3d_device.hpp"
@@ -38,10 +43,7 @@ int pf3d_hip_check(hipError_t e, const char *where);
 
 namespace {
 
-constexpr int BT = 256;        // threads per workgroup
-constexpr int NW = BT / 64;    // waves per workgroup
-constexpr int RPW = 10;        // block rows per wave and pass
-constexpr int RPG = NW * RPW;  // block rows per workgroup pass
+constexpr int RPW = 10;  // block rows per wave and pass
 
 struct BatchArgs {
     int64_t n_var;
@@ -53,9 +55,10 @@ struct BatchArgs {
     const double *b;
     double *x;
     int x0;
-    double rtol, inner_rtol;
-    int max_iter, max_outer;
-    double *r, *z, *p, *q, *d;
+    double rtol;
+    int replace;
+    int max_iter;
+    double *r, *z, *p, *q;
     int *counter;
     int32_t *iters, *outer;
     double *relres;
@@ -70,6 +73,9 @@ __device__ __forceinline__ double wave_sum(double v)
 
 // sums of two values over the workgroup; red has room for 2 buffers of
 // 2*NW doubles, used alternately so one barrier per reduction suffices
+// (a buffer is rewritten only after the next reduction's barrier, which
+// every thread reaches after reading it)
+template <int NW>
 __device__ __forceinline__ void wg_sum2(double &a, double &b, double *red, int &buf)
 {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -113,8 +119,9 @@ __device__ __forceinline__ Map lane_map()
     return m;
 }
 
-// out = K in over rows [r0, r1); returns this thread's part of dot(out, w)
-template <typename MT>
+// out = K in over rows [r0, r1) (RPG rows per pass); returns this thread's
+// part of dot(out, w)
+template <int RPG, typename MT>
 __device__ __forceinline__ double spmv_part(const Map &m, int64_t r0, int64_t r1,
                                             const int32_t *__restrict__ rp,
                                             const int32_t *__restrict__ ci,
@@ -142,9 +149,9 @@ __device__ __forceinline__ double spmv_part(const Map &m, int64_t r0, int64_t r1
     return dot;
 }
 
-// z = Dinv r for the thread's entries (r already final); returns parts of
-// (r.z, r.r).  All lanes of the wave execute the shuffles.
-template <typename MT>
+// z = Dinv r for the thread's entries; adds this thread's parts of (r.z,
+// r.r).  All lanes of the wave execute the shuffles.
+template <int RPG, typename MT>
 __device__ __forceinline__ void precond_part(const Map &m, int64_t r0, int64_t r1,
                                              const MT *__restrict__ dinv,
                                              const double *__restrict__ r, double *__restrict__ z,
@@ -168,62 +175,14 @@ __device__ __forceinline__ void precond_part(const Map &m, int64_t r0, int64_t r
     }
 }
 
-// PCG on operator (val, dinv) for K sol = rhs given the initial residual
-// in r (r = rhs - K sol).  Stops when r.r <= tol2 or after max_it
-// iterations.  Returns iterations done; rr_out = final recursive r.r.
-template <typename MT>
-__device__ int pcg_wg(const Map &m, int64_t r0, int64_t r1, const int32_t *rp, const int32_t *ci,
-                      const MT *val, const MT *dinv, double *sol, double *r, double *z,
-                      double *p, double *q, double tol2, int max_it, double *red, int &buf,
-                      double &rr_out)
-{
-    double rz = 0., rr = 0.;
-    precond_part<MT>(m, r0, r1, dinv, r, z, rz, rr);
-    wg_sum2(rz, rr, red, buf);
-    // p = z
-    if (m.slot >= 0)
-        for (int64_t row = r0 + m.slot; row < r1; row += RPG) {
-            const int64_t t = row * 6 + m.comp;
-            p[t] = z[t];
-        }
-    int it = 0;
-    while (rr > tol2 && it < max_it) {
-        __syncthreads();  // p complete before the product reads it
-        double pq = spmv_part<MT>(m, r0, r1, rp, ci, val, p, q, p), dummy = 0.;
-        wg_sum2(pq, dummy, red, buf);
-        if (!(pq > 0.) || !isfinite(pq)) break;
-        const double alpha = rz / pq;
-        if (m.slot >= 0)
-            for (int64_t row = r0 + m.slot; row < r1; row += RPG) {
-                const int64_t t = row * 6 + m.comp;
-                sol[t] += alpha * p[t];
-                r[t] -= alpha * q[t];
-            }
-        double rz_new = 0.;
-        rr = 0.;
-        precond_part<MT>(m, r0, r1, dinv, r, z, rz_new, rr);
-        wg_sum2(rz_new, rr, red, buf);
-        const double beta = rz_new / rz;
-        rz = rz_new;
-        if (m.slot >= 0)
-            for (int64_t row = r0 + m.slot; row < r1; row += RPG) {
-                const int64_t t = row * 6 + m.comp;
-                p[t] = z[t] + beta * p[t];
-            }
-        it++;
-    }
-    rr_out = rr;
-    __syncthreads();
-    return it;
-}
-
-// r = b - K64 x (true residual); returns r.r and (optionally) b.b
-__device__ __forceinline__ void true_residual(const Map &m, int64_t r0, int64_t r1,
-                                              const BatchArgs &a, double &rr, double &bb,
-                                              double *red, int &buf)
+// r = b - K64 x (FP64 true residual) over the system's rows; returns the
+// thread parts of r.r and b.b
+template <int RPG>
+__device__ __forceinline__ void true_residual_part(const Map &m, int64_t r0, int64_t r1,
+                                                   const BatchArgs &a, double &rr, double &bb)
 {
     __syncthreads();  // x complete
-    spmv_part<double>(m, r0, r1, a.rp, a.ci, a.val64, a.x, a.q, nullptr);
+    spmv_part<RPG, double>(m, r0, r1, a.rp, a.ci, a.val64, a.x, a.q, nullptr);
     rr = 0.;
     bb = 0.;
     if (m.slot >= 0)
@@ -234,17 +193,20 @@ __device__ __forceinline__ void true_residual(const Map &m, int64_t r0, int64_t 
             rr += rv * rv;
             bb += bv * bv;
         }
-    wg_sum2(rr, bb, red, buf);
 }
 
 __global__ void reset_counter(int *c) { *c = 0; }
 
-template <bool MIXED>
+template <int BT, bool MIXED>
 __global__ void __launch_bounds__(BT) batch_solve(BatchArgs a)
 {
+    using MT = typename std::conditional<MIXED, float, double>::type;
+    constexpr int NW = BT / 64, RPG = NW * RPW;
     __shared__ double red[4 * NW];
     __shared__ int sk;
     const Map m = lane_map();
+    const MT *val = MIXED ? (const MT *)a.val32 : (const MT *)a.val64;
+    const MT *dinv = MIXED ? (const MT *)a.dinv32 : (const MT *)a.dinv64;
     int buf = 0;
     for (;;) {
         if (threadIdx.x == 0) sk = atomicAdd(a.counter, 1);
@@ -256,40 +218,129 @@ __global__ void __launch_bounds__(BT) batch_solve(BatchArgs a)
         const int64_t r0 = a.row0[v], r1 = a.row0[v + 1];
         if (!a.x0 && m.slot >= 0)
             for (int64_t row = r0 + m.slot; row < r1; row += RPG) a.x[row * 6 + m.comp] = 0.;
-        double rr, bb;
-        true_residual(m, r0, r1, a, rr, bb, red, buf);
+        double rr, bb, bscratch;
+        true_residual_part<RPG>(m, r0, r1, a, rr, bb);
+        wg_sum2<NW>(rr, bb, red, buf);
         const double tol2 = a.rtol * a.rtol * (bb > 0. ? bb : 1.);
-        int its = 0, outer = 0;
-        while (rr > tol2 && its < a.max_iter && outer < a.max_outer) {
-            double rr_in;
-            if (MIXED) {
-                // d = 0; inner solve K32 d = r to inner_rtol ||r||, x += d
-                if (m.slot >= 0)
-                    for (int64_t row = r0 + m.slot; row < r1; row += RPG) a.d[row * 6 + m.comp] = 0.;
-                double itol2 = a.inner_rtol * a.inner_rtol * rr;
-                if (itol2 < 0.25 * tol2) itol2 = 0.25 * tol2;
-                its += pcg_wg<float>(m, r0, r1, a.rp, a.ci, a.val32, a.dinv32, a.d, a.r, a.z, a.p,
-                                     a.q, itol2, a.max_iter - its, red, buf, rr_in);
-                if (m.slot >= 0)
-                    for (int64_t row = r0 + m.slot; row < r1; row += RPG) {
-                        const int64_t t = row * 6 + m.comp;
-                        a.x[t] += a.d[t];
-                    }
-            } else {
-                // iterate on x directly from its true residual (a restart)
-                its += pcg_wg<double>(m, r0, r1, a.rp, a.ci, a.val64, a.dinv64, a.x, a.r, a.z,
-                                      a.p, a.q, 0.25 * tol2, a.max_iter - its, red, buf, rr_in);
+        int its = 0, replaced = 1;
+        bool rr_true = true;
+        double best = rr;  // smallest true r.r so far
+        // operator of the iteration: MT (FP32 in mixed mode) until the true
+        // residual stagnates, FP64 afterwards
+        bool use64 = !MIXED;
+        auto precond = [&](double &rz_out, double &rr_out) {
+            if (use64)
+                precond_part<RPG, double>(m, r0, r1, a.dinv64, a.r, a.z, rz_out, rr_out);
+            else
+                precond_part<RPG, MT>(m, r0, r1, dinv, a.r, a.z, rz_out, rr_out);
+        };
+        auto product = [&]() {
+            return use64 ? spmv_part<RPG, double>(m, r0, r1, a.rp, a.ci, a.val64, a.p, a.q, a.p)
+                         : spmv_part<RPG, MT>(m, r0, r1, a.rp, a.ci, val, a.p, a.q, a.p);
+        };
+        auto set_p = [&](double beta) {
+            if (m.slot >= 0)
+                for (int64_t row = r0 + m.slot; row < r1; row += RPG) {
+                    const int64_t t = row * 6 + m.comp;
+                    a.p[t] = a.z[t] + beta * a.p[t];
+                }
+        };
+        double rz = 0., dummy = 0.;
+        {
+            double rr2 = 0.;
+            precond(rz, rr2);
+            wg_sum2<NW>(rz, dummy, red, buf);
+        }
+        set_p(0.);
+        // stagnation watch (mixed mode): the true r.r must halve within
+        // every 20 true-residual evaluations, else the FP32 operator has
+        // reached its accuracy for this system
+        double mark = rr;
+        int since = 0;
+        while (its < a.max_iter && isfinite(rr)) {
+            bool restart = false;
+            if (rr <= tol2) {
+                if (rr_true) break;
+                // the recursive residual claims convergence: check it
+                true_residual_part<RPG>(m, r0, r1, a, rr, bscratch);
+                wg_sum2<NW>(rr, bscratch, red, buf);
+                replaced++;
+                rr_true = true;
+                if (rr <= tol2) break;
+                restart = true;
             }
-            outer++;
-            true_residual(m, r0, r1, a, rr, bb, red, buf);
-            if (!isfinite(rr)) break;
+            if (rr_true) {
+                if (rr < best) best = rr;
+                bool switched = false;
+                if (MIXED && !use64) {
+                    if (rr < 0.5 * mark) {
+                        mark = rr;
+                        since = 0;
+                    } else if (++since >= 20) {
+                        use64 = true;  // finish this system with the FP64 operator
+                        switched = restart = true;
+                    }
+                }
+                if (restart) {
+                    // new direction from the true residual; beta = 0 (a plain
+                    // restart) when the operator changed or the residual rose
+                    // far above the best one (lost conjugacy)
+                    double rz_new = 0., rr2 = 0.;
+                    precond(rz_new, rr2);
+                    wg_sum2<NW>(rz_new, dummy, red, buf);
+                    const double beta = (switched || rr > 1e4 * best) ? 0. : rz_new / rz;
+                    rz = rz_new;
+                    set_p(beta);
+                }
+            }
+            __syncthreads();  // p complete before the product reads it
+            double pq = product();
+            wg_sum2<NW>(pq, dummy, red, buf);
+            if (!(pq > 0.) || !isfinite(pq)) break;
+            const double alpha = rz / pq;
+            if (m.slot >= 0)
+                for (int64_t row = r0 + m.slot; row < r1; row += RPG) {
+                    const int64_t t = row * 6 + m.comp;
+                    a.x[t] += alpha * a.p[t];
+                    a.r[t] -= alpha * a.q[t];
+                }
+            its++;
+            rr_true = false;
+            if (MIXED && !use64 && a.replace > 0 && its % a.replace == 0) {
+                // reliable update: r <- b - K64 x (x changed by other threads)
+                true_residual_part<RPG>(m, r0, r1, a, rr, bscratch);
+                replaced++;
+                rr_true = true;
+            }
+            double rz_new = 0.;
+            rr = 0.;
+            precond(rz_new, rr);
+            wg_sum2<NW>(rz_new, rr, red, buf);
+            const double beta = (rr_true && rr > 1e4 * best) ? 0. : rz_new / rz;
+            rz = rz_new;
+            set_p(beta);
+        }
+        if (!rr_true) {
+            true_residual_part<RPG>(m, r0, r1, a, rr, bscratch);
+            wg_sum2<NW>(rr, bscratch, red, buf);
+            replaced++;
         }
         if (threadIdx.x == 0) {
             a.iters[v] = its;
-            a.outer[v] = outer;
+            a.outer[v] = replaced;
             a.relres[v] = sqrt(rr / (bb > 0. ? bb : 1.));
         }
+        __syncthreads();
     }
+}
+
+template <int BT>
+void launch_batch(bool mixed, int groups, hipStream_t s, const BatchArgs &a)
+{
+    if (mixed)
+        hipLaunchKernelGGL((batch_solve<BT, true>), dim3(groups), dim3(BT), 0, s, a);
+    else
+        hipLaunchKernelGGL((batch_solve<BT, false>), dim3(groups), dim3(BT), 0, s, a);
 }
 
 }  // namespace
@@ -299,16 +350,16 @@ extern "C" {
 int pf3d_batch_bytes(int64_t nb, int64_t *bytes)
 {
     if (nb <= 0 || !bytes) return pf3d_fail(PF3D_E_ARG, "pf3d_batch_bytes: bad arguments");
-    *bytes = (int64_t)(5 * 6 * nb * sizeof(double)) + 256;
+    *bytes = (int64_t)(4 * 6 * nb * sizeof(double)) + 256;
     return PF3D_OK;
 }
 
 int pf3d_batch_solve(int64_t n_var, int64_t nb, const int64_t *row0, const int32_t *order,
                      const int32_t *row_ptr, const int32_t *col_idx, const double *val64,
                      const double *dinv64, const float *val32, const float *dinv32,
-                     const double *b, double *x, int x0, double rtol, double inner_rtol,
-                     int max_iter, int max_outer, void *work, int64_t work_bytes,
-                     int32_t *iters, int32_t *outer, double *relres, int n_groups, void *stream)
+                     const double *b, double *x, int x0, double rtol, int replace, int max_iter,
+                     void *work, int64_t work_bytes, int32_t *iters, int32_t *outer,
+                     double *relres, int n_groups, int threads, void *stream)
 {
     if (n_var < 0 || nb < 0) return pf3d_fail(PF3D_E_ARG, "pf3d_batch_solve: negative size");
     if (n_var == 0) return PF3D_OK;
@@ -318,8 +369,11 @@ int pf3d_batch_solve(int64_t n_var, int64_t nb, const int64_t *row0, const int32
     const bool mixed = val32 != nullptr;
     if (mixed && !dinv32) return pf3d_fail(PF3D_E_ARG, "mixed mode needs dinv32");
     if (!mixed && !dinv64) return pf3d_fail(PF3D_E_ARG, "plain mode needs dinv64");
-    if (!(rtol > 0.) || !(inner_rtol > 0.) || max_iter < 0 || max_outer < 1)
+    if (!(rtol > 0.) || max_iter < 0 || replace < 0)
         return pf3d_fail(PF3D_E_ARG, "pf3d_batch_solve: bad tolerance or limits");
+    if (threads == 0) threads = 1024;
+    if (threads != 256 && threads != 512 && threads != 1024)
+        return pf3d_fail(PF3D_E_ARG, "pf3d_batch_solve: threads must be 256, 512 or 1024");
     int64_t need;
     pf3d_batch_bytes(nb, &need);
     if (work_bytes < need)
@@ -328,20 +382,19 @@ int pf3d_batch_solve(int64_t n_var, int64_t nb, const int64_t *row0, const int32
     BatchArgs a{};
     a.n_var = n_var; a.row0 = row0; a.order = order; a.rp = row_ptr; a.ci = col_idx;
     a.val64 = val64; a.dinv64 = dinv64; a.val32 = val32; a.dinv32 = dinv32; a.b = b; a.x = x;
-    a.x0 = x0; a.rtol = rtol; a.inner_rtol = inner_rtol; a.max_iter = max_iter;
-    a.max_outer = max_outer; a.iters = iters; a.outer = outer; a.relres = relres;
+    a.x0 = x0; a.rtol = rtol; a.replace = replace; a.max_iter = max_iter;
+    a.iters = iters; a.outer = outer; a.relres = relres;
     const int64_t n = 6 * nb;
     double *w = (double *)work;
-    a.r = w; a.z = w + n; a.p = w + 2 * n; a.q = w + 3 * n; a.d = w + 4 * n;
-    a.counter = (int *)(w + 5 * n);
+    a.r = w; a.z = w + n; a.p = w + 2 * n; a.q = w + 3 * n;
+    a.counter = (int *)(w + 4 * n);
     hipStream_t s = (hipStream_t)stream;
-    int groups = n_groups > 0 ? n_groups : 2048;
+    int groups = n_groups > 0 ? n_groups : 256 * (1024 / threads);
     if (groups > n_var) groups = (int)n_var;
     hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, s, a.counter);
-    if (mixed)
-        hipLaunchKernelGGL(batch_solve<true>, dim3(groups), dim3(BT), 0, s, a);
-    else
-        hipLaunchKernelGGL(batch_solve<false>, dim3(groups), dim3(BT), 0, s, a);
+    if (threads == 256) launch_batch<256>(mixed, groups, s, a);
+    else if (threads == 512) launch_batch<512>(mixed, groups, s, a);
+    else launch_batch<1024>(mixed, groups, s, a);
     return pf3d_hip_check(hipGetLastError(), "pf3d_batch_solve");
 }
 

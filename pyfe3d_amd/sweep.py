@@ -45,6 +45,14 @@ def smooth_field(nxe, nye, a, b, corr, rng):
     return (g - g.mean()) / (sd if sd > 0 else 1.)
 
 
+def _central(ne):
+    """[lo, hi) of the interior node indices within the central 20% (at
+    least one node) of a line of ne elements."""
+    lo = max(1, int(np.floor(0.4 * ne)))
+    hi = max(lo, min(ne - 1, int(np.ceil(0.6 * ne))))
+    return lo, hi + 1
+
+
 class PlateSweep:
     """n_var clamped Quad4 plates a x b with (nxe, nye) elements each."""
 
@@ -91,7 +99,7 @@ class PlateSweep:
         nbytes = ctypes.c_int64()
         _lib.call('pf3d_batch_bytes', self.n_nodes, ctypes.byref(nbytes))
         self.work = torch.empty(nbytes.value, dtype=torch.uint8, device=dev)
-        self.b = torch.zeros(6 * self.n_nodes, dtype=torch.float64, device=dev)
+        self.f = torch.zeros(6 * self.n_nodes, dtype=torch.float64, device=dev)
         self.u = torch.zeros(6 * self.n_nodes, dtype=torch.float64, device=dev)
         self.iters = torch.zeros(self.n_var, dtype=torch.int32, device=dev)
         self.outer = torch.zeros(self.n_var, dtype=torch.int32, device=dev)
@@ -112,8 +120,8 @@ class PlateSweep:
             g = smooth_field(nxe, nye, self.a, self.b, corr * self.a, rng)
             # element order of PlateMesh: i (x) major, j (y) minor
             h[self.elem0[v]:self.elem0[v + 1]] = np.maximum(h0 + dh * g.ravel(), hmin)
-            i = rng.integers(int(np.ceil(0.4 * nxe)), int(np.floor(0.6 * nxe)) + 1)
-            j = rng.integers(int(np.ceil(0.4 * nye)), int(np.floor(0.6 * nye)) + 1)
+            i = rng.integers(*_central(nxe))
+            j = rng.integers(*_central(nye))
             node = self.node0[v] + i * (nye + 1) + j
             f[6 * node + 2] = -1000.
         return h, f
@@ -125,8 +133,8 @@ class PlateSweep:
 
     def set_loads(self, f):
         """Nodal forces per DOF; fixed DOFs are zeroed (P f)."""
-        self.b.copy_(torch.as_tensor(f, dtype=torch.float64))
-        self.b.mul_(1 - self.fixed.to(torch.float64))
+        self.f.copy_(torch.as_tensor(f, dtype=torch.float64))
+        self.f.mul_(1 - self.fixed.to(torch.float64))
 
     # ------------------------------------------------------------- the step
     def assemble(self, stream=None):
@@ -145,9 +153,11 @@ class PlateSweep:
                   self.dinv.data_ptr(), self.dinv32.data_ptr(), self.singular.data_ptr(), s)
         _lib.call('pf3d_to_f32', self.val.numel(), self.val.data_ptr(), self.val32.data_ptr(), s)
 
-    def solve(self, mode='mixed', rtol=1e-8, inner_rtol=1e-3, max_iter=100000, max_outer=50,
-              x0=False, n_groups=0, stream=None):
-        """Solve every variant into self.u (x0: start from the current u)."""
+    def solve(self, mode='mixed', rtol=1e-8, replace=50, max_iter=200000, x0=False,
+              n_groups=0, threads=0, stream=None):
+        """Solve every variant into self.u (x0: start from the current u).
+        mode 'fp64': FP64 operator; 'mixed': FP32 operator and inverses with
+        an FP64 true-residual replacement every ``replace`` iterations."""
         mixed = mode == 'mixed'
         if mode not in ('mixed', 'fp64'):
             raise ValueError(mode)
@@ -155,11 +165,11 @@ class PlateSweep:
                   self.order.data_ptr(), self.plan.row_ptr.data_ptr(),
                   self.plan.col_idx.data_ptr(), self.val.data_ptr(), self.dinv.data_ptr(),
                   self.val32.data_ptr() if mixed else None,
-                  self.dinv32.data_ptr() if mixed else None, self.b.data_ptr(),
-                  self.u.data_ptr(), 1 if x0 else 0, float(rtol), float(inner_rtol),
-                  int(max_iter), int(max_outer), self.work.data_ptr(), self.work.numel(),
-                  self.iters.data_ptr(), self.outer.data_ptr(), self.relres.data_ptr(),
-                  int(n_groups), _s(stream))
+                  self.dinv32.data_ptr() if mixed else None, self.f.data_ptr(),
+                  self.u.data_ptr(), 1 if x0 else 0, float(rtol), int(replace), int(max_iter),
+                  self.work.data_ptr(), self.work.numel(), self.iters.data_ptr(),
+                  self.outer.data_ptr(), self.relres.data_ptr(), int(n_groups), int(threads),
+                  _s(stream))
 
     def step(self, mode='mixed', rtol=1e-8, stream=None, **kw):
         self.assemble(stream=stream)

@@ -15,8 +15,6 @@ bfloat16 GEMM on the matrix cores with FP32 accumulation, up-sampling,
 losses and Adam); torch only holds the buffers.  Training data are solves
 of the plain GPU solver (:mod:`pyfe3d_amd.sweep`).
 """
-import ctypes
-
 import numpy as np
 import torch
 
@@ -307,19 +305,20 @@ def solve_plates(h, f, mode='fp64', rtol=1e-8, sweep=None, x0=None):
     return sweep.u.view(V, n + 1, n + 1, 6), sweep
 
 
-def make_dataset(n_samples, seed, n=64, chunk=1024, log=None):
-    """Plates, their GPU solutions and the network inputs/targets (float32
+def make_dataset(n_samples, seed, n=64, chunk=1024, rtol=1e-6, log=None):
+    """Plates, their GPU solutions (to rtol: the network's own error is
+    orders of magnitude above it) and the network inputs/targets (float32
     tensors on the GPU; targets normalised with per-channel std)."""
     X, U, PS = [], [], []
     sweep = None
     for c0 in range(0, n_samples, chunk):
         V = min(chunk, n_samples - c0)
         h, p, f = _plates(V, n, seed * 1000003 + c0)
-        u, sweep = solve_plates(h, f, sweep=sweep if sweep is not None and sweep.n_var == V
-                                else None)
+        u, sweep = solve_plates(h, f, rtol=rtol,
+                                sweep=sweep if sweep is not None and sweep.n_var == V else None)
         torch.cuda.synchronize()
         sweep.check()
-        if float(sweep.relres.max()) > 1e-8:
+        if float(sweep.relres.max()) > rtol:
             raise RuntimeError('training solve missed the tolerance')
         x, ps = features(h, p)
         X.append(torch.as_tensor(x, device='cuda'))
@@ -381,10 +380,12 @@ def predict(net, h, p, sc, batch=64):
 
 
 def galerkin_scale(sweep, u0):
-    """Per plate alpha = u0.f / u0.K u0 (K after the Dirichlet rows/columns,
-    u0 with fixed DOFs zeroed); plates where the scaled start is not better
-    than zero (alpha <= 0, or ||f - alpha K u0|| >= ||f||) start from zero.
-    Returns (scaled u0, number of plates that fell back to zero)."""
+    """Per plate alpha = u0.f / u0.K u0 (K after the Dirichlet rows and
+    columns, u0 with its fixed DOFs zeroed): the multiple of u0 closest to
+    the solution in the energy norm, whose error ||u - alpha u0||_K^2 =
+    ||u||_K^2 - (u0.f)^2 / u0.K u0 never exceeds that of a zero start (the
+    norm in which CG converges).  Plates with alpha <= 0 (or no finite
+    alpha) start from zero.  Returns (scaled u0, number of zero starts)."""
     V = sweep.n_var
     u = u0.reshape(-1).clone()
     u.mul_(1 - sweep.fixed.to(torch.float64))
@@ -392,14 +393,93 @@ def galerkin_scale(sweep, u0):
     _lib.call('pf3d_bsr_spmv', sweep.n_nodes, sweep.plan.row_ptr.data_ptr(),
               sweep.plan.col_idx.data_ptr(), sweep.val.data_ptr(), u.data_ptr(), Ku.data_ptr(),
               torch.cuda.current_stream().cuda_stream)
-    uv, kv, fv = u.view(V, -1), Ku.view(V, -1), sweep.b.view(V, -1)
+    uv, kv, fv = u.view(V, -1), Ku.view(V, -1), sweep.f.view(V, -1)
     uf = (uv * fv).sum(dim=1)
     uku = (uv * kv).sum(dim=1)
-    alpha = torch.where(uku > 0, uf / torch.where(uku > 0, uku, torch.ones_like(uku)),
+    ok = (uku > 0) & (uf > 0) & torch.isfinite(uf) & torch.isfinite(uku)
+    alpha = torch.where(ok, uf / torch.where(ok, uku, torch.ones_like(uku)),
                         torch.zeros_like(uku))
-    ff = (fv * fv).sum(dim=1)
-    r2 = ff - 2 * alpha * uf + alpha * alpha * (kv * kv).sum(dim=1)
-    ok = (alpha > 0) & (r2 < ff)
-    alpha = torch.where(ok, alpha, torch.zeros_like(alpha))
     uv.mul_(alpha[:, None])
     return u, int((~ok).sum())
+
+
+def save(net, sc, path):
+    """Parameters and target scales as an .npz (no pickled objects)."""
+    arrs = {k: v for k, v in net.params_numpy().items()}
+    arrs['scales.std'] = np.asarray(sc.std)
+    arrs['scales.s_ref'] = np.asarray([sc.s_ref])
+    np.savez(path, **arrs)
+
+
+def load(path, batch=64, device='cuda'):
+    z = np.load(path, allow_pickle=False)
+    net = WarmStartNet(n=64, batch=batch, device=device)
+    for k in net.p:
+        net.p[k].copy_(torch.as_tensor(z[k], dtype=torch.float32))
+    sc = Scales(std=z['scales.std'])
+    sc.s_ref = float(z['scales.s_ref'][0])
+    return net, sc
+
+
+def evaluate(net, sc, n_eval, seed, max_iter=60000, log=None):
+    """Held-out plates: batched solves to 1e-8 from zero and from the
+    network's (Galerkin-scaled) guess, FP64 PCG and FP32/FP64 refinement.
+    Times are GPU event times of the whole batch (network inference and
+    scaling included in the warm-start figures)."""
+    from .sweep import PlateSweep
+    n = net.n
+    h, p, f = _plates(n_eval, n, seed)
+    sw = PlateSweep([(n, n)] * n_eval, a=1.0, b=1.0)
+    sw.set_thickness(h.reshape(-1))
+    fd = np.zeros((n_eval, (n + 1) * (n + 1), 6))
+    fd[:, :, 2] = f.reshape(n_eval, -1)
+    sw.set_loads(fd.reshape(-1))
+    sw.assemble()
+    torch.cuda.synchronize()
+    sw.check()
+    out = dict(plates=n_eval)
+
+    def timed(fn):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        r = fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1), r
+
+    if log:
+        log('evaluate: %d plates assembled' % n_eval)
+    for mode in ('fp64', 'mixed'):
+        ms0, _ = timed(lambda: sw.solve(mode=mode, rtol=1e-8, max_iter=max_iter))
+        if log:
+            log('%s zero start: %.1f ms' % (mode, ms0))
+        it0 = sw.iters.cpu().numpy().copy()
+        rel0 = float(sw.relres.max())
+        conv0 = int((sw.relres <= 1e-8).sum())
+
+        def warm():
+            u0 = predict(net, h, p, sc)
+            us, fallback = galerkin_scale(sw, u0)
+            sw.u.copy_(us)
+            sw.solve(mode=mode, rtol=1e-8, x0=True, max_iter=max_iter)
+            return fallback
+        ms1, fallback = timed(warm)
+        it1 = sw.iters.cpu().numpy().copy()
+        rel1 = float(sw.relres.max())
+        conv1 = int((sw.relres <= 1e-8).sum())
+        ms_net, _ = timed(lambda: predict(net, h, p, sc))
+        out[mode] = dict(zero_ms=ms0, zero_solves_per_s=n_eval / (ms0 * 1e-3),
+                         zero_iters_mean=float(it0.mean()), zero_max_relres=rel0,
+                         zero_converged=conv0, warm_converged=conv1,
+                         zero_iters_median=float(np.median(it0)),
+                         warm_iters_median=float(np.median(it1)),
+                         warm_ms=ms1, warm_solves_per_s=n_eval / (ms1 * 1e-3),
+                         warm_iters_mean=float(it1.mean()), warm_max_relres=rel1,
+                         network_ms=ms_net, fallback=fallback,
+                         iteration_ratio=float(it0.mean() / max(1e-9, it1.mean())))
+        if log:
+            log('%s: zero %.1f ms (%.0f its), warm %.1f ms (%.0f its, net %.1f ms, %d fallback)'
+                % (mode, ms0, it0.mean(), ms1, it1.mean(), ms_net, fallback))
+    out['mixed_warm_vs_fp64_zero'] = out['fp64']['zero_ms'] / out['mixed']['warm_ms']
+    return out

@@ -67,19 +67,20 @@ def test_im2col_col2im_up2_node_avg_exact():
         assert np.array_equal(col.cpu().numpy(), N.im2col(x, s))
         d = rng.normal(size=col.shape).astype(np.float32)
         dx = torch.empty_like(tx)
-        _call('pf3d_nn_col2im', 2, 8, 8, 5, s, torch.as_tensor(d, device=DEV).data_ptr(),
-              dx.data_ptr(), 0)
+        td = torch.as_tensor(d, device=DEV)
+        _call('pf3d_nn_col2im', 2, 8, 8, 5, s, td.data_ptr(), dx.data_ptr(), 0)
         np.testing.assert_allclose(dx.cpu().numpy(), N.col2im(d.astype(np.float64), x.shape, s),
                                    rtol=1e-6, atol=1e-6)
     lo = rng.normal(size=(2, 4, 4, 3)).astype(np.float32)
     sk = rng.normal(size=(2, 8, 8, 3)).astype(np.float32)
     y = torch.empty((2, 8, 8, 3), dtype=torch.float32, device=DEV)
-    _call('pf3d_nn_up2_add', 2, 8, 8, 3, torch.as_tensor(lo, device=DEV).data_ptr(),
-          torch.as_tensor(sk, device=DEV).data_ptr(), y.data_ptr())
+    tlo, tsk = torch.as_tensor(lo, device=DEV), torch.as_tensor(sk, device=DEV)
+    _call('pf3d_nn_up2_add', 2, 8, 8, 3, tlo.data_ptr(), tsk.data_ptr(), y.data_ptr())
     assert np.array_equal(y.cpu().numpy(), (N.up2(lo, 8, 8) + sk).astype(np.float32))
     e = rng.normal(size=(2, 6, 6, 3)).astype(np.float32)
     u = torch.empty((2, 7, 7, 3), dtype=torch.float32, device=DEV)
-    _call('pf3d_nn_node_avg', 2, 6, 3, torch.as_tensor(e, device=DEV).data_ptr(), u.data_ptr())
+    te = torch.as_tensor(e, device=DEV)
+    _call('pf3d_nn_node_avg', 2, 6, 3, te.data_ptr(), u.data_ptr())
     np.testing.assert_allclose(u.cpu().numpy(), N.node_avg(e.astype(np.float64)), rtol=1e-6,
                                atol=1e-7)
 

@@ -41,10 +41,13 @@ def test_block_plan_bit_identical_to_slot_plan(etype):
     assert bp.nnzb == sp.nnzb
     assert torch.equal(bp.row_ptr, sp.row_ptr) and torch.equal(bp.col_idx, sp.col_idx)
     assert torch.equal(bp.apply(v), sp.apply(v))
-    # accumulate mode adds a second copy
-    out = bp.apply(v)
+    # accumulate mode: from zeros it is the plain apply; from a base it adds
+    out = torch.zeros(bp.nnzb * 36, dtype=torch.float64, device=DEV)
     bp.apply(v, out=out, accumulate=True)
-    assert torch.equal(out, 2 * sp.apply(v))
+    assert torch.equal(out, sp.apply(v))
+    bp.apply(v, out=out, accumulate=True)
+    ref = sp.apply(v)
+    assert (out - 2 * ref).abs().max() <= 1e-14 * ref.abs().max()
 
 
 def test_isotropic_records_match_host():
@@ -54,8 +57,9 @@ def test_isotropic_records_match_host():
     ref = isotropic_records(h, 70e9, 0.33, rho=2700.)
     out = torch.empty((h.size, 32), dtype=torch.float64, device=DEV)
     unit = np.ascontiguousarray(isotropic_plate(1.0, 70e9, 0.33, rho=2700.).record())
-    _lib.call('pf3d_isotropic_records', h.size, T(h, torch.float64).data_ptr(), unit.ctypes.data,
-              out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    th = T(h, torch.float64)
+    _lib.call('pf3d_isotropic_records', h.size, th.data_ptr(), unit.ctypes.data, out.data_ptr(),
+              torch.cuda.current_stream().cuda_stream)
     got = out.cpu().numpy()
     np.testing.assert_allclose(got, ref, rtol=1e-14, atol=0)
     # and the thickness-h plate computed directly (shellprop_utils.py:182-206)
@@ -99,13 +103,13 @@ def _variant_reference(sw, h, f, v):
 def test_sweep_matches_spsolve_per_variant(mode):
     sw, h, f = _sweep()
     sw.assemble()
-    sw.solve(mode=mode, rtol=1e-12)
+    sw.solve(mode=mode, rtol=1e-10)
     torch.cuda.synchronize()
     sw.check()
     relres = sw.relres.cpu().numpy()
     iters = sw.iters.cpu().numpy()
     print(mode, 'iters', iters, 'relres', relres)
-    assert (relres <= 1e-12).all()
+    assert (relres <= 1e-10).all()
     assert (iters > 0).all()
     u = sw.u.cpu().numpy()
     for v in range(sw.n_var):
