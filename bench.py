@@ -26,6 +26,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the values-only Quad4 KC0 kernel on the 1000x1000
+# plate, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of
+# `python bench.py --profile-kc0` (KiB per dispatch), each scaled by the
+# reading of the same counter for a 1 GiB device copy in the same run
+# (FETCH_SIZE reads 0.5 GiB for it: factor 2.0; WRITE_SIZE 1 GiB: 1.0):
+# 60,052 KiB x 2 + 4,500,076 KiB = 4.731e9 B (algorithmic: 4.736e9 B).
+KC0_TRAFFIC_BYTES = (60052 * 2 + 4500076) * 1024
 # algorithmic bytes of one Quad4 KC0 element, values only: 576 doubles out,
 # 4 int64 DOF offsets and 12 coordinates in
 KC0_BYTES_VALUES = 576 * 8 + 4 * 8 + 12 * 8
@@ -166,6 +173,40 @@ def sweep_rates(n_var, log, steps=2):
     return out
 
 
+def profile_kc0(n):
+    """Three values-only KC0 launches on the n x n plate, then three copies
+    of 1 GiB (the calibration of the HBM byte counters)."""
+    import torch
+    from pyfe3d_amd import assembly as A
+    mesh = build_plate(n)
+    prop, _ = laminate_record()
+    E = mesh.quads.shape[0]
+    x = torch.as_tensor(mesh.x, device='cuda')
+    conn = torch.as_tensor(6 * mesh.quads, device='cuda')
+    props = torch.as_tensor(prop.record()[None], device='cuda')
+    v = torch.empty(576 * E, dtype=torch.float64, device='cuda')
+    for _ in range(3):
+        A.shell_kc0('quad4', x, conn, None, None, v, props, values_only=True, overwrite=True,
+                    check=False)
+    torch.cuda.synchronize()
+    src = torch.ones(1 << 27, dtype=torch.float64, device='cuda')
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print('profile-kc0: %d elements, 3 KC0 launches, 3 x 1 GiB copies' % E, flush=True)
+
+
+def warm_start_rates(n_plates, log):
+    """Held-out 64 x 64 plates (smooth thickness fields, Gaussian pressure
+    patches): batched PCG to 1e-8 from zero and from the trained network's
+    Galerkin-scaled guess."""
+    from pyfe3d_amd import warmstart as W
+    path = os.path.join(ROOT, 'pyfe3d_amd', 'warmstart_weights.npz')
+    net, sc = W.load(path)
+    return W.evaluate(net, sc, n_plates, seed=12345, max_iter=40000, log=log)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -174,7 +215,10 @@ def main():
     ap.add_argument('--n', type=int, default=1000, help='elements per side')
     ap.add_argument('--cpu-sample', type=int, default=200000)
     ap.add_argument('--no-solve', action='store_true')
-    ap.add_argument('--sweep-variants', type=int, default=4096)
+    ap.add_argument('--sweep-variants', type=int, default=1024)
+    ap.add_argument('--warm-start-plates', type=int, default=64)
+    ap.add_argument('--profile-kc0', action='store_true',
+                    help='only: 3 KC0 launches and a 1 GiB device copy (counter runs)')
     ap.add_argument('--quiet', action='store_true')
     args = ap.parse_args()
     log = (lambda *a: None) if args.quiet else (lambda *a: print(*a, file=sys.stderr, flush=True))
@@ -187,6 +231,9 @@ def main():
     from pyfe3d_amd import assembly as A
     dev = 'cuda'
     stream = torch.cuda.current_stream()
+    if args.profile_kc0:
+        profile_kc0(args.n)
+        return
 
     t0 = time.perf_counter()
     mesh = build_plate(args.n)
@@ -261,6 +308,14 @@ def main():
             extra['design_sweep'] = dict(error=str(exc))
             log('sweep failed: %s' % exc)
 
+    if not args.no_solve and args.warm_start_plates > 0:
+        torch.cuda.empty_cache()
+        try:
+            extra['warm_start_64x64'] = warm_start_rates(args.warm_start_plates, log)
+        except Exception as exc:
+            extra['warm_start_64x64'] = dict(error=str(exc))
+            log('warm start failed: %s' % exc)
+
     n_sample = min(args.cpu_sample, E)
     base = cpu_baseline(mesh, prop, n_sample)
     log('cpu baseline %.0f elements/s' % base['value'])
@@ -283,7 +338,9 @@ def main():
                    'elements': E, 'nodes': mesh.n_nodes, 'dof': mesh.n_dof,
                    'slots': 576 * E},
         'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
-                     'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
+                     'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
+                     'traffic': KC0_TRAFFIC_BYTES if args.n == 1000 else None,
+                     'algorithmic_bytes': KC0_BYTES_VALUES * E,
                      'bytes_per_element': KC0_BYTES_VALUES},
         'cpu_baseline': base,
         'extra': extra,

@@ -2,13 +2,13 @@
 
 A small convolutional encoder-decoder (about 0.35M parameters, 4 levels,
 32-128 channels) maps a plate's per-element thickness field and pressure
-map (64 x 64 elements) to the six nodal displacements of the 65 x 65 node
-grid.  Its prediction is scaled by a one-dimensional Galerkin projection
-(alpha = u0.f / u0.K u0, one sparse product) and handed to the PCG as x0;
-if that start is not better than zero (alpha <= 0 or ||f - K u0|| >= ||f||)
-the solve starts from zero.  The solve itself, and its FP64 residual
-criterion, are unchanged: the network can only change how many iterations
-are needed.
+map (64 x 64 elements) to the shape of its deflection on the 65 x 65 node
+grid.  The rotations are taken from that deflection (Kirchhoff: rx = w,y,
+ry = -w,x), the amplitude from a one-dimensional Galerkin projection
+(alpha = u0.f / u0.K u0, one sparse product), and the result is the PCG's
+x0; a plate whose alpha is not positive starts from zero.  The solve
+itself, and its FP64 residual criterion, are unchanged: the network can only
+change how many iterations are needed.
 
 Everything runs through the library's own HIP kernels (im2col / col2im,
 bfloat16 GEMM on the matrix cores with FP32 accumulation, up-sampling,
@@ -267,11 +267,18 @@ def features(h, p, h0=0.002, dh=0.001):
 # ----------------------------------------------------------------------
 
 class Scales:
-    """Target normalisation: u_norm[c] = u[c] / (max|p| * s_ref * std[c])."""
+    """Target normalisation.  The network learns the shape of the
+    deflection w only: each training target is w / rms(w) of its plate
+    (then divided by the dataset std), because the amplitude comes from the
+    Galerkin projection at solve time and the rotations from w (Kirchhoff:
+    for thin plates independent rotation guesses carry shear energy that
+    is (L/h)^2 times their bending energy)."""
 
-    def __init__(self, E=70e9, a=1.0, h0=0.002, std=None):
-        self.s_ref = a ** 4 / (E * h0 ** 3)
+    def __init__(self, std=None):
         self.std = np.ones(6) if std is None else np.asarray(std, dtype=np.float64)
+
+
+W_CHANNEL = 2  # w among (u, v, w, rx, ry, rz)
 
 
 def _plates(n_var, n, seed, a=1.0, b=1.0):
@@ -328,20 +335,21 @@ def make_dataset(n_samples, seed, n=64, chunk=1024, rtol=1e-6, log=None):
             log('dataset %d/%d' % (c0 + V, n_samples))
     X = torch.cat(X)
     U = torch.cat(U)
-    ps = np.concatenate(PS)
-    sc = Scales()
-    norm = torch.as_tensor(ps * sc.s_ref, dtype=torch.float32, device='cuda')
-    T = U / norm[:, None, None, None]
+    w = U[:, 1:-1, 1:-1, W_CHANNEL]
+    rms = w.pow(2).mean(dim=(1, 2)).sqrt().clamp_min(1e-30)
+    T = U / rms[:, None, None, None]
     std = T[:, 1:-1, 1:-1].reshape(-1, 6).std(dim=0).double().cpu().numpy()
     std = np.where(std > 0, std, 1.)
-    sc.std = std
+    sc = Scales(std)
     T = T / torch.as_tensor(std, dtype=torch.float32, device='cuda')
     return X, T.contiguous(), sc
 
 
 def train(net, X, T, epochs=10, batch=32, lr=1e-3, seed=0, log=None):
-    """Adam on the standardised MSE; returns the loss history per epoch."""
-    wc = torch.ones(6, dtype=torch.float32, device='cuda')
+    """Adam on the standardised MSE of the deflection channel; returns the
+    loss history per epoch."""
+    wc = torch.zeros(6, dtype=torch.float32, device='cuda')
+    wc[W_CHANNEL] = 1.
     S = X.shape[0]
     g = torch.Generator(device='cpu').manual_seed(seed)
     hist = []
@@ -354,8 +362,8 @@ def train(net, X, T, epochs=10, batch=32, lr=1e-3, seed=0, log=None):
             u = net.forward(xb)
             net.backward(tb, wc)
             net.adam(lr=lr * (0.5 ** (ep // max(1, epochs // 3))))
-            if b0 % (batch * 50) == 0:
-                d = (u - tb)[:, 1:-1, 1:-1]
+            if b0 % (batch * 16) == 0:
+                d = (u - tb)[:, 1:-1, 1:-1, W_CHANNEL]
                 tot += 0.5 * float((d * d).sum()) / (batch * (net.n - 1) ** 2)
                 nb += 1
         hist.append(tot / max(1, nb))
@@ -364,28 +372,50 @@ def train(net, X, T, epochs=10, batch=32, lr=1e-3, seed=0, log=None):
     return hist
 
 
+def kirchhoff_rotations(w, a=1.0, b=1.0):
+    """(V, n+1, n+1) nodal deflections -> (V, n+1, n+1, 6) DOF vectors with
+    rx = w,y and ry = -w,x (zero transverse shear for the element's
+    gyz = w,y - rx, gxz = w,x + ry), central differences inside, one-sided
+    at the edges; u, v, rz = 0."""
+    V, n1 = w.shape[0], w.shape[1]
+    dx, dy = a / (n1 - 1), b / (n1 - 1)
+    wx = torch.empty_like(w)
+    wy = torch.empty_like(w)
+    wx[:, 1:-1] = (w[:, 2:] - w[:, :-2]) / (2 * dx)
+    wx[:, 0] = (w[:, 1] - w[:, 0]) / dx
+    wx[:, -1] = (w[:, -1] - w[:, -2]) / dx
+    wy[:, :, 1:-1] = (w[:, :, 2:] - w[:, :, :-2]) / (2 * dy)
+    wy[:, :, 0] = (w[:, :, 1] - w[:, :, 0]) / dy
+    wy[:, :, -1] = (w[:, :, -1] - w[:, :, -2]) / dy
+    u = torch.zeros((V, n1, n1, 6), dtype=w.dtype, device=w.device)
+    u[..., 2] = w
+    u[..., 3] = wy
+    u[..., 4] = -wx
+    return u
+
+
 def predict(net, h, p, sc, batch=64):
-    """Initial guesses (V, n+1, n+1, 6) float64 on the GPU, before the
-    Galerkin scaling."""
-    x, ps = features(h, p)
+    """Initial-guess shapes (V, n+1, n+1, 6) float64 on the GPU: the
+    network's deflection with Kirchhoff rotations, before the Galerkin
+    scaling (which sets the amplitude)."""
+    x, _ = features(h, p)
     V = x.shape[0]
-    out = torch.empty((V, net.n + 1, net.n + 1, 6), dtype=torch.float64, device='cuda')
-    std = torch.as_tensor(sc.std, dtype=torch.float64, device='cuda')
+    w = torch.empty((V, net.n + 1, net.n + 1), dtype=torch.float64, device='cuda')
     for b0 in range(0, V, batch):
         xb = torch.as_tensor(x[b0:b0 + batch], device='cuda')
         u = net.forward(xb)
-        s = torch.as_tensor(ps[b0:b0 + batch] * sc.s_ref, dtype=torch.float64, device='cuda')
-        out[b0:b0 + xb.shape[0]] = u[:xb.shape[0]].double() * std * s[:, None, None, None]
-    return out
+        w[b0:b0 + xb.shape[0]] = u[:xb.shape[0], :, :, W_CHANNEL].double() * sc.std[W_CHANNEL]
+    return kirchhoff_rotations(w)
 
 
-def galerkin_scale(sweep, u0):
+def galerkin_scale(sweep, u0, stats=None):
     """Per plate alpha = u0.f / u0.K u0 (K after the Dirichlet rows and
     columns, u0 with its fixed DOFs zeroed): the multiple of u0 closest to
     the solution in the energy norm, whose error ||u - alpha u0||_K^2 =
     ||u||_K^2 - (u0.f)^2 / u0.K u0 never exceeds that of a zero start (the
     norm in which CG converges).  Plates with alpha <= 0 (or no finite
-    alpha) start from zero.  Returns (scaled u0, number of zero starts)."""
+    alpha) start from zero.  Returns (scaled u0, number of zero starts);
+    ``stats`` (a dict), if given, receives the per-plate u0.f and u0.K u0."""
     V = sweep.n_var
     u = u0.reshape(-1).clone()
     u.mul_(1 - sweep.fixed.to(torch.float64))
@@ -400,14 +430,15 @@ def galerkin_scale(sweep, u0):
     alpha = torch.where(ok, uf / torch.where(ok, uku, torch.ones_like(uku)),
                         torch.zeros_like(uku))
     uv.mul_(alpha[:, None])
+    if stats is not None:
+        stats['uf'], stats['uku'] = uf, uku
     return u, int((~ok).sum())
 
 
 def save(net, sc, path):
     """Parameters and target scales as an .npz (no pickled objects)."""
-    arrs = {k: v for k, v in net.params_numpy().items()}
+    arrs = {k: v.astype(np.float32) for k, v in net.params_numpy().items()}
     arrs['scales.std'] = np.asarray(sc.std)
-    arrs['scales.s_ref'] = np.asarray([sc.s_ref])
     np.savez(path, **arrs)
 
 
@@ -417,7 +448,6 @@ def load(path, batch=64, device='cuda'):
     for k in net.p:
         net.p[k].copy_(torch.as_tensor(z[k], dtype=torch.float32))
     sc = Scales(std=z['scales.std'])
-    sc.s_ref = float(z['scales.s_ref'][0])
     return net, sc
 
 
@@ -458,13 +488,22 @@ def evaluate(net, sc, n_eval, seed, max_iter=60000, log=None):
         rel0 = float(sw.relres.max())
         conv0 = int((sw.relres <= 1e-8).sum())
 
+        # energy norm of the solution, ||u*||_K^2 = u*.f, for the error of
+        # the warm start: ||u* - alpha u0||_K^2 / ||u*||_K^2 = 1 - (u0.f)^2 /
+        # (u0.K u0 * u*.f)
+        usf = (sw.u.view(n_eval, -1) * sw.f.view(n_eval, -1)).sum(dim=1)
+
+        gstats = {}
+
         def warm():
             u0 = predict(net, h, p, sc)
-            us, fallback = galerkin_scale(sw, u0)
+            us, fallback = galerkin_scale(sw, u0, gstats)
             sw.u.copy_(us)
             sw.solve(mode=mode, rtol=1e-8, x0=True, max_iter=max_iter)
             return fallback
         ms1, fallback = timed(warm)
+        uf, uku = gstats['uf'], gstats['uku']
+        e2 = (1 - uf * uf / (uku * usf)).clamp_min(0).sqrt().cpu().numpy()
         it1 = sw.iters.cpu().numpy().copy()
         rel1 = float(sw.relres.max())
         conv1 = int((sw.relres <= 1e-8).sum())
@@ -477,6 +516,8 @@ def evaluate(net, sc, n_eval, seed, max_iter=60000, log=None):
                          warm_ms=ms1, warm_solves_per_s=n_eval / (ms1 * 1e-3),
                          warm_iters_mean=float(it1.mean()), warm_max_relres=rel1,
                          network_ms=ms_net, fallback=fallback,
+                         warm_energy_error_median=float(np.median(e2)),
+                         warm_energy_error_mean=float(e2.mean()),
                          iteration_ratio=float(it0.mean() / max(1e-9, it1.mean())))
         if log:
             log('%s: zero %.1f ms (%.0f its), warm %.1f ms (%.0f its, net %.1f ms, %d fallback)'
