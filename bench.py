@@ -133,7 +133,9 @@ def sweep_dims(n_var, seed=2026):
 def sweep_rates(n_var, log, steps=2):
     """Batched design sweep: n_var clamped isotropic plates with smooth
     seeded thickness fields and a central point load, assembled and solved
-    to 1e-8 (every variant) per step, as one captured graph per mode."""
+    to 1e-8 (every variant) per step, as one captured graph per mode:
+    banded Cholesky per plate ('direct'), FP64 block-Jacobi PCG ('fp64'),
+    PCG reading the operator in FP32 ('mixed')."""
     import torch
     from pyfe3d_amd.sweep import PlateSweep
     t0 = time.perf_counter()
@@ -144,7 +146,7 @@ def sweep_rates(n_var, log, steps=2):
     log('sweep: %d variants, %d elements, %d DOF, %d blocks (setup %.1f s)'
         % (n_var, sw.n_elem, 6 * sw.n_nodes, sw.plan.nnzb, time.perf_counter() - t0))
     out = dict(variants=n_var, elements=sw.n_elem, dof=6 * sw.n_nodes)
-    for mode in ('fp64', 'mixed'):
+    for mode in ('direct', 'fp64', 'mixed'):
         t0 = time.perf_counter()
         sw.step(mode=mode, rtol=1e-8)
         torch.cuda.synchronize()

@@ -294,6 +294,33 @@ int pf3d_batch_solve(int64_t n_var, int64_t nb, const int64_t *row0, const int32
                      double *relres, int n_groups, int threads, void *stream);
 
 /* ------------------------------------------------------------------ */
+/* Batched banded direct solves (the design-sweep counterpart of the     */
+/* callers' scipy spsolve, tests/test_quad4_static_point_load.py:84-102) */
+/* ------------------------------------------------------------------ */
+
+/* Largest bandwidth (scalars below the diagonal) the kernels accept. */
+int pf3d_band_max_width(void);
+
+/* Copy the systems of a block-diagonal BSR matrix (after pf3d_bsr_dirichlet)
+ * into lower band storage and factorise each in place (Cholesky), one
+ * workgroup per system.  System v owns DOFs [dof0[v], dof0[v+1]) of the
+ * permuted numbering; var[node] is the system of each node, pinv[node] its
+ * permuted (global) node index, bw[v] the bandwidth in scalars and
+ * boff[v] the offset of the system's band (dof count x (bw+1) doubles) in
+ * band (band_size doubles in all).  *bad (device int) counts systems with
+ * a non-positive pivot. */
+int pf3d_band_factor(int64_t n_var, int64_t n_nodes, const int32_t *order, const int64_t *dof0,
+                     const int32_t *bw, const int64_t *boff, int64_t band_size, double *band,
+                     const int32_t *row_ptr, const int32_t *col_idx, const double *val,
+                     const int32_t *var, const int64_t *pinv, int *bad, void *stream);
+
+/* x = A^-1 f per system with the factors of pf3d_band_factor; pnode[p] is
+ * the original node of permuted node p; work: one double per DOF. */
+int pf3d_band_solve(int64_t n_var, const int32_t *order, const int64_t *dof0, const int32_t *bw,
+                    const int64_t *boff, const double *band, const int64_t *pnode,
+                    const double *f, double *x, double *work, void *stream);
+
+/* ------------------------------------------------------------------ */
 /* Warm-start network kernels (no reference counterpart: pyfe3d starts   */
 /* every solve from zero).  FP32 tensors, NHWC activations, weights      */
 /* [c_out][(ky, kx, c_in)]; products on the matrix cores in bfloat16     */

@@ -171,9 +171,64 @@ class PlateSweep:
                   self.outer.data_ptr(), self.relres.data_ptr(), int(n_groups), int(threads),
                   _s(stream))
 
+    # ------------------------------------------------------- direct solves
+    def _band_setup(self):
+        """Node permutations (each plate numbered along its longer side, so
+        the band spans the shorter one), bandwidths and band storage."""
+        var, pinv, bws, sizes = [], [], [], []
+        for v, (nxe, nye) in enumerate(self.dims):
+            nx, ny = nxe + 1, nye + 1
+            i, j = np.divmod(np.arange(nx * ny), ny)  # PlateMesh: node = i*ny + j
+            new = i * ny + j if ny <= nx else j * nx + i
+            pinv.append(self.node0[v] + new)
+            var.append(np.full(nx * ny, v, np.int32))
+            bws.append(6 * (min(nx, ny) + 1) + 5)
+            sizes.append(6 * nx * ny * (bws[-1] + 1))
+        bws = np.array(bws, np.int32)
+        maxb = _lib.call('pf3d_band_max_width')
+        if bws.max() > maxb:
+            raise ValueError('plate too wide for the banded solver (bandwidth %d > %d)'
+                             % (bws.max(), maxb))
+        pinv = np.concatenate(pinv)
+        pnode = np.empty_like(pinv)
+        pnode[pinv] = np.arange(pinv.size)
+        dev = self.device
+        self.band_var = torch.as_tensor(np.concatenate(var), device=dev)
+        self.band_pinv = torch.as_tensor(pinv, device=dev)
+        self.band_pnode = torch.as_tensor(pnode, device=dev)
+        self.band_bw = torch.as_tensor(bws, device=dev)
+        boff = np.concatenate([[0], np.cumsum(sizes)])
+        self.band_off = torch.as_tensor(boff[:-1].astype(np.int64), device=dev)
+        self.band_size = int(boff[-1])
+        self.band = torch.empty(self.band_size, dtype=torch.float64, device=dev)
+        self.band_dof0 = torch.as_tensor(6 * self.node0, device=dev)
+        self.band_work = torch.empty(6 * self.n_nodes, dtype=torch.float64, device=dev)
+        self.band_bad = torch.zeros(1, dtype=torch.int32, device=dev)
+
+    def solve_direct(self, rtol=1e-8, polish_iter=2000, stream=None):
+        """Banded Cholesky of every variant (one workgroup each), forward and
+        back substitution, then the FP64 true residual of each variant
+        (PCG polish from the direct solution if it misses rtol)."""
+        if not hasattr(self, 'band'):
+            self._band_setup()
+        s = _s(stream)
+        _lib.call('pf3d_band_factor', self.n_var, self.n_nodes, self.order.data_ptr(),
+                  self.band_dof0.data_ptr(), self.band_bw.data_ptr(), self.band_off.data_ptr(),
+                  self.band_size, self.band.data_ptr(), self.plan.row_ptr.data_ptr(),
+                  self.plan.col_idx.data_ptr(), self.val.data_ptr(), self.band_var.data_ptr(),
+                  self.band_pinv.data_ptr(), self.band_bad.data_ptr(), s)
+        _lib.call('pf3d_band_solve', self.n_var, self.order.data_ptr(),
+                  self.band_dof0.data_ptr(), self.band_bw.data_ptr(), self.band_off.data_ptr(),
+                  self.band.data_ptr(), self.band_pnode.data_ptr(), self.f.data_ptr(),
+                  self.u.data_ptr(), self.band_work.data_ptr(), s)
+        self.solve(mode='fp64', rtol=rtol, max_iter=polish_iter, x0=True, stream=stream)
+
     def step(self, mode='mixed', rtol=1e-8, stream=None, **kw):
         self.assemble(stream=stream)
-        self.solve(mode=mode, rtol=rtol, stream=stream, **kw)
+        if mode == 'direct':
+            self.solve_direct(rtol=rtol, stream=stream, **kw)
+        else:
+            self.solve(mode=mode, rtol=rtol, stream=stream, **kw)
 
     def capture(self, mode='mixed', rtol=1e-8, **kw):
         """Capture :meth:`step` in a graph; replay with ``self.graph.replay()``
@@ -191,9 +246,12 @@ class PlateSweep:
         return g
 
     def check(self):
-        """Raise if any variant missed the tolerance or a block was singular."""
+        """Raise if a diagonal block was singular or a band factorisation
+        met a non-positive pivot."""
         if int(self.singular.item()):
             raise _lib.Pf3dError('%d singular diagonal blocks' % int(self.singular.item()))
+        if hasattr(self, 'band_bad') and int(self.band_bad.item()):
+            raise _lib.Pf3dError('%d band factorisations failed' % int(self.band_bad.item()))
 
     def variant(self, v):
         """(node range, element range) of variant v."""
