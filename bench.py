@@ -110,12 +110,13 @@ def solve_rates(n, repeats, log):
     sysm = StaticSystem(plan, plan.apply(v), mesh.clamped())
     f = mesh.pressure_load(1e3)
     out = {}
-    for method in ('fp64', 'mixed'):
+    for method in ('direct', 'fp64', 'mixed'):
         u, info = sysm.solve(f, method=method, rtol=1e-8)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(repeats):
-            u, info = sysm.solve(f, method=method, rtol=1e-8)
+            # the direct solve is timed with its factorisation
+            u, info = sysm.solve(f, method=method, rtol=1e-8, refactor=True)
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / repeats
         out[method] = dict(solves_per_s=1. / dt, ms=dt * 1e3, iters=info['iters'],
@@ -130,12 +131,14 @@ def sweep_dims(n_var, seed=2026):
     return [tuple(int(t) for t in rng.integers(8, 65, 2)) for _ in range(n_var)]
 
 
-def sweep_rates(n_var, log, steps=2):
+def sweep_rates(n_var, log, steps=2, modes=('direct', 'fp64', 'mixed')):
     """Batched design sweep: n_var clamped isotropic plates with smooth
     seeded thickness fields and a central point load, assembled and solved
     to 1e-8 (every variant) per step, as one captured graph per mode:
     banded Cholesky per plate ('direct'), FP64 block-Jacobi PCG ('fp64'),
-    PCG reading the operator in FP32 ('mixed')."""
+    PCG reading the operator in FP32 ('mixed').  A step is the whole
+    captured graph: property records, KC0, BSR, Dirichlet, block Jacobi,
+    FP32 copies and the solve."""
     import torch
     from pyfe3d_amd.sweep import PlateSweep
     t0 = time.perf_counter()
@@ -146,13 +149,12 @@ def sweep_rates(n_var, log, steps=2):
     log('sweep: %d variants, %d elements, %d DOF, %d blocks (setup %.1f s)'
         % (n_var, sw.n_elem, 6 * sw.n_nodes, sw.plan.nnzb, time.perf_counter() - t0))
     out = dict(variants=n_var, elements=sw.n_elem, dof=6 * sw.n_nodes)
-    for mode in ('direct', 'fp64', 'mixed'):
+    for mode in modes:
         t0 = time.perf_counter()
-        sw.step(mode=mode, rtol=1e-8)
+        g = sw.capture(mode=mode, rtol=1e-8)  # runs one eager step first
         torch.cuda.synchronize()
         first = time.perf_counter() - t0
         sw.check()
-        g = sw.capture(mode=mode, rtol=1e-8)
         ev0 = torch.cuda.Event(enable_timing=True)
         ev1 = torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
@@ -167,7 +169,7 @@ def sweep_rates(n_var, log, steps=2):
         out[mode] = dict(ms_per_sweep=ms, solves_per_s=n_var / (ms * 1e-3),
                          max_relres=float(rel.max()), converged=int((rel <= 1e-8).sum()),
                          pcg_iters_total=int(its.sum()), pcg_iters_max=int(its.max()),
-                         outer_max=int(sw.outer.cpu().numpy().max()), first_step_s=first)
+                         first_step_s=first)
         log('sweep %s: %.1f ms/sweep, %.1f solves/s, max relres %.2e, iters %d'
             % (mode, ms, n_var / (ms * 1e-3), rel.max(), its.sum()))
         del g
@@ -217,7 +219,10 @@ def main():
     ap.add_argument('--n', type=int, default=1000, help='elements per side')
     ap.add_argument('--cpu-sample', type=int, default=200000)
     ap.add_argument('--no-solve', action='store_true')
-    ap.add_argument('--sweep-variants', type=int, default=1024)
+    ap.add_argument('--sweep-variants', type=int, default=4096,
+                    help='plates of the design sweep solved with the banded direct solver')
+    ap.add_argument('--sweep-pcg-variants', type=int, default=256,
+                    help='plates of the same sweep solved with PCG (FP64 and FP32 operator)')
     ap.add_argument('--warm-start-plates', type=int, default=64)
     ap.add_argument('--profile-kc0', action='store_true',
                     help='only: 3 KC0 launches and a 1 GiB device copy (counter runs)')
@@ -305,10 +310,18 @@ def main():
         del v
         torch.cuda.empty_cache()
         try:
-            extra['design_sweep'] = sweep_rates(args.sweep_variants, log)
+            extra['design_sweep'] = sweep_rates(args.sweep_variants, log, modes=('direct',))
         except Exception as exc:
             extra['design_sweep'] = dict(error=str(exc))
             log('sweep failed: %s' % exc)
+        torch.cuda.empty_cache()
+        if args.sweep_pcg_variants > 0:
+            try:
+                extra['design_sweep_pcg'] = sweep_rates(args.sweep_pcg_variants, log, steps=1,
+                                                        modes=('direct', 'fp64', 'mixed'))
+            except Exception as exc:
+                extra['design_sweep_pcg'] = dict(error=str(exc))
+                log('sweep (pcg) failed: %s' % exc)
 
     if not args.no_solve and args.warm_start_plates > 0:
         torch.cuda.empty_cache()

@@ -47,8 +47,9 @@ struct BandSys {
 // ---------------------------------------------------------------- fill
 __global__ void band_zero(int64_t n, double *__restrict__ p)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) p[t] = 0.;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * blockDim.x)
+        p[t] = 0.;
 }
 
 // one thread per (node row, entry of its 6x6 blocks): lower-triangle
@@ -134,15 +135,44 @@ __global__ void __launch_bounds__(BT) band_chol(BandSys s, int *__restrict__ bad
             if (c < jn && r >= c && r - c <= b) L[(j0 + c) * ld + (r - c)] = P[r][c];
         }
         // trailing update: A[j0+ri][j0+rk] -= sum_c P[ri][c] P[rk][c],
-        // jn <= rk <= ri < rows
+        // jn <= rk <= ri < rows, in 4 x 4 tiles per thread (8 LDS reads
+        // per 16 products); consecutive threads take consecutive row tiles
+        // of one column tile, so the global accesses coalesce
         const int T = rows - jn;
-        for (int64_t t = tid; t < (int64_t)T * T; t += BT) {
-            const int rk = jn + (int)(t / T), ri = jn + (int)(t % T);
-            if (ri < rk) continue;
-            double acc = 0.;
+        const int nt = (T + 3) / 4;
+        for (int64_t t = tid; t < (int64_t)nt * nt; t += BT) {
+            const int tk = (int)(t / nt), ti = (int)(t % nt);
+            if (ti < tk) continue;
+            const int rk0 = jn + 4 * tk, ri0 = jn + 4 * ti;
+            double acc[4][4];
 #pragma unroll
-            for (int c = 0; c < NB; c++) acc += P[ri][c] * P[rk][c];
-            L[(j0 + rk) * ld + (ri - rk)] -= acc;
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) acc[a][q] = 0.;
+#pragma unroll 4
+            for (int c = 0; c < NB; c++) {
+                double pk[4], pi[4];
+#pragma unroll
+                for (int a = 0; a < 4; a++) {
+                    pk[a] = rk0 + a < rows ? P[rk0 + a][c] : 0.;
+                    pi[a] = ri0 + a < rows ? P[ri0 + a][c] : 0.;
+                }
+#pragma unroll
+                for (int a = 0; a < 4; a++)
+#pragma unroll
+                    for (int q = 0; q < 4; q++) acc[a][q] += pi[q] * pk[a];
+            }
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                const int rk = rk0 + a;
+                if (rk >= rows) continue;
+                double *col = L + (j0 + rk) * ld;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int ri = ri0 + q;
+                    if (ri >= rk && ri < rows) col[ri - rk] -= acc[a][q];
+                }
+            }
         }
         __syncthreads();
     }
@@ -260,7 +290,8 @@ int pf3d_band_factor(int64_t n_var, int64_t n_nodes, const int32_t *order, const
         return pf3d_fail(PF3D_E_ARG, "pf3d_band_factor: null argument");
     if (n_var > 2147483647) return pf3d_fail(PF3D_E_ARG, "too many systems");
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(band_zero, dim3(nb256(band_size)), dim3(256), 0, s, band_size, band);
+    const unsigned zb = nb256(band_size) < 1u << 20 ? nb256(band_size) : 1u << 20;
+    hipLaunchKernelGGL(band_zero, dim3(zb), dim3(256), 0, s, band_size, band);
     hipLaunchKernelGGL(band_fill, dim3(nb256(n_nodes * 36)), dim3(256), 0, s, n_nodes, row_ptr,
                        col_idx, val, var, pinv, dof0, bw, boff, band);
     BandSys sys{n_var, order, dof0, bw, boff, band};

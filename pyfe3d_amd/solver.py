@@ -66,9 +66,57 @@ class StaticSystem:
         f[self.fixed.bool()] = 0.
         return f
 
+    def factor_direct(self, stream=None):
+        """Banded Cholesky of the system on the GPU (one workgroup), in the
+        node numbering (given, or reverse Cuthill-McKee; host, once) with the
+        narrower band.  Raises if the band is wider than the kernels accept."""
+        import scipy.sparse as ss
+        from scipy.sparse.csgraph import reverse_cuthill_mckee
+        nb = self.nb
+        if not hasattr(self, '_band'):
+            rp = self.plan.row_ptr.cpu().numpy()
+            ci = self.plan.col_idx.cpu().numpy()
+            g = ss.csr_matrix((np.ones(ci.size), ci, rp), shape=(nb, nb))
+            rows = np.repeat(np.arange(nb), np.diff(rp))
+            best = None
+            # the given numbering or reverse Cuthill-McKee, whichever band is narrower
+            for pnode in (np.arange(nb, dtype=np.int64),
+                          np.asarray(reverse_cuthill_mckee(g, symmetric_mode=True), np.int64)):
+                pinv = np.empty_like(pnode)
+                pinv[pnode] = np.arange(nb)
+                w = 6 * int(np.abs(pinv[rows] - pinv[ci]).max()) + 5
+                if best is None or w < best[0]:
+                    best = (w, pnode, pinv)
+            bw, pnode, pinv = best
+            maxb = _lib.call('pf3d_band_max_width')
+            if bw > maxb:
+                raise ValueError('bandwidth %d after renumbering exceeds %d: use PCG' % (bw, maxb))
+            dev = self.val.device
+            self._band_bw = torch.tensor([bw], dtype=torch.int32, device=dev)
+            self._band_off = torch.zeros(1, dtype=torch.int64, device=dev)
+            self._band_dof0 = torch.tensor([0, 6 * nb], dtype=torch.int64, device=dev)
+            self._band_var = torch.zeros(nb, dtype=torch.int32, device=dev)
+            self._band_pinv = torch.as_tensor(pinv, device=dev)
+            self._band_pnode = torch.as_tensor(pnode, device=dev)
+            self._band_size = 6 * nb * (bw + 1)
+            self._band = torch.empty(self._band_size, dtype=torch.float64, device=dev)
+            self._band_work = torch.empty(6 * nb, dtype=torch.float64, device=dev)
+            self._band_bad = torch.zeros(1, dtype=torch.int32, device=dev)
+            self.bandwidth = bw
+        _lib.call('pf3d_band_factor', 1, nb, None, self._band_dof0.data_ptr(),
+                  self._band_bw.data_ptr(), self._band_off.data_ptr(), self._band_size,
+                  self._band.data_ptr(), self.plan.row_ptr.data_ptr(),
+                  self.plan.col_idx.data_ptr(), self.val.data_ptr(), self._band_var.data_ptr(),
+                  self._band_pinv.data_ptr(), self._band_bad.data_ptr(), _s(stream))
+        self._factored = True
+
     def solve(self, f, method='fp64', rtol=1e-8, x0=None, maxiter=200000, inner_rtol=1e-4,
-              max_outer=60, stream=None):
-        """Returns (u, info) with info = dict(iters, outer, relres)."""
+              max_outer=60, refactor=False, stream=None):
+        """Returns (u, info) with info = dict(iters, outer, relres).
+
+        method: 'fp64' (block-Jacobi PCG), 'mixed' (FP64 refinement around
+        PCG on the FP32-stored matrix) or 'direct' (banded Cholesky, factor
+        kept for later right-hand sides unless refactor)."""
         b = self.rhs(f)
         x = torch.zeros_like(b) if x0 is None else x0.clone()
         pr, pc = self.plan.row_ptr.data_ptr(), self.plan.col_idx.data_ptr()
@@ -88,6 +136,16 @@ class StaticSystem:
                       int(maxiter), int(x0 is not None), self._wir.data_ptr(),
                       self._wir.numel(), stats, ctypes.byref(relres), _s(stream))
             info = dict(iters=int(stats[1]), outer=int(stats[0]))
+        elif method == 'direct':
+            if refactor or not getattr(self, '_factored', False):
+                self.factor_direct(stream=stream)
+            _lib.call('pf3d_band_solve', 1, None, self._band_dof0.data_ptr(),
+                      self._band_bw.data_ptr(), self._band_off.data_ptr(),
+                      self._band.data_ptr(), self._band_pnode.data_ptr(), b.data_ptr(),
+                      x.data_ptr(), self._band_work.data_ptr(), _s(stream))
+            if int(self._band_bad.item()):
+                raise _lib.Pf3dError('band factorisation met a non-positive pivot')
+            info = dict(iters=0, outer=0)
         else:
             raise ValueError(method)
         # true FP64 residual on the FP64 matrix
