@@ -415,45 +415,43 @@ __device__ __forceinline__ void membrane_forces(const Shell &s, const double *Nx
 }
 
 // ------------------------------------------------------------------------
-// kernel
+// KG, KG given stress and M: the node-pair blocks of these matrices are a
+// scalar per pair times one 6x6 pattern per element (KG: G_IJ n n^T on the
+// translations, n the element normal; M: H_IJ R6 m R6^T), so the work is
+// split in two phases.  Phase 1: one lane per element (64 elements per
+// wave; 32 for M) computes the geometry, the per-pair scalars and the
+// pattern into LDS.  Phase 2: the wave streams its elements' slots in the
+// reference layout, consecutive lanes computing consecutive slots from LDS,
+// so the stores are contiguous across lanes.
 // ------------------------------------------------------------------------
-constexpr int WAVES_PER_BLOCK = 2;
-
 template <int ET, int MAT>
-__global__ void __launch_bounds__(64 * WAVES_PER_BLOCK)
-shell_assemble(AsmArgs a)
+__global__ void __launch_bounds__(64) shell_stream(AsmArgs a)
 {
     constexpr int NN = (ET == TRIA3R) ? 3 : 4;
-    constexpr int NB = NN * NN;
-    constexpr int EPW = 64 / NB;
-    constexpr int NDOF = 6 * NN;
-    constexpr int TILE = NDOF * NDOF;
-    __shared__ double tile[WAVES_PER_BLOCK][EPW][TILE];
-    __shared__ int64_t sconn[WAVES_PER_BLOCK][EPW][NN];
-    __shared__ int64_t sk0[WAVES_PER_BLOCK][EPW];
-
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int slot = lane / NB, sub = lane - slot * NB;
-    const int I = sub / NN, J = sub - (sub / NN) * NN;
-    const int64_t e0 = ((int64_t)blockIdx.x * WAVES_PER_BLOCK + wave) * EPW;
-    const int64_t e = e0 + slot;
-    const bool active = slot < EPW && e < a.n_elem;
-    const int nslot = slot_count<NN>(MAT, a.mtype);
-
-    if (active) {
+    constexpr int NP = NN * NN;
+    constexpr int W = (MAT == MASS) ? NP + 36 : NP + 3;  // per-element LDS doubles
+    // elements per 64-lane block: the mass pattern needs 4x the LDS of KG,
+    // so mass blocks take 32 elements to keep several waves per CU
+    constexpr int EPB = (MAT == MASS) ? 32 : 64;
+    __shared__ double sd[EPB][W];
+    __shared__ int64_t sconn[EPB][NN];
+    __shared__ int64_t sk0[EPB];
+    const int lane = threadIdx.x;
+    const int64_t e0 = (int64_t)blockIdx.x * EPB;
+    const int64_t e = e0 + lane;
+    const int mtype = a.mtype;
+    const int nslot = slot_count<NN>(MAT, mtype);
+    if (lane < EPB && e < a.n_elem) {
         V3 X[NN];
         int64_t cn[NN];
 #pragma unroll
         for (int i = 0; i < NN; i++) {
             cn[i] = a.conn[e * NN + i];
+            sconn[lane][i] = cn[i];
             const double *px = a.x + cn[i] / 2;
             X[i] = {px[0], px[1], px[2]};
         }
-        if (sub == 0) {
-#pragma unroll
-            for (int i = 0; i < NN; i++) sconn[wave][slot][i] = cn[i];
-            sk0[wave][slot] = a.init_k ? a.init_k[e] : (int64_t)nslot * e;
-        }
+        sk0[lane] = a.init_k ? a.init_k[e] : (int64_t)nslot * e;
         V3 xm = {0., 0., 0.};
         if (a.evec) xm = {a.evec[3 * e], a.evec[3 * e + 1], a.evec[3 * e + 2]};
         Geom g;
@@ -462,30 +460,14 @@ shell_assemble(AsmArgs a)
         Shell s;
         if (MAT != KGS)  // the given-stress path reads no property
             load_shell(a.props + (a.pid ? (int64_t)a.pid[e] : 0) * SHELL_STRIDE, g.m, s);
-
-        double K[6][6];
+        double P[NP];
 #pragma unroll
-        for (int i = 0; i < 6; i++)
-#pragma unroll
-            for (int j = 0; j < 6; j++) K[i][j] = 0.;
-
-        if (MAT == KC0) {
-            const double k6 = a.k6rot ? a.k6rot[e] : 100.;
-            if (ET == QUAD4) quad4_kc0(K, g, s, I, J);
-            else if (ET == QUAD4R) {
-                double hgf[5] = {1., 1., 1., 1., 1.};
-                if (a.hg)
-#pragma unroll
-                    for (int d = 0; d < 5; d++) hgf[d] = a.hg[5 * e + d];
-                quad4r_kc0(K, g, s, I, J, k6, hgf);
-            } else {
-                tria3r_kc0(K, g, s, I, J, k6, a.alpha ? a.alpha[e] : 0.7);
-            }
-        } else if (MAT == KG || MAT == KGS) {
+        for (int k = 0; k < NP; k++) P[k] = 0.;
+        if (MAT == KG || MAT == KGS) {
+            // quad4.pyx:1365-2257 / 2259-3081, tria3r.pyx:3020-4062
             double ue[6 * NN];
             double Nf[3] = {0., 0., 0.};
             if (MAT == KG) {
-                // ue = blockdiag(R^T) u (quad4.pyx:627-680)
 #pragma unroll
                 for (int i = 0; i < NN; i++)
 #pragma unroll
@@ -501,95 +483,135 @@ shell_assemble(AsmArgs a)
                 const double *ps = a.stress + (a.stress_per_elem ? 3 * e : 0);
                 Nf[0] = ps[0]; Nf[1] = ps[1]; Nf[2] = ps[2];
             }
-            double G = 0.;
-            if (ET == TRIA3R) {  // 1 point (tria3r.pyx:3020-3575)
+            if (ET == TRIA3R) {
                 double Nx[3], Ny[3];
                 tria_grad(g, Nx, Ny);
                 if (MAT == KG) membrane_forces<3>(s, Nx, Ny, ue, Nf);
                 const double w = 0.5 * 2 * g.area;
-                const double nxi = pick(Nx, I), nyi = pick(Ny, I);
-                const double nxj = pick(Nx, J), nyj = pick(Ny, J);
-                G = w * (nxj * (nxi * Nf[0] + nyi * Nf[2]) + nyj * (nxi * Nf[2] + nyi * Nf[1]));
-            } else {  // 2x2 Gauss (quad4.pyx:1933-2257)
+#pragma unroll
+                for (int I = 0; I < 3; I++)
+#pragma unroll
+                    for (int J = 0; J < 3; J++)
+                        P[I * 3 + J] = w * (Nx[J] * (Nx[I] * Nf[0] + Ny[I] * Nf[2]) +
+                                            Ny[J] * (Nx[I] * Nf[2] + Ny[I] * Nf[1]));
+            } else {
 #pragma unroll
                 for (int p = 0; p < 4; p++) {
                     QShape q;
                     quad_shape(g.xe, (p >> 1) ? GP2 : -GP2, (p & 1) ? GP2 : -GP2, q);
                     if (MAT == KG) membrane_forces<4>(s, q.Nx, q.Ny, ue, Nf);
-                    const Pair n = quad_pair(g.xe, (p >> 1) ? GP2 : -GP2, (p & 1) ? GP2 : -GP2,
-                                             node_sx(I), node_sy(I), node_sx(J), node_sy(J));
-                    G += q.det * (n.nxj * (n.nxi * Nf[0] + n.nyi * Nf[2]) +
-                                  n.nyj * (n.nxi * Nf[2] + n.nyi * Nf[1]));
+#pragma unroll
+                    for (int I = 0; I < 4; I++) {
+                        const double fx = q.Nx[I] * Nf[0] + q.Ny[I] * Nf[2];
+                        const double fy = q.Nx[I] * Nf[2] + q.Ny[I] * Nf[1];
+#pragma unroll
+                        for (int J = 0; J < 4; J++)
+                            P[I * 4 + J] += q.det * (q.Nx[J] * fx + q.Ny[J] * fy);
+                    }
                 }
             }
-            K[2][2] = G;
-        } else {  // mass (quad4.pyx:3083-9489, tria3r.pyx:4063-7723)
-            double H = 0.;
-            const int mtype = a.mtype;
+#pragma unroll
+            for (int k = 0; k < NP; k++) sd[lane][k] = P[k];
+            // element normal: third local axis in global coordinates
+            sd[lane][NP + 0] = g.r[2];
+            sd[lane][NP + 1] = g.r[5];
+            sd[lane][NP + 2] = g.r[8];
+        } else {
+            // mass: quad4.pyx:3083-9489, tria3r.pyx:4063-7723
             if (ET == TRIA3R) {
                 const double detJ = 2 * g.area;
-                if (mtype == 1) H = detJ / 18.;
-                else {
-                    const double pc[3][3] = {{2. / 3, 1. / 6, 1. / 6}, {1. / 6, 1. / 6, 2. / 3},
-                                             {1. / 6, 2. / 3, 1. / 6}};
-                    const double pl[3][3] = {{1, 0, 0}, {0, 0, 1}, {0, 1, 0}};
+                const double pc[3][3] = {{2. / 3, 1. / 6, 1. / 6}, {1. / 6, 1. / 6, 2. / 3},
+                                         {1. / 6, 2. / 3, 1. / 6}};
+                const double pl[3][3] = {{1, 0, 0}, {0, 0, 1}, {0, 1, 0}};
 #pragma unroll
-                    for (int t = 0; t < 3; t++) {
-                        const double ni = mtype == 0 ? pick(pc[t], I) : pick(pl[t], I);
-                        const double nj = mtype == 0 ? pick(pc[t], J) : pick(pl[t], J);
-                        H += ni * nj * detJ * (0.5 / 3.);
+                for (int I = 0; I < 3; I++)
+#pragma unroll
+                    for (int J = 0; J < 3; J++) {
+                        double H = 0.;
+                        if (mtype == 1) H = detJ / 18.;
+                        else
+#pragma unroll
+                            for (int t = 0; t < 3; t++) {
+                                const double ni = mtype == 0 ? pc[t][I] : pl[t][I];
+                                const double nj = mtype == 0 ? pc[t][J] : pl[t][J];
+                                H += ni * nj * detJ * (0.5 / 3.);
+                            }
+                        P[I * 3 + J] = H;
                     }
-                }
+            } else if (mtype == 1) {
+#pragma unroll
+                for (int k = 0; k < 16; k++) P[k] = 0.0625 * g.area;
             } else {
-                if (mtype == 1) H = 0.0625 * g.area;
-                else {
-                    const double pt = mtype == 0 ? GP2 : 1.0;
+                const double pt = mtype == 0 ? GP2 : 1.0;
 #pragma unroll
-                    for (int p = 0; p < 4; p++) {
-                        const Pair n = quad_pair(g.xe, (p >> 1) ? pt : -pt, (p & 1) ? pt : -pt,
-                                                 node_sx(I), node_sy(I), node_sx(J), node_sy(J));
-                        H += n.Ni * n.Nj * n.det;
-                    }
+                for (int p = 0; p < 4; p++) {
+                    QShape q;
+                    quad_shape(g.xe, (p >> 1) ? pt : -pt, (p & 1) ? pt : -pt, q);
+#pragma unroll
+                    for (int I = 0; I < 4; I++)
+#pragma unroll
+                        for (int J = 0; J < 4; J++) P[I * 4 + J] += q.N[I] * q.N[J] * q.det;
                 }
             }
-            K[0][0] = K[1][1] = K[2][2] = H * s.intrho;
-            K[3][3] = K[4][4] = H * s.intrhoz2;
+            double M[6][6];
+#pragma unroll
+            for (int i = 0; i < 6; i++)
+#pragma unroll
+                for (int j = 0; j < 6; j++) M[i][j] = 0.;
+            M[0][0] = M[1][1] = M[2][2] = s.intrho;
+            M[3][3] = M[4][4] = s.intrhoz2;
             if (mtype != 2) {
-                K[0][4] = K[4][0] = H * s.intrhoz;
-                K[1][3] = K[3][1] = -H * s.intrhoz;
+                M[0][4] = M[4][0] = s.intrhoz;
+                M[1][3] = M[3][1] = -s.intrhoz;
             }
+            rotate_block(M, g.r);
+#pragma unroll
+            for (int k = 0; k < NP; k++) sd[lane][k] = P[k];
+#pragma unroll
+            for (int i = 0; i < 6; i++)
+#pragma unroll
+                for (int j = 0; j < 6; j++) sd[lane][NP + 6 * i + j] = M[i][j];
         }
-        rotate_block(K, g.r);
-        double *t = &tile[wave][slot][0];
-#pragma unroll
-        for (int i = 0; i < 6; i++)
-#pragma unroll
-            for (int j = 0; j < 6; j++) t[(6 * I + i) * NDOF + 6 * J + j] = K[i][j];
     }
     __syncthreads();
-
-    // stream the wave's elements out in slot order
-    const int nel = (int)((a.n_elem - e0) < EPW ? (a.n_elem - e0) : EPW);
-    if (nel <= 0) return;
+    const int nel = (int)((a.n_elem - e0) < EPB ? (a.n_elem - e0) : EPB);
     const bool write_idx = !(a.flags & F_VALUES_ONLY);
     const bool overwrite = (a.flags & F_OVERWRITE);
-    for (int el = 0; el < nel; el++) {
-        const int64_t k0 = sk0[wave][el];
-        const double *t = &tile[wave][el][0];
-        for (int k = lane; k < nslot; k += 64) {
-            int rl, cl;
-            slot_layout<NN>(MAT, a.mtype, k, rl, cl);
-            const double val = t[rl * NDOF + cl];
-            if (overwrite) a.v[k0 + k] = val;
-            else a.v[k0 + k] += val;
-            if (write_idx) {
-                a.r[k0 + k] = sconn[wave][el][rl / 6] + rl % 6;
-                a.c[k0 + k] = sconn[wave][el][cl / 6] + cl % 6;
-            }
+    // consecutive lanes write consecutive slots of the block's elements
+    const int total = nel * nslot;
+    for (int t = lane; t < total; t += 64) {
+        const int el = t / nslot, k = t - el * nslot;
+        int I, J, dr, dc;  // node pair and DOFs within the 6x6 block
+        if (MAT == KG || MAT == KGS) {  // (I, a, J, b) over translations
+            const int row = k / (3 * NN), col = k - row * 3 * NN;
+            I = row / 3; dr = row - 3 * I;
+            J = col / 3; dc = col - 3 * J;
+        } else if (mtype == 2) {  // lumped: (I, g, d, J, t)
+            const int Ib = k / (18 * NN), rem = k - Ib * 18 * NN;
+            const int gg = rem / (9 * NN), rem2 = rem - gg * 9 * NN;
+            const int d = rem2 / (3 * NN), col = rem2 - d * 3 * NN;
+            I = Ib; dr = 3 * gg + d;
+            J = col / 3; dc = 3 * gg + (col - 3 * J);
+        } else {  // row d skips column (d + 3) mod 6
+            const int row = k / (5 * NN), col = k - row * 5 * NN;
+            I = row / 6; dr = row - 6 * I;
+            J = col / 5;
+            const int tt = col - 5 * J, skip = (dr + 3) % 6;
+            dc = tt < skip ? tt : tt + 1;
+        }
+        const double *d = sd[el];
+        double val;
+        if (MAT == KG || MAT == KGS) val = (d[NP + dr] * d[I * NN + J]) * d[NP + dc];
+        else val = d[I * NN + J] * d[NP + 6 * dr + dc];
+        const int64_t slot = sk0[el] + k;
+        if (overwrite) __builtin_nontemporal_store(val, a.v + slot);
+        else a.v[slot] += val;
+        if (write_idx) {
+            a.r[slot] = sconn[el][I] + dr;
+            a.c[slot] = sconn[el][J] + dc;
         }
     }
 }
-
 
 // ------------------------------------------------------------------------
 // KC0 with block symmetry.  K_e is symmetric, so a lane owns one node-pair
@@ -1221,7 +1243,7 @@ template <int ET, int MAT>
 static hipError_t launch_shell(const AsmArgs &a, hipStream_t s)
 {
     constexpr int NN = (ET == TRIA3R) ? 3 : 4;
-    if (MAT == KC0) {
+    if constexpr (MAT == KC0) {
         constexpr int EPW = 64 / (NN * (NN + 1) / 2);
         const int64_t blocks = (a.n_elem + EPW - 1) / EPW;
         if (blocks == 0) return hipSuccess;
@@ -1241,13 +1263,13 @@ static hipError_t launch_shell(const AsmArgs &a, hipStream_t s)
         default: hipLaunchKernelGGL((shell_kc0_sym<ET, 7>), grid, blk, 0, s, a);
         }
         return hipGetLastError();
+    } else {
+        constexpr int EPB = (MAT == MASS) ? 32 : 64;
+        const int64_t blocks = (a.n_elem + EPB - 1) / EPB;
+        if (blocks == 0) return hipSuccess;
+        hipLaunchKernelGGL((shell_stream<ET, MAT>), dim3((unsigned)blocks), dim3(64), 0, s, a);
+        return hipGetLastError();
     }
-    constexpr int EPB = (64 / (NN * NN)) * WAVES_PER_BLOCK;
-    const int64_t blocks = (a.n_elem + EPB - 1) / EPB;
-    if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL((shell_assemble<ET, MAT>), dim3((unsigned)blocks),
-                       dim3(64 * WAVES_PER_BLOCK), 0, s, a);
-    return hipGetLastError();
 }
 
 template <int ET>
