@@ -236,6 +236,12 @@ int pf3d_bsr_spmv_f32(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx
 int pf3d_bsr_dirichlet(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx, double *val,
                        const uint8_t *fixed, void *stream);
 
+/* Same with a chosen diagonal on the fixed DOFs: 0 for a mass or
+ * geometric-stiffness matrix (P B P), so the fixed DOFs drop out of
+ * A x = lambda B x (their eigenvalues move to infinity). */
+int pf3d_bsr_dirichlet_diag(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx,
+                            double *val, const uint8_t *fixed, double diag, void *stream);
+
 /* b[i] = (float)a[i] */
 int pf3d_to_f32(int64_t n, const double *a, float *b, void *stream);
 
@@ -319,6 +325,19 @@ int pf3d_band_factor(int64_t n_var, int64_t n_nodes, const int32_t *order, const
 int pf3d_band_solve(int64_t n_var, const int32_t *order, const int64_t *dof0, const int32_t *bw,
                     const int64_t *boff, const double *band, const int64_t *pnode,
                     const double *f, double *x, double *work, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* FP64 dense helpers of the eigen-solvers (Krylov bases, column-major  */
+/* n x m with leading dimension ld; fixed-order reductions)             */
+/* ------------------------------------------------------------------ */
+/* c = alpha V^T x + beta c */
+int pf3d_gemv_t(int64_t n, int m, const double *V, int64_t ld, const double *x, double *c,
+                double alpha, double beta, void *stream);
+/* y = alpha V c + beta y */
+int pf3d_gemv_n(int64_t n, int m, const double *V, int64_t ld, const double *c, double *y,
+                double alpha, double beta, void *stream);
+/* y = a x + b y */
+int pf3d_axpby(int64_t n, double a, const double *x, double b, double *y, void *stream);
 
 /* ------------------------------------------------------------------ */
 /* Warm-start network kernels (no reference counterpart: pyfe3d starts   */

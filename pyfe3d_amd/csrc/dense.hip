@@ -1,0 +1,89 @@
+// Small FP64 dense kernels for the eigen-solvers' Krylov bases: a basis is
+// m columns of length n, column-major with leading dimension ld.  Every
+// reduction has a fixed order (per-column workgroup tree), so reruns give
+// the same bits.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/pf3d.h"
+
+int pf3d_fail(int code, const char *fmt, ...);
+int pf3d_hip_check(hipError_t e, const char *where);
+
+namespace {
+
+// c[j] = alpha * V[:, j] . x + beta * c[j]   (one workgroup per column)
+__global__ void __launch_bounds__(256) gemv_t(int64_t n, int m, const double *__restrict__ V,
+                                              int64_t ld, const double *__restrict__ x,
+                                              double *__restrict__ c, double alpha, double beta)
+{
+    __shared__ double sh[256];
+    const int j = blockIdx.x;
+    const double *v = V + (int64_t)j * ld;
+    double s = 0.;
+    for (int64_t i = threadIdx.x; i < n; i += 256) s += v[i] * x[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) c[j] = alpha * sh[0] + (beta != 0. ? beta * c[j] : 0.);
+}
+
+// y = alpha * V c + beta * y
+__global__ void gemv_n(int64_t n, int m, const double *__restrict__ V, int64_t ld,
+                       const double *__restrict__ c, double *__restrict__ y, double alpha,
+                       double beta)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.;
+    for (int j = 0; j < m; j++) s += V[(int64_t)j * ld + i] * c[j];
+    y[i] = alpha * s + (beta != 0. ? beta * y[i] : 0.);
+}
+
+// y = a * x + b * y
+__global__ void axpby(int64_t n, double a, const double *__restrict__ x, double b,
+                      double *__restrict__ y)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = a * x[i] + (b != 0. ? b * y[i] : 0.);
+}
+
+}  // namespace
+
+extern "C" {
+
+int pf3d_gemv_t(int64_t n, int m, const double *V, int64_t ld, const double *x, double *c,
+                double alpha, double beta, void *stream)
+{
+    if (n < 0 || m < 0 || ld < n || (m && (!V || !x || !c)))
+        return pf3d_fail(PF3D_E_ARG, "pf3d_gemv_t: bad arguments");
+    if (!m) return PF3D_OK;
+    hipLaunchKernelGGL(gemv_t, dim3(m), dim3(256), 0, (hipStream_t)stream, n, m, V, ld, x, c,
+                       alpha, beta);
+    return pf3d_hip_check(hipGetLastError(), "pf3d_gemv_t");
+}
+
+int pf3d_gemv_n(int64_t n, int m, const double *V, int64_t ld, const double *c, double *y,
+                double alpha, double beta, void *stream)
+{
+    if (n < 0 || m < 0 || ld < n || (n && (!V || !c || !y)))
+        return pf3d_fail(PF3D_E_ARG, "pf3d_gemv_n: bad arguments");
+    if (!n) return PF3D_OK;
+    hipLaunchKernelGGL(gemv_n, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, n, m, V, ld, c, y, alpha, beta);
+    return pf3d_hip_check(hipGetLastError(), "pf3d_gemv_n");
+}
+
+int pf3d_axpby(int64_t n, double a, const double *x, double b, double *y, void *stream)
+{
+    if (n < 0 || (n && (!x || !y))) return pf3d_fail(PF3D_E_ARG, "pf3d_axpby: bad arguments");
+    if (!n) return PF3D_OK;
+    hipLaunchKernelGGL(axpby, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, n, a, x, b, y);
+    return pf3d_hip_check(hipGetLastError(), "pf3d_axpby");
+}
+
+}  // extern "C"

@@ -66,7 +66,7 @@ __global__ void __launch_bounds__(256) bsr_spmv(int64_t nb, const int32_t *__res
 // ------------------------------------------------------ boundary conditions
 // K <- P K P + (I - P) with P = diag(1 - fixed)
 __global__ void bsr_bc(int64_t nb, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                       double *__restrict__ val, const uint8_t *__restrict__ fixed)
+                       double *__restrict__ val, const uint8_t *__restrict__ fixed, double diag)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t row = t / 36;
@@ -77,7 +77,7 @@ __global__ void bsr_bc(int64_t nb, const int32_t *__restrict__ rp, const int32_t
         const int64_t col = ci[b];
         const bool fc = fixed[col * 6 + k];
         double *v = val + (int64_t)b * 36 + e;
-        if (fr || fc) *v = (col == row && i == k) ? 1. : 0.;
+        if (fr || fc) *v = (col == row && i == k) ? diag : 0.;
     }
 }
 
@@ -386,8 +386,18 @@ int pf3d_bsr_dirichlet(int64_t nb, const int32_t *row_ptr, const int32_t *col_id
     if (nb <= 0 || !row_ptr || !col_idx || !val || !fixed)
         return pf3d_fail(PF3D_E_ARG, "pf3d_bsr_dirichlet: bad arguments");
     hipLaunchKernelGGL(bsr_bc, dim3(nblk(nb * 36, 256)), dim3(256), 0, (hipStream_t)stream, nb,
-                       row_ptr, col_idx, val, fixed);
+                       row_ptr, col_idx, val, fixed, 1.);
     return pf3d_hip_check(hipGetLastError(), "pf3d_bsr_dirichlet");
+}
+
+int pf3d_bsr_dirichlet_diag(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx,
+                            double *val, const uint8_t *fixed, double diag, void *stream)
+{
+    if (nb <= 0 || !row_ptr || !col_idx || !val || !fixed)
+        return pf3d_fail(PF3D_E_ARG, "pf3d_bsr_dirichlet_diag: bad arguments");
+    hipLaunchKernelGGL(bsr_bc, dim3(nblk(nb * 36, 256)), dim3(256), 0, (hipStream_t)stream, nb,
+                       row_ptr, col_idx, val, fixed, diag);
+    return pf3d_hip_check(hipGetLastError(), "pf3d_bsr_dirichlet_diag");
 }
 
 int pf3d_to_f32(int64_t n, const double *a, float *b, void *stream)
