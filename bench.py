@@ -177,6 +177,50 @@ def sweep_rates(n_var, log, steps=2, modes=('direct', 'fp64', 'mixed')):
     return out
 
 
+def modal_rates(n, log, k=6):
+    """First k natural frequencies of an n x n simply supported isotropic
+    plate (u, v, w fixed on the edges, consistent mass): assembly of KC0 and
+    M, BSR, banded Cholesky of K + M and shift-invert Lanczos."""
+    import torch
+    from pyfe3d_amd import assembly as A
+    from pyfe3d_amd.eigen import GeneralizedEigen
+    from pyfe3d_amd.mesh import PlateMesh
+    from pyfe3d_amd.shellprop import isotropic_plate
+    from pyfe3d_amd.sparse import BSRPlan
+    m = PlateMesh(n + 1, n + 1, 0.3, 0.5)
+    rec = isotropic_plate(thickness=0.01, E=203e9, nu=0.33, rho=7.83e3).record()
+    E = m.quads.shape[0]
+    dev = 'cuda'
+    x = torch.as_tensor(m.x, device=dev)
+    conn = torch.as_tensor(6 * m.quads, device=dev)
+    props = torch.as_tensor(rec[None], device=dev)
+    xx, yy = m.ncoords[:, 0], m.ncoords[:, 1]
+    edge = (np.isclose(xx, 0) | np.isclose(xx, 0.3) | np.isclose(yy, 0) | np.isclose(yy, 0.5))
+    fixed = np.zeros(m.n_dof, bool)
+    for d in range(3):
+        fixed[d::6] = edge
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rk = torch.empty(576 * E, dtype=torch.int64, device=dev)
+    ck = torch.empty_like(rk)
+    vk = torch.zeros(576 * E, dtype=torch.float64, device=dev)
+    A.shell_kc0('quad4', x, conn, rk, ck, vk, props)
+    rm = torch.empty(480 * E, dtype=torch.int64, device=dev)
+    cm = torch.empty_like(rm)
+    vm = torch.zeros(480 * E, dtype=torch.float64, device=dev)
+    A.shell_m('quad4', x, conn, rm, cm, vm, props, mtype=0)
+    pk = BSRPlan(rk, ck, m.n_nodes)
+    pm = BSRPlan(rm, cm, m.n_nodes)
+    ev = GeneralizedEigen(pk, pk.apply(vk), pm.apply(vm), fixed, sigma=-1.)
+    lam, X, info = ev.solve(k=k, tol=1e-10)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    out = dict(ms=dt * 1e3, omega=[float(v) for v in np.sqrt(lam)], lanczos_steps=info['m'],
+               max_residual=float(np.max(info['residual'])), dof=m.n_dof)
+    log('modal %dx%d: %.1f ms, omega %s' % (n, n, dt * 1e3, np.sqrt(lam)[:3]))
+    return out
+
+
 def profile_kc0(n):
     """Three values-only KC0 launches on the n x n plate, then three copies
     of 1 GiB (the calibration of the HBM byte counters)."""
@@ -299,6 +343,20 @@ def main():
     del vm
     log('KG %.3f ms, M %.3f ms' % (ms_g, ms_m))
 
+    # internal forces (update_fint) of the same plate: element forces + ordered node gather
+    u = torch.randn(mesh.n_dof, dtype=torch.float64, device=dev,
+                    generator=torch.Generator(device=dev).manual_seed(3)) * 1e-4
+    fint = torch.zeros(mesh.n_dof, dtype=torch.float64, device=dev)
+    fe = torch.empty((E, 4, 6), dtype=torch.float64, device=dev)
+    gather = A.NodeGather(conn, mesh.n_nodes)
+    ms_f, _ = timed(lambda: A.update_fint('quad4', x, conn, u, fint, props, gather=gather,
+                                          fe=fe, check=False),
+                    max(3, args.steps // 4), 1, stream)
+    extra['fint'] = dict(ms=ms_f, elements_per_s=E / (ms_f * 1e-3),
+                         gbs=(24 * 8 + 4 * 8 + 12 * 8 + 24 * 8 * 2 + 24 * 8) * E / (ms_f * 1e-3) / 1e9)
+    del fe, gather, fint
+    log('fint %.3f ms' % ms_f)
+
     if not args.no_solve:
         try:
             extra['solve_64x64'] = solve_rates(64, 3, log)
@@ -322,6 +380,13 @@ def main():
             except Exception as exc:
                 extra['design_sweep_pcg'] = dict(error=str(exc))
                 log('sweep (pcg) failed: %s' % exc)
+
+    if not args.no_solve:
+        try:
+            extra['modal_64x64'] = modal_rates(64, log)
+        except Exception as exc:
+            extra['modal_64x64'] = dict(error=str(exc))
+            log('modal failed: %s' % exc)
 
     if not args.no_solve and args.warm_start_plates > 0:
         torch.cuda.empty_cache()

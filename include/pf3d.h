@@ -19,7 +19,9 @@
  *     B11..B66, D11..D66, E44 E45 E55, scf_k13, scf_k23, h, offset,
  *     intrho, intrhoz, intrhoz2, 4 unused  (shellprop.pxd:38-60)
  *   - beam property record (16 doubles): A E G Iyy Izz Iyz J Ay Az intrho
- *     intrhoy intrhoz intrhoy2 intrhoz2 intrhoyz, 1 unused (beamprop.pyx:15-93)
+ *     intrhoy intrhoz intrhoy2 intrhoz2 intrhoyz, 1 unused (beamprop.pyx:15-93);
+ *     BeamC and Truss read it
+ *   - spring record (6 doubles): kxe kye kze krxe krye krze (spring.pyx:97-125)
  */
 #ifndef PF3D_H
 #define PF3D_H
@@ -39,6 +41,8 @@ extern "C" {
 #define PF3D_QUAD4R 1
 #define PF3D_TRIA3R 2
 #define PF3D_BEAMC 3
+#define PF3D_TRUSS 4
+#define PF3D_SPRING 5
 
 #define PF3D_KC0 0
 #define PF3D_KG 1
@@ -53,7 +57,8 @@ const char *pf3d_last_error(void);
 
 /* Slots written per element: Quad4Data/Quad4RData/Tria3RData/BeamCData
  * KC0_SPARSE_SIZE, KG_SPARSE_SIZE, M_SPARSE_SIZE (quad4.pyx:174-180,
- * quad4r.pyx:140-146, tria3r.pyx:148-151, beamc.pyx:43-46); for lumped
+ * quad4r.pyx:140-146, tria3r.pyx:148-151, beamc.pyx:43-46,
+ * truss.pyx:45-47, spring.pyx:40-43); for lumped
  * masses the used prefix of M_SPARSE_SIZE. */
 int pf3d_slot_count(int etype, int matrix, int mtype);
 
@@ -62,7 +67,10 @@ int pf3d_slot_count(int etype, int matrix, int mtype);
  * area (shells) or length (beams); xe[e*3nn ..] = local x, y, z per node.
  * Replaces update_rotation_matrix + update_probe_xe + update_area/length:
  * quad4.pyx:491-752, quad4r.pyx:286-548, tria3r.pyx:294-548,
- * beamc.pyx:158-350.  evec = xmat (shells, may be NULL) or vxy (beams). */
+ * beamc.pyx:158-350, truss.pyx:203-338, spring.pyx:147-206.  evec = xmat
+ * (shells, may be NULL), vxy (beams), unused (trusses: axes from the nodes)
+ * or x axis + XY-plane vector, 6 per element (springs); springs have no
+ * length (0). */
 int pf3d_geometry(int etype, int64_t n_elem, const double *x, const int64_t *conn,
                   const double *evec, double *geom, double *xe, void *stream);
 
@@ -124,6 +132,27 @@ int pf3d_beam_m(int64_t n_elem, const double *x, const int64_t *conn, const int6
                 const double *vxy, const int32_t *pid, const double *props, int mtype,
                 int64_t *r, int64_t *c, double *v, int flags, void *stream);
 
+/* Truss KC0 (72 slots: axial E A / L and torsion G J / L), props = beam
+ * records.  Replaces update_rotation_matrix(x) -> update_probe_xe(x) ->
+ * update_length() -> update_KC0(...): truss.pyx:203-338, 386-801. */
+int pf3d_truss_kc0(int64_t n_elem, const double *x, const int64_t *conn, const int64_t *init_k,
+                   const int32_t *pid, const double *props, int64_t *r, int64_t *c, double *v,
+                   int flags, void *stream);
+
+/* Truss mass, mtype 0 consistent (144 slots) or 1 lumped (36 slots).
+ * Replaces update_M(..., mtype): truss.pyx:838-1800. */
+int pf3d_truss_m(int64_t n_elem, const double *x, const int64_t *conn, const int64_t *init_k,
+                 const int32_t *pid, const double *props, int mtype, int64_t *r, int64_t *c,
+                 double *v, int flags, void *stream);
+
+/* Spring KC0 (72 slots), axes[e*6..] = x axis (xi, xj, xk) and XY-plane
+ * vector (vxyi, vxyj, vxyk), k = spring records (6 doubles, by pid or
+ * record 0).  Replaces update_rotation_matrix(xi, xj, xk, vxyi, vxyj,
+ * vxyk) -> update_KC0(...): spring.pyx:147-206, 295-707. */
+int pf3d_spring_kc0(int64_t n_elem, const double *x, const int64_t *conn, const int64_t *init_k,
+                    const double *axes, const int32_t *pid, const double *k, int64_t *r,
+                    int64_t *c, double *v, int flags, void *stream);
+
 /* Shell property records (32 doubles each) of isotropic plates, one per
  * element, from a thickness field h[n] (device) and the record of the same
  * material at unit thickness (host, 32 doubles, h field = 1, no offset):
@@ -154,6 +183,15 @@ int pf3d_shell_fe(int etype, int64_t n_elem, const double *x, const int64_t *con
 int pf3d_beam_fe(int64_t n_elem, const double *x, const int64_t *conn, const double *vxy,
                  const int32_t *pid, const double *props, const double *u, double *fe,
                  void *stream);
+
+/* Truss (etype 4, props = beam records, axes unused) or Spring (etype 5,
+ * props = spring records, axes as in pf3d_spring_kc0) element internal
+ * forces, fe[(e*2 + i)*6 + d].  Replaces update_probe_ue +
+ * update_probe_finte + update_fint: truss.pyx:150-202, 339-385, 802-837;
+ * spring.pyx:208-294, 708-739. */
+int pf3d_rod_fe(int etype, int64_t n_elem, const double *x, const int64_t *conn,
+                const double *axes, const int32_t *pid, const double *props, const double *u,
+                double *fe, void *stream);
 
 /* Node incidence plan of n_entries (element, node) entries whose DOF
  * offsets are conn[0..n_entries): ids (n_entries) lists the entries node by

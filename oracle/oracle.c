@@ -24,7 +24,9 @@
 
 #define DOF 6
 
-enum { ET_QUAD4 = 0, ET_QUAD4R = 1, ET_TRIA3R = 2, ET_BEAMC = 3 };
+enum { ET_QUAD4 = 0, ET_QUAD4R = 1, ET_TRIA3R = 2, ET_BEAMC = 3, ET_TRUSS = 4, ET_SPRING = 5 };
+#define IS_ROD(t) ((t) == ET_TRUSS || (t) == ET_SPRING)
+#define SPR_STRIDE 8 /* spring record: kxe kye kze krxe krye krze, 2 unused */
 enum { MAT_KC0 = 0, MAT_KG = 1, MAT_KGS = 2, MAT_M = 3, MAT_FINT = 4 };
 
 /* shell property record (doubles): A11 A12 A16 A22 A26 A66, B.., D.., E44 E45
@@ -656,6 +658,55 @@ static void beamc_m_local(double L, const double *bp, int mtype, double *K)
 
 /* membrane forces from local displacements: N = A eps + B kappa
  * (quad4.pyx:1965-1967) */
+/* Truss (truss.pyx:386-801, 838-1800): axial E A / L and torsion G J / L;
+ * consistent mass (two-point Gauss) and lumped mass (Gauss-Lobatto) with
+ * the translations, the torsional inertia and the translation-torsion
+ * coupling through the section's first moments. */
+static void truss_kc0_local(double L, const double *bp, double *K)
+{
+    memset(K, 0, sizeof(double) * 144);
+    const double ka = bp[0] * bp[1] / L, kt = bp[2] * bp[6] / L;
+    for (int I = 0; I < 2; I++)
+        for (int J = 0; J < 2; J++) {
+            const double s = I == J ? 1. : -1.;
+            K[(6 * I + 0) * 12 + 6 * J + 0] = s * ka;
+            K[(6 * I + 3) * 12 + 6 * J + 3] = s * kt;
+        }
+}
+
+static void truss_m_local(double L, const double *bp, int mtype, double *K)
+{
+    memset(K, 0, sizeof(double) * 144);
+    const double rho = bp[9], ry = bp[10], rz = bp[11], rt = bp[12] + bp[13];
+    if (mtype == 1) {
+        for (int I = 0; I < 2; I++) {
+            const int o = 6 * I;
+            K[(o + 0) * 12 + o + 0] = K[(o + 1) * 12 + o + 1] = K[(o + 2) * 12 + o + 2] =
+                L * rho / 2;
+            K[(o + 3) * 12 + o + 3] = L * rt / 2;
+        }
+        return;
+    }
+    for (int I = 0; I < 2; I++)
+        for (int J = 0; J < 2; J++) {
+            const double f = L * (I == J ? 1. / 3. : 1. / 6.);
+            const int a = 6 * I, b = 6 * J;
+            K[(a + 0) * 12 + b + 0] = K[(a + 1) * 12 + b + 1] = K[(a + 2) * 12 + b + 2] = f * rho;
+            K[(a + 1) * 12 + b + 3] = K[(a + 3) * 12 + b + 1] = -f * rz;
+            K[(a + 2) * 12 + b + 3] = K[(a + 3) * 12 + b + 2] = f * ry;
+            K[(a + 3) * 12 + b + 3] = f * rt;
+        }
+}
+
+/* Spring (spring.pyx:295-707): k on each local DOF between the nodes */
+static void spring_kc0_local(const double *kp, double *K)
+{
+    memset(K, 0, sizeof(double) * 144);
+    for (int I = 0; I < 2; I++)
+        for (int J = 0; J < 2; J++)
+            for (int d = 0; d < 6; d++) K[(6 * I + d) * 12 + 6 * J + d] = (I == J ? 1. : -1.) * kp[d];
+}
+
 static void membrane_forces(int nn, const shell_t *s, const double *Nx, const double *Ny,
                             const double *ue, double Nf[3])
 {
@@ -770,6 +821,22 @@ static void shell_mass_block(const shell_t *s, int lumped, double mb[6][6])
 static int layout(int etype, int mat, int mtype, int nn, int *rl, int *cl)
 {
     int k = 0;
+    if (IS_ROD(etype) && mat == MAT_KC0) {
+        /* truss.pyx:432-800, spring.pyx:341-560: (I, group, d) rows of (J, e) */
+        for (int I = 0; I < 2; I++)
+            for (int g = 0; g < 2; g++)
+                for (int d = 0; d < 3; d++)
+                    for (int J = 0; J < 2; J++)
+                        for (int e = 0; e < 3; e++) {
+                            rl[k] = 6 * I + 3 * g + d; cl[k] = 6 * J + 3 * g + e; k++;
+                        }
+        return k;
+    }
+    if (etype == ET_TRUSS && mat == MAT_M && mtype == 0) {
+        for (int i = 0; i < 12; i++)
+            for (int j = 0; j < 12; j++) { rl[k] = i; cl[k] = j; k++; }
+        return k;
+    }
     if (mat == MAT_KC0 || (etype == ET_BEAMC && (mat == MAT_KG || (mat == MAT_M && mtype == 0)))) {
         for (int I = 0; I < nn; I++)
             for (int d = 0; d < 6; d++)
@@ -785,7 +852,7 @@ static int layout(int etype, int mat, int mtype, int nn, int *rl, int *cl)
         return k;
     }
     /* mass */
-    if (etype == ET_BEAMC) { /* lumped: node-diagonal translations + rotations */
+    if (etype == ET_BEAMC || etype == ET_TRUSS) { /* lumped: node-diagonal translations + rotations */
         for (int I = 0; I < nn; I++)
             for (int g = 0; g < 2; g++)
                 for (int d = 0; d < 3; d++)
@@ -817,7 +884,7 @@ static int layout(int etype, int mat, int mtype, int nn, int *rl, int *cl)
 int orc_slot_count(int etype, int mat, int mtype)
 {
     int rl[576], cl[576];
-    int nn = etype == ET_TRIA3R ? 3 : (etype == ET_BEAMC ? 2 : 4);
+    int nn = etype == ET_TRIA3R ? 3 : ((etype == ET_BEAMC || IS_ROD(etype)) ? 2 : 4);
     return layout(etype, mat, mtype, nn, rl, cl);
 }
 
@@ -825,7 +892,13 @@ int orc_slot_count(int etype, int mat, int mtype)
 /* per-element driver                                                    */
 /* ------------------------------------------------------------------ */
 
-static int nnodes(int etype) { return etype == ET_TRIA3R ? 3 : (etype == ET_BEAMC ? 2 : 4); }
+static int nnodes(int etype)
+{
+    return etype == ET_TRIA3R ? 3 : ((etype == ET_BEAMC || IS_ROD(etype)) ? 2 : 4);
+}
+
+/* per-element orientation vectors: xmat / vxy (3), spring axis + vxy (6), truss none */
+static int evec_stride(int etype) { return etype == ET_SPRING ? 6 : 3; }
 
 /* geometry record: r11..r33 (9), m11 m12 m21 m22 (4), area or length (1) */
 static void element_geometry(int etype, const double *x, const int64_t *cn,
@@ -838,10 +911,31 @@ static void element_geometry(int etype, const double *x, const int64_t *cn,
     double m[4] = {1, 0, 0, 1};
     if (etype == ET_QUAD4 || etype == ET_QUAD4R) quad_axes(X, evec, g, m);
     else if (etype == ET_TRIA3R) tria_axes(X, evec, g, m);
-    else beam_axes(X, evec, g);
+    else if (etype == ET_TRUSS) {
+        /* truss.pyx:203-272: x along the element, off-axis vector (xj, xk, xi) */
+        double ex[3] = {X[3] - X[0], X[4] - X[1], X[5] - X[2]}, t;
+        normalize3(ex, &t);
+        const double v[3] = {ex[1], ex[2], ex[0]};
+        double ez[3], ey[3];
+        cross3(ex, v, ez);
+        normalize3(ez, &t);
+        cross3(ez, ex, ey);
+        normalize3(ey, &t);
+        set_r(g, ex, ey, ez);
+    } else if (etype == ET_SPRING) {
+        /* spring.pyx:147-206: x axis and XY-plane vector given */
+        double ex[3] = {evec[0], evec[1], evec[2]}, t;
+        normalize3(ex, &t);
+        double ez[3], ey[3];
+        cross3(ex, evec + 3, ez);
+        normalize3(ez, &t);
+        cross3(ez, ex, ey);
+        normalize3(ey, &t);
+        set_r(g, ex, ey, ez);
+    } else beam_axes(X, evec, g);
     memcpy(g + 9, m, sizeof(m));
     local_coords(nn, X, g, xe);
-    if (etype == ET_BEAMC) {
+    if (etype == ET_BEAMC || etype == ET_TRUSS) {
         double d[3] = {X[3] - X[0], X[4] - X[1], X[5] - X[2]};
         g[13] = pow(d[0] * d[0] + d[1] * d[1] + d[2] * d[2], 0.5);
     } else {
@@ -855,7 +949,8 @@ int orc_geometry(int etype, int64_t E, const double *x, const int64_t *conn,
     int nn = nnodes(etype);
     for (int64_t e = 0; e < E; e++) {
         double xe[12];
-        element_geometry(etype, x, conn + e * nn, evec + 3 * e, geom + 14 * e, xe);
+        element_geometry(etype, x, conn + e * nn, evec ? evec + evec_stride(etype) * e : NULL,
+                         geom + 14 * e, xe);
         if (xe_out) memcpy(xe_out + 3 * nn * e, xe, sizeof(double) * 3 * nn);
     }
     return 0;
@@ -880,6 +975,17 @@ static int element_local(int etype, int mat, int mtype, const double *g, const d
 {
     int nn = nnodes(etype), n = 6 * nn;
     double area = g[13];
+    if (etype == ET_TRUSS) {
+        if (mat == MAT_KC0 || mat == MAT_FINT) truss_kc0_local(area, prop, K);
+        else if (mat == MAT_M) truss_m_local(area, prop, mtype, K);
+        else return -1;
+        return 0;
+    }
+    if (etype == ET_SPRING) {
+        if (mat == MAT_KC0 || mat == MAT_FINT) spring_kc0_local(prop, K);
+        else return -1;
+        return 0;
+    }
     if (etype == ET_BEAMC) {
         if (mat == MAT_KC0 || mat == MAT_FINT) beamc_kc0_local(area, prop, K);
         else if (mat == MAT_KG) /* N = A E (u2 - u1) / L, beamc.pyx:1889 */
@@ -931,12 +1037,13 @@ int orc_assemble(int etype, int mat, int mtype, int64_t E, const double *x,
     int nn = nnodes(etype), n = 6 * nn;
     int rl[576], cl[576];
     int ns = layout(etype, mat, mtype, nn, rl, cl);
-    int stride = etype == ET_BEAMC ? BP_STRIDE : SP_STRIDE;
+    int stride = (etype == ET_BEAMC || etype == ET_TRUSS) ? BP_STRIDE
+                 : (etype == ET_SPRING ? SPR_STRIDE : SP_STRIDE);
     static const double hg_default[5] = {1., 1., 1., 1., 1.};
     for (int64_t e = 0; e < E; e++) {
         const int64_t *cn = conn + e * nn;
         double g[14], xe[12], ue[24], K[576], Kg[576];
-        element_geometry(etype, x, cn, evec + 3 * e, g, xe);
+        element_geometry(etype, x, cn, evec ? evec + evec_stride(etype) * e : NULL, g, xe);
         if (mat == MAT_KG && u) local_displ(nn, u, cn, g, ue);
         int st = element_local(etype, mat, mtype, g, xe, props + pid[e] * stride,
                                k6rot ? k6rot[e] : 100., alpha ? alpha[e] : 0.7,
@@ -975,12 +1082,13 @@ int orc_fint(int etype, int64_t E, const double *x, const int64_t *conn, const d
              const double *hg, const double *u, double *fint)
 {
     int nn = nnodes(etype), n = 6 * nn;
-    int stride = etype == ET_BEAMC ? BP_STRIDE : SP_STRIDE;
+    int stride = (etype == ET_BEAMC || etype == ET_TRUSS) ? BP_STRIDE
+                 : (etype == ET_SPRING ? SPR_STRIDE : SP_STRIDE);
     static const double hg_default[5] = {1., 1., 1., 1., 1.};
     for (int64_t e = 0; e < E; e++) {
         const int64_t *cn = conn + e * nn;
         double g[14], xe[12], ue[24], K[576], fe[24];
-        element_geometry(etype, x, cn, evec + 3 * e, g, xe);
+        element_geometry(etype, x, cn, evec ? evec + evec_stride(etype) * e : NULL, g, xe);
         local_displ(nn, u, cn, g, ue);
         int st = element_local(etype, MAT_FINT, 0, g, xe, props + pid[e] * stride,
                                k6rot ? k6rot[e] : 100., alpha ? alpha[e] : 0.7,

@@ -30,6 +30,10 @@ SIGNATURES = {
     'pf3d_isotropic_records': [I64, P, P, P, P],
     'pf3d_shell_fe': [I32, I64, P, P, P, P, P, P, P, P, P, P, P],
     'pf3d_beam_fe': [I64, P, P, P, P, P, P, P, P],
+    'pf3d_truss_kc0': [I64, P, P, P, P, P, P, P, P, I32, P],
+    'pf3d_truss_m': [I64, P, P, P, P, P, I32, P, P, P, I32, P],
+    'pf3d_spring_kc0': [I64, P, P, P, P, P, P, P, P, P, I32, P],
+    'pf3d_rod_fe': [I32, I64, P, P, P, P, P, P, P, P],
     'pf3d_incidence_bytes': [I64, I64, P],
     'pf3d_incidence_plan': [I64, P, I64, P, I64, P, P, P],
     'pf3d_gather_fint': [I64, P, P, P, P, P],

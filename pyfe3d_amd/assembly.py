@@ -20,13 +20,14 @@ import torch
 
 from . import _lib
 
-ETYPE = {'quad4': 0, 'quad4r': 1, 'tria3r': 2, 'beamc': 3}
+ETYPE = {'quad4': 0, 'quad4r': 1, 'tria3r': 2, 'beamc': 3, 'truss': 4, 'spring': 5}
 MATRIX = {'KC0': 0, 'KG': 1, 'KG_stress': 2, 'M': 3}
-NNODES = {'quad4': 4, 'quad4r': 4, 'tria3r': 3, 'beamc': 2}
+NNODES = {'quad4': 4, 'quad4r': 4, 'tria3r': 3, 'beamc': 2, 'truss': 2, 'spring': 2}
 VALUES_ONLY = 1
 OVERWRITE = 2
 SHELL_STRIDE = 32
 BEAM_STRIDE = 16
+SPRING_STRIDE = 6
 
 
 def _ptr(t, dtype, name, numel=None, optional=False):
@@ -127,11 +128,14 @@ def _props(props, pid, E, stride, check=True):
 
 
 def geometry(etype, x, conn, evec=None, stream=None):
-    """Element axes/material rotation/area (quad4.pyx:491-752 and peers).
+    """Element axes/material rotation/area (quad4.pyx:491-752 and peers);
+    evec: xmat (shells), vxy (beams), None (trusses), (E, 6) axes (springs).
 
     Returns (geom (E, 14): r11..r33, m11, m12, m21, m22, area|length;
              xe (E, 3*nn) local coordinates)."""
     E, px, pc, _, pe = _common(etype, x, conn, None, evec)
+    if etype == 'spring':
+        pe = _spring_axes(evec, E)
     geom = torch.empty((E, 14), dtype=torch.float64, device=x.device)
     xe = torch.empty((E, 3 * NNODES[etype]), dtype=torch.float64, device=x.device)
     _lib.call('pf3d_geometry', ETYPE[etype], E, px, pc, pe, geom.data_ptr(), xe.data_ptr(),
@@ -238,6 +242,50 @@ def beam_m(x, conn, vxy, r, c, v, props, mtype=0, pid=None, init_k=None, values_
               _flags(values_only, overwrite), _stream(stream))
 
 
+def truss_kc0(x, conn, r, c, v, props, pid=None, init_k=None, values_only=False,
+              overwrite=False, stream=None, check=True):
+    """Truss KC0 of a batch (update_rotation_matrix(x) + update_probe_xe +
+    update_length + update_KC0; truss.pyx:203-338, 386-801); props are beam
+    records (A, E, G, J used)."""
+    E, px, pc, pk, _ = _common('truss', x, conn, init_k, None, check)
+    _check_span(72 * E, r, c, v, values_only, init_k, check)
+    pr, pcc, pv = _outputs(r, c, v, True, values_only)
+    pp, ppid = _props(props, pid, E, BEAM_STRIDE, check)
+    _lib.call('pf3d_truss_kc0', E, px, pc, pk, ppid, pp, pr, pcc, pv,
+              _flags(values_only, overwrite), _stream(stream))
+
+
+def truss_m(x, conn, r, c, v, props, mtype=0, pid=None, init_k=None, values_only=False,
+            overwrite=False, stream=None, check=True):
+    """Truss mass matrix (update_M, truss.pyx:838-1800), mtype 0 consistent
+    (144 slots), 1 lumped (36 slots)."""
+    E, px, pc, pk, _ = _common('truss', x, conn, init_k, None, check)
+    _check_span(slot_count('truss', 'M', mtype) * E, r, c, v, values_only, init_k, check)
+    pr, pcc, pv = _outputs(r, c, v, True, values_only)
+    pp, ppid = _props(props, pid, E, BEAM_STRIDE, check)
+    _lib.call('pf3d_truss_m', E, px, pc, pk, ppid, pp, int(mtype), pr, pcc, pv,
+              _flags(values_only, overwrite), _stream(stream))
+
+
+def _spring_axes(axes, E):
+    return _ptr(axes, torch.float64, 'axes', 6 * E)
+
+
+def spring_kc0(x, conn, axes, r, c, v, k, pid=None, init_k=None, values_only=False,
+               overwrite=False, stream=None, check=True):
+    """Spring KC0 of a batch (update_rotation_matrix(xi, xj, xk, vxyi, vxyj,
+    vxyk) + update_KC0; spring.pyx:147-206, 295-707).  axes (E, 6): element
+    x axis and XY-plane vector; k: spring records (kxe, kye, kze, krxe, krye,
+    krze), one per pid (or one shared record)."""
+    E, px, pc, pk, _ = _common('spring', x, conn, init_k, None, check)
+    pa = _spring_axes(axes, E)
+    _check_span(72 * E, r, c, v, values_only, init_k, check)
+    pr, pcc, pv = _outputs(r, c, v, True, values_only)
+    pp, ppid = _props(k, pid, E, SPRING_STRIDE, check)
+    _lib.call('pf3d_spring_kc0', E, px, pc, pk, pa, ppid, pp, pr, pcc, pv,
+              _flags(values_only, overwrite), _stream(stream))
+
+
 # ----------------------------------------------------------------------
 # internal forces (update_fint)
 # ----------------------------------------------------------------------
@@ -278,6 +326,26 @@ def beam_fe(x, conn, vxy, u, props, pid=None, fe=None, stream=None, check=True):
         fe = torch.empty((E, 2, 6), dtype=torch.float64, device=x.device)
     _lib.call('pf3d_beam_fe', E, px, pc, pe, ppid, pp, _ptr(u, torch.float64, 'u'),
               _ptr(fe, torch.float64, 'fe', 12 * E), _stream(stream))
+    return fe
+
+
+def rod_fe(etype, x, conn, u, props, axes=None, pid=None, fe=None, stream=None, check=True):
+    """Truss or Spring element internal forces in global axes, shape
+    (E, 2, 6) (truss.pyx:150-202, 339-385, 802-837; spring.pyx:208-294,
+    708-739).  Truss: props are beam records; Spring: props are spring
+    records and axes (E, 6) is required."""
+    if etype not in ('truss', 'spring'):
+        raise ValueError('rod_fe takes truss or spring, got %r' % etype)
+    E, px, pc, _, _ = _common(etype, x, conn, None, None, check)
+    pa = _spring_axes(axes, E) if etype == 'spring' else None
+    if check:
+        _check_nodes(conn, x, u)
+    pp, ppid = _props(props, pid, E, BEAM_STRIDE if etype == 'truss' else SPRING_STRIDE, check)
+    if fe is None:
+        fe = torch.empty((E, 2, 6), dtype=torch.float64, device=x.device)
+    _lib.call('pf3d_rod_fe', ETYPE[etype], E, px, pc, pa, ppid, pp,
+              _ptr(u, torch.float64, 'u'), _ptr(fe, torch.float64, 'fe', 12 * E),
+              _stream(stream))
     return fe
 
 
@@ -322,13 +390,17 @@ class NodeGather:
         return fint
 
 
-def update_fint(etype, x, conn, u, fint, props, gather=None, vxy=None, stream=None, **kw):
+def update_fint(etype, x, conn, u, fint, props, gather=None, vxy=None, axes=None, stream=None,
+                **kw):
     """fint += internal forces of a batch of elements, the loop
     ``for e in elems: e.update_probe_ue(u); e.update_fint(fint, prop)``.
 
-    ``gather`` is a :class:`NodeGather` for ``conn`` (built here if None)."""
+    ``gather`` is a :class:`NodeGather` for ``conn`` (built here if None);
+    vxy for beams, axes for springs."""
     if etype == 'beamc':
         fe = beam_fe(x, conn, vxy, u, props, stream=stream, **kw)
+    elif etype in ('truss', 'spring'):
+        fe = rod_fe(etype, x, conn, u, props, axes=axes, stream=stream, **kw)
     else:
         fe = shell_fe(etype, x, conn, u, props, stream=stream, **kw)
     if gather is None:

@@ -895,7 +895,7 @@ template <int ET>
 __global__ void element_geometry(int64_t n, const double *x, const int64_t *conn,
                                  const double *evec, double *geom, double *xe)
 {
-    constexpr int NN = ET == TRIA3R ? 3 : (ET == BEAMC ? 2 : 4);
+    constexpr int NN = ET == TRIA3R ? 3 : ((ET == BEAMC || ET == TRUSS || ET == SPRING) ? 2 : 4);
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
     V3 X[NN];
@@ -905,10 +905,11 @@ __global__ void element_geometry(int64_t n, const double *x, const int64_t *conn
         X[i] = {px[0], px[1], px[2]};
     }
     V3 ev = {0., 0., 0.};
-    if (evec) ev = {evec[3 * e], evec[3 * e + 1], evec[3 * e + 2]};
+    if (evec && ET != TRUSS && ET != SPRING) ev = {evec[3 * e], evec[3 * e + 1], evec[3 * e + 2]};
     Geom g;
     if (ET == TRIA3R) tria_geometry(X, ev, g);
     else if (ET == BEAMC) beam_geometry(X, ev, g);
+    else if (ET == TRUSS || ET == SPRING) rod_geometry<ET>(X, evec ? evec + 6 * e : nullptr, g);
     else quad_geometry(X, ev, g);
     double *o = geom + 14 * e;
     for (int i = 0; i < 9; i++) o[i] = g.r[i];
@@ -1232,6 +1233,168 @@ __global__ void __launch_bounds__(64) beam_fe(AsmArgs a, double *__restrict__ fe
     }
 }
 
+// ------------------------------------------------------------------------
+// Truss and Spring: 64 rods per block, one wave.  Each lane forms the few
+// numbers that fix its rod's global blocks (the rotated 3x3 stiffness blocks,
+// or the mass scalars and axis products) in LDS; the wave then streams the
+// block's slots, consecutive lanes on consecutive slots.
+//   KC0 (72 slots, truss.pyx:386-801, spring.pyx:295-707): node pair (I, J),
+//     group g (translations, rotations), slot 36 I + 18 g + 6 d + 3 J + e,
+//     value (I == J ? 1 : -1) B_g[d][e] with B_g = R diag(k_3g..k_3g+2) R^T
+//     (truss: k = (E A / L, 0, 0, G J / L, 0, 0)).
+//   Truss consistent mass (144 slots, row-major 12x12, truss.pyx:894-1616):
+//     node block (I, J) = f_IJ [[rho I3, w ex^T], [ex w^T, rt ex ex^T]],
+//     f = L / 3 on the diagonal blocks, L / 6 across, w = ry ez - rz ey.
+//   Truss lumped mass (36 slots, truss.pyx:1617-1800): node-diagonal
+//     L rho / 2 I3 and L rt / 2 ex ex^T.
+// ------------------------------------------------------------------------
+constexpr int ROD_VALS = 19;
+
+template <int ET>
+__device__ __forceinline__ void rod_load(const AsmArgs &a, int64_t e, int64_t *cn, Geom &g)
+{
+    V3 X[2];
+    for (int i = 0; i < 2; i++) {
+        cn[i] = a.conn[2 * e + i];
+        const double *px = a.x + cn[i] / 2;
+        X[i] = {px[0], px[1], px[2]};
+    }
+    rod_geometry<ET>(X, ET == SPRING ? a.evec + 6 * e : nullptr, g);
+}
+
+// local stiffness per DOF of a rod: k[0..5] along x, y, z, rx, ry, rz
+template <int ET>
+__device__ __forceinline__ void rod_k(const AsmArgs &a, int64_t e, double L, double *k)
+{
+    const int64_t p = a.pid ? (int64_t)a.pid[e] : 0;
+    if (ET == TRUSS) {
+        const double *bp = a.props + p * BEAM_STRIDE;
+        k[0] = bp[0] * bp[1] / L;
+        k[1] = k[2] = 0.;
+        k[3] = bp[2] * bp[6] / L;
+        k[4] = k[5] = 0.;
+    } else {
+        const double *kp = a.props + p * SPRING_STRIDE;
+        for (int i = 0; i < 6; i++) k[i] = kp[i];
+    }
+}
+
+template <int ET, int MAT>
+__global__ void __launch_bounds__(64) rod_assemble(AsmArgs a)
+{
+    __shared__ double sv[64][ROD_VALS + 2];
+    __shared__ int64_t sconn[64][2];
+    __shared__ int64_t sk0[64];
+    const int lane = threadIdx.x;
+    const int64_t e0 = (int64_t)blockIdx.x * 64, e = e0 + lane;
+    const bool lumped = (MAT == MASS) && a.mtype == 1;
+    const int nslot = MAT == KC0 ? 72 : (lumped ? 36 : 144);
+    if (e < a.n_elem) {
+        int64_t cn[2];
+        Geom g;
+        rod_load<ET>(a, e, cn, g);
+        sconn[lane][0] = cn[0];
+        sconn[lane][1] = cn[1];
+        sk0[lane] = a.init_k ? a.init_k[e] : (int64_t)nslot * e;
+        double *o = sv[lane];
+        if (MAT == KC0) {
+            double k[6];
+            rod_k<ET>(a, e, g.area, k);
+            for (int grp = 0; grp < 2; grp++)
+                for (int d = 0; d < 3; d++)
+                    for (int q = 0; q < 3; q++) {
+                        double s = 0.;
+                        for (int t = 0; t < 3; t++)
+                            s += k[3 * grp + t] * g.r[3 * d + t] * g.r[3 * q + t];
+                        o[9 * grp + 3 * d + q] = s;
+                    }
+        } else {
+            const double *bp = a.props + (a.pid ? (int64_t)a.pid[e] : 0) * BEAM_STRIDE;
+            const double L = g.area, rho = bp[9], ry = bp[10], rz = bp[11];
+            const double rt = bp[12] + bp[13];
+            const double f = lumped ? L / 2 : L / 3;
+            o[0] = f * rho;
+            for (int d = 0; d < 3; d++) {
+                o[1 + d] = g.r[3 * d];                                          // ex
+                o[4 + d] = f * (ry * g.r[3 * d + 2] - rz * g.r[3 * d + 1]);     // f w
+            }
+            for (int d = 0; d < 3; d++)
+                for (int q = 0; q < 3; q++) o[7 + 3 * d + q] = f * rt * g.r[3 * d] * g.r[3 * q];
+        }
+    }
+    __syncthreads();
+    const int nel = (int)((a.n_elem - e0) < 64 ? (a.n_elem - e0) : 64);
+    const bool write_idx = !(a.flags & F_VALUES_ONLY);
+    const bool overwrite = (a.flags & F_OVERWRITE);
+    const int total = nel * nslot;
+    for (int idx = lane; idx < total; idx += 64) {
+        const int el = idx / nslot, k = idx - el * nslot;
+        const double *o = sv[el];
+        int rl, cl;
+        double val;
+        if (MAT == KC0) {
+            const int I = k / 36, grp = (k / 18) & 1, d = (k % 18) / 6, J = (k % 6) / 3,
+                      q = k % 3;
+            rl = 6 * I + 3 * grp + d;
+            cl = 6 * J + 3 * grp + q;
+            val = (I == J ? 1. : -1.) * o[9 * grp + 3 * d + q];
+        } else if (lumped) {
+            const int n = k / 18, grp = (k % 18) / 9, t = k % 9, d = t / 3, q = t % 3;
+            rl = 6 * n + 3 * grp + d;
+            cl = 6 * n + 3 * grp + q;
+            val = grp ? o[7 + 3 * d + q] : (d == q ? o[0] : 0.);
+        } else {
+            rl = k / 12;
+            cl = k - 12 * rl;
+            const int I = rl / 6, J = cl / 6, p = rl % 6, q = cl % 6;
+            const double f = I == J ? 1. : 0.5;
+            if (p < 3 && q < 3) val = p == q ? f * o[0] : 0.;
+            else if (p < 3) val = f * o[4 + p] * o[1 + q - 3];
+            else if (q < 3) val = f * o[1 + p - 3] * o[4 + q];
+            else val = f * o[7 + 3 * (p - 3) + q - 3];
+        }
+        const int64_t k0 = sk0[el] + k;
+        if (overwrite) a.v[k0] = val;
+        else a.v[k0] += val;
+        if (write_idx) {
+            a.r[k0] = sconn[el][rl / 6] + rl % 6;
+            a.c[k0] = sconn[el][cl / 6] + cl % 6;
+        }
+    }
+}
+
+// Rod internal forces in global axes (truss.pyx:150-202 ue, :339-385 finte,
+// :802-837 rotation; spring.pyx:208-294, 708-739): local forces
+// k_t (ue_0 - ue_1) at node 0 and their negative at node 1.
+template <int ET>
+__global__ void __launch_bounds__(64) rod_fe(AsmArgs a, double *__restrict__ fe)
+{
+    const int64_t e = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (e >= a.n_elem) return;
+    int64_t cn[2];
+    Geom g;
+    rod_load<ET>(a, e, cn, g);
+    double k[6];
+    rod_k<ET>(a, e, g.area, k);
+    double fl[6];
+    for (int b = 0; b < 2; b++)
+        for (int t = 0; t < 3; t++) {
+            double u0 = 0., u1 = 0.;
+            for (int q = 0; q < 3; q++) {
+                u0 += g.r[3 * q + t] * a.u[cn[0] + 3 * b + q];
+                u1 += g.r[3 * q + t] * a.u[cn[1] + 3 * b + q];
+            }
+            fl[3 * b + t] = k[3 * b + t] * (u0 - u1);
+        }
+    for (int b = 0; b < 2; b++)
+        for (int q = 0; q < 3; q++) {
+            const double s = g.r[3 * q] * fl[3 * b] + g.r[3 * q + 1] * fl[3 * b + 1] +
+                             g.r[3 * q + 2] * fl[3 * b + 2];
+            fe[e * 12 + 3 * b + q] = s;
+            fe[e * 12 + 6 + 3 * b + q] = -s;
+        }
+}
+
 }  // namespace pf3d
 
 // ------------------------------------------------------------------------
@@ -1320,10 +1483,43 @@ hipError_t pf3d_launch_geometry(int etype, int64_t n, const double *x, const int
         hipLaunchKernelGGL(element_geometry<TRIA3R>, dim3(blocks), dim3(128), 0, s, n, x, conn,
                            evec, geom, xe);
         break;
+    case TRUSS:
+        hipLaunchKernelGGL(element_geometry<TRUSS>, dim3(blocks), dim3(128), 0, s, n, x, conn,
+                           evec, geom, xe);
+        break;
+    case SPRING:
+        hipLaunchKernelGGL(element_geometry<SPRING>, dim3(blocks), dim3(128), 0, s, n, x, conn,
+                           evec, geom, xe);
+        break;
     default:
         hipLaunchKernelGGL(element_geometry<BEAMC>, dim3(blocks), dim3(128), 0, s, n, x, conn,
                            evec, geom, xe);
     }
+    return hipGetLastError();
+}
+
+hipError_t pf3d_launch_rod(int etype, int mat, const AsmArgs &a, hipStream_t s)
+{
+    const int64_t blocks = (a.n_elem + 63) / 64;
+    if (blocks == 0) return hipSuccess;
+    const dim3 grid((unsigned)blocks), blk(64);
+    if (etype == SPRING)
+        hipLaunchKernelGGL((rod_assemble<SPRING, KC0>), grid, blk, 0, s, a);
+    else if (mat == KC0)
+        hipLaunchKernelGGL((rod_assemble<TRUSS, KC0>), grid, blk, 0, s, a);
+    else
+        hipLaunchKernelGGL((rod_assemble<TRUSS, MASS>), grid, blk, 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t pf3d_launch_rod_fe(int etype, const AsmArgs &a, double *fe, hipStream_t s)
+{
+    const int64_t blocks = (a.n_elem + 63) / 64;
+    if (blocks == 0) return hipSuccess;
+    if (etype == SPRING)
+        hipLaunchKernelGGL(rod_fe<SPRING>, dim3((unsigned)blocks), dim3(64), 0, s, a, fe);
+    else
+        hipLaunchKernelGGL(rod_fe<TRUSS>, dim3((unsigned)blocks), dim3(64), 0, s, a, fe);
     return hipGetLastError();
 }
 

@@ -11,6 +11,8 @@ hipError_t pf3d_launch_shell(int etype, int mat, const pf3d::AsmArgs &a, hipStre
 hipError_t pf3d_launch_beam(int mat, const pf3d::AsmArgs &a, hipStream_t s);
 hipError_t pf3d_launch_shell_fe(int etype, const pf3d::AsmArgs &a, double *fe, hipStream_t s);
 hipError_t pf3d_launch_beam_fe(const pf3d::AsmArgs &a, double *fe, hipStream_t s);
+hipError_t pf3d_launch_rod(int etype, int mat, const pf3d::AsmArgs &a, hipStream_t s);
+hipError_t pf3d_launch_rod_fe(int etype, const pf3d::AsmArgs &a, double *fe, hipStream_t s);
 hipError_t pf3d_launch_geometry(int etype, int64_t n, const double *x, const int64_t *conn,
                                 const double *evec, double *geom, double *xe, hipStream_t s);
 
@@ -41,6 +43,18 @@ const char *pf3d_last_error(void) { return g_err.c_str(); }
 
 int pf3d_slot_count(int etype, int matrix, int mtype)
 {
+    if (etype == PF3D_TRUSS) {
+        if (matrix == PF3D_KC0) return 72;
+        if (matrix == PF3D_M) {
+            if (mtype < 0 || mtype > 1) return pf3d_fail(PF3D_E_ARG, "Truss mtype must be 0 or 1");
+            return mtype == 0 ? 144 : 36;
+        }
+        return pf3d_fail(PF3D_E_ARG, "Truss has KC0 and M only");
+    }
+    if (etype == PF3D_SPRING) {
+        if (matrix == PF3D_KC0) return 72;
+        return pf3d_fail(PF3D_E_ARG, "Spring has KC0 only");
+    }
     if (etype == PF3D_BEAMC) {
         if (matrix == PF3D_KC0 || matrix == PF3D_KG) return 144;
         if (matrix == PF3D_M) {
@@ -65,11 +79,12 @@ int pf3d_slot_count(int etype, int matrix, int mtype)
 int pf3d_geometry(int etype, int64_t n_elem, const double *x, const int64_t *conn,
                   const double *evec, double *geom, double *xe, void *stream)
 {
-    if (etype < 0 || etype > PF3D_BEAMC) return pf3d_fail(PF3D_E_ARG, "bad etype %d", etype);
+    if (etype < 0 || etype > PF3D_SPRING) return pf3d_fail(PF3D_E_ARG, "bad etype %d", etype);
     if (n_elem < 0) return pf3d_fail(PF3D_E_ARG, "negative n_elem");
     if (n_elem == 0) return PF3D_OK;
     if (!x || !conn || !geom) return pf3d_fail(PF3D_E_ARG, "null x/conn/geom");
     if (etype == PF3D_BEAMC && !evec) return pf3d_fail(PF3D_E_ARG, "BeamC needs vxy");
+    if (etype == PF3D_SPRING && !evec) return pf3d_fail(PF3D_E_ARG, "Spring needs its axes");
     return pf3d_hip_check(pf3d_launch_geometry(etype, n_elem, x, conn, evec, geom, xe,
                                                (hipStream_t)stream),
                           "pf3d_geometry");
@@ -237,6 +252,78 @@ int pf3d_beam_fe(int64_t n_elem, const double *x, const int64_t *conn, const dou
     a.n_elem = n_elem; a.x = x; a.conn = conn; a.evec = vxy; a.pid = pid; a.props = props;
     a.u = u;
     return pf3d_hip_check(pf3d_launch_beam_fe(a, fe, (hipStream_t)stream), "pf3d_beam_fe");
+}
+
+static int rod_check(int64_t n_elem, const double *x, const int64_t *conn, const double *props,
+                     int64_t *r, int64_t *c, double *v, int flags)
+{
+    if (n_elem < 0) return pf3d_fail(PF3D_E_ARG, "negative n_elem");
+    if (n_elem > (int64_t)2147483647 * 64) return pf3d_fail(PF3D_E_ARG, "n_elem too large");
+    if (n_elem == 0) return 1;
+    if (!x || !conn || !v || !props) return pf3d_fail(PF3D_E_ARG, "null argument");
+    if (!(flags & PF3D_VALUES_ONLY) && (!r || !c))
+        return pf3d_fail(PF3D_E_ARG, "null r/c without PF3D_VALUES_ONLY");
+    if (flags & ~(PF3D_VALUES_ONLY | PF3D_OVERWRITE))
+        return pf3d_fail(PF3D_E_ARG, "unknown flags 0x%x", flags);
+    return 0;
+}
+
+int pf3d_truss_kc0(int64_t n_elem, const double *x, const int64_t *conn, const int64_t *init_k,
+                   const int32_t *pid, const double *props, int64_t *r, int64_t *c, double *v,
+                   int flags, void *stream)
+{
+    int st = rod_check(n_elem, x, conn, props, r, c, v, flags);
+    if (st) return st > 0 ? PF3D_OK : st;
+    pf3d::AsmArgs a{};
+    a.n_elem = n_elem; a.x = x; a.conn = conn; a.init_k = init_k; a.pid = pid; a.props = props;
+    a.flags = flags; a.r = r; a.c = c; a.v = v;
+    return pf3d_hip_check(pf3d_launch_rod(PF3D_TRUSS, PF3D_KC0, a, (hipStream_t)stream),
+                          "pf3d_truss_kc0");
+}
+
+int pf3d_truss_m(int64_t n_elem, const double *x, const int64_t *conn, const int64_t *init_k,
+                 const int32_t *pid, const double *props, int mtype, int64_t *r, int64_t *c,
+                 double *v, int flags, void *stream)
+{
+    int st = rod_check(n_elem, x, conn, props, r, c, v, flags);
+    if (st) return st > 0 ? PF3D_OK : st;
+    if (mtype < 0 || mtype > 1) return pf3d_fail(PF3D_E_ARG, "Truss mtype must be 0 or 1");
+    pf3d::AsmArgs a{};
+    a.n_elem = n_elem; a.x = x; a.conn = conn; a.init_k = init_k; a.pid = pid; a.props = props;
+    a.mtype = mtype; a.flags = flags; a.r = r; a.c = c; a.v = v;
+    return pf3d_hip_check(pf3d_launch_rod(PF3D_TRUSS, PF3D_M, a, (hipStream_t)stream),
+                          "pf3d_truss_m");
+}
+
+int pf3d_spring_kc0(int64_t n_elem, const double *x, const int64_t *conn, const int64_t *init_k,
+                    const double *axes, const int32_t *pid, const double *k, int64_t *r,
+                    int64_t *c, double *v, int flags, void *stream)
+{
+    int st = rod_check(n_elem, x, conn, k, r, c, v, flags);
+    if (st) return st > 0 ? PF3D_OK : st;
+    if (!axes) return pf3d_fail(PF3D_E_ARG, "null axes");
+    pf3d::AsmArgs a{};
+    a.n_elem = n_elem; a.x = x; a.conn = conn; a.init_k = init_k; a.evec = axes; a.pid = pid;
+    a.props = k; a.flags = flags; a.r = r; a.c = c; a.v = v;
+    return pf3d_hip_check(pf3d_launch_rod(PF3D_SPRING, PF3D_KC0, a, (hipStream_t)stream),
+                          "pf3d_spring_kc0");
+}
+
+int pf3d_rod_fe(int etype, int64_t n_elem, const double *x, const int64_t *conn,
+                const double *axes, const int32_t *pid, const double *props, const double *u,
+                double *fe, void *stream)
+{
+    if (etype != PF3D_TRUSS && etype != PF3D_SPRING)
+        return pf3d_fail(PF3D_E_ARG, "bad rod etype %d", etype);
+    if (n_elem < 0) return pf3d_fail(PF3D_E_ARG, "negative n_elem");
+    if (n_elem > (int64_t)2147483647 * 64) return pf3d_fail(PF3D_E_ARG, "n_elem too large");
+    if (n_elem == 0) return PF3D_OK;
+    if (!x || !conn || !props || !u || !fe) return pf3d_fail(PF3D_E_ARG, "null argument");
+    if (etype == PF3D_SPRING && !axes) return pf3d_fail(PF3D_E_ARG, "null axes");
+    pf3d::AsmArgs a{};
+    a.n_elem = n_elem; a.x = x; a.conn = conn; a.evec = axes; a.pid = pid; a.props = props;
+    a.u = u;
+    return pf3d_hip_check(pf3d_launch_rod_fe(etype, a, fe, (hipStream_t)stream), "pf3d_rod_fe");
 }
 
 }  // extern "C"

@@ -10,10 +10,11 @@
 namespace pf3d {
 
 constexpr int SHELL_STRIDE = 32;  // doubles per shell property record
-constexpr int BEAM_STRIDE = 16;   // doubles per beam property record
+constexpr int BEAM_STRIDE = 16;   // doubles per beam property record (BeamC, Truss)
+constexpr int SPRING_STRIDE = 6;  // kxe kye kze krxe krye krze per spring record
 constexpr double GP2 = 0.5773502691896257645092;
 
-enum ElemType { QUAD4 = 0, QUAD4R = 1, TRIA3R = 2, BEAMC = 3 };
+enum ElemType { QUAD4 = 0, QUAD4R = 1, TRIA3R = 2, BEAMC = 3, TRUSS = 4, SPRING = 5 };
 enum Matrix { KC0 = 0, KG = 1, KGS = 2, MASS = 3 };
 
 // flags of the assembly entry points
@@ -26,7 +27,8 @@ struct AsmArgs {
     const double *x;        // [3*n_nodes]
     const int64_t *conn;    // [n_elem*nn] DOF offsets c_I (6*node position)
     const int64_t *init_k;  // [n_elem] first slot, or null for size*e
-    const double *evec;     // [n_elem*3] xmat (shells) or vxy (beams); may be null
+    const double *evec;     // [n_elem*3] xmat (shells) or vxy (beams), [n_elem*6] x axis +
+                            // vxy (springs); may be null where optional
     const int32_t *pid;     // [n_elem] property index, or null for 0
     const double *props;    // property records
     const double *k6rot;    // [n_elem] or null (100.)
@@ -162,6 +164,30 @@ __device__ __forceinline__ void beam_geometry(const V3 *X, V3 vxy, Geom &g)
     set_axes(g, ex, ey, ez);
     g.m[0] = 1.; g.m[1] = 0.; g.m[2] = 0.; g.m[3] = 1.;
     g.area = sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+}
+
+// Truss: truss.pyx:203-272 (x along the element, off-axis vector
+// (xj, xk, xi) for the plane of y), :323-342 (length).
+// Spring: spring.pyx:147-206 (x axis and XY-plane vector given per element).
+template <int ET>
+__device__ __forceinline__ void rod_geometry(const V3 *X, const double *ev, Geom &g)
+{
+#pragma clang fp contract(off)
+    V3 ex, v;
+    if (ET == TRUSS) {
+        const V3 d = sub(X[1], X[0]);
+        ex = unit(d);
+        v = {ex.y, ex.z, ex.x};
+        g.area = sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+    } else {
+        ex = unit(V3{ev[0], ev[1], ev[2]});
+        v = {ev[3], ev[4], ev[5]};
+        g.area = 0.;
+    }
+    const V3 ez = unit(cross(ex, v));
+    const V3 ey = unit(cross(ez, ex));
+    set_axes(g, ex, ey, ez);
+    g.m[0] = 1.; g.m[1] = 0.; g.m[2] = 0.; g.m[3] = 1.;
 }
 
 // Laminate tables in element axes: A' = T A T^T when m12 != 0 (quad4.pyx:845-899);

@@ -24,7 +24,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF = '/root/reference'
 BUILD = os.path.join(HERE, '_refbuild')
-MODULES = ['shellprop', 'beamprop', 'quad4', 'quad4r', 'tria3r', 'beamc']
+MODULES = ['shellprop', 'beamprop', 'quad4', 'quad4r', 'tria3r', 'beamc', 'truss', 'spring']
 
 
 def build_reference():
@@ -330,6 +330,109 @@ BEAM_CASES = [
 ]
 
 
+def run_truss_case(mods, case, rng):
+    tr = mods['truss']
+    BeamProp = mods['beamprop'].BeamProp
+    n = case['n']
+    t = np.linspace(0, 1, n + 1)
+    P = np.vstack([case['L'] * t, 0.3 * np.sin(3 * t) + 0.01 * rng.uniform(-1, 1, t.size),
+                   0.2 * t ** 2 - 0.1 * t]).T
+    P = P @ rot3(*case['rot']).T
+    x = P.ravel().astype(np.float64)
+    N = n + 1
+    conn = np.array([[i, i + 1] for i in range(n)], dtype=np.int64)
+    E = n
+    props, precs = [], []
+    for e in range(E):
+        p = BeamProp()
+        r_ = 0.01 * (1 + 0.2 * e)
+        p.A = np.pi * r_ ** 2
+        p.E = 200e9 * (1 + 0.1 * e)
+        p.G = p.E / 2 / 1.3
+        p.J = np.pi * r_ ** 4 / 2
+        p.Iyy = p.Izz = p.J / 2
+        rho = 7850.
+        p.intrho = rho * p.A
+        p.intrhoy = rho * p.A * 1e-3 * (e - 2)
+        p.intrhoz = rho * p.A * 2e-3 * (1 - e)
+        p.intrhoy2 = rho * p.Izz
+        p.intrhoz2 = rho * p.Iyy * 1.1
+        props.append(p)
+        precs.append(beam_record(p))
+    precs = np.array(precs)
+    perm = rng.permutation(E)
+    d = tr.TrussData()
+    sizes = dict(KC0=d.KC0_SPARSE_SIZE, M=d.M_SPARSE_SIZE)
+    u = rng.uniform(-1, 1, 6 * N) * 1e-3
+    out = {}
+    for key in ['KC0', 'M0', 'M1']:
+        base = 'M' if key.startswith('M') else key
+        m = sizes[base] * E
+        out[key + 'r'] = np.zeros(m, dtype=np.int64)
+        out[key + 'c'] = np.zeros(m, dtype=np.int64)
+        out[key + 'v'] = np.zeros(m, dtype=np.float64)
+    fint = np.zeros(6 * N)
+    geom = np.zeros((E, 10))
+    probe = tr.TrussProbe()
+    for e in range(E):
+        el = tr.Truss(probe)
+        el.n1, el.n2 = int(conn[e, 0]), int(conn[e, 1])
+        el.c1, el.c2 = int(6 * conn[e, 0]), int(6 * conn[e, 1])
+        el.init_k_KC0 = int(sizes['KC0'] * perm[e])
+        el.init_k_M = int(sizes['M'] * perm[e])
+        el.update_rotation_matrix(x)
+        el.update_probe_xe(x)
+        el.update_length()
+        geom[e] = [el.r11, el.r12, el.r13, el.r21, el.r22, el.r23, el.r31, el.r32,
+                   el.r33, el.length]
+        p = props[e]
+        el.update_KC0(out['KC0r'], out['KC0c'], out['KC0v'], p)
+        el.update_M(out['M0r'], out['M0c'], out['M0v'], p, 0)
+        el.update_M(out['M1r'], out['M1c'], out['M1v'], p, 1)
+        el.update_probe_ue(u)
+        el.update_fint(fint, p)
+    return dict(x=x, conn=conn, perm=perm, props=precs, u=u, geom=geom, fint=fint, **out)
+
+
+def run_spring_case(mods, case, rng):
+    sp_ = mods['spring']
+    N = case['nodes']
+    x = rng.uniform(-1, 1, 3 * N)
+    conn = np.array(case['pairs'], dtype=np.int64)
+    E = conn.shape[0]
+    kk = np.exp(rng.uniform(np.log(1e2), np.log(1e7), (E, 6)))
+    axes = rng.normal(size=(E, 3))
+    vxy = rng.normal(size=(E, 3))
+    perm = rng.permutation(E)
+    d = sp_.SpringData()
+    size = d.KC0_SPARSE_SIZE
+    u = rng.uniform(-1, 1, 6 * N) * 1e-3
+    out = dict(KC0r=np.zeros(size * E, np.int64), KC0c=np.zeros(size * E, np.int64),
+               KC0v=np.zeros(size * E))
+    fint = np.zeros(6 * N)
+    geom = np.zeros((E, 9))
+    probe = sp_.SpringProbe()
+    for e in range(E):
+        el = sp_.Spring(probe)
+        el.n1, el.n2 = int(conn[e, 0]), int(conn[e, 1])
+        el.c1, el.c2 = int(6 * conn[e, 0]), int(6 * conn[e, 1])
+        el.init_k_KC0 = int(size * perm[e])
+        el.kxe, el.kye, el.kze, el.krxe, el.krye, el.krze = [float(v) for v in kk[e]]
+        el.update_rotation_matrix(*[float(v) for v in axes[e]], *[float(v) for v in vxy[e]])
+        geom[e] = [el.r11, el.r12, el.r13, el.r21, el.r22, el.r23, el.r31, el.r32, el.r33]
+        el.update_KC0(out['KC0r'], out['KC0c'], out['KC0v'])
+        el.update_probe_ue(u)
+        el.update_fint(fint)
+    return dict(x=x, conn=conn, perm=perm, k=kk, axes=axes, vxy=vxy, u=u, geom=geom,
+                fint=fint, **out)
+
+
+TRUSS_CASES = [dict(name='polyline_3d', n=5, L=1.2, rot=(0.4, -0.3, 0.9)),
+               dict(name='short_rotated', n=3, L=0.2, rot=(-1.1, 0.6, 2.0))]
+SPRING_CASES = [dict(name='mixed', nodes=5, pairs=[[0, 1], [1, 2], [2, 3], [3, 4], [0, 4], [1, 3]]),
+                dict(name='chain', nodes=4, pairs=[[0, 1], [1, 2], [2, 3]])]
+
+
 def shellprop_cases(spu):
     """Laminate builder vectors (ABD, E, scf, integrals) for the host table builder."""
     cases = []
@@ -351,8 +454,26 @@ def shellprop_cases(spu):
     return specs, np.array(recs)
 
 
+def write_rods(mods):
+    """Truss and Spring fixtures (their own seed: the other files are not
+    touched when only these are regenerated)."""
+    rng = np.random.default_rng(20261203)
+    for name, cases, fn in (('truss', TRUSS_CASES, run_truss_case),
+                            ('spring', SPRING_CASES, run_spring_case)):
+        res = {}
+        for case in cases:
+            r = fn(mods, case, rng)
+            for k, v in r.items():
+                res[case['name'] + '__' + k] = v
+        np.savez_compressed(os.path.join(HERE, 'golden_%s.npz' % name), **res)
+        print(name, 'cases', [c['name'] for c in cases])
+
+
 if __name__ == '__main__':
     mods = build_reference()
+    write_rods(mods)
+    if '--rods-only' in sys.argv:
+        sys.exit(0)
     spu = load_shellprop_utils(mods)
     rng = np.random.default_rng(20261202)
     for etype in ['quad4', 'quad4r', 'tria3r']:

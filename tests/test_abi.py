@@ -40,6 +40,11 @@ def test_host_only_queries():
     assert lib.pf3d_slot_count(0, 1, 0) == 144
     assert lib.pf3d_slot_count(0, 3, 2) == 288
     assert lib.pf3d_slot_count(2, 3, 0) == 270
+    assert lib.pf3d_slot_count(4, 0, 0) == 72    # Truss KC0
+    assert lib.pf3d_slot_count(4, 3, 0) == 144   # Truss consistent mass
+    assert lib.pf3d_slot_count(4, 3, 1) == 36    # Truss lumped mass
+    assert lib.pf3d_slot_count(5, 0, 0) == 72    # Spring KC0
+    assert lib.pf3d_slot_count(5, 3, 0) < 0      # Spring has no mass
     assert lib.pf3d_slot_count(9, 0, 0) < 0
     assert b'etype' in lib.pf3d_last_error()
 

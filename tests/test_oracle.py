@@ -233,3 +233,78 @@ def test_fint_is_kc0_times_u(etype):
         ku = K @ d['u']
         f = _fint_oracle(etype, d)
         assert np.abs(f - ku).max() <= 1e-11 * np.abs(ku).max(), (etype, name)
+
+
+def _rod_evec(etype, d):
+    """Truss: no orientation input (axes from the nodes); Spring: x axis and
+    XY-plane vector per element."""
+    E = d['conn'].shape[0]
+    if etype == 'truss':
+        return np.zeros((E, 3))
+    return np.concatenate([d['axes'], d['vxy']], axis=1)
+
+
+def _rod_props(etype, d):
+    return d['props'] if etype == 'truss' else d['k']
+
+
+ROD_MATS = {'truss': (('KC0', 'KC0', 0), ('M0', 'M', 0), ('M1', 'M', 1)),
+            'spring': (('KC0', 'KC0', 0),)}
+
+
+@pytest.mark.parametrize('etype', ['truss', 'spring'])
+def test_rod_golden(etype):
+    """Truss (truss.pyx:203-1800) and Spring (spring.pyx:147-707): rotation
+    matrix, length, every KC0 / mass slot and the internal forces."""
+    for name, d in cases(etype):
+        E = d['conn'].shape[0]
+        ev = _rod_evec(etype, d)
+        geom, _ = O.geometry(etype, d['x'], d['conn'], ev)
+        np.testing.assert_allclose(geom[:, :9], d['geom'][:, :9], rtol=0, atol=1e-14)
+        if etype == 'truss':
+            np.testing.assert_allclose(geom[:, 13], d['geom'][:, 9], rtol=1e-14)
+        for key, mat, mtype in ROD_MATS[etype]:
+            size = d[key + 'v'].size // E
+            r = np.zeros(size * E, np.int64); c = np.zeros_like(r); v = np.zeros(size * E)
+            O.assemble(etype, mat, d['x'], d['conn'], size * d['perm'], ev, np.arange(E),
+                       _rod_props(etype, d), r, c, v, mtype=mtype)
+            assert (r == d[key + 'r']).all() and (c == d[key + 'c']).all(), (name, key)
+            assert element_errors(v, d[key + 'v'], size, d['perm']) <= TOL, (name, key)
+        f = O.fint(etype, d['x'], d['conn'], ev, np.arange(E), _rod_props(etype, d), d['u'])
+        assert np.abs(f - d['fint']).max() <= FINT_TOL * np.abs(d['fint']).max(), name
+        # linear elements: the internal forces are KC0 u
+        import scipy.sparse as ss
+        size = d['KC0v'].size // E
+        r = np.zeros(size * E, np.int64); c = np.zeros_like(r); v = np.zeros(size * E)
+        O.assemble(etype, 'KC0', d['x'], d['conn'], size * d['perm'], ev, np.arange(E),
+                   _rod_props(etype, d), r, c, v)
+        n = d['u'].size
+        ku = ss.coo_matrix((v, (r, c)), shape=(n, n)).tocsr() @ d['u']
+        assert np.abs(f - ku).max() <= 1e-12 * np.abs(ku).max(), name
+
+
+def test_rod_invariants():
+    """Truss: a rigid translation produces no force, total translational
+    mass is rho A L for both mass types, the axial stiffness of a rotated
+    bar is E A / L along its axis; Spring: stiffness along the given axes."""
+    L, A, Em = 1.3, 2e-4, 70e9
+    bp = np.zeros(16)
+    bp[:7] = [A, Em, Em / 2.6, 0, 0, 0, 3e-8]
+    bp[9:14] = [2.7e3 * A, 1e-3, -2e-3, 4e-6, 5e-6]
+    d = np.array([0.3, -0.5, 0.8]); d /= np.linalg.norm(d)
+    X = np.array([[0.1, 0.2, 0.3], [0.1, 0.2, 0.3] + L * d])
+    K = _dense_element('truss', 'KC0', X, np.zeros(3), bp)
+    t = np.zeros(12); t[0::6] = 1
+    assert np.abs(K @ t).max() <= 1e-9 * np.abs(K).max()
+    s = np.zeros(12); s[6:9] = d  # stretch along the axis
+    assert np.isclose(s @ K @ s, Em * A / L, rtol=1e-13)
+    for mtype in (0, 1):
+        M = _dense_element('truss', 'M', X, np.zeros(3), bp, mtype=mtype)
+        assert np.isclose(t @ M @ t, bp[9] * L, rtol=1e-14)
+        assert np.allclose(M, M.T, rtol=0, atol=1e-12 * np.abs(M).max())
+    kk = np.array([1e3, 2e3, 3e3, 4e3, 5e3, 6e3, 0, 0])
+    ax = np.array([0., 0., 1.]); vxy = np.array([1., 0., 0.])  # x_e = Z, y_e = X, z_e = Y
+    Ks = _dense_element('spring', 'KC0', X, np.concatenate([ax, vxy]), kk)
+    assert np.isclose(Ks[2, 2], 1e3) and np.isclose(Ks[0, 0], 2e3) and np.isclose(Ks[1, 1], 3e3)
+    assert np.isclose(Ks[5, 5], 4e3) and np.isclose(Ks[3, 3], 5e3) and np.isclose(Ks[4, 4], 6e3)
+    assert np.isclose(Ks[2, 8], -1e3)
