@@ -221,6 +221,69 @@ def modal_rates(n, log, k=6):
     return out
 
 
+# algorithmic bytes per element of the values-only KC0 launch (values out,
+# int64 DOF offsets and coordinates in)
+PANEL_KC0_BYTES = {'quad4': 576 * 8 + 4 * 8 + 12 * 8, 'tria3r': 324 * 8 + 3 * 8 + 9 * 8,
+                   'beamc': 144 * 8 + 2 * 8 + 6 * 8 + 3 * 8}
+PANEL_KG_BYTES = {'quad4': 144 * 8 + 4 * 8 + 12 * 8 + 24 * 8,
+                  'tria3r': 81 * 8 + 3 * 8 + 9 * 8 + 18 * 8,
+                  'beamc': 144 * 8 + 2 * 8 + 6 * 8 + 3 * 8 + 12 * 8}
+
+
+def panel_rates(n, log, steps=10, maxiter=200000):
+    """Stiffened panel (n x n Quad4 skin, Tria3R cut-out fill, BeamC
+    stringers on every 16th node row): KC0 and KG of all three element
+    types in one launch each, and the static solve to 1e-8 (plain FP64 PCG
+    vs FP32/FP64 refinement)."""
+    import torch
+    from pyfe3d_amd import assembly as A
+    from pyfe3d_amd.panel import StiffenedPanel
+    from pyfe3d_amd.solver import StaticSystem
+    from pyfe3d_amd.sparse import BSRPlan
+    p = StiffenedPanel(n=n)
+    stream = torch.cuda.current_stream()
+    x = torch.as_tensor(p.mesh.x, device='cuda')
+    groups, total = p.groups('KC0')
+    counts = {g.etype: g.conn.shape[0] for g in groups}
+    r = torch.empty(total, dtype=torch.int64, device='cuda')
+    c = torch.empty_like(r)
+    v = torch.empty(total, dtype=torch.float64, device='cuda')
+    A.mixed_assemble('KC0', groups, x, r, c, v, overwrite=True)
+    ms_k, _ = timed(lambda: A.mixed_assemble('KC0', groups, x, None, None, v, values_only=True,
+                                             overwrite=True, check=False), steps, 2, stream)
+    kbytes = sum(PANEL_KC0_BYTES[t] * e for t, e in counts.items())
+    out = dict(elements=counts, dof=p.n_dof, slots=total,
+               kc0_mixed=dict(ms=ms_k, elements_per_s=sum(counts.values()) / (ms_k * 1e-3),
+                              gbs=kbytes / (ms_k * 1e-3) / 1e9))
+    log('panel %dx%d: %s, KC0 one launch %.3f ms (%.0f GB/s)'
+        % (n, n, counts, ms_k, kbytes / (ms_k * 1e-3) / 1e9))
+    plan = BSRPlan(r, c, p.n_nodes)
+    sysm = StaticSystem(plan, plan.apply(v), p.fixed())
+    del r, c, v
+    f = p.loads(Nxx=1e3, p=1e3)
+    sol = {}
+    for method in ('fp64', 'mixed'):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        u, info = sysm.solve(f, method=method, rtol=1e-8, maxiter=maxiter)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        sol[method] = dict(ms=dt * 1e3, solves_per_s=1. / dt, iters=info['iters'],
+                           outer=info['outer'], relres=info['relres'])
+        log('panel solve %s: %.1f ms, %s' % (method, dt * 1e3, info))
+    out['solve'] = sol
+    gk, totk = p.groups('KG')
+    vg = torch.empty(totk, dtype=torch.float64, device='cuda')
+    ms_g, _ = timed(lambda: A.mixed_assemble('KG', gk, x, None, None, vg, u=u, values_only=True,
+                                             overwrite=True, check=False), steps, 2, stream)
+    gbytes = sum(PANEL_KG_BYTES[t] * e for t, e in counts.items())
+    out['kg_mixed'] = dict(ms=ms_g, gbs=gbytes / (ms_g * 1e-3) / 1e9)
+    log('panel KG(u) one launch %.3f ms' % ms_g)
+    out['warm_start'] = ('not applicable: the network is trained on 64x64 isotropic plates '
+                         'and has no input for stringers or cut-outs')
+    return out
+
+
 def profile_kc0(n):
     """Three values-only KC0 launches on the n x n plate, then three copies
     of 1 GiB (the calibration of the HBM byte counters)."""
@@ -268,6 +331,10 @@ def main():
     ap.add_argument('--sweep-pcg-variants', type=int, default=256,
                     help='plates of the same sweep solved with PCG (FP64 and FP32 operator)')
     ap.add_argument('--warm-start-plates', type=int, default=64)
+    ap.add_argument('--panel', type=int, default=512,
+                    help='stiffened panel skin elements per side (0: skip)')
+    ap.add_argument('--panel-only', action='store_true',
+                    help='run only the stiffened-panel lines (no headline)')
     ap.add_argument('--profile-kc0', action='store_true',
                     help='only: 3 KC0 launches and a 1 GiB device copy (counter runs)')
     ap.add_argument('--quiet', action='store_true')
@@ -284,6 +351,9 @@ def main():
     stream = torch.cuda.current_stream()
     if args.profile_kc0:
         profile_kc0(args.n)
+        return
+    if args.panel_only:
+        print(json.dumps({'stiffened_panel': panel_rates(args.panel or 512, log)}), flush=True)
         return
 
     t0 = time.perf_counter()
@@ -387,6 +457,14 @@ def main():
         except Exception as exc:
             extra['modal_64x64'] = dict(error=str(exc))
             log('modal failed: %s' % exc)
+
+    if args.panel > 0:
+        torch.cuda.empty_cache()
+        try:
+            extra['stiffened_panel'] = panel_rates(args.panel, log)
+        except Exception as exc:
+            extra['stiffened_panel'] = dict(error=str(exc))
+            log('panel failed: %s' % exc)
 
     if not args.no_solve and args.warm_start_plates > 0:
         torch.cuda.empty_cache()

@@ -132,6 +132,33 @@ int pf3d_beam_m(int64_t n_elem, const double *x, const int64_t *conn, const int6
                 const double *vxy, const int32_t *pid, const double *props, int mtype,
                 int64_t *r, int64_t *c, double *v, int flags, void *stream);
 
+/* One group of elements of one type for pf3d_mixed_assemble. */
+typedef struct pf3d_elem_group {
+    int etype;              /* PF3D_QUAD4, PF3D_QUAD4R, PF3D_TRIA3R or PF3D_BEAMC */
+    int64_t n_elem;
+    const int64_t *conn;    /* n_elem x nn DOF offsets */
+    int64_t k0;             /* added to every slot of the group */
+    const int64_t *init_k;  /* first slot per element (+ k0), or NULL for SIZE*e (+ k0) */
+    const double *evec;     /* xmat (shells, may be NULL) or vxy (beams, required) */
+    const int32_t *pid;     /* property index per element, or NULL for record 0 */
+    const double *props;    /* shell (32) or beam (16) records */
+    const double *k6rot;    /* shells, may be NULL (100) */
+    const double *alpha;    /* Tria3R, may be NULL (0.7) */
+    const double *hg;       /* Quad4R, may be NULL (1) */
+} pf3d_elem_group;
+
+/* KC0 (matrix PF3D_KC0), KG from displacements u (PF3D_KG) or mass
+ * (PF3D_M, mtype 0 or 1; 2 without beams) of up to four groups of different
+ * element types (each type at most once) in ONE kernel launch, into the
+ * shared COO arrays r, c, v; the slot layouts and values are those of the
+ * per-type entry points.  Replaces the caller's loops over mixed element
+ * objects, e.g. a stiffened panel's quads, triangles and beams
+ * (quad4.pyx:1204-1313, tria3r.pyx:950-2974, beamc.pyx:489-1353 and the
+ * KG/M counterparts). */
+int pf3d_mixed_assemble(int matrix, int mtype, int n_groups, const pf3d_elem_group *groups,
+                        const double *x, const double *u, int64_t *r, int64_t *c, double *v,
+                        int flags, void *stream);
+
 /* Truss KC0 (72 slots: axial E A / L and torsion G J / L), props = beam
  * records.  Replaces update_rotation_matrix(x) -> update_probe_xe(x) ->
  * update_length() -> update_KC0(...): truss.pyx:203-338, 386-801. */

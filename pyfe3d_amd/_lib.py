@@ -34,6 +34,7 @@ SIGNATURES = {
     'pf3d_truss_m': [I64, P, P, P, P, P, I32, P, P, P, I32, P],
     'pf3d_spring_kc0': [I64, P, P, P, P, P, P, P, P, P, I32, P],
     'pf3d_rod_fe': [I32, I64, P, P, P, P, P, P, P, P],
+    'pf3d_mixed_assemble': [I32, I32, I32, P, P, P, P, P, P, I32, P],
     'pf3d_incidence_bytes': [I64, I64, P],
     'pf3d_incidence_plan': [I64, P, I64, P, I64, P, P, P],
     'pf3d_gather_fint': [I64, P, P, P, P, P],

@@ -287,6 +287,71 @@ def spring_kc0(x, conn, axes, r, c, v, k, pid=None, init_k=None, values_only=Fal
 
 
 # ----------------------------------------------------------------------
+# several element types in one launch
+# ----------------------------------------------------------------------
+
+class _CGroup(ctypes.Structure):
+    """struct pf3d_elem_group of include/pf3d.h"""
+    _fields_ = [('etype', ctypes.c_int), ('n_elem', ctypes.c_int64), ('conn', ctypes.c_void_p),
+                ('k0', ctypes.c_int64), ('init_k', ctypes.c_void_p), ('evec', ctypes.c_void_p),
+                ('pid', ctypes.c_void_p), ('props', ctypes.c_void_p), ('k6rot', ctypes.c_void_p),
+                ('alpha', ctypes.c_void_p), ('hg', ctypes.c_void_p)]
+
+
+class ElementGroup:
+    """Elements of one type for :func:`mixed_assemble`: the arguments of the
+    per-type entry point, plus ``k0``, the first slot of the group (the
+    running ``init_k`` counter a caller keeps over its element lists)."""
+
+    def __init__(self, etype, conn, props, pid=None, k0=0, init_k=None, evec=None, k6rot=None,
+                 alpha=None, hg=None):
+        if etype not in ('quad4', 'quad4r', 'tria3r', 'beamc'):
+            raise ValueError('mixed groups take shells and beamc, got %r' % etype)
+        self.etype, self.conn, self.props, self.pid = etype, conn, props, pid
+        self.k0, self.init_k, self.evec = int(k0), init_k, evec
+        self.k6rot, self.alpha, self.hg = k6rot, alpha, hg
+
+    def slots(self, matrix='KC0', mtype=0):
+        return slot_count(self.etype, matrix, mtype) * self.conn.shape[0]
+
+
+def mixed_assemble(matrix, groups, x, r, c, v, u=None, mtype=0, values_only=False,
+                   overwrite=False, stream=None, check=True):
+    """KC0 / KG (from u) / M of several element types in ONE kernel launch
+    (pf3d_mixed_assemble): each group writes the slots the per-type call
+    would write, shifted by its k0."""
+    mat = {'KC0': 0, 'KG': 1, 'M': 3}[matrix]
+    cg = (_CGroup * max(1, len(groups)))()
+    for i, g in enumerate(groups):
+        beam = g.etype == 'beamc'
+        E, px, pc, pk, pe = _common(g.etype, x, g.conn, g.init_k, g.evec, check)
+        if beam and pe is None:
+            raise ValueError('beamc group needs vxy (evec)')
+        if check:
+            if u is not None:
+                _check_nodes(g.conn, x, u)
+            size = slot_count(g.etype, matrix, mtype)
+            outs = (('v', v),) + (() if values_only else (('r', r), ('c', c)))
+            hi = g.k0 + (size * E if g.init_k is None else
+                         (int(g.init_k.max()) + size if E else 0))
+            lo = g.k0 + (0 if g.init_k is None or not E else int(g.init_k.min()))
+            for name, t in outs:
+                if lo < 0 or hi > t.numel():
+                    raise ValueError('group %d writes slots [%d, %d) but %s has %d'
+                                     % (i, lo, hi, name, t.numel()))
+        pp, ppid = _props(g.props, g.pid, E, BEAM_STRIDE if beam else SHELL_STRIDE, check)
+        cg[i] = _CGroup(ETYPE[g.etype], E, pc, g.k0, pk, pe, ppid, pp,
+                        _ptr(g.k6rot, torch.float64, 'k6rot', E, optional=True),
+                        _ptr(g.alpha, torch.float64, 'alpha', E, optional=True),
+                        _ptr(g.hg, torch.float64, 'hg', 5 * E, optional=True))
+    pr, pcc, pv = _outputs(r, c, v, True, values_only)
+    pu = _ptr(u, torch.float64, 'u', optional=matrix != 'KG')
+    _lib.call('pf3d_mixed_assemble', mat, int(mtype), len(groups), ctypes.cast(cg, ctypes.c_void_p),
+              _ptr(x, torch.float64, 'x'), pu, pr, pcc, pv, _flags(values_only, overwrite),
+              _stream(stream))
+
+
+# ----------------------------------------------------------------------
 # internal forces (update_fint)
 # ----------------------------------------------------------------------
 

@@ -424,20 +424,28 @@ __device__ __forceinline__ void membrane_forces(const Shell &s, const double *Nx
 // reference layout, consecutive lanes computing consecutive slots from LDS,
 // so the stores are contiguous across lanes.
 // ------------------------------------------------------------------------
+// LDS layout of one shell_stream block (also sizes the mixed-type kernels)
 template <int ET, int MAT>
-__global__ void __launch_bounds__(64) shell_stream(AsmArgs a)
-{
-    constexpr int NN = (ET == TRIA3R) ? 3 : 4;
-    constexpr int NP = NN * NN;
-    constexpr int W = (MAT == MASS) ? NP + 36 : NP + 3;  // per-element LDS doubles
+struct StreamLds {
+    static constexpr int NN = (ET == TRIA3R) ? 3 : 4;
+    static constexpr int NP = NN * NN;
+    static constexpr int W = (MAT == MASS) ? NP + 36 : NP + 3;  // per-element LDS doubles
     // elements per 64-lane block: the mass pattern needs 4x the LDS of KG,
     // so mass blocks take 32 elements to keep several waves per CU
-    constexpr int EPB = (MAT == MASS) ? 32 : 64;
-    __shared__ double sd[EPB][W];
-    __shared__ int64_t sconn[EPB][NN];
-    __shared__ int64_t sk0[EPB];
+    static constexpr int EPB = (MAT == MASS) ? 32 : 64;
+    static constexpr int BYTES = EPB * (W * 8 + NN * 8 + 8);
+};
+
+template <int ET, int MAT>
+__device__ __forceinline__ void shell_stream_body(const AsmArgs &a, int64_t blk, char *lds)
+{
+    using L = StreamLds<ET, MAT>;
+    constexpr int NN = L::NN, NP = L::NP, W = L::W, EPB = L::EPB;
+    double (*sd)[W] = reinterpret_cast<double (*)[W]>(lds);
+    int64_t (*sconn)[NN] = reinterpret_cast<int64_t (*)[NN]>(lds + EPB * W * 8);
+    int64_t *sk0 = reinterpret_cast<int64_t *>(lds + EPB * (W + NN) * 8);
     const int lane = threadIdx.x;
-    const int64_t e0 = (int64_t)blockIdx.x * EPB;
+    const int64_t e0 = blk * EPB;
     const int64_t e = e0 + lane;
     const int mtype = a.mtype;
     const int nslot = slot_count<NN>(MAT, mtype);
@@ -654,29 +662,42 @@ __device__ __forceinline__ void store2(T *p, T v)
 #endif
 }
 
+template <int ET, int MAT>
+__global__ void __launch_bounds__(64) shell_stream(AsmArgs a)
+{
+    __shared__ __align__(16) char lds[StreamLds<ET, MAT>::BYTES];
+    shell_stream_body<ET, MAT>(a, blockIdx.x, lds);
+}
+
 // waves per SIMD the register allocation of the KC0 kernel targets
 #ifndef PF3D_KC0_WAVES
 #define PF3D_KC0_WAVES 3
 #endif
 
+template <int ET>
+struct Kc0Lds {
+    static constexpr int NN = (ET == TRIA3R) ? 3 : 4;
+    static constexpr int NUP = NN * (NN + 1) / 2;
+    static constexpr int EPW = 64 / NUP;
+    static constexpr int NDOF = 6 * NN;
+    static constexpr int RW = 6 * NDOF;  // slots per node row
+    static constexpr int BYTES = EPW * (RW * 8 + NN * 8 + 8);
+};
+
 template <int ET, int FLAGS>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PF3D_KC0_WAVES)))
-shell_kc0_sym(AsmArgs a)
+__device__ __forceinline__ void shell_kc0_body(const AsmArgs &a, int64_t blk, char *lds)
 {
-    constexpr int NN = (ET == TRIA3R) ? 3 : 4;
-    constexpr int NUP = NN * (NN + 1) / 2;
-    constexpr int EPW = 64 / NUP;
-    constexpr int NDOF = 6 * NN;
-    constexpr int RW = 6 * NDOF;  // slots per node row
-    __shared__ double strip[EPW][RW];
-    __shared__ int64_t sconn[EPW][NN];
-    __shared__ int64_t sk0[EPW];
+    using L = Kc0Lds<ET>;
+    constexpr int NN = L::NN, NUP = L::NUP, EPW = L::EPW, NDOF = L::NDOF, RW = L::RW;
+    double (*strip)[RW] = reinterpret_cast<double (*)[RW]>(lds);
+    int64_t (*sconn)[NN] = reinterpret_cast<int64_t (*)[NN]>(lds + EPW * RW * 8);
+    int64_t *sk0 = reinterpret_cast<int64_t *>(lds + EPW * (RW + NN) * 8);
 
     const int lane = threadIdx.x;
     const int slot = lane / NUP, sub = lane - slot * NUP;
     int I, J;
     upper_pair<NN>(sub, I, J);
-    const int64_t e0 = (int64_t)blockIdx.x * EPW;
+    const int64_t e0 = blk * EPW;
     const int64_t e = e0 + slot;
     const bool active = slot < EPW && e < a.n_elem;
 
@@ -778,6 +799,14 @@ shell_kc0_sym(AsmArgs a)
             }
         }
     }
+}
+
+template <int ET, int FLAGS>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PF3D_KC0_WAVES)))
+shell_kc0_sym(AsmArgs a)
+{
+    __shared__ __align__(16) char lds[Kc0Lds<ET>::BYTES];
+    shell_kc0_body<ET, FLAGS>(a, blockIdx.x, lds);
 }
 
 // ------------------------------------------------------------------------
@@ -1114,19 +1143,26 @@ __device__ __forceinline__ void beam_slot(int lumped, int k, int &rl, int &cl)
     }
 }
 
-template <int MAT>
-__global__ void __launch_bounds__(64) beam_assemble(AsmArgs a)
+// beams per block BPB (4 lanes each, BPB <= 16)
+template <int BPB>
+struct BeamLds {
+    static constexpr int BYTES = BPB * (144 * 8 + 2 * 8 + 8 + 9 * 8);
+};
+
+template <int MAT, int BPB>
+__device__ __forceinline__ void beam_body(const AsmArgs &a, int64_t blk, char *lds)
 {
-    __shared__ double tile[16][144];
-    __shared__ int64_t sconn[16][2];
-    __shared__ int64_t sk0[16];
-    __shared__ double sr[16][9];
+    double (*tile)[144] = reinterpret_cast<double (*)[144]>(lds);
+    int64_t (*sconn)[2] = reinterpret_cast<int64_t (*)[2]>(lds + BPB * 144 * 8);
+    int64_t *sk0 = reinterpret_cast<int64_t *>(lds + BPB * 146 * 8);
+    double (*sr)[9] = reinterpret_cast<double (*)[9]>(lds + BPB * 147 * 8);
     const int lane = threadIdx.x, slot = lane >> 2, I = (lane >> 1) & 1, J = lane & 1;
-    const int64_t e0 = (int64_t)blockIdx.x * 16;
+    const int64_t e0 = blk * BPB;
     const int64_t e = e0 + slot;
+    const bool mine = slot < BPB && e < a.n_elem;
     const bool lumped = (MAT == MASS) && a.mtype == 1;
     const int nslot = lumped ? 36 : 144;
-    if (e < a.n_elem && (lane & 3) == 0) {
+    if (mine && (lane & 3) == 0) {
         V3 X[2];
         int64_t cn[2];
         for (int i = 0; i < 2; i++) {
@@ -1156,7 +1192,7 @@ __global__ void __launch_bounds__(64) beam_assemble(AsmArgs a)
     }
     __syncthreads();
     double K[6][6];
-    if (e < a.n_elem) {
+    if (mine) {
         for (int i = 0; i < 6; i++)
             for (int j = 0; j < 6; j++) K[i][j] = tile[slot][(6 * I + i) * 12 + 6 * J + j];
         double r[9];
@@ -1164,11 +1200,11 @@ __global__ void __launch_bounds__(64) beam_assemble(AsmArgs a)
         rotate_block(K, r);
     }
     __syncthreads();
-    if (e < a.n_elem)
+    if (mine)
         for (int i = 0; i < 6; i++)
             for (int j = 0; j < 6; j++) tile[slot][(6 * I + i) * 12 + 6 * J + j] = K[i][j];
     __syncthreads();
-    const int nel = (int)((a.n_elem - e0) < 16 ? (a.n_elem - e0) : 16);
+    const int nel = (int)((a.n_elem - e0) < BPB ? (a.n_elem - e0) : BPB);
     const bool write_idx = !(a.flags & F_VALUES_ONLY);
     const bool overwrite = (a.flags & F_OVERWRITE);
     for (int el = 0; el < nel; el++) {
@@ -1185,6 +1221,13 @@ __global__ void __launch_bounds__(64) beam_assemble(AsmArgs a)
             }
         }
     }
+}
+
+template <int MAT>
+__global__ void __launch_bounds__(64) beam_assemble(AsmArgs a)
+{
+    __shared__ __align__(16) char lds[BeamLds<16>::BYTES];
+    beam_body<MAT, 16>(a, blockIdx.x, lds);
 }
 
 // BeamC element internal forces (beamc.pyx:352-488, 1354-1391)
@@ -1395,6 +1438,57 @@ __global__ void __launch_bounds__(64) rod_fe(AsmArgs a, double *__restrict__ fe)
         }
 }
 
+// ------------------------------------------------------------------------
+// Mixed element types in one launch (e.g. a stiffened panel: Quad4 skin,
+// Tria3R fill, BeamC stringers).  Blocks [blk0[t], blk0[t+1]) run the block
+// body of element type t = Quad4, Quad4R, Tria3R, BeamC on group t, in one
+// LDS buffer sized for the largest body.  The shell bodies keep their own
+// block shapes; beams take 8 per block so that their LDS tile does not
+// lower the occupancy of the shell blocks.
+// ------------------------------------------------------------------------
+struct MixedArgs {
+    AsmArgs g[4];
+    int64_t blk0[5];
+};
+
+constexpr int MIXED_BPB = 8;
+
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+
+constexpr int MIXED_KC0_LDS =
+    cmax(cmax(Kc0Lds<QUAD4>::BYTES, Kc0Lds<TRIA3R>::BYTES), BeamLds<MIXED_BPB>::BYTES);
+
+template <int MAT>
+constexpr int mixed_stream_lds()
+{
+    return cmax(cmax(StreamLds<QUAD4, MAT>::BYTES, StreamLds<TRIA3R, MAT>::BYTES),
+                BeamLds<MIXED_BPB>::BYTES);
+}
+
+template <int FLAGS>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PF3D_KC0_WAVES)))
+mixed_kc0(MixedArgs m)
+{
+    __shared__ __align__(16) char lds[MIXED_KC0_LDS];
+    const int64_t b = blockIdx.x;
+    if (b < m.blk0[1]) shell_kc0_body<QUAD4, FLAGS>(m.g[0], b, lds);
+    else if (b < m.blk0[2]) shell_kc0_body<QUAD4R, FLAGS>(m.g[1], b - m.blk0[1], lds);
+    else if (b < m.blk0[3]) shell_kc0_body<TRIA3R, FLAGS>(m.g[2], b - m.blk0[2], lds);
+    else beam_body<KC0, MIXED_BPB>(m.g[3], b - m.blk0[3], lds);
+}
+
+// KG from displacements or mass (MAT = KG or MASS)
+template <int MAT>
+__global__ void __launch_bounds__(64) mixed_stream(MixedArgs m)
+{
+    __shared__ __align__(16) char lds[mixed_stream_lds<MAT>()];
+    const int64_t b = blockIdx.x;
+    if (b < m.blk0[1]) shell_stream_body<QUAD4, MAT>(m.g[0], b, lds);
+    else if (b < m.blk0[2]) shell_stream_body<QUAD4R, MAT>(m.g[1], b - m.blk0[1], lds);
+    else if (b < m.blk0[3]) shell_stream_body<TRIA3R, MAT>(m.g[2], b - m.blk0[2], lds);
+    else beam_body<MAT, MIXED_BPB>(m.g[3], b - m.blk0[3], lds);
+}
+
 }  // namespace pf3d
 
 // ------------------------------------------------------------------------
@@ -1494,6 +1588,51 @@ hipError_t pf3d_launch_geometry(int etype, int64_t n, const double *x, const int
     default:
         hipLaunchKernelGGL(element_geometry<BEAMC>, dim3(blocks), dim3(128), 0, s, n, x, conn,
                            evec, geom, xe);
+    }
+    return hipGetLastError();
+}
+
+// groups: per element type Quad4, Quad4R, Tria3R, BeamC (n_elem 0 when absent)
+hipError_t pf3d_launch_mixed(int mat, const AsmArgs *groups, hipStream_t s)
+{
+    MixedArgs m;
+    int64_t per_kc0[4] = {Kc0Lds<QUAD4>::EPW, Kc0Lds<QUAD4R>::EPW, Kc0Lds<TRIA3R>::EPW,
+                          MIXED_BPB};
+    const int epb = mat == MASS ? 32 : 64;
+    int64_t per_stream[4] = {epb, epb, epb, MIXED_BPB};
+    m.blk0[0] = 0;
+    bool vec = true;
+    for (int t = 0; t < 4; t++) {
+        m.g[t] = groups[t];
+        const int64_t per = mat == KC0 ? per_kc0[t] : per_stream[t];
+        m.blk0[t + 1] = m.blk0[t] + (groups[t].n_elem + per - 1) / per;
+        if (t < 3 && groups[t].n_elem > 0) {
+            const AsmArgs &a = groups[t];
+            vec = vec && !a.init_k && (reinterpret_cast<uintptr_t>(a.v) & 15) == 0 &&
+                  ((a.flags & F_VALUES_ONLY) ||
+                   ((reinterpret_cast<uintptr_t>(a.r) & 15) == 0 &&
+                    (reinterpret_cast<uintptr_t>(a.c) & 15) == 0));
+        }
+    }
+    const int64_t blocks = m.blk0[4];
+    if (blocks == 0) return hipSuccess;
+    if (blocks > 0xffffffffLL) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)blocks), blk(64);
+    if (mat == KC0) {
+        switch ((groups[0].flags & (F_VALUES_ONLY | F_OVERWRITE)) | (vec ? F_VEC : 0)) {
+        case 0: hipLaunchKernelGGL(mixed_kc0<0>, grid, blk, 0, s, m); break;
+        case 1: hipLaunchKernelGGL(mixed_kc0<1>, grid, blk, 0, s, m); break;
+        case 2: hipLaunchKernelGGL(mixed_kc0<2>, grid, blk, 0, s, m); break;
+        case 3: hipLaunchKernelGGL(mixed_kc0<3>, grid, blk, 0, s, m); break;
+        case 4: hipLaunchKernelGGL(mixed_kc0<4>, grid, blk, 0, s, m); break;
+        case 5: hipLaunchKernelGGL(mixed_kc0<5>, grid, blk, 0, s, m); break;
+        case 6: hipLaunchKernelGGL(mixed_kc0<6>, grid, blk, 0, s, m); break;
+        default: hipLaunchKernelGGL(mixed_kc0<7>, grid, blk, 0, s, m);
+        }
+    } else if (mat == KG) {
+        hipLaunchKernelGGL(mixed_stream<KG>, grid, blk, 0, s, m);
+    } else {
+        hipLaunchKernelGGL(mixed_stream<MASS>, grid, blk, 0, s, m);
     }
     return hipGetLastError();
 }

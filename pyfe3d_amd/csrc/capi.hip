@@ -12,6 +12,7 @@ hipError_t pf3d_launch_beam(int mat, const pf3d::AsmArgs &a, hipStream_t s);
 hipError_t pf3d_launch_shell_fe(int etype, const pf3d::AsmArgs &a, double *fe, hipStream_t s);
 hipError_t pf3d_launch_beam_fe(const pf3d::AsmArgs &a, double *fe, hipStream_t s);
 hipError_t pf3d_launch_rod(int etype, int mat, const pf3d::AsmArgs &a, hipStream_t s);
+hipError_t pf3d_launch_mixed(int mat, const pf3d::AsmArgs *groups, hipStream_t s);
 hipError_t pf3d_launch_rod_fe(int etype, const pf3d::AsmArgs &a, double *fe, hipStream_t s);
 hipError_t pf3d_launch_geometry(int etype, int64_t n, const double *x, const int64_t *conn,
                                 const double *evec, double *geom, double *xe, hipStream_t s);
@@ -252,6 +253,48 @@ int pf3d_beam_fe(int64_t n_elem, const double *x, const int64_t *conn, const dou
     a.n_elem = n_elem; a.x = x; a.conn = conn; a.evec = vxy; a.pid = pid; a.props = props;
     a.u = u;
     return pf3d_hip_check(pf3d_launch_beam_fe(a, fe, (hipStream_t)stream), "pf3d_beam_fe");
+}
+
+int pf3d_mixed_assemble(int matrix, int mtype, int n_groups, const pf3d_elem_group *groups,
+                        const double *x, const double *u, int64_t *r, int64_t *c, double *v,
+                        int flags, void *stream)
+{
+    if (matrix != PF3D_KC0 && matrix != PF3D_KG && matrix != PF3D_M)
+        return pf3d_fail(PF3D_E_ARG, "mixed assembly takes KC0, KG or M (matrix %d)", matrix);
+    if (n_groups < 0 || n_groups > 4) return pf3d_fail(PF3D_E_ARG, "n_groups %d", n_groups);
+    if (n_groups > 0 && !groups) return pf3d_fail(PF3D_E_ARG, "null groups");
+    if (!x || !v) return pf3d_fail(PF3D_E_ARG, "null x/v");
+    if (!(flags & PF3D_VALUES_ONLY) && (!r || !c))
+        return pf3d_fail(PF3D_E_ARG, "null r/c without PF3D_VALUES_ONLY");
+    if (flags & ~(PF3D_VALUES_ONLY | PF3D_OVERWRITE))
+        return pf3d_fail(PF3D_E_ARG, "unknown flags 0x%x", flags);
+    if (matrix == PF3D_KG && !u) return pf3d_fail(PF3D_E_ARG, "null u");
+    pf3d::AsmArgs a[4] = {};
+    bool seen[4] = {false, false, false, false};
+    for (int i = 0; i < n_groups; i++) {
+        const pf3d_elem_group &g = groups[i];
+        if (g.etype < PF3D_QUAD4 || g.etype > PF3D_BEAMC)
+            return pf3d_fail(PF3D_E_ARG, "group %d: etype %d is not a shell or BeamC", i, g.etype);
+        if (seen[g.etype]) return pf3d_fail(PF3D_E_ARG, "group %d: etype %d repeated", i, g.etype);
+        seen[g.etype] = true;
+        if (g.n_elem < 0 || g.k0 < 0) return pf3d_fail(PF3D_E_ARG, "group %d: negative size", i);
+        if (g.n_elem == 0) continue;
+        if (!g.conn || !g.props) return pf3d_fail(PF3D_E_ARG, "group %d: null conn/props", i);
+        if (g.etype == PF3D_BEAMC && !g.evec)
+            return pf3d_fail(PF3D_E_ARG, "group %d: BeamC needs vxy", i);
+        if (matrix == PF3D_M && (mtype < 0 || mtype > (g.etype == PF3D_BEAMC ? 1 : 2)))
+            return pf3d_fail(PF3D_E_ARG, "group %d: bad mtype %d", i, mtype);
+        pf3d::AsmArgs &t = a[g.etype];
+        t.n_elem = g.n_elem; t.x = x; t.conn = g.conn; t.init_k = g.init_k; t.evec = g.evec;
+        t.pid = g.pid; t.props = g.props; t.k6rot = g.k6rot; t.alpha = g.alpha; t.hg = g.hg;
+        t.u = u; t.mtype = mtype; t.flags = flags;
+        t.v = v + g.k0;
+        t.r = r ? r + g.k0 : nullptr;
+        t.c = c ? c + g.k0 : nullptr;
+    }
+    for (int t = 0; t < 4; t++) a[t].flags = flags;
+    return pf3d_hip_check(pf3d_launch_mixed(matrix, a, (hipStream_t)stream),
+                          "pf3d_mixed_assemble");
 }
 
 static int rod_check(int64_t n_elem, const double *x, const int64_t *conn, const double *props,
