@@ -110,7 +110,7 @@ def solve_rates(n, repeats, log):
     sysm = StaticSystem(plan, plan.apply(v), mesh.clamped())
     f = mesh.pressure_load(1e3)
     out = {}
-    for method in ('direct', 'fp64', 'mixed'):
+    for method in ('direct', 'fp64', 'mixed', 'ir'):
         u, info = sysm.solve(f, method=method, rtol=1e-8)
         torch.cuda.synchronize()
         t0 = time.perf_counter()

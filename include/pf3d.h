@@ -43,6 +43,7 @@ extern "C" {
 #define PF3D_BEAMC 3
 #define PF3D_TRUSS 4
 #define PF3D_SPRING 5
+#define PF3D_BEAMLR 6
 
 #define PF3D_KC0 0
 #define PF3D_KG 1
@@ -58,7 +59,7 @@ const char *pf3d_last_error(void);
 /* Slots written per element: Quad4Data/Quad4RData/Tria3RData/BeamCData
  * KC0_SPARSE_SIZE, KG_SPARSE_SIZE, M_SPARSE_SIZE (quad4.pyx:174-180,
  * quad4r.pyx:140-146, tria3r.pyx:148-151, beamc.pyx:43-46,
- * truss.pyx:45-47, spring.pyx:40-43); for lumped
+ * truss.pyx:45-47, spring.pyx:40-43, beamlr.pyx:43-46); for lumped
  * masses the used prefix of M_SPARSE_SIZE. */
 int pf3d_slot_count(int etype, int matrix, int mtype);
 
@@ -180,6 +181,27 @@ int pf3d_spring_kc0(int64_t n_elem, const double *x, const int64_t *conn, const 
                     const double *axes, const int32_t *pid, const double *k, int64_t *r,
                     int64_t *c, double *v, int flags, void *stream);
 
+/* BeamLR (linear Timoshenko beam, reduced integration) KC0 (144 slots,
+ * 12x12 row-major).  Replaces update_rotation_matrix(vxyi, vxyj, vxyk, x)
+ * -> update_probe_xe(x) -> update_length() -> update_KC0(...):
+ * beamlr.pyx:160-353, 407-1187. */
+int pf3d_beamlr_kc0(int64_t n_elem, const double *x, const int64_t *conn, const int64_t *init_k,
+                    const double *vxy, const int32_t *pid, const double *props, int64_t *r,
+                    int64_t *c, double *v, int flags, void *stream);
+
+/* BeamLR KG from global displacements u (36 slots over the rotations of
+ * both nodes).  Replaces update_probe_ue(u) -> update_KG(...):
+ * beamlr.pyx:236-288, 1226-1463. */
+int pf3d_beamlr_kg(int64_t n_elem, const double *x, const int64_t *conn, const int64_t *init_k,
+                   const double *vxy, const int32_t *pid, const double *props, const double *u,
+                   int64_t *r, int64_t *c, double *v, int flags, void *stream);
+
+/* BeamLR mass, mtype 0 consistent (144 slots) or 1 lumped (36 slots).
+ * Replaces update_M(..., mtype): beamlr.pyx:1464-2424. */
+int pf3d_beamlr_m(int64_t n_elem, const double *x, const int64_t *conn, const int64_t *init_k,
+                  const double *vxy, const int32_t *pid, const double *props, int mtype,
+                  int64_t *r, int64_t *c, double *v, int flags, void *stream);
+
 /* Shell property records (32 doubles each) of isotropic plates, one per
  * element, from a thickness field h[n] (device) and the record of the same
  * material at unit thickness (host, 32 doubles, h field = 1, no offset):
@@ -210,6 +232,12 @@ int pf3d_shell_fe(int etype, int64_t n_elem, const double *x, const int64_t *con
 int pf3d_beam_fe(int64_t n_elem, const double *x, const int64_t *conn, const double *vxy,
                  const int32_t *pid, const double *props, const double *u, double *fe,
                  void *stream);
+
+/* BeamLR element internal forces, fe[(e*2 + i)*6 + d].  Replaces
+ * beamlr.pyx:236-288 (ue), 354-406 (finte), 1188-1225 (rotation). */
+int pf3d_beamlr_fe(int64_t n_elem, const double *x, const int64_t *conn, const double *vxy,
+                   const int32_t *pid, const double *props, const double *u, double *fe,
+                   void *stream);
 
 /* Truss (etype 4, props = beam records, axes unused) or Spring (etype 5,
  * props = spring records, axes as in pf3d_spring_kc0) element internal
@@ -333,6 +361,20 @@ int pf3d_ir_solve(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx,
                   const double *b, double *x, double rtol, double inner_rtol, int max_outer,
                   int max_inner, int x0, void *work, int64_t work_bytes, int64_t *stats,
                   double *relres, void *stream);
+
+/* Mixed-precision PCG with reliable updates: the Krylov recursion reads K
+ * and the block-Jacobi inverses in FP32 with FP64 vectors and arithmetic;
+ * every `replace` iterations the recursive residual is replaced by the true
+ * FP64 residual b - K64 x (search direction kept), and after 20
+ * replacements without halving the true residual the solve finishes with
+ * FP64 PCG (dinv64) from the current x.  Stops at ||b - K64 x|| <= rtol ||b||.
+ * stats[3]: iterations, replacements, FP64 finishing iterations.  work:
+ * pf3d_ir_bytes(nb).  Replaces the same spsolve / cg call as pf3d_pcg. */
+int pf3d_pcg_mixed(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx,
+                   const double *val64, const double *dinv64, const float *val32,
+                   const float *dinv32, const double *b, double *x, double rtol, int64_t maxiter,
+                   int replace, int x0, void *work, int64_t work_bytes, int64_t *stats,
+                   double *relres, void *stream);
 
 /* ------------------------------------------------------------------ */
 /* Batched solves of a design sweep: one launch for many independent   */

@@ -24,7 +24,9 @@
 
 #define DOF 6
 
-enum { ET_QUAD4 = 0, ET_QUAD4R = 1, ET_TRIA3R = 2, ET_BEAMC = 3, ET_TRUSS = 4, ET_SPRING = 5 };
+enum { ET_QUAD4 = 0, ET_QUAD4R = 1, ET_TRIA3R = 2, ET_BEAMC = 3, ET_TRUSS = 4, ET_SPRING = 5,
+       ET_BEAMLR = 6 };
+#define IS_BEAM(t) ((t) == ET_BEAMC || (t) == ET_BEAMLR)
 #define IS_ROD(t) ((t) == ET_TRUSS || (t) == ET_SPRING)
 #define SPR_STRIDE 8 /* spring record: kxe kye kze krxe krye krze, 2 unused */
 enum { MAT_KC0 = 0, MAT_KG = 1, MAT_KGS = 2, MAT_M = 3, MAT_FINT = 4 };
@@ -698,6 +700,84 @@ static void truss_m_local(double L, const double *bp, int mtype, double *K)
         }
 }
 
+/* BeamLR (beamlr.pyx): linear Timoshenko beam, every field interpolated
+ * linearly, one-point (reduced) integration.  At the midpoint
+ *   eps = u', ky = ry', kz = rz', gxy = v' - rz_m, gxz = w' + ry_m, phi = rx'
+ * with section stiffness D = E [[A, Az, -Ay], [Az, Iyy, -Iyz], [-Ay, -Iyz, Izz]]
+ * on (eps, ky, kz) and S = G [[A, 0, -Az], [0, A, Ay], [-Az, Ay, J]] on
+ * (gxy, gxz, phi); K = L (Bm^T D Bm + Bs^T S Bs) (beamlr.pyx:354-406, the
+ * internal forces K ue; 407-1187 the scattered K). */
+static void beamlr_kc0_local(double L, const double *bp, double *K)
+{
+    const double A = bp[0], E = bp[1], G = bp[2], Iyy = bp[3], Izz = bp[4], Iyz = bp[5];
+    const double J = bp[6], Ay = bp[7], Az = bp[8];
+    const double D[3][3] = {{E * A, E * Az, -E * Ay}, {E * Az, E * Iyy, -E * Iyz},
+                            {-E * Ay, -E * Iyz, E * Izz}};
+    const double S[3][3] = {{G * A, 0., -G * Az}, {0., G * A, G * Ay}, {-G * Az, G * Ay, G * J}};
+    double Bm[3][12], Bs[3][12];
+    memset(Bm, 0, sizeof(Bm));
+    memset(Bs, 0, sizeof(Bs));
+    Bm[0][0] = -1 / L; Bm[0][6] = 1 / L;
+    Bm[1][4] = -1 / L; Bm[1][10] = 1 / L;
+    Bm[2][5] = -1 / L; Bm[2][11] = 1 / L;
+    Bs[0][1] = -1 / L; Bs[0][7] = 1 / L; Bs[0][5] = -0.5; Bs[0][11] = -0.5;
+    Bs[1][2] = -1 / L; Bs[1][8] = 1 / L; Bs[1][4] = 0.5; Bs[1][10] = 0.5;
+    Bs[2][3] = -1 / L; Bs[2][9] = 1 / L;
+    for (int i = 0; i < 12; i++)
+        for (int j = 0; j < 12; j++) {
+            double s = 0.;
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++)
+                    s += Bm[a][i] * D[a][b] * Bm[b][j] + Bs[a][i] * S[a][b] * Bs[b][j];
+            K[i * 12 + j] = L * s;
+        }
+}
+
+/* BeamLR KG (beamlr.pyx:1226-1463): N = A E (u2 - u1) / L on the bending
+ * rotations, N int(N_I N_J) [[1, -1], [-1, 1]] over (ry, rz) of nodes I, J. */
+static void beamlr_kg_local(double L, double N, double *K)
+{
+    memset(K, 0, sizeof(double) * 144);
+    for (int I = 0; I < 2; I++)
+        for (int J = 0; J < 2; J++) {
+            const double f = N * L * (I == J ? 1. / 3. : 1. / 6.);
+            for (int a = 0; a < 2; a++)
+                for (int b = 0; b < 2; b++)
+                    K[(6 * I + 4 + a) * 12 + 6 * J + 4 + b] = (a == b ? f : -f);
+        }
+}
+
+/* BeamLR mass (beamlr.pyx:1464-2424): section mass matrix m of
+ * u = u0 + z ry - y rz, v = v0 - z rx, w = w0 + y rx; consistent: the
+ * two-point Gauss integral of N_I N_J (L/3, L/6) times m; lumped (Gauss-
+ * Lobatto at the nodes): L/2 times the diagonal of m. */
+static void beamlr_m_local(double L, const double *bp, int mtype, double *K)
+{
+    const double r = bp[9], ry = bp[10], rz = bp[11], ry2 = bp[12], rz2 = bp[13], ryz = bp[14];
+    double m[6][6];
+    memset(m, 0, sizeof(m));
+    m[0][0] = m[1][1] = m[2][2] = r;
+    m[0][4] = m[4][0] = rz;
+    m[0][5] = m[5][0] = -ry;
+    m[1][3] = m[3][1] = -rz;
+    m[2][3] = m[3][2] = ry;
+    m[3][3] = ry2 + rz2;
+    m[4][4] = rz2;
+    m[5][5] = ry2;
+    m[4][5] = m[5][4] = -ryz;
+    memset(K, 0, sizeof(double) * 144);
+    for (int I = 0; I < 2; I++)
+        for (int J = 0; J < 2; J++) {
+            if (mtype == 1 && I != J) continue;
+            const double f = mtype == 1 ? L / 2 : L * (I == J ? 1. / 3. : 1. / 6.);
+            for (int d = 0; d < 6; d++)
+                for (int e = 0; e < 6; e++) {
+                    if (mtype == 1 && d != e) continue;
+                    K[(6 * I + d) * 12 + 6 * J + e] = f * m[d][e];
+                }
+        }
+}
+
 /* Spring (spring.pyx:295-707): k on each local DOF between the nodes */
 static void spring_kc0_local(const double *kp, double *K)
 {
@@ -837,7 +917,15 @@ static int layout(int etype, int mat, int mtype, int nn, int *rl, int *cl)
             for (int j = 0; j < 12; j++) { rl[k] = i; cl[k] = j; k++; }
         return k;
     }
-    if (mat == MAT_KC0 || (etype == ET_BEAMC && (mat == MAT_KG || (mat == MAT_M && mtype == 0)))) {
+    if (etype == ET_BEAMLR && mat == MAT_KG) {
+        /* beamlr.pyx:1276-1386: bending rotations (I, 3..5, J, 3..5) */
+        for (int I = 0; I < 2; I++)
+            for (int d = 3; d < 6; d++)
+                for (int J = 0; J < 2; J++)
+                    for (int e = 3; e < 6; e++) { rl[k] = 6 * I + d; cl[k] = 6 * J + e; k++; }
+        return k;
+    }
+    if (mat == MAT_KC0 || (IS_BEAM(etype) && (mat == MAT_KG || (mat == MAT_M && mtype == 0)))) {
         for (int I = 0; I < nn; I++)
             for (int d = 0; d < 6; d++)
                 for (int J = 0; J < nn; J++)
@@ -852,7 +940,7 @@ static int layout(int etype, int mat, int mtype, int nn, int *rl, int *cl)
         return k;
     }
     /* mass */
-    if (etype == ET_BEAMC || etype == ET_TRUSS) { /* lumped: node-diagonal translations + rotations */
+    if (IS_BEAM(etype) || etype == ET_TRUSS) { /* lumped: node-diagonal translations + rotations */
         for (int I = 0; I < nn; I++)
             for (int g = 0; g < 2; g++)
                 for (int d = 0; d < 3; d++)
@@ -884,7 +972,7 @@ static int layout(int etype, int mat, int mtype, int nn, int *rl, int *cl)
 int orc_slot_count(int etype, int mat, int mtype)
 {
     int rl[576], cl[576];
-    int nn = etype == ET_TRIA3R ? 3 : ((etype == ET_BEAMC || IS_ROD(etype)) ? 2 : 4);
+    int nn = etype == ET_TRIA3R ? 3 : ((IS_BEAM(etype) || IS_ROD(etype)) ? 2 : 4);
     return layout(etype, mat, mtype, nn, rl, cl);
 }
 
@@ -894,7 +982,7 @@ int orc_slot_count(int etype, int mat, int mtype)
 
 static int nnodes(int etype)
 {
-    return etype == ET_TRIA3R ? 3 : ((etype == ET_BEAMC || IS_ROD(etype)) ? 2 : 4);
+    return etype == ET_TRIA3R ? 3 : ((IS_BEAM(etype) || IS_ROD(etype)) ? 2 : 4);
 }
 
 /* per-element orientation vectors: xmat / vxy (3), spring axis + vxy (6), truss none */
@@ -935,7 +1023,7 @@ static void element_geometry(int etype, const double *x, const int64_t *cn,
     } else beam_axes(X, evec, g);
     memcpy(g + 9, m, sizeof(m));
     local_coords(nn, X, g, xe);
-    if (etype == ET_BEAMC || etype == ET_TRUSS) {
+    if (IS_BEAM(etype) || etype == ET_TRUSS) {
         double d[3] = {X[3] - X[0], X[4] - X[1], X[5] - X[2]};
         g[13] = pow(d[0] * d[0] + d[1] * d[1] + d[2] * d[2], 0.5);
     } else {
@@ -983,6 +1071,14 @@ static int element_local(int etype, int mat, int mtype, const double *g, const d
     }
     if (etype == ET_SPRING) {
         if (mat == MAT_KC0 || mat == MAT_FINT) spring_kc0_local(prop, K);
+        else return -1;
+        return 0;
+    }
+    if (etype == ET_BEAMLR) {
+        if (mat == MAT_KC0 || mat == MAT_FINT) beamlr_kc0_local(area, prop, K);
+        else if (mat == MAT_KG) /* N = A E (u2 - u1) / L, beamlr.pyx:1388 */
+            beamlr_kg_local(area, prop[0] * prop[1] * (-ue[0] + ue[6]) / area, K);
+        else if (mat == MAT_M) beamlr_m_local(area, prop, mtype, K);
         else return -1;
         return 0;
     }
@@ -1037,7 +1133,7 @@ int orc_assemble(int etype, int mat, int mtype, int64_t E, const double *x,
     int nn = nnodes(etype), n = 6 * nn;
     int rl[576], cl[576];
     int ns = layout(etype, mat, mtype, nn, rl, cl);
-    int stride = (etype == ET_BEAMC || etype == ET_TRUSS) ? BP_STRIDE
+    int stride = (IS_BEAM(etype) || etype == ET_TRUSS) ? BP_STRIDE
                  : (etype == ET_SPRING ? SPR_STRIDE : SP_STRIDE);
     static const double hg_default[5] = {1., 1., 1., 1., 1.};
     for (int64_t e = 0; e < E; e++) {
@@ -1082,7 +1178,7 @@ int orc_fint(int etype, int64_t E, const double *x, const int64_t *conn, const d
              const double *hg, const double *u, double *fint)
 {
     int nn = nnodes(etype), n = 6 * nn;
-    int stride = (etype == ET_BEAMC || etype == ET_TRUSS) ? BP_STRIDE
+    int stride = (IS_BEAM(etype) || etype == ET_TRUSS) ? BP_STRIDE
                  : (etype == ET_SPRING ? SPR_STRIDE : SP_STRIDE);
     static const double hg_default[5] = {1., 1., 1., 1., 1.};
     for (int64_t e = 0; e < E; e++) {

@@ -14,9 +14,11 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, 'liboracle.so')
 
-ETYPES = {'quad4': 0, 'quad4r': 1, 'tria3r': 2, 'beamc': 3, 'truss': 4, 'spring': 5}
+ETYPES = {'quad4': 0, 'quad4r': 1, 'tria3r': 2, 'beamc': 3, 'truss': 4, 'spring': 5,
+          'beamlr': 6}
 MATS = {'KC0': 0, 'KG': 1, 'KGs': 2, 'M': 3}
-NNODES = {'quad4': 4, 'quad4r': 4, 'tria3r': 3, 'beamc': 2, 'truss': 2, 'spring': 2}
+NNODES = {'quad4': 4, 'quad4r': 4, 'tria3r': 3, 'beamc': 2, 'truss': 2, 'spring': 2,
+          'beamlr': 2}
 SHELL_STRIDE = 32
 BEAM_STRIDE = 16
 SPRING_STRIDE = 8
@@ -53,7 +55,7 @@ def slot_count(etype, mat, mtype=0):
 
 def props_table(records, etype):
     """Pad (n, k) property records to the oracle's stride."""
-    stride = {'beamc': BEAM_STRIDE, 'truss': BEAM_STRIDE,
+    stride = {'beamc': BEAM_STRIDE, 'truss': BEAM_STRIDE, 'beamlr': BEAM_STRIDE,
               'spring': SPRING_STRIDE}.get(etype, SHELL_STRIDE)
     rec = np.atleast_2d(np.asarray(records, dtype=np.float64))
     out = np.zeros((rec.shape[0], stride))

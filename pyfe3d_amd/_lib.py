@@ -30,6 +30,10 @@ SIGNATURES = {
     'pf3d_isotropic_records': [I64, P, P, P, P],
     'pf3d_shell_fe': [I32, I64, P, P, P, P, P, P, P, P, P, P, P],
     'pf3d_beam_fe': [I64, P, P, P, P, P, P, P, P],
+    'pf3d_beamlr_kc0': [I64, P, P, P, P, P, P, P, P, P, I32, P],
+    'pf3d_beamlr_kg': [I64, P, P, P, P, P, P, P, P, P, P, I32, P],
+    'pf3d_beamlr_m': [I64, P, P, P, P, P, P, I32, P, P, P, I32, P],
+    'pf3d_beamlr_fe': [I64, P, P, P, P, P, P, P, P],
     'pf3d_truss_kc0': [I64, P, P, P, P, P, P, P, P, I32, P],
     'pf3d_truss_m': [I64, P, P, P, P, P, I32, P, P, P, I32, P],
     'pf3d_spring_kc0': [I64, P, P, P, P, P, P, P, P, P, I32, P],
@@ -59,6 +63,7 @@ SIGNATURES = {
     'pf3d_pcg': [I64, P, P, P, P, P, P, F64, I32, I32, P, I64, P, P, P],
     'pf3d_ir_bytes': [I64, P],
     'pf3d_ir_solve': [I64, P, P, P, P, P, P, P, F64, F64, I32, I32, I32, P, I64, P, P, P],
+    'pf3d_pcg_mixed': [I64, P, P, P, P, P, P, P, P, F64, I64, I32, I32, P, I64, P, P, P],
     'pf3d_batch_bytes': [I64, P],
     'pf3d_band_max_width': [],
     'pf3d_band_factor': [I64, I64, P, P, P, P, I64, P, P, P, P, P, P, P, P],

@@ -20,9 +20,11 @@ import torch
 
 from . import _lib
 
-ETYPE = {'quad4': 0, 'quad4r': 1, 'tria3r': 2, 'beamc': 3, 'truss': 4, 'spring': 5}
+ETYPE = {'quad4': 0, 'quad4r': 1, 'tria3r': 2, 'beamc': 3, 'truss': 4, 'spring': 5,
+         'beamlr': 6}
 MATRIX = {'KC0': 0, 'KG': 1, 'KG_stress': 2, 'M': 3}
-NNODES = {'quad4': 4, 'quad4r': 4, 'tria3r': 3, 'beamc': 2, 'truss': 2, 'spring': 2}
+NNODES = {'quad4': 4, 'quad4r': 4, 'tria3r': 3, 'beamc': 2, 'truss': 2, 'spring': 2,
+          'beamlr': 2}
 VALUES_ONLY = 1
 OVERWRITE = 2
 SHELL_STRIDE = 32
@@ -201,44 +203,55 @@ def shell_m(etype, x, conn, r, c, v, props, mtype=0, pid=None, init_k=None, xmat
               pr, pcc, pv, _flags(values_only, overwrite), _stream(stream))
 
 
+def _beam_entry(etype, name):
+    if etype not in ('beamc', 'beamlr'):
+        raise ValueError('beam etype must be beamc or beamlr, got %r' % etype)
+    return ('pf3d_beam_' if etype == 'beamc' else 'pf3d_beamlr_') + name
+
+
 def beam_kc0(x, conn, vxy, r, c, v, props, pid=None, init_k=None, values_only=False,
-             overwrite=False, stream=None, check=True):
-    """BeamC KC0 of a batch (update_rotation_matrix(vxy, x) + update_KC0)."""
-    E, px, pc, pk, pe = _common('beamc', x, conn, init_k, vxy, check)
+             overwrite=False, stream=None, check=True, etype='beamc'):
+    """BeamC (or BeamLR, etype='beamlr') KC0 of a batch
+    (update_rotation_matrix(vxy, x) + update_KC0)."""
+    fn = _beam_entry(etype, 'kc0')
+    E, px, pc, pk, pe = _common(etype, x, conn, init_k, vxy, check)
     if pe is None:
         raise ValueError('vxy is required')
     _check_span(144 * E, r, c, v, values_only, init_k, check)
     pr, pcc, pv = _outputs(r, c, v, True, values_only)
     pp, ppid = _props(props, pid, E, BEAM_STRIDE, check)
-    _lib.call('pf3d_beam_kc0', E, px, pc, pk, pe, ppid, pp, pr, pcc, pv,
+    _lib.call(fn, E, px, pc, pk, pe, ppid, pp, pr, pcc, pv,
               _flags(values_only, overwrite), _stream(stream))
 
 
 def beam_kg(x, conn, vxy, u, r, c, v, props, pid=None, init_k=None, values_only=False,
-            overwrite=False, stream=None, check=True):
-    """BeamC KG from global displacements (update_probe_ue + update_KG)."""
-    E, px, pc, pk, pe = _common('beamc', x, conn, init_k, vxy, check)
+            overwrite=False, stream=None, check=True, etype='beamc'):
+    """BeamC (or BeamLR) KG from global displacements (update_probe_ue +
+    update_KG): 144 slots for BeamC, 36 (rotations) for BeamLR."""
+    fn = _beam_entry(etype, 'kg')
+    E, px, pc, pk, pe = _common(etype, x, conn, init_k, vxy, check)
     if pe is None:
         raise ValueError('vxy is required')
-    _check_span(144 * E, r, c, v, values_only, init_k, check)
+    _check_span(slot_count(etype, 'KG') * E, r, c, v, values_only, init_k, check)
     pr, pcc, pv = _outputs(r, c, v, True, values_only)
     pp, ppid = _props(props, pid, E, BEAM_STRIDE, check)
     if check:
         _check_nodes(conn, x, u)
-    _lib.call('pf3d_beam_kg', E, px, pc, pk, pe, ppid, pp, _ptr(u, torch.float64, 'u'),
+    _lib.call(fn, E, px, pc, pk, pe, ppid, pp, _ptr(u, torch.float64, 'u'),
               pr, pcc, pv, _flags(values_only, overwrite), _stream(stream))
 
 
 def beam_m(x, conn, vxy, r, c, v, props, mtype=0, pid=None, init_k=None, values_only=False,
-           overwrite=False, stream=None, check=True):
-    """BeamC mass matrix (update_M), mtype 0 consistent, 1 lumped."""
-    E, px, pc, pk, pe = _common('beamc', x, conn, init_k, vxy, check)
+           overwrite=False, stream=None, check=True, etype='beamc'):
+    """BeamC (or BeamLR) mass matrix (update_M), mtype 0 consistent, 1 lumped."""
+    fn = _beam_entry(etype, 'm')
+    E, px, pc, pk, pe = _common(etype, x, conn, init_k, vxy, check)
     if pe is None:
         raise ValueError('vxy is required')
-    _check_span(slot_count('beamc', 'M', mtype) * E, r, c, v, values_only, init_k, check)
+    _check_span(slot_count(etype, 'M', mtype) * E, r, c, v, values_only, init_k, check)
     pr, pcc, pv = _outputs(r, c, v, True, values_only)
     pp, ppid = _props(props, pid, E, BEAM_STRIDE, check)
-    _lib.call('pf3d_beam_m', E, px, pc, pk, pe, ppid, pp, int(mtype), pr, pcc, pv,
+    _lib.call(fn, E, px, pc, pk, pe, ppid, pp, int(mtype), pr, pcc, pv,
               _flags(values_only, overwrite), _stream(stream))
 
 
@@ -378,10 +391,12 @@ def shell_fe(etype, x, conn, u, props, pid=None, xmat=None, k6rot=None, alpha=No
     return fe
 
 
-def beam_fe(x, conn, vxy, u, props, pid=None, fe=None, stream=None, check=True):
-    """BeamC element internal forces in global axes, shape (E, 2, 6)
-    (beamc.pyx:234-286, 352-488, 1354-1391)."""
-    E, px, pc, _, pe = _common('beamc', x, conn, None, vxy, check)
+def beam_fe(x, conn, vxy, u, props, pid=None, fe=None, stream=None, check=True, etype='beamc'):
+    """BeamC (or BeamLR) element internal forces in global axes, shape
+    (E, 2, 6) (beamc.pyx:234-286, 352-488, 1354-1391; beamlr.pyx:236-288,
+    354-406, 1188-1225)."""
+    fn = _beam_entry(etype, 'fe')
+    E, px, pc, _, pe = _common(etype, x, conn, None, vxy, check)
     if pe is None:
         raise ValueError('vxy is required')
     if check:
@@ -389,7 +404,7 @@ def beam_fe(x, conn, vxy, u, props, pid=None, fe=None, stream=None, check=True):
     pp, ppid = _props(props, pid, E, BEAM_STRIDE, check)
     if fe is None:
         fe = torch.empty((E, 2, 6), dtype=torch.float64, device=x.device)
-    _lib.call('pf3d_beam_fe', E, px, pc, pe, ppid, pp, _ptr(u, torch.float64, 'u'),
+    _lib.call(fn, E, px, pc, pe, ppid, pp, _ptr(u, torch.float64, 'u'),
               _ptr(fe, torch.float64, 'fe', 12 * E), _stream(stream))
     return fe
 
@@ -462,8 +477,8 @@ def update_fint(etype, x, conn, u, fint, props, gather=None, vxy=None, axes=None
 
     ``gather`` is a :class:`NodeGather` for ``conn`` (built here if None);
     vxy for beams, axes for springs."""
-    if etype == 'beamc':
-        fe = beam_fe(x, conn, vxy, u, props, stream=stream, **kw)
+    if etype in ('beamc', 'beamlr'):
+        fe = beam_fe(x, conn, vxy, u, props, stream=stream, etype=etype, **kw)
     elif etype in ('truss', 'spring'):
         fe = rod_fe(etype, x, conn, u, props, axes=axes, stream=stream, **kw)
     else:

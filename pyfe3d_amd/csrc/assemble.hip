@@ -924,7 +924,7 @@ template <int ET>
 __global__ void element_geometry(int64_t n, const double *x, const int64_t *conn,
                                  const double *evec, double *geom, double *xe)
 {
-    constexpr int NN = ET == TRIA3R ? 3 : ((ET == BEAMC || ET == TRUSS || ET == SPRING) ? 2 : 4);
+    constexpr int NN = ET == TRIA3R ? 3 : ((ET == BEAMC || ET == BEAMLR || ET == TRUSS || ET == SPRING) ? 2 : 4);
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
     V3 X[NN];
@@ -937,7 +937,7 @@ __global__ void element_geometry(int64_t n, const double *x, const int64_t *conn
     if (evec && ET != TRUSS && ET != SPRING) ev = {evec[3 * e], evec[3 * e + 1], evec[3 * e + 2]};
     Geom g;
     if (ET == TRIA3R) tria_geometry(X, ev, g);
-    else if (ET == BEAMC) beam_geometry(X, ev, g);
+    else if (ET == BEAMC || ET == BEAMLR) beam_geometry(X, ev, g);
     else if (ET == TRUSS || ET == SPRING) rod_geometry<ET>(X, evec ? evec + 6 * e : nullptr, g);
     else quad_geometry(X, ev, g);
     double *o = geom + 14 * e;
@@ -1129,11 +1129,96 @@ __device__ void beam_m_local(double L, const BeamSec &s, const double *bp, int m
     beam_sym(U);
 }
 
+// BeamLR (beamlr.pyx): linear Timoshenko beam, all fields linear, one-point
+// (reduced) integration.  Midpoint strains eps = u', ky = ry', kz = rz',
+// gxy = v' - rz_m, gxz = w' + ry_m, phi = rx'; section stiffness
+// D = E [[A, Az, -Ay], [Az, Iyy, -Iyz], [-Ay, -Iyz, Izz]] on (eps, ky, kz),
+// S = G [[A, 0, -Az], [0, A, Ay], [-Az, Ay, J]] on (gxy, gxz, phi);
+// K = L (Bm^T D Bm + Bs^T S Bs) (beamlr.pyx:354-406 internal forces K ue,
+// 407-1187 the scattered K).  Each strain row has 2 or 4 terms.
+__device__ void beamlr_kc0_local(double L, const double *bp, double *U)
+{
+    const double A = bp[0], E = bp[1], G = bp[2], Iyy = bp[3], Izz = bp[4], Iyz = bp[5];
+    const double J = bp[6], Ay = bp[7], Az = bp[8];
+    const double D[6][6] = {{E * A, E * Az, -E * Ay, 0., 0., 0.},
+                            {E * Az, E * Iyy, -E * Iyz, 0., 0., 0.},
+                            {-E * Ay, -E * Iyz, E * Izz, 0., 0., 0.},
+                            {0., 0., 0., G * A, 0., -G * Az},
+                            {0., 0., 0., 0., G * A, G * Ay},
+                            {0., 0., 0., -G * Az, G * Ay, G * J}};
+    const double il = 1. / L;
+    // strain rows: DOF index and coefficient (4 terms, unused ones with 0)
+    const int bi[6][4] = {{0, 6, 0, 6}, {4, 10, 4, 10}, {5, 11, 5, 11},
+                          {1, 7, 5, 11}, {2, 8, 4, 10}, {3, 9, 3, 9}};
+    const double bc[6][4] = {{-il, il, 0., 0.}, {-il, il, 0., 0.}, {-il, il, 0., 0.},
+                             {-il, il, -0.5, -0.5}, {-il, il, 0.5, 0.5}, {-il, il, 0., 0.}};
+    for (int k = 0; k < 144; k++) U[k] = 0.;
+#pragma unroll
+    for (int a = 0; a < 6; a++)
+#pragma unroll
+        for (int b = 0; b < 6; b++) {
+            const double d = D[a][b];
+            if (d == 0.) continue;
+#pragma unroll
+            for (int p = 0; p < 4; p++)
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    U[bi[a][p] * 12 + bi[b][q]] += L * d * bc[a][p] * bc[b][q];
+        }
+}
+
+// BeamLR KG (beamlr.pyx:1226-1463): N int(N_I N_J) [[1, -1], [-1, 1]] on
+// (ry, rz) of nodes I, J (L/3 on a node, L/6 across)
+__device__ void beamlr_kg_local(double L, double N, double *U)
+{
+    for (int k = 0; k < 144; k++) U[k] = 0.;
+    for (int I = 0; I < 2; I++)
+        for (int J = 0; J < 2; J++) {
+            const double f = N * L * (I == J ? 1. / 3. : 1. / 6.);
+            for (int a = 0; a < 2; a++)
+                for (int b = 0; b < 2; b++) U[(6 * I + 4 + a) * 12 + 6 * J + 4 + b] = a == b ? f : -f;
+        }
+}
+
+// BeamLR mass (beamlr.pyx:1464-2424): section mass m of u = u0 + z ry - y rz,
+// v = v0 - z rx, w = w0 + y rx; consistent: L/3, L/6 times m; lumped
+// (Gauss-Lobatto at the nodes): L/2 times the diagonal of m
+__device__ void beamlr_m_local(double L, const double *bp, int mtype, double *U)
+{
+    const double r = bp[9], ry = bp[10], rz = bp[11], ry2 = bp[12], rz2 = bp[13], ryz = bp[14];
+    double m[6][6];
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) m[i][j] = 0.;
+    m[0][0] = m[1][1] = m[2][2] = r;
+    m[0][4] = m[4][0] = rz;
+    m[0][5] = m[5][0] = -ry;
+    m[1][3] = m[3][1] = -rz;
+    m[2][3] = m[3][2] = ry;
+    m[3][3] = ry2 + rz2;
+    m[4][4] = rz2;
+    m[5][5] = ry2;
+    m[4][5] = m[5][4] = -ryz;
+    for (int k = 0; k < 144; k++) U[k] = 0.;
+    for (int I = 0; I < 2; I++)
+        for (int J = 0; J < 2; J++) {
+            if (mtype == 1 && I != J) continue;
+            const double f = mtype == 1 ? L / 2 : L * (I == J ? 1. / 3. : 1. / 6.);
+            for (int d = 0; d < 6; d++)
+                for (int e = 0; e < 6; e++)
+                    if (mtype != 1 || d == e) U[(6 * I + d) * 12 + 6 * J + e] = f * m[d][e];
+        }
+}
+
 // slot -> (local row, local col) for beams: 12x12 row-major, or the lumped
 // mass's node-diagonal 3x3 translation and rotation blocks (beamc.pyx:2969+)
 __device__ __forceinline__ void beam_slot(int lumped, int k, int &rl, int &cl)
 {
-    if (!lumped) {
+    if (lumped == 2) {
+        // BeamLR KG: bending rotations (I, 3..5, J, 3..5), beamlr.pyx:1276-1386
+        const int I = k / 18, d = (k % 18) / 6, J = (k % 6) / 3, e = k % 3;
+        rl = 6 * I + 3 + d;
+        cl = 6 * J + 3 + e;
+    } else if (!lumped) {
         rl = k / 12;
         cl = k - rl * 12;
     } else {
@@ -1149,7 +1234,7 @@ struct BeamLds {
     static constexpr int BYTES = BPB * (144 * 8 + 2 * 8 + 8 + 9 * 8);
 };
 
-template <int MAT, int BPB>
+template <int ET, int MAT, int BPB>
 __device__ __forceinline__ void beam_body(const AsmArgs &a, int64_t blk, char *lds)
 {
     double (*tile)[144] = reinterpret_cast<double (*)[144]>(lds);
@@ -1160,7 +1245,8 @@ __device__ __forceinline__ void beam_body(const AsmArgs &a, int64_t blk, char *l
     const int64_t e0 = blk * BPB;
     const int64_t e = e0 + slot;
     const bool mine = slot < BPB && e < a.n_elem;
-    const bool lumped = (MAT == MASS) && a.mtype == 1;
+    // slot map: 0 full 12x12, 1 lumped node-diagonal, 2 BeamLR KG rotations
+    const int lumped = ((MAT == MASS) && a.mtype == 1) ? 1 : ((ET == BEAMLR && MAT == KG) ? 2 : 0);
     const int nslot = lumped ? 36 : 144;
     if (mine && (lane & 3) == 0) {
         V3 X[2];
@@ -1178,17 +1264,27 @@ __device__ __forceinline__ void beam_body(const AsmArgs &a, int64_t blk, char *l
         beam_geometry(X, vxy, g);
         for (int i = 0; i < 9; i++) sr[slot][i] = g.r[i];
         const double *bp = a.props + (a.pid ? (int64_t)a.pid[e] : 0) * BEAM_STRIDE;
-        const BeamSec sec = beam_section(bp, g.area);
-        if (MAT == KC0) beam_kc0_local(g.area, sec, tile[slot]);
-        else if (MAT == KG) {
-            // axial force from the local axial displacements (beamc.pyx:1889)
+        double N = 0.;
+        if (MAT == KG) {
+            // axial force from the local axial displacements (beamc.pyx:1889,
+            // beamlr.pyx:1388)
             double u1 = 0., u2 = 0.;
             for (int i = 0; i < 3; i++) {
                 u1 += g.r[3 * i] * a.u[cn[0] + i];
                 u2 += g.r[3 * i] * a.u[cn[1] + i];
             }
-            beam_kg_local(g.area, sec, sec.A * sec.E * (-u1 + u2) / g.area, tile[slot]);
-        } else beam_m_local(g.area, sec, bp, a.mtype, tile[slot]);
+            N = bp[0] * bp[1] * (-u1 + u2) / g.area;
+        }
+        if (ET == BEAMLR) {
+            if (MAT == KC0) beamlr_kc0_local(g.area, bp, tile[slot]);
+            else if (MAT == KG) beamlr_kg_local(g.area, N, tile[slot]);
+            else beamlr_m_local(g.area, bp, a.mtype, tile[slot]);
+        } else {
+            const BeamSec sec = beam_section(bp, g.area);
+            if (MAT == KC0) beam_kc0_local(g.area, sec, tile[slot]);
+            else if (MAT == KG) beam_kg_local(g.area, sec, N, tile[slot]);
+            else beam_m_local(g.area, sec, bp, a.mtype, tile[slot]);
+        }
     }
     __syncthreads();
     double K[6][6];
@@ -1223,14 +1319,16 @@ __device__ __forceinline__ void beam_body(const AsmArgs &a, int64_t blk, char *l
     }
 }
 
-template <int MAT>
+template <int ET, int MAT>
 __global__ void __launch_bounds__(64) beam_assemble(AsmArgs a)
 {
     __shared__ __align__(16) char lds[BeamLds<16>::BYTES];
-    beam_body<MAT, 16>(a, blockIdx.x, lds);
+    beam_body<ET, MAT, 16>(a, blockIdx.x, lds);
 }
 
-// BeamC element internal forces (beamc.pyx:352-488, 1354-1391)
+// BeamC / BeamLR element internal forces (beamc.pyx:352-488, 1354-1391;
+// beamlr.pyx:354-406, 1188-1225)
+template <int ET>
 __global__ void __launch_bounds__(64) beam_fe(AsmArgs a, double *__restrict__ fe)
 {
     __shared__ double tile[16][144];
@@ -1258,7 +1356,8 @@ __global__ void __launch_bounds__(64) beam_fe(AsmArgs a, double *__restrict__ fe
                     sue[slot][6 * n + 3 * b + t] = s;
                 }
         const double *bp = a.props + (a.pid ? (int64_t)a.pid[e] : 0) * BEAM_STRIDE;
-        beam_kc0_local(g.area, beam_section(bp, g.area), tile[slot]);
+        if (ET == BEAMLR) beamlr_kc0_local(g.area, bp, tile[slot]);
+        else beam_kc0_local(g.area, beam_section(bp, g.area), tile[slot]);
     }
     __syncthreads();
     if (e < a.n_elem && I < 2) {
@@ -1474,7 +1573,7 @@ mixed_kc0(MixedArgs m)
     if (b < m.blk0[1]) shell_kc0_body<QUAD4, FLAGS>(m.g[0], b, lds);
     else if (b < m.blk0[2]) shell_kc0_body<QUAD4R, FLAGS>(m.g[1], b - m.blk0[1], lds);
     else if (b < m.blk0[3]) shell_kc0_body<TRIA3R, FLAGS>(m.g[2], b - m.blk0[2], lds);
-    else beam_body<KC0, MIXED_BPB>(m.g[3], b - m.blk0[3], lds);
+    else beam_body<BEAMC, KC0, MIXED_BPB>(m.g[3], b - m.blk0[3], lds);
 }
 
 // KG from displacements or mass (MAT = KG or MASS)
@@ -1486,7 +1585,7 @@ __global__ void __launch_bounds__(64) mixed_stream(MixedArgs m)
     if (b < m.blk0[1]) shell_stream_body<QUAD4, MAT>(m.g[0], b, lds);
     else if (b < m.blk0[2]) shell_stream_body<QUAD4R, MAT>(m.g[1], b - m.blk0[1], lds);
     else if (b < m.blk0[3]) shell_stream_body<TRIA3R, MAT>(m.g[2], b - m.blk0[2], lds);
-    else beam_body<MAT, MIXED_BPB>(m.g[3], b - m.blk0[3], lds);
+    else beam_body<BEAMC, MAT, MIXED_BPB>(m.g[3], b - m.blk0[3], lds);
 }
 
 }  // namespace pf3d
@@ -1549,16 +1648,23 @@ hipError_t pf3d_launch_shell(int etype, int mat, const AsmArgs &a, hipStream_t s
     }
 }
 
-hipError_t pf3d_launch_beam(int mat, const AsmArgs &a, hipStream_t s)
+template <int ET>
+static void launch_beam_et(int mat, const AsmArgs &a, dim3 grid, hipStream_t s)
+{
+    if (mat == KC0)
+        hipLaunchKernelGGL((beam_assemble<ET, KC0>), grid, dim3(64), 0, s, a);
+    else if (mat == KG)
+        hipLaunchKernelGGL((beam_assemble<ET, KG>), grid, dim3(64), 0, s, a);
+    else
+        hipLaunchKernelGGL((beam_assemble<ET, MASS>), grid, dim3(64), 0, s, a);
+}
+
+hipError_t pf3d_launch_beam(int etype, int mat, const AsmArgs &a, hipStream_t s)
 {
     const int64_t blocks = (a.n_elem + 15) / 16;
     if (blocks == 0) return hipSuccess;
-    if (mat == KC0)
-        hipLaunchKernelGGL(beam_assemble<KC0>, dim3((unsigned)blocks), dim3(64), 0, s, a);
-    else if (mat == KG)
-        hipLaunchKernelGGL(beam_assemble<KG>, dim3((unsigned)blocks), dim3(64), 0, s, a);
-    else
-        hipLaunchKernelGGL(beam_assemble<MASS>, dim3((unsigned)blocks), dim3(64), 0, s, a);
+    if (etype == BEAMLR) launch_beam_et<BEAMLR>(mat, a, dim3((unsigned)blocks), s);
+    else launch_beam_et<BEAMC>(mat, a, dim3((unsigned)blocks), s);
     return hipGetLastError();
 }
 
@@ -1680,10 +1786,13 @@ hipError_t pf3d_launch_shell_fe(int etype, const AsmArgs &a, double *fe, hipStre
     return hipGetLastError();
 }
 
-hipError_t pf3d_launch_beam_fe(const AsmArgs &a, double *fe, hipStream_t s)
+hipError_t pf3d_launch_beam_fe(int etype, const AsmArgs &a, double *fe, hipStream_t s)
 {
     const int64_t blocks = (a.n_elem + 15) / 16;
     if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(beam_fe, dim3((unsigned)blocks), dim3(64), 0, s, a, fe);
+    if (etype == BEAMLR)
+        hipLaunchKernelGGL(beam_fe<BEAMLR>, dim3((unsigned)blocks), dim3(64), 0, s, a, fe);
+    else
+        hipLaunchKernelGGL(beam_fe<BEAMC>, dim3((unsigned)blocks), dim3(64), 0, s, a, fe);
     return hipGetLastError();
 }

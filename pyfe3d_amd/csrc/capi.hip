@@ -8,9 +8,9 @@
 
 // launchers defined next to the kernels
 hipError_t pf3d_launch_shell(int etype, int mat, const pf3d::AsmArgs &a, hipStream_t s);
-hipError_t pf3d_launch_beam(int mat, const pf3d::AsmArgs &a, hipStream_t s);
+hipError_t pf3d_launch_beam(int etype, int mat, const pf3d::AsmArgs &a, hipStream_t s);
 hipError_t pf3d_launch_shell_fe(int etype, const pf3d::AsmArgs &a, double *fe, hipStream_t s);
-hipError_t pf3d_launch_beam_fe(const pf3d::AsmArgs &a, double *fe, hipStream_t s);
+hipError_t pf3d_launch_beam_fe(int etype, const pf3d::AsmArgs &a, double *fe, hipStream_t s);
 hipError_t pf3d_launch_rod(int etype, int mat, const pf3d::AsmArgs &a, hipStream_t s);
 hipError_t pf3d_launch_mixed(int mat, const pf3d::AsmArgs *groups, hipStream_t s);
 hipError_t pf3d_launch_rod_fe(int etype, const pf3d::AsmArgs &a, double *fe, hipStream_t s);
@@ -52,6 +52,15 @@ int pf3d_slot_count(int etype, int matrix, int mtype)
         }
         return pf3d_fail(PF3D_E_ARG, "Truss has KC0 and M only");
     }
+    if (etype == PF3D_BEAMLR) {
+        if (matrix == PF3D_KC0) return 144;
+        if (matrix == PF3D_KG) return 36;
+        if (matrix == PF3D_M) {
+            if (mtype < 0 || mtype > 1) return pf3d_fail(PF3D_E_ARG, "BeamLR mtype must be 0 or 1");
+            return mtype == 0 ? 144 : 36;
+        }
+        return pf3d_fail(PF3D_E_ARG, "BeamLR has no KG_given_stress");
+    }
     if (etype == PF3D_SPRING) {
         if (matrix == PF3D_KC0) return 72;
         return pf3d_fail(PF3D_E_ARG, "Spring has KC0 only");
@@ -80,11 +89,12 @@ int pf3d_slot_count(int etype, int matrix, int mtype)
 int pf3d_geometry(int etype, int64_t n_elem, const double *x, const int64_t *conn,
                   const double *evec, double *geom, double *xe, void *stream)
 {
-    if (etype < 0 || etype > PF3D_SPRING) return pf3d_fail(PF3D_E_ARG, "bad etype %d", etype);
+    if (etype < 0 || etype > PF3D_BEAMLR) return pf3d_fail(PF3D_E_ARG, "bad etype %d", etype);
     if (n_elem < 0) return pf3d_fail(PF3D_E_ARG, "negative n_elem");
     if (n_elem == 0) return PF3D_OK;
     if (!x || !conn || !geom) return pf3d_fail(PF3D_E_ARG, "null x/conn/geom");
-    if (etype == PF3D_BEAMC && !evec) return pf3d_fail(PF3D_E_ARG, "BeamC needs vxy");
+    if ((etype == PF3D_BEAMC || etype == PF3D_BEAMLR) && !evec)
+        return pf3d_fail(PF3D_E_ARG, "beams need vxy");
     if (etype == PF3D_SPRING && !evec) return pf3d_fail(PF3D_E_ARG, "Spring needs its axes");
     return pf3d_hip_check(pf3d_launch_geometry(etype, n_elem, x, conn, evec, geom, xe,
                                                (hipStream_t)stream),
@@ -182,7 +192,7 @@ int pf3d_beam_kc0(int64_t n_elem, const double *x, const int64_t *conn, const in
     pf3d::AsmArgs a{};
     a.n_elem = n_elem; a.x = x; a.conn = conn; a.init_k = init_k; a.evec = vxy; a.pid = pid;
     a.props = props; a.flags = flags; a.r = r; a.c = c; a.v = v;
-    return pf3d_hip_check(pf3d_launch_beam(PF3D_KC0, a, (hipStream_t)stream), "pf3d_beam_kc0");
+    return pf3d_hip_check(pf3d_launch_beam(PF3D_BEAMC, PF3D_KC0, a, (hipStream_t)stream), "pf3d_beam_kc0");
 }
 
 static int beam_check(int64_t n_elem, const double *x, const int64_t *conn, const double *vxy,
@@ -208,7 +218,7 @@ int pf3d_beam_kg(int64_t n_elem, const double *x, const int64_t *conn, const int
     pf3d::AsmArgs a{};
     a.n_elem = n_elem; a.x = x; a.conn = conn; a.init_k = init_k; a.evec = vxy; a.pid = pid;
     a.props = props; a.u = u; a.flags = flags; a.r = r; a.c = c; a.v = v;
-    return pf3d_hip_check(pf3d_launch_beam(PF3D_KG, a, (hipStream_t)stream), "pf3d_beam_kg");
+    return pf3d_hip_check(pf3d_launch_beam(PF3D_BEAMC, PF3D_KG, a, (hipStream_t)stream), "pf3d_beam_kg");
 }
 
 int pf3d_beam_m(int64_t n_elem, const double *x, const int64_t *conn, const int64_t *init_k,
@@ -221,7 +231,63 @@ int pf3d_beam_m(int64_t n_elem, const double *x, const int64_t *conn, const int6
     pf3d::AsmArgs a{};
     a.n_elem = n_elem; a.x = x; a.conn = conn; a.init_k = init_k; a.evec = vxy; a.pid = pid;
     a.props = props; a.mtype = mtype; a.flags = flags; a.r = r; a.c = c; a.v = v;
-    return pf3d_hip_check(pf3d_launch_beam(PF3D_M, a, (hipStream_t)stream), "pf3d_beam_m");
+    return pf3d_hip_check(pf3d_launch_beam(PF3D_BEAMC, PF3D_M, a, (hipStream_t)stream), "pf3d_beam_m");
+}
+
+int pf3d_beamlr_kc0(int64_t n_elem, const double *x, const int64_t *conn, const int64_t *init_k,
+                    const double *vxy, const int32_t *pid, const double *props, int64_t *r,
+                    int64_t *c, double *v, int flags, void *stream)
+{
+    int st = beam_check(n_elem, x, conn, vxy, props, r, c, v, flags);
+    if (st) return st > 0 ? PF3D_OK : st;
+    pf3d::AsmArgs a{};
+    a.n_elem = n_elem; a.x = x; a.conn = conn; a.init_k = init_k; a.evec = vxy; a.pid = pid;
+    a.props = props; a.flags = flags; a.r = r; a.c = c; a.v = v;
+    return pf3d_hip_check(pf3d_launch_beam(PF3D_BEAMLR, PF3D_KC0, a, (hipStream_t)stream),
+                          "pf3d_beamlr_kc0");
+}
+
+int pf3d_beamlr_kg(int64_t n_elem, const double *x, const int64_t *conn, const int64_t *init_k,
+                   const double *vxy, const int32_t *pid, const double *props, const double *u,
+                   int64_t *r, int64_t *c, double *v, int flags, void *stream)
+{
+    int st = beam_check(n_elem, x, conn, vxy, props, r, c, v, flags);
+    if (st) return st > 0 ? PF3D_OK : st;
+    if (!u) return pf3d_fail(PF3D_E_ARG, "null u");
+    pf3d::AsmArgs a{};
+    a.n_elem = n_elem; a.x = x; a.conn = conn; a.init_k = init_k; a.evec = vxy; a.pid = pid;
+    a.props = props; a.u = u; a.flags = flags; a.r = r; a.c = c; a.v = v;
+    return pf3d_hip_check(pf3d_launch_beam(PF3D_BEAMLR, PF3D_KG, a, (hipStream_t)stream),
+                          "pf3d_beamlr_kg");
+}
+
+int pf3d_beamlr_m(int64_t n_elem, const double *x, const int64_t *conn, const int64_t *init_k,
+                  const double *vxy, const int32_t *pid, const double *props, int mtype,
+                  int64_t *r, int64_t *c, double *v, int flags, void *stream)
+{
+    int st = beam_check(n_elem, x, conn, vxy, props, r, c, v, flags);
+    if (st) return st > 0 ? PF3D_OK : st;
+    if (mtype < 0 || mtype > 1) return pf3d_fail(PF3D_E_ARG, "BeamLR mtype must be 0 or 1");
+    pf3d::AsmArgs a{};
+    a.n_elem = n_elem; a.x = x; a.conn = conn; a.init_k = init_k; a.evec = vxy; a.pid = pid;
+    a.props = props; a.mtype = mtype; a.flags = flags; a.r = r; a.c = c; a.v = v;
+    return pf3d_hip_check(pf3d_launch_beam(PF3D_BEAMLR, PF3D_M, a, (hipStream_t)stream),
+                          "pf3d_beamlr_m");
+}
+
+int pf3d_beamlr_fe(int64_t n_elem, const double *x, const int64_t *conn, const double *vxy,
+                   const int32_t *pid, const double *props, const double *u, double *fe,
+                   void *stream)
+{
+    if (n_elem < 0) return pf3d_fail(PF3D_E_ARG, "negative n_elem");
+    if (n_elem > (int64_t)2147483647 * 16) return pf3d_fail(PF3D_E_ARG, "n_elem too large");
+    if (n_elem == 0) return PF3D_OK;
+    if (!x || !conn || !vxy || !props || !u || !fe) return pf3d_fail(PF3D_E_ARG, "null argument");
+    pf3d::AsmArgs a{};
+    a.n_elem = n_elem; a.x = x; a.conn = conn; a.evec = vxy; a.pid = pid; a.props = props;
+    a.u = u;
+    return pf3d_hip_check(pf3d_launch_beam_fe(PF3D_BEAMLR, a, fe, (hipStream_t)stream),
+                          "pf3d_beamlr_fe");
 }
 
 int pf3d_shell_fe(int etype, int64_t n_elem, const double *x, const int64_t *conn,
@@ -252,7 +318,7 @@ int pf3d_beam_fe(int64_t n_elem, const double *x, const int64_t *conn, const dou
     pf3d::AsmArgs a{};
     a.n_elem = n_elem; a.x = x; a.conn = conn; a.evec = vxy; a.pid = pid; a.props = props;
     a.u = u;
-    return pf3d_hip_check(pf3d_launch_beam_fe(a, fe, (hipStream_t)stream), "pf3d_beam_fe");
+    return pf3d_hip_check(pf3d_launch_beam_fe(PF3D_BEAMC, a, fe, (hipStream_t)stream), "pf3d_beam_fe");
 }
 
 int pf3d_mixed_assemble(int matrix, int mtype, int n_groups, const pf3d_elem_group *groups,

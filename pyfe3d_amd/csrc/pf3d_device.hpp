@@ -14,7 +14,7 @@ constexpr int BEAM_STRIDE = 16;   // doubles per beam property record (BeamC, Tr
 constexpr int SPRING_STRIDE = 6;  // kxe kye kze krxe krye krze per spring record
 constexpr double GP2 = 0.5773502691896257645092;
 
-enum ElemType { QUAD4 = 0, QUAD4R = 1, TRIA3R = 2, BEAMC = 3, TRUSS = 4, SPRING = 5 };
+enum ElemType { QUAD4 = 0, QUAD4R = 1, TRIA3R = 2, BEAMC = 3, TRUSS = 4, SPRING = 5, BEAMLR = 6 };
 enum Matrix { KC0 = 0, KG = 1, KGS = 2, MASS = 3 };
 
 // flags of the assembly entry points

@@ -356,6 +356,112 @@ int pcg_run(const Mat &A, const VT *b, VT *x, double rtol, int maxiter, int x0, 
     return PF3D_OK;
 }
 
+// FP64 PCG restarted from the true residual until the true residual meets
+// rtol (the recursive residual of long runs drifts from b - A x)
+int pcg64_restarted(const Mat &A, const double *b, double *x, double rtol, int64_t maxiter,
+                    int x0, void *work, hipStream_t s, int64_t *iters, double *relres)
+{
+    int64_t total = 0;
+    double rel = 0.;
+    for (int restart = 0; restart < 8; restart++) {
+        int64_t it = 0;
+        int st = pcg_run<double, double>(A, b, x, rtol, (int)(maxiter - total), x0 || restart,
+                                         work, s, &it, &rel, 25);
+        if (st) return st;
+        total += it;
+        if (it == 0 || total >= maxiter) break;  // true residual already below rtol
+    }
+    *iters = total;
+    *relres = rel;
+    return PF3D_OK;
+}
+
+// Mixed-precision PCG with reliable updates: the Krylov recursion reads K
+// and the block inverses in FP32 (vectors and arithmetic FP64); every
+// `replace` iterations the recursive residual is replaced by the true FP64
+// residual b - K64 x (search direction kept, r.z recomputed), so the
+// iteration converges to the FP64 system's solution without the restarts
+// of iterative refinement.  When `stall` consecutive replacements fail to
+// halve the true residual the solve finishes with FP64 PCG from the current
+// x.  stats: [0] iterations, [1] replacements, [2] FP64 iterations.
+int pcg_mixed_run(const Mat &A64, const Mat &A32, const double *b, double *x, double rtol,
+                  int64_t maxiter, int replace, int stall, int x0, void *work, hipStream_t s,
+                  int64_t *stats, double *relres)
+{
+    const int64_t n = A64.nb * 6;
+    char *w = (char *)work;
+    double *r = (double *)w, *z = r + n, *p = z + n, *q = p + n;
+    double *partial = (double *)(w + ((4 * n * 8 + 255) / 256) * 256);
+    PcgState *st = (PcgState *)(partial + 2 * RED_BLOCKS);
+    double *y = (double *)(w + pcg_bytes(n, sizeof(double)));
+    const unsigned nv = nblk(n, VB), nr = red_blocks(A64.nb), nd = red_blocks(n);
+    hipError_t e;
+    hipLaunchKernelGGL(dot2<double>, dim3(nd), dim3(VB), 0, s, n, b, b, (const double *)nullptr,
+                       (const double *)nullptr, partial);
+    hipLaunchKernelGGL(finish, dim3(1), dim3(VB), 0, s, (int)nd, partial, st, 3);
+    if (!x0) hipLaunchKernelGGL(fill0<double>, dim3(nv), dim3(VB), 0, s, n, x);
+    hipLaunchKernelGGL(fill0<double>, dim3(nv), dim3(VB), 0, s, n, p);
+    // r = b - K64 x ; z = M r ; (r.z, r.r) ; keep p (zero at the start)
+    auto true_residual = [&]() -> hipError_t {
+        hipError_t e2 = spmv<double, double>(A64, x, y, s);
+        if (e2 != hipSuccess) return e2;
+        hipLaunchKernelGGL(residual<double>, dim3(nv), dim3(VB), 0, s, n, b, y, r,
+                           (double *)nullptr);
+        hipLaunchKernelGGL(set_alpha, dim3(1), dim3(1), 0, s, st, 0.);
+        hipLaunchKernelGGL((update_xrz<float, double>), dim3(nr), dim3(VB), 0, s, A64.nb, st, x,
+                           r, p, q, (const float *)A32.dinv, z, partial);
+        hipLaunchKernelGGL(finish, dim3(1), dim3(VB), 0, s, (int)nr, partial, st, 2);
+        return hipGetLastError();
+    };
+    if ((e = true_residual()) != hipSuccess) return pf3d_hip_check(e, "mixed pcg init");
+    hipMemcpyAsync(p, z, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
+    PcgState h;
+    e = hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return pf3d_hip_check(e, "mixed pcg init");
+    const double bb = h.bb > 0 ? h.bb : 1.;
+    double rel = sqrt(h.rr / bb), best = rel;
+    int64_t it = 0, nrep = 0;
+    int since = 0;
+    while (rel > rtol && it < maxiter && since < stall) {
+        for (int k = 0; k < replace && it < maxiter; k++, it++) {
+            spmv<float, double>(A32, p, q, s);
+            hipLaunchKernelGGL(dot2<double>, dim3(nd), dim3(VB), 0, s, n, p, q,
+                               (const double *)nullptr, (const double *)nullptr, partial);
+            hipLaunchKernelGGL(finish, dim3(1), dim3(VB), 0, s, (int)nd, partial, st, 0);
+            hipLaunchKernelGGL((update_xrz<float, double>), dim3(nr), dim3(VB), 0, s, A64.nb, st,
+                               x, r, p, q, (const float *)A32.dinv, z, partial);
+            hipLaunchKernelGGL(finish, dim3(1), dim3(VB), 0, s, (int)nr, partial, st, 1);
+            hipLaunchKernelGGL(update_p<double>, dim3(nv), dim3(VB), 0, s, n, st, z, p);
+        }
+        if ((e = true_residual()) != hipSuccess) return pf3d_hip_check(e, "mixed pcg replace");
+        nrep++;
+        e = hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return pf3d_hip_check(e, "mixed pcg iterate");
+        rel = sqrt(h.rr / bb);
+        if (!isfinite(rel)) break;
+        if (rel <= 0.5 * best) {
+            best = rel;
+            since = 0;
+        } else {
+            since++;
+        }
+    }
+    stats[0] = it;
+    stats[1] = nrep;
+    stats[2] = 0;
+    if (rel > rtol && isfinite(rel) && it < maxiter) {
+        // stagnation of the FP32 operator: finish in FP64 from x
+        int64_t it64 = 0;
+        int st2 = pcg64_restarted(A64, b, x, rtol, maxiter - it, 1, work, s, &it64, &rel);
+        if (st2) return st2;
+        stats[2] = it64;
+    }
+    *relres = rel;
+    return PF3D_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -436,21 +542,7 @@ int pf3d_pcg(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx, const d
     if ((int64_t)pcg_bytes(6 * nb, sizeof(double)) > work_bytes)
         return pf3d_fail(PF3D_E_ARG, "pf3d_pcg: workspace too small");
     Mat A{nb, row_ptr, col_idx, val, dinv};
-    // restart from the true residual until it meets the tolerance (the
-    // recursive residual of long runs drifts from b - A x)
-    int64_t total = 0;
-    double rel = 0.;
-    for (int restart = 0; restart < 8; restart++) {
-        int64_t it = 0;
-        int st = pcg_run<double, double>(A, b, x, rtol, (int)(maxiter - total), x0 || restart,
-                                         work, (hipStream_t)stream, &it, &rel, 25);
-        if (st) return st;
-        total += it;
-        if (it == 0 || total >= maxiter) break;  // true residual already below rtol
-    }
-    *iters = total;
-    *relres = rel;
-    return PF3D_OK;
+    return pcg64_restarted(A, b, x, rtol, maxiter, x0, work, (hipStream_t)stream, iters, relres);
 }
 
 /* FP64 iterative refinement around a PCG on the FP32-stored matrix:
@@ -526,6 +618,24 @@ int pf3d_ir_solve(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx,
     *relres = rel;
     if (!(rel <= rtol)) return pf3d_fail(PF3D_E_SOLVE, "IR stopped at relres %.3e", rel);
     return PF3D_OK;
+}
+
+int pf3d_pcg_mixed(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx,
+                   const double *val64, const double *dinv64, const float *val32,
+                   const float *dinv32, const double *b, double *x, double rtol, int64_t maxiter,
+                   int replace, int x0, void *work, int64_t work_bytes, int64_t *stats,
+                   double *relres, void *stream)
+{
+    if (nb <= 0 || !row_ptr || !col_idx || !val64 || !dinv64 || !val32 || !dinv32 || !b || !x ||
+        !work || !stats || !relres || replace < 1 || maxiter < 0)
+        return pf3d_fail(PF3D_E_ARG, "pf3d_pcg_mixed: bad arguments");
+    int64_t need;
+    pf3d_ir_bytes(nb, &need);
+    if (need > work_bytes) return pf3d_fail(PF3D_E_ARG, "pf3d_pcg_mixed: workspace too small");
+    Mat A64{nb, row_ptr, col_idx, val64, dinv64};
+    Mat A32{nb, row_ptr, col_idx, val32, dinv32};
+    return pcg_mixed_run(A64, A32, b, x, rtol, maxiter, replace, 20, x0, work,
+                         (hipStream_t)stream, stats, relres);
 }
 
 }  // extern "C"

@@ -111,12 +111,15 @@ class StaticSystem:
         self._factored = True
 
     def solve(self, f, method='fp64', rtol=1e-8, x0=None, maxiter=200000, inner_rtol=1e-4,
-              max_outer=60, refactor=False, stream=None):
+              max_outer=60, replace=50, refactor=False, stream=None):
         """Returns (u, info) with info = dict(iters, outer, relres).
 
-        method: 'fp64' (block-Jacobi PCG), 'mixed' (FP64 refinement around
-        PCG on the FP32-stored matrix) or 'direct' (banded Cholesky, factor
-        kept for later right-hand sides unless refactor)."""
+        method: 'fp64' (block-Jacobi PCG), 'mixed' (PCG on the FP32-stored
+        matrix with FP64 vectors and a true FP64 residual replacing the
+        recursive one every `replace` iterations; info['outer'] counts the
+        replacements), 'ir' (FP64 iterative refinement around restarted
+        FP32-matrix PCG solves to inner_rtol) or 'direct' (banded Cholesky,
+        factor kept for later right-hand sides unless refactor)."""
         b = self.rhs(f)
         x = torch.zeros_like(b) if x0 is None else x0.clone()
         pr, pc = self.plan.row_ptr.data_ptr(), self.plan.col_idx.data_ptr()
@@ -129,6 +132,15 @@ class StaticSystem:
                       ctypes.byref(iters), ctypes.byref(relres), _s(stream))
             info = dict(iters=iters.value, outer=0)
         elif method == 'mixed':
+            stats = (ctypes.c_int64 * 3)()
+            _lib.call('pf3d_pcg_mixed', self.nb, pr, pc, self.val.data_ptr(),
+                      self.dinv.data_ptr(), self.val32.data_ptr(), self.dinv32.data_ptr(),
+                      b.data_ptr(), x.data_ptr(), float(rtol), int(maxiter), int(replace),
+                      int(x0 is not None), self._wir.data_ptr(), self._wir.numel(), stats,
+                      ctypes.byref(relres), _s(stream))
+            info = dict(iters=int(stats[0]) + int(stats[2]), outer=int(stats[1]),
+                        fp64_iters=int(stats[2]))
+        elif method == 'ir':
             stats = (ctypes.c_int64 * 2)()
             _lib.call('pf3d_ir_solve', self.nb, pr, pc, self.val.data_ptr(),
                       self.val32.data_ptr(), self.dinv32.data_ptr(), b.data_ptr(),

@@ -24,7 +24,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF = '/root/reference'
 BUILD = os.path.join(HERE, '_refbuild')
-MODULES = ['shellprop', 'beamprop', 'quad4', 'quad4r', 'tria3r', 'beamc', 'truss', 'spring']
+MODULES = ['shellprop', 'beamprop', 'quad4', 'quad4r', 'tria3r', 'beamc', 'truss', 'spring',
+           'beamlr']
 
 
 def build_reference():
@@ -322,6 +323,91 @@ def run_beam_case(mods, case, rng):
                 fint=fint, **out)
 
 
+def run_beamlr_case(mods, case, rng):
+    """BeamLR: the BeamC case's geometry and section properties through the
+    reduced-integration Timoshenko element (beamlr.pyx)."""
+    bl = mods['beamlr']
+    BeamProp = mods['beamprop'].BeamProp
+    n = case['n']
+    t = np.linspace(0, 1, n + 1)
+    P = np.vstack([case['L'] * t, 0.2 * np.sin(2 * t) + 0.01 * rng.uniform(-1, 1, t.size),
+                   0.1 * t ** 2]).T
+    if case['rot'] is not None:
+        P = P @ rot3(*case['rot']).T
+    x = P.ravel().astype(np.float64)
+    N = n + 1
+    conn = np.array([[i, i + 1] for i in range(n)], dtype=np.int64)
+    E = n
+    props, precs = [], []
+    for e in range(E):
+        p = BeamProp()
+        b_, h_ = 0.02 * (1 + 0.1 * e), 0.004 * (1 + 0.05 * e)
+        E_, nu = 70e9, 0.33
+        p.A = b_ * h_
+        p.E = E_
+        p.G = E_ / 2 / (1 + nu) * 5 / 6
+        p.Izz = b_ ** 3 * h_ / 12
+        p.Iyy = b_ * h_ ** 3 / 12
+        p.Iyz = case['Iyz'] * p.Iyy
+        p.J = p.Izz + p.Iyy
+        p.Ay = case['Ay'] * p.A
+        p.Az = case['Az'] * p.A
+        rho = 2700.
+        p.intrho = rho * p.A
+        p.intrhoy = rho * case['Ay'] * 1e-3
+        p.intrhoz = rho * case['Az'] * 1e-3
+        p.intrhoy2 = rho * p.Izz
+        p.intrhoz2 = rho * p.Iyy
+        p.intrhoyz = rho * p.Iyz
+        props.append(p)
+        precs.append(beam_record(p))
+    precs = np.array(precs)
+    vxy = np.tile(case['vxy'], (E, 1)) + 0.1 * rng.uniform(-1, 1, (E, 3))
+    perm = rng.permutation(E)
+    d = bl.BeamLRData()
+    sizes = dict(KC0=d.KC0_SPARSE_SIZE, KG=d.KG_SPARSE_SIZE, M=d.M_SPARSE_SIZE)
+    u = rng.uniform(-1, 1, 6 * N) * 1e-3
+    out = {}
+    for key in ['KC0', 'KG', 'M0', 'M1']:
+        base = 'M' if key.startswith('M') else key
+        m = sizes[base] * E
+        out[key + 'r'] = np.zeros(m, dtype=np.int64)
+        out[key + 'c'] = np.zeros(m, dtype=np.int64)
+        out[key + 'v'] = np.zeros(m, dtype=np.float64)
+    fint = np.zeros(6 * N)
+    geom = np.zeros((E, 10))
+    probe = bl.BeamLRProbe()
+    for e in range(E):
+        el = bl.BeamLR(probe)
+        el.n1, el.n2 = int(conn[e, 0]), int(conn[e, 1])
+        el.c1, el.c2 = int(6 * conn[e, 0]), int(6 * conn[e, 1])
+        el.init_k_KC0 = int(sizes['KC0'] * perm[e])
+        el.init_k_KG = int(sizes['KG'] * perm[e])
+        el.init_k_M = int(sizes['M'] * perm[e])
+        el.update_rotation_matrix(*vxy[e], x)
+        el.update_probe_xe(x)
+        el.update_length()
+        geom[e] = [el.r11, el.r12, el.r13, el.r21, el.r22, el.r23, el.r31, el.r32,
+                   el.r33, el.length]
+        p = props[e]
+        el.update_KC0(out['KC0r'], out['KC0c'], out['KC0v'], p)
+        el.update_M(out['M0r'], out['M0c'], out['M0v'], p, 0)
+        el.update_M(out['M1r'], out['M1c'], out['M1v'], p, 1)
+        el.update_probe_ue(u)
+        el.update_KG(out['KGr'], out['KGc'], out['KGv'], p)
+        el.update_fint(fint, p)
+    return dict(x=x, conn=conn, perm=perm, props=precs, vxy=vxy, u=u, geom=geom,
+                fint=fint, **out)
+
+
+BEAMLR_CASES = [
+    dict(name='curved_3d', n=5, L=1.0, rot=(0.3, 0.2, -0.4), vxy=(0., 1., 0.3),
+         Iyz=0.0, Ay=0.0, Az=0.0),
+    dict(name='coupled', n=4, L=0.5, rot=(1.2, -0.7, 0.1), vxy=(0.1, 0.2, 1.0),
+         Iyz=0.3, Ay=0.2, Az=-0.1),
+]
+
+
 BEAM_CASES = [
     dict(name='curved_3d', n=5, L=1.0, rot=(0.3, 0.2, -0.4), vxy=(0., 1., 0.3),
          Iyz=0.0, Ay=0.0, Az=0.0),
@@ -459,7 +545,8 @@ def write_rods(mods):
     touched when only these are regenerated)."""
     rng = np.random.default_rng(20261203)
     for name, cases, fn in (('truss', TRUSS_CASES, run_truss_case),
-                            ('spring', SPRING_CASES, run_spring_case)):
+                            ('spring', SPRING_CASES, run_spring_case),
+                            ('beamlr', BEAMLR_CASES, run_beamlr_case)):
         res = {}
         for case in cases:
             r = fn(mods, case, rng)

@@ -45,6 +45,9 @@ def test_host_only_queries():
     assert lib.pf3d_slot_count(4, 3, 1) == 36    # Truss lumped mass
     assert lib.pf3d_slot_count(5, 0, 0) == 72    # Spring KC0
     assert lib.pf3d_slot_count(5, 3, 0) < 0      # Spring has no mass
+    assert lib.pf3d_slot_count(6, 0, 0) == 144   # BeamLR KC0
+    assert lib.pf3d_slot_count(6, 1, 0) == 36    # BeamLR KG (rotations)
+    assert lib.pf3d_slot_count(6, 3, 1) == 36    # BeamLR lumped mass
     assert lib.pf3d_slot_count(9, 0, 0) < 0
     assert b'etype' in lib.pf3d_last_error()
 

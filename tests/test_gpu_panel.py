@@ -142,7 +142,7 @@ def test_panel_static_and_buckling_vs_scipy():
     assert np.linalg.norm((ud - uref)[keep]) <= 1e-10 * np.linalg.norm(uref[keep])
     spread = np.linalg.norm((ud - uref)[~keep]) / np.linalg.norm(uref[~keep])
     print('scipy dense vs sparse on rz: %.2e' % spread)
-    for method in ('fp64', 'mixed', 'direct'):
+    for method in ('fp64', 'mixed', 'ir', 'direct'):
         u, info = sysm.solve(f, method=method, rtol=1e-10)
         ug = u.cpu().numpy()
         err = np.linalg.norm((ug - uref)[keep]) / np.linalg.norm(uref[keep])

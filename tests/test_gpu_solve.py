@@ -68,7 +68,7 @@ def _reference_plate():
     return m, prop, fixed, f
 
 
-@pytest.mark.parametrize('method', ['fp64', 'mixed', 'direct'])
+@pytest.mark.parametrize('method', ['fp64', 'mixed', 'ir', 'direct'])
 def test_reference_point_load_known_answer(method):
     from pyfe3d_amd.sparse import BSRPlan
     from pyfe3d_amd.solver import StaticSystem
@@ -87,7 +87,7 @@ def test_reference_point_load_known_answer(method):
     assert u[fixed].max() == 0 and u[fixed].min() == 0
 
 
-@pytest.mark.parametrize('method', ['fp64', 'mixed', 'direct'])
+@pytest.mark.parametrize('method', ['fp64', 'mixed', 'ir', 'direct'])
 @pytest.mark.parametrize('etype', ['quad4', 'quad4r', 'tria3r'])
 def test_clamped_plate_point_load(etype, method):
     """20x20 plate, clamped, centre point load (the CPU-runnable case)."""

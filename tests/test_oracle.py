@@ -308,3 +308,53 @@ def test_rod_invariants():
     assert np.isclose(Ks[2, 2], 1e3) and np.isclose(Ks[0, 0], 2e3) and np.isclose(Ks[1, 1], 3e3)
     assert np.isclose(Ks[5, 5], 4e3) and np.isclose(Ks[3, 3], 5e3) and np.isclose(Ks[4, 4], 6e3)
     assert np.isclose(Ks[2, 8], -1e3)
+
+
+def test_beamlr_golden():
+    """BeamLR (beamlr.pyx): axes, length, KC0, KG (bending rotations only),
+    consistent and lumped mass, internal forces."""
+    for name, d in cases('beamlr'):
+        E = d['conn'].shape[0]
+        geom, _ = O.geometry('beamlr', d['x'], d['conn'], d['vxy'])
+        np.testing.assert_allclose(geom[:, :9], d['geom'][:, :9], rtol=0, atol=1e-14)
+        np.testing.assert_allclose(geom[:, 13], d['geom'][:, 9], rtol=1e-14)
+        for key, mat, mtype in (('KC0', 'KC0', 0), ('KG', 'KG', 0), ('M0', 'M', 0), ('M1', 'M', 1)):
+            size = d[key + 'v'].size // E
+            r = np.zeros(size * E, np.int64); c = np.zeros_like(r); v = np.zeros(size * E)
+            O.assemble('beamlr', mat, d['x'], d['conn'], size * d['perm'], d['vxy'], np.arange(E),
+                       d['props'], r, c, v, mtype=mtype, u=d['u'])
+            assert (r == d[key + 'r']).all() and (c == d[key + 'c']).all(), (name, key)
+            assert element_errors(v, d[key + 'v'], size, d['perm']) <= TOL, (name, key)
+        f = O.fint('beamlr', d['x'], d['conn'], d['vxy'], np.arange(E), d['props'], d['u'])
+        assert np.abs(f - d['fint']).max() <= FINT_TOL * np.abs(d['fint']).max(), name
+
+
+def test_beamlr_invariants():
+    """Rigid-body modes give no force (the reduced integration keeps the
+    shear terms consistent with rotations); the bending stiffness of a
+    cantilever tip load matches Timoshenko's w = P L^3 / 3EI + P L / (G A)
+    for one element with 1-point shear and linear fields (exactly: the
+    element's flexibility L^3/(4EI) + L/(GA)); total mass rho A L."""
+    L, A, Em, G = 0.9, 2e-4, 70e9, 26e9
+    Iyy, Izz = 3e-9, 5e-9
+    bp = np.zeros(16)
+    bp[:9] = [A, Em, G, Iyy, Izz, 0., Iyy + Izz, 0., 0.]
+    bp[9:15] = [2700 * A, 0., 0., 2700 * Izz, 2700 * Iyy, 0.]
+    X = np.array([[0.1, -0.2, 0.3], [0.1 + L, -0.2, 0.3]])
+    vxy = np.array([0., 1., 0.])
+    K = _dense_element('beamlr', 'KC0', X, vxy, bp)
+    # rigid translations and rotation about z through node 1
+    for t in range(3):
+        m = np.zeros(12); m[t::6] = 1.
+        assert np.abs(K @ m).max() <= 1e-6 * np.abs(K).max()
+    m = np.zeros(12); m[5] = m[11] = 1.; m[7] = L   # rz = 1, v = x
+    assert np.abs(K @ m).max() <= 1e-6 * np.abs(K).max()
+    # cantilever: node 1 clamped, tip force in y
+    Kf = K[6:, 6:]
+    f = np.zeros(6); f[1] = 1.
+    w = np.linalg.solve(Kf, f)[1]
+    assert np.isclose(w, L ** 3 / (4 * Em * Izz) + L / (G * A), rtol=1e-12)
+    for mtype in (0, 1):
+        M = _dense_element('beamlr', 'M', X, vxy, bp, mtype=mtype)
+        t = np.zeros(12); t[0::6] = 1
+        assert np.isclose(t @ M @ t, 2700 * A * L, rtol=1e-14)
