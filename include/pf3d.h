@@ -49,6 +49,9 @@ extern "C" {
 #define PF3D_KG 1
 #define PF3D_KG_STRESS 2
 #define PF3D_M 3
+#define PF3D_KA_BETA 4  /* piston theory (Quad4, Quad4R) */
+#define PF3D_KA_GAMMA 5
+#define PF3D_CA 6
 
 #define PF3D_VALUES_ONLY 1 /* update_KC0v_only != 0 (quad4.pyx:1278) */
 #define PF3D_OVERWRITE 2   /* store instead of accumulate */
@@ -59,7 +62,8 @@ const char *pf3d_last_error(void);
 /* Slots written per element: Quad4Data/Quad4RData/Tria3RData/BeamCData
  * KC0_SPARSE_SIZE, KG_SPARSE_SIZE, M_SPARSE_SIZE (quad4.pyx:174-180,
  * quad4r.pyx:140-146, tria3r.pyx:148-151, beamc.pyx:43-46,
- * truss.pyx:45-47, spring.pyx:40-43, beamlr.pyx:43-46); for lumped
+ * truss.pyx:45-47, spring.pyx:40-43, beamlr.pyx:43-46; KA_BETA/KA_GAMMA/CA
+ * quad4.pyx:157-180); for lumped
  * masses the used prefix of M_SPARSE_SIZE. */
 int pf3d_slot_count(int etype, int matrix, int mtype);
 
@@ -201,6 +205,18 @@ int pf3d_beamlr_kg(int64_t n_elem, const double *x, const int64_t *conn, const i
 int pf3d_beamlr_m(int64_t n_elem, const double *x, const int64_t *conn, const int64_t *init_k,
                   const double *vxy, const int32_t *pid, const double *props, int mtype,
                   int64_t *r, int64_t *c, double *v, int flags, void *stream);
+
+/* Piston-theory aerodynamic matrices of Quad4 / Quad4R (matrix =
+ * PF3D_KA_BETA, PF3D_KA_GAMMA or PF3D_CA; 144 slots over the translations
+ * (I, a, J, b), values P_IJ n_a n_b with the element normal n):
+ * KA_beta = -int N_I dN_J/dX dA (X: global x, the flow direction),
+ * KA_gamma = int N_I N_J dA, CA = -int N_I N_J dA, 2x2 Gauss.  Replaces
+ * update_KA_beta / update_KA_gamma / update_CA (quad4.pyx:9491-11917,
+ * quad4r.pyx:12789-15200); the caller's init_k is init_k_KA_beta,
+ * init_k_KA_gamma or init_k_CA. */
+int pf3d_shell_piston(int etype, int matrix, int64_t n_elem, const double *x,
+                      const int64_t *conn, const int64_t *init_k, int64_t *r, int64_t *c,
+                      double *v, int flags, void *stream);
 
 /* Shell property records (32 doubles each) of isotropic plates, one per
  * element, from a thickness field h[n] (device) and the record of the same

@@ -29,7 +29,9 @@ enum { ET_QUAD4 = 0, ET_QUAD4R = 1, ET_TRIA3R = 2, ET_BEAMC = 3, ET_TRUSS = 4, E
 #define IS_BEAM(t) ((t) == ET_BEAMC || (t) == ET_BEAMLR)
 #define IS_ROD(t) ((t) == ET_TRUSS || (t) == ET_SPRING)
 #define SPR_STRIDE 8 /* spring record: kxe kye kze krxe krye krze, 2 unused */
-enum { MAT_KC0 = 0, MAT_KG = 1, MAT_KGS = 2, MAT_M = 3, MAT_FINT = 4 };
+enum { MAT_KC0 = 0, MAT_KG = 1, MAT_KGS = 2, MAT_M = 3, MAT_FINT = 4, MAT_KAB = 5, MAT_KAG = 6,
+       MAT_CA = 7 };
+#define IS_PISTON(m) ((m) >= MAT_KAB && (m) <= MAT_CA)
 
 /* shell property record (doubles): A11 A12 A16 A22 A26 A66, B.., D.., E44 E45
  * E55, scf_k13, scf_k23, h, offset, intrho, intrhoz, intrhoz2 */
@@ -837,6 +839,29 @@ static void shell_kg_ww(int etype, const double *xe, double area, const shell_t 
         }
 }
 
+/* Piston-theory aerodynamic matrices of Quad4 / Quad4R on the normal
+ * displacement, 2x2 Gauss (quad4.pyx:9491-11917, quad4r.pyx:12789-15200):
+ *   KA_beta:  -int N_I dN_J/dX dA   (X = global x, the flow direction:
+ *              dN/dX = N,x r11 + N,y r21 in element axes)
+ *   KA_gamma:  int N_I N_J dA       CA: -int N_I N_J dA
+ * G[I][J] multiplies n_a n_b of the element normal in the scatter. */
+static void shell_piston_ww(int mat, const double *xe, const double *r, double G[4][4])
+{
+    memset(G, 0, sizeof(double) * 16);
+    double N[4], Nx[4], Ny[4];
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) {
+            double xi = a ? GP2 : -GP2, eta = b ? GP2 : -GP2;
+            double det = quad_shape(xe, xi, eta, N, Nx, Ny);
+            for (int I = 0; I < 4; I++)
+                for (int J = 0; J < 4; J++) {
+                    if (mat == MAT_KAB) G[I][J] -= N[I] * det * (Nx[J] * r[0] + Ny[J] * r[3]);
+                    else if (mat == MAT_KAG) G[I][J] += N[I] * N[J] * det;
+                    else G[I][J] -= N[I] * N[J] * det;
+                }
+        }
+}
+
 /* H[I][J] mass weights: quad4.pyx:3124, 4598-4640, 8873-8902;
  * tria3r.pyx:4107-4108 (valH1 = detJ/18), 3-point rule, vertex rule */
 static void shell_mass_h(int etype, const double *xe, double area, int mtype, double H[4][4])
@@ -932,7 +957,7 @@ static int layout(int etype, int mat, int mtype, int nn, int *rl, int *cl)
                     for (int e = 0; e < 6; e++) { rl[k] = 6 * I + d; cl[k] = 6 * J + e; k++; }
         return k;
     }
-    if (mat == MAT_KG || mat == MAT_KGS) {
+    if (mat == MAT_KG || mat == MAT_KGS || IS_PISTON(mat)) {
         for (int I = 0; I < nn; I++)
             for (int d = 0; d < 3; d++)
                 for (int J = 0; J < nn; J++)
@@ -1099,6 +1124,14 @@ static int element_local(int etype, int mat, int mtype, const double *g, const d
         return 0;
     }
     memset(K, 0, sizeof(double) * n * n);
+    if (IS_PISTON(mat)) {
+        if (etype == ET_TRIA3R) return -1;
+        double G[4][4];
+        shell_piston_ww(mat, xe, g, G);
+        for (int I = 0; I < nn; I++)
+            for (int J = 0; J < nn; J++) K[(6 * I + 2) * n + 6 * J + 2] = G[I][J];
+        return 0;
+    }
     if (mat == MAT_KG || mat == MAT_KGS) {
         double G[4][4];
         shell_kg_ww(etype, xe, area, &s, mat == MAT_KG ? ue : NULL, stress, G);

@@ -16,7 +16,7 @@ LIB = os.path.join(HERE, 'liboracle.so')
 
 ETYPES = {'quad4': 0, 'quad4r': 1, 'tria3r': 2, 'beamc': 3, 'truss': 4, 'spring': 5,
           'beamlr': 6}
-MATS = {'KC0': 0, 'KG': 1, 'KGs': 2, 'M': 3}
+MATS = {'KC0': 0, 'KG': 1, 'KGs': 2, 'M': 3, 'KAB': 5, 'KAG': 6, 'CA': 7}
 NNODES = {'quad4': 4, 'quad4r': 4, 'tria3r': 3, 'beamc': 2, 'truss': 2, 'spring': 2,
           'beamlr': 2}
 SHELL_STRIDE = 32

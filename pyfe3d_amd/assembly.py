@@ -22,7 +22,7 @@ from . import _lib
 
 ETYPE = {'quad4': 0, 'quad4r': 1, 'tria3r': 2, 'beamc': 3, 'truss': 4, 'spring': 5,
          'beamlr': 6}
-MATRIX = {'KC0': 0, 'KG': 1, 'KG_stress': 2, 'M': 3}
+MATRIX = {'KC0': 0, 'KG': 1, 'KG_stress': 2, 'M': 3, 'KA_beta': 4, 'KA_gamma': 5, 'CA': 6}
 NNODES = {'quad4': 4, 'quad4r': 4, 'tria3r': 3, 'beamc': 2, 'truss': 2, 'spring': 2,
           'beamlr': 2}
 VALUES_ONLY = 1
@@ -201,6 +201,22 @@ def shell_m(etype, x, conn, r, c, v, props, mtype=0, pid=None, init_k=None, xmat
     pp, ppid = _props(props, pid, E, SHELL_STRIDE, check)
     _lib.call('pf3d_shell_m', ETYPE[etype], E, px, pc, pk, pe, ppid, pp, int(mtype),
               pr, pcc, pv, _flags(values_only, overwrite), _stream(stream))
+
+
+def shell_piston(etype, matrix, x, conn, r, c, v, init_k=None, values_only=False,
+                 overwrite=False, stream=None, check=True):
+    """Piston-theory aerodynamic matrix of a batch of Quad4 / Quad4R
+    elements: matrix 'KA_beta', 'KA_gamma' or 'CA' (update_KA_beta,
+    update_KA_gamma, update_CA; quad4.pyx:9491-11917, quad4r.pyx:12789)."""
+    if etype not in ('quad4', 'quad4r'):
+        raise ValueError('piston theory is defined for quad4 and quad4r')
+    if matrix not in ('KA_beta', 'KA_gamma', 'CA'):
+        raise ValueError('matrix must be KA_beta, KA_gamma or CA')
+    E, px, pc, pk, _ = _common(etype, x, conn, init_k, None, check)
+    _check_span(144 * E, r, c, v, values_only, init_k, check)
+    pr, pcc, pv = _outputs(r, c, v, True, values_only)
+    _lib.call('pf3d_shell_piston', ETYPE[etype], MATRIX[matrix], E, px, pc, pk, pr, pcc, pv,
+              _flags(values_only, overwrite), _stream(stream))
 
 
 def _beam_entry(etype, name):

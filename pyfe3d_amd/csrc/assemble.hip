@@ -49,7 +49,7 @@ template <int NN>
 __host__ __device__ constexpr int slot_count(int mat, int mtype)
 {
     return mat == KC0 ? 36 * NN * NN
-         : (mat == KG || mat == KGS) ? 9 * NN * NN
+         : kg_like(mat) ? 9 * NN * NN
          : (mtype == 2 ? 18 * NN * NN : 30 * NN * NN);
 }
 
@@ -466,12 +466,33 @@ __device__ __forceinline__ void shell_stream_body(const AsmArgs &a, int64_t blk,
         if (ET == TRIA3R) tria_geometry(X, xm, g);
         else quad_geometry(X, xm, g);
         Shell s;
-        if (MAT != KGS)  // the given-stress path reads no property
+        if (MAT == KG || MAT == MASS)  // given stress and piston theory read no property
             load_shell(a.props + (a.pid ? (int64_t)a.pid[e] : 0) * SHELL_STRIDE, g.m, s);
         double P[NP];
 #pragma unroll
         for (int k = 0; k < NP; k++) P[k] = 0.;
-        if (MAT == KG || MAT == KGS) {
+        if (piston(MAT)) {
+            // piston theory on the normal displacement, 2x2 Gauss
+            // (quad4.pyx:9491-11917, quad4r.pyx:12789-15200): KA_beta
+            // -int N_I dN_J/dX, KA_gamma int N_I N_J, CA -int N_I N_J, with
+            // dN/dX = N,x r11 + N,y r21 (X = global x, the flow direction)
+#pragma unroll
+            for (int p = 0; p < 4; p++) {
+                QShape q;
+                quad_shape(g.xe, (p >> 1) ? GP2 : -GP2, (p & 1) ? GP2 : -GP2, q);
+#pragma unroll
+                for (int I = 0; I < 4; I++)
+#pragma unroll
+                    for (int J = 0; J < 4; J++) {
+                        if (MAT == KAB)
+                            P[I * 4 + J] -= q.N[I] * q.det * (q.Nx[J] * g.r[0] + q.Ny[J] * g.r[3]);
+                        else if (MAT == KAG) P[I * 4 + J] += q.N[I] * q.N[J] * q.det;
+                        else P[I * 4 + J] -= q.N[I] * q.N[J] * q.det;
+                    }
+            }
+        }
+        if (kg_like(MAT)) {
+            if (MAT == KG || MAT == KGS) {
             // quad4.pyx:1365-2257 / 2259-3081, tria3r.pyx:3020-4062
             double ue[6 * NN];
             double Nf[3] = {0., 0., 0.};
@@ -517,6 +538,7 @@ __device__ __forceinline__ void shell_stream_body(const AsmArgs &a, int64_t blk,
                             P[I * 4 + J] += q.det * (q.Nx[J] * fx + q.Ny[J] * fy);
                     }
                 }
+            }
             }
 #pragma unroll
             for (int k = 0; k < NP; k++) sd[lane][k] = P[k];
@@ -590,7 +612,7 @@ __device__ __forceinline__ void shell_stream_body(const AsmArgs &a, int64_t blk,
     for (int t = lane; t < total; t += 64) {
         const int el = t / nslot, k = t - el * nslot;
         int I, J, dr, dc;  // node pair and DOFs within the 6x6 block
-        if (MAT == KG || MAT == KGS) {  // (I, a, J, b) over translations
+        if (kg_like(MAT)) {  // (I, a, J, b) over translations
             const int row = k / (3 * NN), col = k - row * 3 * NN;
             I = row / 3; dr = row - 3 * I;
             J = col / 3; dc = col - 3 * J;
@@ -609,7 +631,7 @@ __device__ __forceinline__ void shell_stream_body(const AsmArgs &a, int64_t blk,
         }
         const double *d = sd[el];
         double val;
-        if (MAT == KG || MAT == KGS) val = (d[NP + dr] * d[I * NN + J]) * d[NP + dc];
+        if (kg_like(MAT)) val = (d[NP + dr] * d[I * NN + J]) * d[NP + dc];
         else val = d[I * NN + J] * d[NP + 6 * dr + dc];
         const int64_t slot = sk0[el] + k;
         if (overwrite) __builtin_nontemporal_store(val, a.v + slot);
@@ -1267,7 +1289,9 @@ __device__ __forceinline__ void beam_body(const AsmArgs &a, int64_t blk, char *l
         double N = 0.;
         if (MAT == KG) {
             // axial force from the local axial displacements (beamc.pyx:1889,
-            // beamlr.pyx:1388)
+            // beamlr.pyx:1388), without FMA contraction: N is a difference of
+            // nearly equal projections, so the reference's rounding is kept
+#pragma clang fp contract(off)
             double u1 = 0., u2 = 0.;
             for (int i = 0; i < 3; i++) {
                 u1 += g.r[3 * i] * a.u[cn[0] + i];
@@ -1631,6 +1655,14 @@ static hipError_t launch_shell(const AsmArgs &a, hipStream_t s)
 template <int ET>
 static hipError_t launch_shell_mat(int mat, const AsmArgs &a, hipStream_t s)
 {
+    if constexpr (ET != TRIA3R) {
+        switch (mat) {
+        case KAB: return launch_shell<ET, KAB>(a, s);
+        case KAG: return launch_shell<ET, KAG>(a, s);
+        case CAM: return launch_shell<ET, CAM>(a, s);
+        default: break;
+        }
+    }
     switch (mat) {
     case KC0: return launch_shell<ET, KC0>(a, s);
     case KG: return launch_shell<ET, KG>(a, s);

@@ -75,6 +75,10 @@ int pf3d_slot_count(int etype, int matrix, int mtype)
     }
     if (etype < 0 || etype > PF3D_TRIA3R) return pf3d_fail(PF3D_E_ARG, "bad etype %d", etype);
     const int nn = etype == PF3D_TRIA3R ? 3 : 4;
+    if (matrix >= PF3D_KA_BETA && matrix <= PF3D_CA) {
+        if (etype == PF3D_TRIA3R) return pf3d_fail(PF3D_E_ARG, "Tria3R has no piston theory");
+        return 144;
+    }
     switch (matrix) {
     case PF3D_KC0: return 36 * nn * nn;
     case PF3D_KG:
@@ -232,6 +236,23 @@ int pf3d_beam_m(int64_t n_elem, const double *x, const int64_t *conn, const int6
     a.n_elem = n_elem; a.x = x; a.conn = conn; a.init_k = init_k; a.evec = vxy; a.pid = pid;
     a.props = props; a.mtype = mtype; a.flags = flags; a.r = r; a.c = c; a.v = v;
     return pf3d_hip_check(pf3d_launch_beam(PF3D_BEAMC, PF3D_M, a, (hipStream_t)stream), "pf3d_beam_m");
+}
+
+int pf3d_shell_piston(int etype, int matrix, int64_t n_elem, const double *x,
+                      const int64_t *conn, const int64_t *init_k, int64_t *r, int64_t *c,
+                      double *v, int flags, void *stream)
+{
+    if (etype != PF3D_QUAD4 && etype != PF3D_QUAD4R)
+        return pf3d_fail(PF3D_E_ARG, "piston theory: etype %d is not Quad4/Quad4R", etype);
+    if (matrix < PF3D_KA_BETA || matrix > PF3D_CA)
+        return pf3d_fail(PF3D_E_ARG, "piston theory: bad matrix %d", matrix);
+    int st = check_common(etype, n_elem, x, conn, r, c, v, flags);
+    if (st) return st > 0 ? PF3D_OK : st;
+    pf3d::AsmArgs a{};
+    a.n_elem = n_elem; a.x = x; a.conn = conn; a.init_k = init_k; a.flags = flags;
+    a.r = r; a.c = c; a.v = v;
+    return pf3d_hip_check(pf3d_launch_shell(etype, matrix, a, (hipStream_t)stream),
+                          "pf3d_shell_piston");
 }
 
 int pf3d_beamlr_kc0(int64_t n_elem, const double *x, const int64_t *conn, const int64_t *init_k,

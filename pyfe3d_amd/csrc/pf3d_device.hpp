@@ -15,7 +15,15 @@ constexpr int SPRING_STRIDE = 6;  // kxe kye kze krxe krye krze per spring recor
 constexpr double GP2 = 0.5773502691896257645092;
 
 enum ElemType { QUAD4 = 0, QUAD4R = 1, TRIA3R = 2, BEAMC = 3, TRUSS = 4, SPRING = 5, BEAMLR = 6 };
-enum Matrix { KC0 = 0, KG = 1, KGS = 2, MASS = 3 };
+enum Matrix { KC0 = 0, KG = 1, KGS = 2, MASS = 3, KAB = 4, KAG = 5, CAM = 6 };
+
+// matrices on the translations (I, a, J, b) with value P_IJ n_a n_b: KG,
+// KG given stress and the piston-theory KA_beta, KA_gamma, CA
+__host__ __device__ constexpr bool kg_like(int mat)
+{
+    return mat == KG || mat == KGS || mat == KAB || mat == KAG || mat == CAM;
+}
+__host__ __device__ constexpr bool piston(int mat) { return mat == KAB || mat == KAG || mat == CAM; }
 
 // flags of the assembly entry points
 constexpr int F_VALUES_ONLY = 1;  // do not write row/col indices

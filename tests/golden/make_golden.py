@@ -195,8 +195,14 @@ def run_shell_case(mods, spu, etype, case, rng):
     Nxx, Nyy, Nxy = -1.3e3, 0.4e3, 0.25e3
 
     out = {}
-    for key in ['KC0', 'KG', 'KGs', 'M0', 'M1', 'M2']:
-        base = 'M' if key.startswith('M') else ('KG' if key.startswith('KG') else 'KC0')
+    keys = ['KC0', 'KG', 'KGs', 'M0', 'M1', 'M2']
+    piston = etype != 'tria3r'   # piston theory: Quad4 / Quad4R only
+    if piston:
+        sizes.update(KAB=data.KA_BETA_SPARSE_SIZE, KAG=data.KA_GAMMA_SPARSE_SIZE,
+                     CA=data.CA_SPARSE_SIZE)
+        keys += ['KAB', 'KAG', 'CA']
+    for key in keys:
+        base = 'M' if key.startswith('M') else ('KG' if key.startswith('KG') else key)
         n = sizes[base] * E
         out[key + 'r'] = np.zeros(n, dtype=np.int64)
         out[key + 'c'] = np.zeros(n, dtype=np.int64)
@@ -227,6 +233,13 @@ def run_shell_case(mods, spu, etype, case, rng):
             el.update_KC0(out['KC0r'], out['KC0c'], out['KC0v'], p, 0, *hg[e])
         else:
             el.update_KC0(out['KC0r'], out['KC0c'], out['KC0v'], p)
+        if piston:
+            el.init_k_KA_beta = int(sizes['KAB'] * perm[e])
+            el.init_k_KA_gamma = int(sizes['KAG'] * perm[e])
+            el.init_k_CA = int(sizes['CA'] * perm[e])
+            el.update_KA_beta(out['KABr'], out['KABc'], out['KABv'])
+            el.update_KA_gamma(out['KAGr'], out['KAGc'], out['KAGv'])
+            el.update_CA(out['CAr'], out['CAc'], out['CAv'])
         el.update_M(out['M0r'], out['M0c'], out['M0v'], p, 0)
         el.update_M(out['M1r'], out['M1c'], out['M1v'], p, 1)
         el.update_M(out['M2r'], out['M2c'], out['M2v'], p, 2)

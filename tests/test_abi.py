@@ -48,6 +48,9 @@ def test_host_only_queries():
     assert lib.pf3d_slot_count(6, 0, 0) == 144   # BeamLR KC0
     assert lib.pf3d_slot_count(6, 1, 0) == 36    # BeamLR KG (rotations)
     assert lib.pf3d_slot_count(6, 3, 1) == 36    # BeamLR lumped mass
+    assert lib.pf3d_slot_count(0, 4, 0) == 144   # Quad4 KA_beta
+    assert lib.pf3d_slot_count(1, 6, 0) == 144   # Quad4R CA
+    assert lib.pf3d_slot_count(2, 5, 0) < 0      # no piston theory for Tria3R
     assert lib.pf3d_slot_count(9, 0, 0) < 0
     assert b'etype' in lib.pf3d_last_error()
 

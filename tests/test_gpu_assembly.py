@@ -179,3 +179,44 @@ def test_empty_batch_is_a_no_op():
     v = torch.zeros(0, dtype=torch.float64, device=DEV)
     A.shell_kc0('quad4', x, conn, v.long(), v.long(), v, torch.zeros((1, 32), dtype=torch.float64,
                                                                           device=DEV))
+
+
+@pytest.mark.parametrize('etype', ['quad4', 'quad4r'])
+def test_piston_theory_golden_gpu(etype):
+    """KA_beta, KA_gamma, CA against the reference's vectors (indices exact,
+    values within 1e-12), and a random 3-D mesh against the oracle."""
+    from pyfe3d_amd import assembly as A
+    bad = []
+    for name, d in cases(etype):
+        E = d['conn'].shape[0]
+        for key, mat in (('KAB', 'KA_beta'), ('KAG', 'KA_gamma'), ('CA', 'CA')):
+            r = torch.zeros(144 * E, dtype=torch.int64, device=DEV)
+            c = torch.zeros_like(r)
+            v = torch.zeros(144 * E, dtype=torch.float64, device=DEV)
+            A.shell_piston(etype, mat, T(d['x'], torch.float64), T(6 * d['conn'], torch.int64),
+                           r, c, v, init_k=T(144 * d['perm'], torch.int64))
+            torch.cuda.synchronize()
+            if not ((r.cpu().numpy() == d[key + 'r']).all() and
+                    (c.cpu().numpy() == d[key + 'c']).all()):
+                bad.append((name, key, 'indices'))
+            err = element_errors(v.cpu().numpy(), d[key + 'v'], 144, d['perm'])
+            print(etype, name, key, 'max rel err %.2e' % err)
+            if not err <= TOL_CASE.get((etype, name), TOL):
+                bad.append((name, key, err))
+    x, quads, rng = _random_mesh(23, 19, 8)
+    E = quads.shape[0]
+    for key, mat in (('KAB', 'KA_beta'), ('KAG', 'KA_gamma'), ('CA', 'CA')):
+        r = torch.zeros(144 * E, dtype=torch.int64, device=DEV)
+        c = torch.zeros_like(r)
+        v = torch.zeros(144 * E, dtype=torch.float64, device=DEV)
+        A.shell_piston(etype, mat, T(x, torch.float64), T(6 * quads, torch.int64), r, c, v)
+        torch.cuda.synchronize()
+        rr = np.zeros(144 * E, np.int64); cc = np.zeros_like(rr); vv = np.zeros(144 * E)
+        O.assemble(etype, key, x, quads, 144 * np.arange(E), np.zeros((E, 3)),
+                   np.zeros(E, np.int64), np.zeros((1, 32)), rr, cc, vv)
+        assert (r.cpu().numpy() == rr).all() and (c.cpu().numpy() == cc).all(), key
+        err = element_errors(v.cpu().numpy(), vv, 144, np.arange(E))
+        print(etype, 'random', key, 'max rel err %.2e' % err)
+        if not err <= TOL:
+            bad.append(('random', key, err))
+    assert not bad, bad

@@ -358,3 +358,40 @@ def test_beamlr_invariants():
         M = _dense_element('beamlr', 'M', X, vxy, bp, mtype=mtype)
         t = np.zeros(12); t[0::6] = 1
         assert np.isclose(t @ M @ t, 2700 * A * L, rtol=1e-14)
+
+
+@pytest.mark.parametrize('etype', ['quad4', 'quad4r'])
+def test_piston_theory_golden(etype):
+    """KA_beta, KA_gamma, CA (quad4.pyx:9491-11917, quad4r.pyx:12789-15200):
+    translations (I, a, J, b), 144 slots, against the reference's vectors."""
+    for name, d in cases(etype):
+        E = d['conn'].shape[0]
+        for key in ('KAB', 'KAG', 'CA'):
+            r, c, v, size = _run(etype, d, key, key, 0)
+            assert size == 144
+            assert (r == d[key + 'r']).all() and (c == d[key + 'c']).all(), (name, key)
+            err = element_errors(v, d[key + 'v'], size, d['perm'])
+            assert err <= TOL_CASE.get((etype, name), TOL), (name, key, err)
+
+
+def test_piston_theory_invariants():
+    """On a flat rectangle in the XY plane: KA_gamma is the consistent
+    'mass' pattern area*(4,2,1)/36 on w, CA = -KA_gamma, and KA_beta's rows
+    sum to zero (a uniform w has no slope) while its columns applied to
+    w = X give -int N_I dA = -area/4 per node."""
+    a, b = 0.3, 0.2
+    X = np.array([[0., 0., 0.], [a, 0., 0.], [a, b, 0.], [0., b, 0.]])
+    p = _iso_prop()
+    G = _dense_element('quad4', 'KAG', X, np.zeros(3), p)[2::6, 2::6]
+    area = a * b
+    pat = np.array([[4, 2, 1, 2], [2, 4, 2, 1], [1, 2, 4, 2], [2, 1, 2, 4]]) * area / 36
+    np.testing.assert_allclose(G, pat, rtol=1e-13)
+    C = _dense_element('quad4', 'CA', X, np.zeros(3), p)[2::6, 2::6]
+    np.testing.assert_allclose(C, -pat, rtol=1e-13)
+    B = _dense_element('quad4', 'KAB', X, np.zeros(3), p)[2::6, 2::6]
+    assert np.abs(B.sum(axis=1)).max() <= 1e-15
+    np.testing.assert_allclose(B @ X[:, 0], -np.full(4, area / 4), rtol=1e-13)
+    # only the w DOFs of a flat element in the XY plane are touched
+    full = _dense_element('quad4', 'KAB', X, np.zeros(3), p)
+    mask = np.zeros((24, 24), bool); mask[2::6, 2::6] = True
+    assert np.abs(full[~mask]).max() == 0
