@@ -453,6 +453,13 @@ int pf3d_band_solve(int64_t n_var, const int32_t *order, const int64_t *dof0, co
 /* FP64 dense helpers of the eigen-solvers (Krylov bases, column-major  */
 /* n x m with leading dimension ld; fixed-order reductions)             */
 /* ------------------------------------------------------------------ */
+/* out[0] = a . b (FP64, fixed reduction order: reruns give the same bits);
+ * work: 1024 doubles of device scratch.  The inner products of the
+ * multigrid-preconditioned CG (no reference counterpart: the reference
+ * calls scipy's solvers, tests/test_quad4_static_point_load.py:84-102). */
+int pf3d_dot(int64_t n, const double *a, const double *b, double *out, double *work,
+             void *stream);
+
 /* c = alpha V^T x + beta c */
 int pf3d_gemv_t(int64_t n, int m, const double *V, int64_t ld, const double *x, double *c,
                 double alpha, double beta, void *stream);

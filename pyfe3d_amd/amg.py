@@ -1,0 +1,331 @@
+"""Smoothed-aggregation algebraic multigrid (SA-AMG) preconditioned CG for
+the Dirichlet-treated 6-DOF-per-node stiffness (a stronger preconditioner
+than block Jacobi for the static solve that pyfe3d's callers run with
+``spsolve``/``cg``, tests/test_quad4_static_point_load.py:84-102).
+
+Setup (host, numpy/scipy, once per matrix):
+  * geometric aggregation: nodes (then aggregates, by centroid) grouped in
+    boxes of ``box`` times the typical edge length;
+  * near-null space: the six rigid-body modes of each node (translations,
+    rotations about the centroid, with the rotational DOFs themselves),
+    zero on fixed DOFs;
+  * tentative prolongator per aggregate by a rank-revealing orthonormal
+    factorisation (SVD) of the aggregate's modes, coarse modes = its R;
+  * smoothed prolongator P = (I - w D^-1 A) P0 with w = 4/3 / lambda_max
+    (power iteration), Galerkin coarse operator P^T A P, all in 6x6 BSR.
+Solve (GPU, this library's kernels only): PCG whose preconditioner is one
+symmetric V-cycle with Chebyshev(D^-1 A) pre/post smoothing, restriction
+P^T and prolongation P as 6x6-block sparse products (pf3d_bsr_spmv), and
+a dense inverse on the coarsest level (pf3d_gemv_n); dot products with
+pf3d_gemv_t, vector updates with pf3d_axpby.
+"""
+import time
+
+import numpy as np
+import scipy.sparse as ss
+import torch
+
+from . import _lib
+
+
+def _s():
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _bsr_host(row_ptr, col_idx, val, n_rows, n_cols):
+    return ss.bsr_matrix((val.reshape(-1, 6, 6), col_idx, row_ptr),
+                         shape=(6 * n_rows, 6 * n_cols))
+
+
+def rigid_body_modes(xyz, fixed):
+    """(6 n, 6) near-null space: translations and small rotations about the
+    centroid, rows of fixed DOFs zero."""
+    n = xyz.shape[0]
+    c = xyz - xyz.mean(axis=0)
+    B = np.zeros((n, 6, 6))
+    for d in range(3):
+        B[:, d, d] = 1.
+    x, y, z = c[:, 0], c[:, 1], c[:, 2]
+    # rotation about x: (v, w) = (-z, y); about y: (u, w) = (z, -x); about z: (u, v) = (-y, x)
+    B[:, 1, 3] = -z; B[:, 2, 3] = y; B[:, 3, 3] = 1.
+    B[:, 0, 4] = z; B[:, 2, 4] = -x; B[:, 4, 4] = 1.
+    B[:, 0, 5] = -y; B[:, 1, 5] = x; B[:, 5, 5] = 1.
+    B = B.reshape(6 * n, 6)
+    B[np.asarray(fixed, bool)] = 0.
+    return B
+
+
+def box_aggregates(xyz, size):
+    """Aggregate id per point: boxes of edge `size` in space (compressed)."""
+    lo = xyz.min(axis=0)
+    k = np.floor((xyz - lo) / size).astype(np.int64)
+    dims = k.max(axis=0) + 1
+    key = (k[:, 0] * dims[1] + k[:, 1]) * dims[2] + k[:, 2]
+    uniq, agg = np.unique(key, return_inverse=True)
+    return agg.astype(np.int64), uniq.size
+
+
+def tentative(agg, n_agg, B, tol=1e-10):
+    """P0 (BSR, fine nodes x aggregates) with orthonormal columns per
+    aggregate and the coarse near-null space Bc (6 n_agg x 6)."""
+    n = agg.size
+    order = np.argsort(agg, kind='stable')
+    counts = np.bincount(agg, minlength=n_agg)
+    start = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    pos = np.empty(n, np.int64)
+    pos[order] = np.arange(n) - start[agg[order]]
+    m = counts.max()
+    Bn = B.reshape(n, 6, 6)
+    Bp = np.zeros((n_agg, m, 6, 6))
+    Bp[agg, pos] = Bn
+    U, S, Vt = np.linalg.svd(Bp.reshape(n_agg, 6 * m, 6), full_matrices=False)
+    keep = S > tol * np.maximum(S[:, :1], 1e-300)
+    U = U * keep[:, None, :]
+    R = (S * keep)[:, :, None] * Vt                 # (n_agg, 6, 6): U R = Bp
+    Q = U.reshape(n_agg, m, 6, 6)
+    blocks = Q[agg, pos]                            # (n, 6, 6) block of node i
+    P0 = ss.bsr_matrix((blocks, agg, np.arange(n + 1)), shape=(6 * n, 6 * n_agg))
+    dead = ~keep.reshape(-1)                        # coarse DOFs without a mode
+    return P0, R.reshape(6 * n_agg, 6), dead
+
+
+def _block_diag_inv(A):
+    """Inverses of the 6x6 diagonal blocks of a square 6x6-BSR matrix."""
+    n = A.shape[0] // 6
+    A = A.tobsr(blocksize=(6, 6))
+    A.sort_indices()
+    D = np.zeros((n, 6, 6))
+    rows = np.repeat(np.arange(n), np.diff(A.indptr))
+    diag = rows == A.indices
+    D[rows[diag]] = A.data[diag]
+    bad = np.abs(np.linalg.det(D)) == 0
+    D[bad] = np.eye(6)
+    return np.linalg.inv(D)
+
+
+def _lambda_max(A, Dinv, iters=20, seed=0):
+    """Power iteration on D^-1 A (host)."""
+    n = A.shape[0]
+    x = np.random.default_rng(seed).uniform(-1, 1, n)
+    lam = 1.
+    for _ in range(iters):
+        y = A @ x
+        y = np.einsum('nij,nj->ni', Dinv, y.reshape(-1, 6)).ravel()
+        lam = np.linalg.norm(y) / np.linalg.norm(x)
+        x = y / np.linalg.norm(y)
+    return lam
+
+
+class _Dev:
+    """A 6x6-BSR operator on the GPU (rows x cols in nodes)."""
+
+    def __init__(self, M, dev):
+        M = M.tobsr(blocksize=(6, 6))
+        M.sort_indices()
+        self.nb = M.shape[0] // 6
+        self.row_ptr = torch.as_tensor(M.indptr.astype(np.int32), device=dev)
+        self.col_idx = torch.as_tensor(M.indices.astype(np.int32), device=dev)
+        self.val = torch.as_tensor(np.ascontiguousarray(M.data.reshape(-1)), device=dev)
+
+    def __call__(self, x, y):
+        _lib.call('pf3d_bsr_spmv', self.nb, self.row_ptr.data_ptr(), self.col_idx.data_ptr(),
+                  self.val.data_ptr(), x.data_ptr(), y.data_ptr(), _s())
+        return y
+
+
+def _diag_dev(Dinv, dev):
+    n = Dinv.shape[0]
+    return _Dev(ss.bsr_matrix((Dinv, np.arange(n), np.arange(n + 1)), shape=(6 * n, 6 * n)), dev)
+
+
+def _axpby(a, x, b, y):
+    _lib.call('pf3d_axpby', y.numel(), float(a), x.data_ptr(), float(b), y.data_ptr(), _s())
+
+
+def setup_levels(A, fixed, xyz, box=3.0, coarse_nodes=300, max_levels=8, log=None):
+    """Host setup: list of levels dict(A, Dinv, P, lam) (scipy CSR, block
+    inverses) and the coarsest matrix (dense)."""
+    t0 = time.perf_counter()
+    A = A.tocsr()
+    n = A.shape[0] // 6
+    xyz = np.asarray(xyz, dtype=np.float64).reshape(-1, 3)
+    B = rigid_body_modes(xyz, fixed)
+    Ab = A.tobsr(blocksize=(6, 6))
+    rows = np.repeat(np.arange(n), np.diff(Ab.indptr))
+    off = rows != Ab.indices
+    h = np.median(np.linalg.norm(xyz[rows[off]] - xyz[Ab.indices[off]], axis=1))
+    size = box * h
+    levels = []
+    while A.shape[0] // 6 > coarse_nodes and len(levels) < max_levels - 1:
+        Dinv = _block_diag_inv(A)
+        lam = _lambda_max(A, Dinv)
+        agg, n_agg = box_aggregates(xyz, size)
+        if n_agg >= A.shape[0] // 6:
+            break
+        P0, Bc, dead = tentative(agg, n_agg, B)
+        nb = Dinv.shape[0]
+        Dm = ss.bsr_matrix((Dinv, np.arange(nb), np.arange(nb + 1)), shape=A.shape).tocsr()
+        P0 = P0.tocsr()
+        P = (P0 - (4. / 3. / lam) * (Dm @ (A @ P0))).tocsr()
+        P.eliminate_zeros()
+        Ac = (P.T @ (A @ P)).tocsr()
+        if dead.any():
+            d = np.zeros(Ac.shape[0]); d[dead] = 1.
+            Ac = Ac + ss.diags(d)
+        Ac = (0.5 * (Ac + Ac.T)).tocsr()
+        levels.append(dict(A=A, Dinv=Dinv, P=P, lam=lam))
+        cnt = np.bincount(agg, minlength=n_agg)[:, None]
+        cen = np.zeros((n_agg, 3))
+        np.add.at(cen, agg, xyz)
+        xyz = cen / cnt
+        A, B = Ac, Bc
+        size *= box
+        if log:
+            log('amg level %d: %d -> %d DOF (%.1f s)' % (len(levels) - 1, P.shape[0], P.shape[1],
+                                                           time.perf_counter() - t0))
+    return levels, A.toarray()
+
+
+class SAAMG:
+    """plan: BSR structure (row_ptr, col_idx, n_nodes) of A; val: A's BSR
+    values after the Dirichlet treatment (identity rows/cols on fixed
+    DOFs); fixed: bool per DOF; xyz: (n_nodes, 3) node coordinates."""
+
+    def __init__(self, plan, val, fixed, xyz, box=3.0, coarse_nodes=300, max_levels=8,
+                 cheb_degree=2, cheb_lower=1. / 30, log=None):
+        t0 = time.perf_counter()
+        dev = val.device
+        n = plan.n_nodes
+        A = _bsr_host(plan.row_ptr.cpu().numpy(), plan.col_idx.cpu().numpy(),
+                      val.cpu().numpy(), n, n)
+        host, Ad = setup_levels(A, fixed, xyz, box, coarse_nodes, max_levels, log)
+        self.cheb_degree, self.cheb_lower = cheb_degree, cheb_lower
+        self.levels = []
+        for i, H in enumerate(host):
+            self.levels.append(dict(A=_Dev(H['A'], dev), Dinv=_diag_dev(H['Dinv'], dev),
+                                    P=_Dev(H['P'], dev), R=_Dev(H['P'].T.tocsr(), dev),
+                                    lam=H['lam'], n=H['A'].shape[0]))
+        self.nc = Ad.shape[0]
+        self.coarse_inv = torch.as_tensor(np.ascontiguousarray(np.linalg.inv(Ad).T), device=dev)
+        zeros = lambda m: torch.zeros(m, dtype=torch.float64, device=dev)
+        for i, L in enumerate(self.levels):
+            m = L['n']
+            L['r'], L['d'], L['t'] = zeros(m), zeros(m), zeros(m)
+            # right-hand side and solution of the next level
+            mc = self.levels[i + 1]['n'] if i + 1 < len(self.levels) else self.nc
+            L['bc'], L['xc'] = zeros(mc), zeros(mc)
+        self.setup_s = time.perf_counter() - t0
+
+    def _cheb(self, L, b, x):
+        """x += Chebyshev(D^-1 A) correction for A x = b (degree k, bounds
+        [lower, 1.1] lambda_max)."""
+        lmax = 1.1 * L['lam']
+        lmin = self.cheb_lower * L['lam']
+        theta, delta = 0.5 * (lmax + lmin), 0.5 * (lmax - lmin)
+        sigma = theta / delta
+        rho_old = 1. / sigma
+        r, d, t = L['r'], L['d'], L['t']
+        L['A'](x, t)
+        r.copy_(b)
+        _axpby(-1., t, 1., r)                      # r = b - A x
+        L['Dinv'](r, d)
+        _axpby(0., d, 1. / theta, d)               # d = D^-1 r / theta
+        for k in range(self.cheb_degree):
+            _axpby(1., d, 1., x)                   # x += d
+            if k == self.cheb_degree - 1:
+                break
+            L['A'](d, t)
+            _axpby(-1., t, 1., r)                  # r -= A d
+            rho = 1. / (2. * sigma - rho_old)
+            L['Dinv'](r, t)
+            _axpby(2. * rho / delta, t, rho * rho_old, d)
+            rho_old = rho
+
+    def vcycle(self, b, x, lvl=0):
+        """x = M^-1 b (x is overwritten)."""
+        if lvl == len(self.levels):
+            _lib.call('pf3d_gemv_n', self.nc, self.nc, self.coarse_inv.data_ptr(), self.nc,
+                      b.data_ptr(), x.data_ptr(), 1.0, 0.0, _s())
+            return x
+        L = self.levels[lvl]
+        x.zero_()
+        self._cheb(L, b, x)
+        r, t = L['r'], L['t']
+        L['A'](x, t)
+        r.copy_(b)
+        _axpby(-1., t, 1., r)                      # r = b - A x
+        bc, xc = L['bc'], L['xc']
+        L['R'](r, bc)
+        self.vcycle(bc, xc, lvl + 1)
+        L['P'](xc, t)
+        _axpby(1., t, 1., x)                       # x += P xc
+        self._cheb(L, b, x)
+        return x
+
+
+def _dot(a, b, out):
+    """a . b through pf3d_dot; out holds the result (element 0) and the
+    1024-double workspace after it."""
+    _lib.call('pf3d_dot', a.numel(), a.data_ptr(), b.data_ptr(), out.data_ptr(),
+              out[1:].data_ptr(), _s())
+    return float(out[0].item())
+
+
+def amg_pcg(system, amg, f, rtol=1e-8, maxiter=2000, x0=None, log=None):
+    """PCG with one V-cycle per iteration on a StaticSystem's matrix.
+    Returns (u, info) like StaticSystem.solve (true FP64 residual)."""
+    b = system.rhs(f)
+    x = torch.zeros_like(b) if x0 is None else x0.clone()
+    r = torch.empty_like(b)
+    q = torch.empty_like(b)
+    z = torch.empty_like(b)
+    s = torch.empty(1025, dtype=torch.float64, device=b.device)
+    bb = _dot(b, b, s) or 1.
+    it = 0
+    restarts = 0
+    best = float('inf')
+    while True:
+        # (re)start from the true residual: the recursive one drifts on
+        # these ill-conditioned systems.  Stop when a restart no longer
+        # halves the true residual: it has reached the FP64 rounding floor
+        # of b - A x (about eps |A| |x| / |b|)
+        system.matvec(x, q)
+        r.copy_(b)
+        _axpby(-1., q, 1., r)
+        rel = (_dot(r, r, s) / bb) ** 0.5
+        if rel <= rtol or it >= maxiter or restarts > 8 or rel > 0.5 * best:
+            break
+        best = rel
+        restarts += 1
+        amg.vcycle(r, z)
+        p = z.clone()
+        rz = _dot(r, z, s)
+        while it < maxiter:
+            system.matvec(p, q)
+            alpha = rz / _dot(p, q, s)
+            _axpby(alpha, p, 1., x)
+            _axpby(-alpha, q, 1., r)
+            it += 1
+            if (_dot(r, r, s) / bb) ** 0.5 <= rtol:
+                break
+            amg.vcycle(r, z)
+            rz_new = _dot(r, z, s)
+            _axpby(1., z, rz_new / rz, p)
+            rz = rz_new
+    return x, dict(iters=it, outer=restarts, relres=rel)
+
+
+def residual_floor(system, x):
+    """eps * || |A| |x| || / ||b||-style estimate of the smallest relative
+    residual b - A x that FP64 arithmetic can show for the solution x (the
+    rounding of the row sums of A x), with |b| taken as ||P f|| = ||A x||."""
+    dev = x.device
+    absval = system.val.abs()
+    ax = x.abs()
+    y = torch.empty_like(x)
+    _lib.call('pf3d_bsr_spmv', system.nb, system.plan.row_ptr.data_ptr(),
+              system.plan.col_idx.data_ptr(), absval.data_ptr(), ax.data_ptr(), y.data_ptr(), _s())
+    s = torch.empty(1025, dtype=torch.float64, device=dev)
+    big = _dot(y, y, s) ** 0.5
+    system.matvec(x, y)
+    return 2. ** -53 * big / (_dot(y, y, s) ** 0.5 or 1.)

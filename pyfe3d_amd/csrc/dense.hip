@@ -51,9 +51,61 @@ __global__ void axpby(int64_t n, double a, const double *__restrict__ x, double 
     if (i < n) y[i] = a * x[i] + (b != 0. ? b * y[i] : 0.);
 }
 
+// dot product in two fixed-order stages: DOT_BLOCKS partial sums over a
+// grid stride, then one workgroup adds the partials in index order
+constexpr int DOT_BLOCKS = 1024;
+
+__global__ void __launch_bounds__(256) dot_partial(int64_t n, const double *__restrict__ a,
+                                                   const double *__restrict__ b,
+                                                   double *__restrict__ part)
+{
+    __shared__ double sh[256];
+    double s = 0.;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * 256)
+        s += a[i] * b[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void __launch_bounds__(256) dot_finish(int nparts, const double *__restrict__ part,
+                                                  double *__restrict__ out)
+{
+    __shared__ double sh[256];
+    double s = 0.;
+    for (int k = threadIdx.x; k < nparts; k += 256) s += part[k];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0];
+}
+
 }  // namespace
 
 extern "C" {
+
+int pf3d_dot(int64_t n, const double *a, const double *b, double *out, double *work,
+             void *stream)
+{
+    if (n < 0 || (n && (!a || !b)) || !out || !work)
+        return pf3d_fail(PF3D_E_ARG, "pf3d_dot: bad arguments");
+    int64_t nb = (n + 255) / 256;
+    const int blocks = (int)(nb < DOT_BLOCKS ? (nb < 1 ? 1 : nb) : DOT_BLOCKS);
+    hipLaunchKernelGGL(dot_partial, dim3(blocks), dim3(256), 0, (hipStream_t)stream, n, a, b,
+                       work);
+    hipLaunchKernelGGL(dot_finish, dim3(1), dim3(256), 0, (hipStream_t)stream, blocks, work,
+                       out);
+    return pf3d_hip_check(hipGetLastError(), "pf3d_dot");
+}
+
 
 int pf3d_gemv_t(int64_t n, int m, const double *V, int64_t ld, const double *x, double *c,
                 double alpha, double beta, void *stream)

@@ -110,6 +110,14 @@ class StaticSystem:
                   self._band_pinv.data_ptr(), self._band_bad.data_ptr(), _s(stream))
         self._factored = True
 
+    def setup_amg(self, xyz, log=None, **kw):
+        """Smoothed-aggregation multigrid for method='amg' (host setup from
+        the node coordinates xyz (n_nodes, 3); see pyfe3d_amd.amg)."""
+        from .amg import SAAMG
+        self.amg = SAAMG(self.plan, self.val, self.fixed.cpu().numpy().astype(bool), xyz,
+                         log=log, **kw)
+        return self.amg
+
     def solve(self, f, method='fp64', rtol=1e-8, x0=None, maxiter=200000, inner_rtol=1e-4,
               max_outer=60, replace=50, refactor=False, stream=None):
         """Returns (u, info) with info = dict(iters, outer, relres).
@@ -119,7 +127,9 @@ class StaticSystem:
         recursive one every `replace` iterations; info['outer'] counts the
         replacements), 'ir' (FP64 iterative refinement around restarted
         FP32-matrix PCG solves to inner_rtol) or 'direct' (banded Cholesky,
-        factor kept for later right-hand sides unless refactor)."""
+        factor kept for later right-hand sides unless refactor) or 'amg' (CG
+        preconditioned by one smoothed-aggregation V-cycle per iteration,
+        after setup_amg)."""
         b = self.rhs(f)
         x = torch.zeros_like(b) if x0 is None else x0.clone()
         pr, pc = self.plan.row_ptr.data_ptr(), self.plan.col_idx.data_ptr()
@@ -148,6 +158,12 @@ class StaticSystem:
                       int(maxiter), int(x0 is not None), self._wir.data_ptr(),
                       self._wir.numel(), stats, ctypes.byref(relres), _s(stream))
             info = dict(iters=int(stats[1]), outer=int(stats[0]))
+        elif method == 'amg':
+            from .amg import amg_pcg
+            if getattr(self, 'amg', None) is None:
+                raise ValueError("method='amg' needs setup_amg(xyz) first")
+            return amg_pcg(self, self.amg, f, rtol=rtol, maxiter=min(int(maxiter), 100000),
+                           x0=x0)
         elif method == 'direct':
             if refactor or not getattr(self, '_factored', False):
                 self.factor_direct(stream=stream)
