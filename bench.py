@@ -111,9 +111,9 @@ def solve_rates(n, repeats, log):
     f = mesh.pressure_load(1e3)
     out = {}
     t0 = time.perf_counter()
-    sysm.setup_amg(mesh.ncoords)
+    sysm.setup_amg(mesh.ncoords, precisions=('fp64', 'fp32'))
     amg_setup = time.perf_counter() - t0
-    for method in ('direct', 'fp64', 'mixed', 'ir', 'amg'):
+    for method in ('direct', 'fp64', 'mixed', 'ir', 'amg', 'amg32'):
         u, info = sysm.solve(f, method=method, rtol=1e-8)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -267,10 +267,10 @@ def panel_rates(n, log, steps=10, maxiter=200000):
     f = p.loads(Nxx=1e3, p=1e3)
     sol = {}
     t0 = time.perf_counter()
-    sysm.setup_amg(p.mesh.ncoords)
+    sysm.setup_amg(p.mesh.ncoords, precisions=('fp64', 'fp32'))
     amg_setup = time.perf_counter() - t0
     log('panel amg setup %.1f s' % amg_setup)
-    for method in ('amg', 'fp64'):
+    for method in ('amg', 'amg32', 'fp64'):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         u, info = sysm.solve(f, method=method, rtol=1e-8, maxiter=maxiter)
@@ -319,23 +319,25 @@ def large_solve(n, log):
     torch.cuda.empty_cache()
     f = mesh.pressure_load(1e3)
     t0 = time.perf_counter()
-    amg = sysm.setup_amg(mesh.ncoords, log=log)
+    amg = sysm.setup_amg(mesh.ncoords, log=log, precisions=('fp64', 'fp32'))
     setup = time.perf_counter() - t0
     log('large solve %dx%d: amg setup %.1f s, %d levels' % (n, n, setup, len(amg.levels)))
     out = dict(dof=mesh.n_dof, amg_setup_s=setup, levels=len(amg.levels))
-    for rep in range(2):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        u, info = sysm.solve(f, method='amg', rtol=1e-8)
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        log('large solve %dx%d amg: %.2f s, %s' % (n, n, dt, info))
+    for method in ('amg', 'amg32'):
+        for rep in range(2):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            u, info = sysm.solve(f, method=method, rtol=1e-8, maxiter=4000)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            log('large solve %dx%d %s: %.2f s, %s' % (n, n, method, dt, info))
+        out[method] = dict(s=dt, solves_per_s=1. / dt, iters=info['iters'],
+                           relres=info['relres'], restarts=info['outer'],
+                           ms_per_iter=dt * 1e3 / max(info['iters'], 1))
     from pyfe3d_amd.amg import residual_floor
     floor = residual_floor(sysm, u)
     log('large solve: FP64 residual floor eps |A| |u| / |A u| = %.2e' % floor)
-    out.update(amg=dict(s=dt, solves_per_s=1. / dt, iters=info['iters'], relres=info['relres'],
-                        restarts=info['outer'], ms_per_iter=dt * 1e3 / max(info['iters'], 1)),
-               fp64_residual_floor=floor,
+    out.update(fp64_residual_floor=floor,
                note='1e-8 is below the FP64 rounding floor of b - K u on this system; the '
                     'solve stops when a restart from the true residual no longer halves it')
     return out

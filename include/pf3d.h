@@ -453,6 +453,12 @@ int pf3d_band_solve(int64_t n_var, const int32_t *order, const int64_t *dof0, co
 /* FP64 dense helpers of the eigen-solvers (Krylov bases, column-major  */
 /* n x m with leading dimension ld; fixed-order reductions)             */
 /* ------------------------------------------------------------------ */
+/* y = a x + b y on FP32 vectors, and FP32 -> FP64 widening: the
+ * single-precision multigrid preconditioner inside FP64 CG (no reference
+ * counterpart). */
+int pf3d_axpby_f32(int64_t n, float a, const float *x, float b, float *y, void *stream);
+int pf3d_to_f64(int64_t n, const float *a, double *b, void *stream);
+
 /* out[0] = a . b (FP64, fixed reduction order: reruns give the same bits);
  * work: 1024 doubles of device scratch.  The inner products of the
  * multigrid-preconditioned CG (no reference counterpart: the reference

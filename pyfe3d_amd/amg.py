@@ -17,7 +17,9 @@ Solve (GPU, this library's kernels only): PCG whose preconditioner is one
 symmetric V-cycle with Chebyshev(D^-1 A) pre/post smoothing, restriction
 P^T and prolongation P as 6x6-block sparse products (pf3d_bsr_spmv), and
 a dense inverse on the coarsest level (pf3d_gemv_n); dot products with
-pf3d_gemv_t, vector updates with pf3d_axpby.
+pf3d_dot, vector updates with pf3d_axpby.  With precision 'fp32' the whole
+V-cycle runs in single precision (pf3d_bsr_spmv_f32, pf3d_axpby_f32) and
+the CG around it, in FP64, uses the flexible (Polak-Ribiere) beta.
 """
 import time
 
@@ -117,29 +119,37 @@ def _lambda_max(A, Dinv, iters=20, seed=0):
 
 
 class _Dev:
-    """A 6x6-BSR operator on the GPU (rows x cols in nodes)."""
+    """A 6x6-BSR operator on the GPU (rows x cols in nodes), values and
+    vectors in FP64 or FP32."""
 
-    def __init__(self, M, dev):
+    def __init__(self, M, dev, dtype=torch.float64):
         M = M.tobsr(blocksize=(6, 6))
         M.sort_indices()
         self.nb = M.shape[0] // 6
         self.row_ptr = torch.as_tensor(M.indptr.astype(np.int32), device=dev)
         self.col_idx = torch.as_tensor(M.indices.astype(np.int32), device=dev)
-        self.val = torch.as_tensor(np.ascontiguousarray(M.data.reshape(-1)), device=dev)
+        self.val = torch.as_tensor(np.ascontiguousarray(M.data.reshape(-1)), device=dev,
+                                   dtype=dtype)
+        self.fn = 'pf3d_bsr_spmv' if dtype == torch.float64 else 'pf3d_bsr_spmv_f32'
 
     def __call__(self, x, y):
-        _lib.call('pf3d_bsr_spmv', self.nb, self.row_ptr.data_ptr(), self.col_idx.data_ptr(),
+        _lib.call(self.fn, self.nb, self.row_ptr.data_ptr(), self.col_idx.data_ptr(),
                   self.val.data_ptr(), x.data_ptr(), y.data_ptr(), _s())
         return y
 
 
-def _diag_dev(Dinv, dev):
+def _diag_dev(Dinv, dev, dtype=torch.float64):
     n = Dinv.shape[0]
-    return _Dev(ss.bsr_matrix((Dinv, np.arange(n), np.arange(n + 1)), shape=(6 * n, 6 * n)), dev)
+    return _Dev(ss.bsr_matrix((Dinv, np.arange(n), np.arange(n + 1)), shape=(6 * n, 6 * n)), dev,
+                dtype)
 
 
 def _axpby(a, x, b, y):
-    _lib.call('pf3d_axpby', y.numel(), float(a), x.data_ptr(), float(b), y.data_ptr(), _s())
+    if y.dtype == torch.float32:
+        _lib.call('pf3d_axpby_f32', y.numel(), float(a), x.data_ptr(), float(b), y.data_ptr(),
+                  _s())
+    else:
+        _lib.call('pf3d_axpby', y.numel(), float(a), x.data_ptr(), float(b), y.data_ptr(), _s())
 
 
 def setup_levels(A, fixed, xyz, box=3.0, coarse_nodes=300, max_levels=8, log=None):
@@ -192,29 +202,54 @@ class SAAMG:
     DOFs); fixed: bool per DOF; xyz: (n_nodes, 3) node coordinates."""
 
     def __init__(self, plan, val, fixed, xyz, box=3.0, coarse_nodes=300, max_levels=8,
-                 cheb_degree=2, cheb_lower=1. / 30, log=None):
+                 cheb_degree=2, cheb_lower=1. / 30, precision='fp64', log=None, host=None):
+        """precision 'fp32': every level's matrices, smoother and transfer
+        operators in single precision (half the bytes of the V-cycle); the
+        coarsest solve and the CG around it stay FP64.  host: the result of
+        setup_levels for this matrix, to share one setup between precisions."""
         t0 = time.perf_counter()
         dev = val.device
         n = plan.n_nodes
-        A = _bsr_host(plan.row_ptr.cpu().numpy(), plan.col_idx.cpu().numpy(),
-                      val.cpu().numpy(), n, n)
-        host, Ad = setup_levels(A, fixed, xyz, box, coarse_nodes, max_levels, log)
+        if host is None:
+            A = _bsr_host(plan.row_ptr.cpu().numpy(), plan.col_idx.cpu().numpy(),
+                          val.cpu().numpy(), n, n)
+            host = setup_levels(A, fixed, xyz, box, coarse_nodes, max_levels, log)
+        host, Ad = host
         self.cheb_degree, self.cheb_lower = cheb_degree, cheb_lower
+        if precision not in ('fp64', 'fp32'):
+            raise ValueError(precision)
+        self.precision = precision
+        dt = torch.float64 if precision == 'fp64' else torch.float32
         self.levels = []
         for i, H in enumerate(host):
-            self.levels.append(dict(A=_Dev(H['A'], dev), Dinv=_diag_dev(H['Dinv'], dev),
-                                    P=_Dev(H['P'], dev), R=_Dev(H['P'].T.tocsr(), dev),
+            self.levels.append(dict(A=_Dev(H['A'], dev, dt), Dinv=_diag_dev(H['Dinv'], dev, dt),
+                                    P=_Dev(H['P'], dev, dt), R=_Dev(H['P'].T.tocsr(), dev, dt),
                                     lam=H['lam'], n=H['A'].shape[0]))
         self.nc = Ad.shape[0]
         self.coarse_inv = torch.as_tensor(np.ascontiguousarray(np.linalg.inv(Ad).T), device=dev)
-        zeros = lambda m: torch.zeros(m, dtype=torch.float64, device=dev)
+        zeros = lambda m, t=dt: torch.zeros(m, dtype=t, device=dev)
         for i, L in enumerate(self.levels):
             m = L['n']
             L['r'], L['d'], L['t'] = zeros(m), zeros(m), zeros(m)
             # right-hand side and solution of the next level
             mc = self.levels[i + 1]['n'] if i + 1 < len(self.levels) else self.nc
             L['bc'], L['xc'] = zeros(mc), zeros(mc)
+        # FP64 staging of the coarsest solve and of the fine-level vectors
+        self.c64 = (zeros(self.nc, torch.float64), zeros(self.nc, torch.float64))
+        if dt == torch.float32 and self.levels:
+            m0 = self.levels[0]['n']
+            self.f32 = (zeros(m0), zeros(m0))
         self.setup_s = time.perf_counter() - t0
+
+    def apply(self, r, z):
+        """z = M^-1 r for FP64 vectors r, z (one V-cycle)."""
+        if self.precision == 'fp64' or not self.levels:
+            return self.vcycle(r, z)
+        r32, z32 = self.f32
+        _lib.call('pf3d_to_f32', r.numel(), r.data_ptr(), r32.data_ptr(), _s())
+        self.vcycle(r32, z32)
+        _lib.call('pf3d_to_f64', z.numel(), z32.data_ptr(), z.data_ptr(), _s())
+        return z
 
     def _cheb(self, L, b, x):
         """x += Chebyshev(D^-1 A) correction for A x = b (degree k, bounds
@@ -244,8 +279,15 @@ class SAAMG:
     def vcycle(self, b, x, lvl=0):
         """x = M^-1 b (x is overwritten)."""
         if lvl == len(self.levels):
-            _lib.call('pf3d_gemv_n', self.nc, self.nc, self.coarse_inv.data_ptr(), self.nc,
-                      b.data_ptr(), x.data_ptr(), 1.0, 0.0, _s())
+            if b.dtype == torch.float64:
+                _lib.call('pf3d_gemv_n', self.nc, self.nc, self.coarse_inv.data_ptr(), self.nc,
+                          b.data_ptr(), x.data_ptr(), 1.0, 0.0, _s())
+            else:
+                b64, x64 = self.c64
+                _lib.call('pf3d_to_f64', self.nc, b.data_ptr(), b64.data_ptr(), _s())
+                _lib.call('pf3d_gemv_n', self.nc, self.nc, self.coarse_inv.data_ptr(), self.nc,
+                          b64.data_ptr(), x64.data_ptr(), 1.0, 0.0, _s())
+                _lib.call('pf3d_to_f32', self.nc, x64.data_ptr(), x.data_ptr(), _s())
             return x
         L = self.levels[lvl]
         x.zero_()
@@ -271,15 +313,19 @@ def _dot(a, b, out):
     return float(out[0].item())
 
 
-def amg_pcg(system, amg, f, rtol=1e-8, maxiter=2000, x0=None, log=None):
+def amg_pcg(system, amg, f, rtol=1e-8, maxiter=2000, x0=None, log=None, stall=100):
     """PCG with one V-cycle per iteration on a StaticSystem's matrix.
-    Returns (u, info) like StaticSystem.solve (true FP64 residual)."""
+    Returns (u, info) like StaticSystem.solve (true FP64 residual).  A run
+    whose recursive residual has not halved in `stall` iterations restarts
+    from the true residual (and ends when a restart does not halve that)."""
     b = system.rhs(f)
     x = torch.zeros_like(b) if x0 is None else x0.clone()
     r = torch.empty_like(b)
     q = torch.empty_like(b)
     z = torch.empty_like(b)
     s = torch.empty(1025, dtype=torch.float64, device=b.device)
+    flexible = amg.precision != 'fp64'
+    zold = torch.empty_like(b) if flexible else None
     bb = _dot(b, b, s) or 1.
     it = 0
     restarts = 0
@@ -297,20 +343,39 @@ def amg_pcg(system, amg, f, rtol=1e-8, maxiter=2000, x0=None, log=None):
             break
         best = rel
         restarts += 1
-        amg.vcycle(r, z)
+        if log:
+            log('amg pcg: restart %d at %d iterations, true relres %.3e' % (restarts, it, rel))
+        amg.apply(r, z)
         p = z.clone()
         rz = _dot(r, z, s)
+        low, since = rel, 0
         while it < maxiter:
             system.matvec(p, q)
             alpha = rz / _dot(p, q, s)
             _axpby(alpha, p, 1., x)
             _axpby(-alpha, q, 1., r)
             it += 1
-            if (_dot(r, r, s) / bb) ** 0.5 <= rtol:
+            rr = (_dot(r, r, s) / bb) ** 0.5
+            if rr <= rtol:
                 break
-            amg.vcycle(r, z)
-            rz_new = _dot(r, z, s)
-            _axpby(1., z, rz_new / rz, p)
+            if rr <= 0.5 * low:
+                low, since = rr, 0
+            else:
+                since += 1
+                if since >= stall:
+                    break
+            if flexible:
+                # beta = r.(z_new - z_old) / r_old.z_old keeps CG's rate with
+                # a preconditioner that is linear only up to FP32 rounding
+                zold.copy_(z)
+                amg.apply(r, z)
+                rz_new = _dot(r, z, s)
+                beta = (rz_new - _dot(r, zold, s)) / rz
+            else:
+                amg.apply(r, z)
+                rz_new = _dot(r, z, s)
+                beta = rz_new / rz
+            _axpby(1., z, beta, p)
             rz = rz_new
     return x, dict(iters=it, outer=restarts, relres=rel)
 

@@ -51,6 +51,20 @@ __global__ void axpby(int64_t n, double a, const double *__restrict__ x, double 
     if (i < n) y[i] = a * x[i] + (b != 0. ? b * y[i] : 0.);
 }
 
+// FP32 vectors (the multigrid preconditioner in single precision)
+__global__ void axpby_f32(int64_t n, float a, const float *__restrict__ x, float b,
+                          float *__restrict__ y)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = a * x[i] + (b != 0.f ? b * y[i] : 0.f);
+}
+
+__global__ void widen(int64_t n, const float *__restrict__ a, double *__restrict__ b)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (double)a[i];
+}
+
 // dot product in two fixed-order stages: DOT_BLOCKS partial sums over a
 // grid stride, then one workgroup adds the partials in index order
 constexpr int DOT_BLOCKS = 1024;
@@ -91,6 +105,24 @@ __global__ void __launch_bounds__(256) dot_finish(int nparts, const double *__re
 }  // namespace
 
 extern "C" {
+
+int pf3d_axpby_f32(int64_t n, float a, const float *x, float b, float *y, void *stream)
+{
+    if (n < 0 || (n && (!x || !y))) return pf3d_fail(PF3D_E_ARG, "pf3d_axpby_f32: bad arguments");
+    if (!n) return PF3D_OK;
+    hipLaunchKernelGGL(axpby_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, n, a, x, b, y);
+    return pf3d_hip_check(hipGetLastError(), "pf3d_axpby_f32");
+}
+
+int pf3d_to_f64(int64_t n, const float *a, double *b, void *stream)
+{
+    if (n < 0 || (n && (!a || !b))) return pf3d_fail(PF3D_E_ARG, "pf3d_to_f64: bad arguments");
+    if (!n) return PF3D_OK;
+    hipLaunchKernelGGL(widen, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, n, a, b);
+    return pf3d_hip_check(hipGetLastError(), "pf3d_to_f64");
+}
 
 int pf3d_dot(int64_t n, const double *a, const double *b, double *out, double *work,
              void *stream)

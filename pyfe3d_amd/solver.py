@@ -110,13 +110,27 @@ class StaticSystem:
                   self._band_pinv.data_ptr(), self._band_bad.data_ptr(), _s(stream))
         self._factored = True
 
-    def setup_amg(self, xyz, log=None, **kw):
-        """Smoothed-aggregation multigrid for method='amg' (host setup from
-        the node coordinates xyz (n_nodes, 3); see pyfe3d_amd.amg)."""
-        from .amg import SAAMG
-        self.amg = SAAMG(self.plan, self.val, self.fixed.cpu().numpy().astype(bool), xyz,
-                         log=log, **kw)
-        return self.amg
+    def setup_amg(self, xyz, log=None, precisions=('fp64',), **kw):
+        """Smoothed-aggregation multigrid for method='amg' (FP64 V-cycle)
+        and/or 'amg32' (FP32 V-cycle): one host setup from the node
+        coordinates xyz (n_nodes, 3) (see pyfe3d_amd.amg).  Returns the
+        last one built."""
+        from .amg import SAAMG, _bsr_host, setup_levels
+        fixed = self.fixed.cpu().numpy().astype(bool)
+        box = kw.pop('box', 3.0)
+        coarse_nodes = kw.pop('coarse_nodes', 300)
+        max_levels = kw.pop('max_levels', 8)
+        A = _bsr_host(self.plan.row_ptr.cpu().numpy(), self.plan.col_idx.cpu().numpy(),
+                      self.val.cpu().numpy(), self.nb, self.nb)
+        host = setup_levels(A, fixed, xyz, box, coarse_nodes, max_levels, log)
+        m = None
+        for prec in precisions:
+            m = SAAMG(self.plan, self.val, fixed, xyz, precision=prec, host=host, **kw)
+            if prec == 'fp64':
+                self.amg = m
+            else:
+                self.amg32 = m
+        return m
 
     def solve(self, f, method='fp64', rtol=1e-8, x0=None, maxiter=200000, inner_rtol=1e-4,
               max_outer=60, replace=50, refactor=False, stream=None):
@@ -127,9 +141,10 @@ class StaticSystem:
         recursive one every `replace` iterations; info['outer'] counts the
         replacements), 'ir' (FP64 iterative refinement around restarted
         FP32-matrix PCG solves to inner_rtol) or 'direct' (banded Cholesky,
-        factor kept for later right-hand sides unless refactor) or 'amg' (CG
+        factor kept for later right-hand sides unless refactor), 'amg' (CG
         preconditioned by one smoothed-aggregation V-cycle per iteration,
-        after setup_amg)."""
+        after setup_amg) or 'amg32' (the same with the V-cycle in FP32, after
+        setup_amg(xyz, precision='fp32'))."""
         b = self.rhs(f)
         x = torch.zeros_like(b) if x0 is None else x0.clone()
         pr, pc = self.plan.row_ptr.data_ptr(), self.plan.col_idx.data_ptr()
@@ -158,12 +173,23 @@ class StaticSystem:
                       int(maxiter), int(x0 is not None), self._wir.data_ptr(),
                       self._wir.numel(), stats, ctypes.byref(relres), _s(stream))
             info = dict(iters=int(stats[1]), outer=int(stats[0]))
-        elif method == 'amg':
+        elif method in ('amg', 'amg32'):
             from .amg import amg_pcg
-            if getattr(self, 'amg', None) is None:
-                raise ValueError("method='amg' needs setup_amg(xyz) first")
-            return amg_pcg(self, self.amg, f, rtol=rtol, maxiter=min(int(maxiter), 100000),
-                           x0=x0)
+            name = 'amg' if method == 'amg' else 'amg32'
+            if getattr(self, name, None) is None:
+                raise ValueError("method=%r needs setup_amg(xyz%s) first"
+                                 % (method, '' if method == 'amg' else ", precision='fp32'"))
+            u, info = amg_pcg(self, getattr(self, name), f, rtol=rtol,
+                              maxiter=min(int(maxiter), 100000), x0=x0)
+            if (method == 'amg32' and info['relres'] > rtol
+                    and getattr(self, 'amg', None) is not None):
+                # the FP32 V-cycle stalled: finish with the FP64 one from u
+                u, info64 = amg_pcg(self, self.amg, f, rtol=rtol,
+                                    maxiter=min(int(maxiter), 100000), x0=u)
+                info = dict(iters=info['iters'] + info64['iters'],
+                            outer=info['outer'] + info64['outer'], relres=info64['relres'],
+                            fp64_iters=info64['iters'])
+            return u, info
         elif method == 'direct':
             if refactor or not getattr(self, '_factored', False):
                 self.factor_direct(stream=stream)

@@ -42,8 +42,9 @@ def _spsolve(m, conn, etype, rec, fixed, f):
     return u
 
 
+@pytest.mark.parametrize('method', ['amg', 'amg32'])
 @pytest.mark.parametrize('etype', ['quad4', 'tria3r'])
-def test_amg_pcg_matches_spsolve(etype):
+def test_amg_pcg_matches_spsolve(etype, method):
     """20x20 clamped plate, centre load (the reference's static test
     set-up): displacements within 1e-8 of spsolve at residual 1e-10."""
     from pyfe3d_amd.mesh import PlateMesh
@@ -55,12 +56,12 @@ def test_amg_pcg_matches_spsolve(etype):
     f = np.zeros(m.n_dof)
     f[6 * m.centre_node() + 2] = -1000.
     sysm = _system(m, conn, etype, rec, fixed)
-    sysm.setup_amg(m.ncoords, coarse_nodes=50)
-    u, info = sysm.solve(f, method='amg', rtol=1e-10)
+    sysm.setup_amg(m.ncoords, coarse_nodes=50, precisions=('fp64', 'fp32'))
+    u, info = sysm.solve(f, method=method, rtol=1e-10)
     _, info_bj = sysm.solve(f, method='fp64', rtol=1e-10)
     uref = _spsolve(m, conn, etype, rec, fixed, f)
     err = np.linalg.norm(u.cpu().numpy() - uref) / np.linalg.norm(uref)
-    print(etype, 'amg', info, 'block-Jacobi', info_bj['iters'], 'err %.2e' % err)
+    print(etype, method, info, 'block-Jacobi', info_bj['iters'], 'err %.2e' % err)
     assert info['relres'] <= 1e-10 and err <= 1e-8
     assert info['iters'] < info_bj['iters']
 
@@ -88,7 +89,8 @@ def test_vcycle_is_symmetric_and_positive():
     assert float(Ma @ a) > 0 and float(Mb @ b) > 0
 
 
-def test_amg_pcg_laminate_64():
+@pytest.mark.parametrize('method', ['amg', 'amg32'])
+def test_amg_pcg_laminate_64(method):
     """The 64x64 jittered laminate plate of the solve benchmark at 1e-8:
     the drilling rotations aside (ill-determined by the element's small
     drilling stiffness: scipy's spsolve and dense solves differ there too),
@@ -99,14 +101,14 @@ def test_amg_pcg_laminate_64():
     prop, _ = B.laminate_record()
     fixed = m.clamped()
     sysm = _system(m, m.quads, 'quad4', prop.record(), fixed)
-    sysm.setup_amg(m.ncoords)
+    sysm.setup_amg(m.ncoords, precisions=('fp64', 'fp32'))
     f = m.pressure_load(1e3)
-    u, info = sysm.solve(f, method='amg', rtol=1e-8)
+    u, info = sysm.solve(f, method=method, rtol=1e-8)
     u2, info2 = sysm.solve(f, method='fp64', rtol=1e-10)
     keep = np.ones(m.n_dof, bool)
     keep[5::6] = False
     ua, ub = u.cpu().numpy(), u2.cpu().numpy()
     err = np.linalg.norm((ua - ub)[keep]) / np.linalg.norm(ub[keep])
-    print('amg', info, 'bj', info2['iters'], 'err %.2e' % err)
+    print(method, info, 'bj', info2['iters'], 'err %.2e' % err)
     assert info['relres'] <= 1e-8 and err <= 1e-7
     assert info['iters'] * 20 < info2['iters']
