@@ -31,8 +31,8 @@ HBM_PEAK_GBS = 8000.0
 # `python bench.py --profile-kc0` (KiB per dispatch), each scaled by the
 # reading of the same counter for a 1 GiB device copy in the same run
 # (FETCH_SIZE reads 0.5 GiB for it: factor 2.0; WRITE_SIZE 1 GiB: 1.0):
-# 60,052 KiB x 2 + 4,500,076 KiB = 4.731e9 B (algorithmic: 4.736e9 B).
-KC0_TRAFFIC_BYTES = (60052 * 2 + 4500076) * 1024
+# 60,070 KiB x 2 + 4,500,577 KiB = 4.732e9 B (algorithmic: 4.736e9 B).
+KC0_TRAFFIC_BYTES = (60070 * 2 + 4500577) * 1024
 # algorithmic bytes of one Quad4 KC0 element, values only: 576 doubles out,
 # 4 int64 DOF offsets and 12 coordinates in
 KC0_BYTES_VALUES = 576 * 8 + 4 * 8 + 12 * 8

@@ -241,14 +241,45 @@ class SAAMG:
             self.f32 = (zeros(m0), zeros(m0))
         self.setup_s = time.perf_counter() - t0
 
-    def apply(self, r, z):
-        """z = M^-1 r for FP64 vectors r, z (one V-cycle)."""
+    def _apply(self, r, z):
         if self.precision == 'fp64' or not self.levels:
             return self.vcycle(r, z)
         r32, z32 = self.f32
         _lib.call('pf3d_to_f32', r.numel(), r.data_ptr(), r32.data_ptr(), _s())
         self.vcycle(r32, z32)
         _lib.call('pf3d_to_f64', z.numel(), z32.data_ptr(), z.data_ptr(), _s())
+        return z
+
+    def capture(self):
+        """Record one V-cycle (every launch with fixed buffers and
+        coefficients; no host synchronisation inside) in a HIP graph that
+        apply() replays: one graph launch instead of tens of kernel
+        launches per CG iteration."""
+        n = self.levels[0]['n'] if self.levels else self.nc
+        dev = self.coarse_inv.device
+        self.g_in = torch.zeros(n, dtype=torch.float64, device=dev)
+        self.g_out = torch.zeros(n, dtype=torch.float64, device=dev)
+        s = torch.cuda.Stream(device=dev)
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            self._apply(self.g_in, self.g_out)      # warm-up outside the graph
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self._apply(self.g_in, self.g_out)
+        self.graph = g
+        return g
+
+    def apply(self, r, z):
+        """z = M^-1 r for FP64 vectors r, z (one V-cycle; the captured
+        graph when capture() was called)."""
+        g = getattr(self, 'graph', None)
+        if g is None:
+            return self._apply(r, z)
+        _axpby(1., r, 0., self.g_in)
+        g.replay()
+        _axpby(1., self.g_out, 0., z)
         return z
 
     def _cheb(self, L, b, x):
@@ -261,7 +292,7 @@ class SAAMG:
         rho_old = 1. / sigma
         r, d, t = L['r'], L['d'], L['t']
         L['A'](x, t)
-        r.copy_(b)
+        _axpby(1., b, 0., r)
         _axpby(-1., t, 1., r)                      # r = b - A x
         L['Dinv'](r, d)
         _axpby(0., d, 1. / theta, d)               # d = D^-1 r / theta
@@ -290,11 +321,11 @@ class SAAMG:
                 _lib.call('pf3d_to_f32', self.nc, x64.data_ptr(), x.data_ptr(), _s())
             return x
         L = self.levels[lvl]
-        x.zero_()
+        _axpby(0., b, 0., x)                       # x = 0
         self._cheb(L, b, x)
         r, t = L['r'], L['t']
         L['A'](x, t)
-        r.copy_(b)
+        _axpby(1., b, 0., r)
         _axpby(-1., t, 1., r)                      # r = b - A x
         bc, xc = L['bc'], L['xc']
         L['R'](r, bc)

@@ -110,11 +110,12 @@ class StaticSystem:
                   self._band_pinv.data_ptr(), self._band_bad.data_ptr(), _s(stream))
         self._factored = True
 
-    def setup_amg(self, xyz, log=None, precisions=('fp64',), **kw):
+    def setup_amg(self, xyz, log=None, precisions=('fp64',), graph=True, **kw):
         """Smoothed-aggregation multigrid for method='amg' (FP64 V-cycle)
         and/or 'amg32' (FP32 V-cycle): one host setup from the node
-        coordinates xyz (n_nodes, 3) (see pyfe3d_amd.amg).  Returns the
-        last one built."""
+        coordinates xyz (n_nodes, 3) (see pyfe3d_amd.amg), the V-cycle
+        captured in a HIP graph unless graph=False.  Returns the last one
+        built."""
         from .amg import SAAMG, _bsr_host, setup_levels
         fixed = self.fixed.cpu().numpy().astype(bool)
         box = kw.pop('box', 3.0)
@@ -126,6 +127,8 @@ class StaticSystem:
         m = None
         for prec in precisions:
             m = SAAMG(self.plan, self.val, fixed, xyz, precision=prec, host=host, **kw)
+            if graph:
+                m.capture()
             if prec == 'fp64':
                 self.amg = m
             else:

@@ -112,3 +112,27 @@ def test_amg_pcg_laminate_64(method):
     print(method, info, 'bj', info2['iters'], 'err %.2e' % err)
     assert info['relres'] <= 1e-8 and err <= 1e-7
     assert info['iters'] * 20 < info2['iters']
+
+
+def test_vcycle_graph_replays_match_eager():
+    """The captured V-cycle replayed several times, with eager work and
+    synchronisations between the replays, gives the eager result bit for
+    bit (FP64 and FP32 V-cycles)."""
+    from pyfe3d_amd.mesh import PlateMesh
+    from pyfe3d_amd.shellprop import isotropic_plate
+    m = PlateMesh(29, 21, 1.0, 0.7, jitter=0.2, seed=6)
+    rec = isotropic_plate(thickness=0.004, E=70e9, nu=0.33, calc_scf=True).record()
+    sysm = _system(m, m.quads, 'quad4', rec, m.clamped())
+    for prec in ('fp64', 'fp32'):
+        amg = sysm.setup_amg(m.ncoords, coarse_nodes=40, precisions=(prec,), graph=False)
+        g = torch.Generator(device='cpu').manual_seed(3)
+        rs = [(torch.rand(m.n_dof, generator=g, dtype=torch.float64) - 0.5).to(DEV)
+              for _ in range(3)]
+        eager = [amg.apply(r, torch.empty_like(r)).clone() for r in rs]
+        amg.capture()
+        for rep in range(2):
+            for r, e in zip(rs, eager):
+                z = amg.apply(r, torch.empty_like(r))
+                torch.cuda.synchronize()
+                assert torch.equal(z, e), (prec, rep)
+                _ = (r * 2).sum().item()   # eager work between replays
