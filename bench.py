@@ -153,6 +153,17 @@ def sweep_rates(n_var, log, steps=2, modes=('direct', 'fp64', 'mixed')):
     log('sweep: %d variants, %d elements, %d DOF, %d blocks (setup %.1f s)'
         % (n_var, sw.n_elem, 6 * sw.n_nodes, sw.plan.nnzb, time.perf_counter() - t0))
     out = dict(variants=n_var, elements=sw.n_elem, dof=6 * sw.n_nodes)
+    tuned = None
+    if 'direct' in modes:
+        # launch configuration of the banded solver chosen by timing
+        sw.assemble()
+        t0 = time.perf_counter()
+        table = sw.autotune_direct(log=log, repeats=1)
+        tuned = dict(config=dict(zip(('factor_threads', 'panel_bandwidth', 'solve_threads'),
+                                     sw.direct_config)),
+                     tuning_s=time.perf_counter() - t0,
+                     candidates=[dict(c, ms=round(ms, 3)) for c, ms in table])
+        log('sweep direct: autotuned config %s' % (sw.direct_config,))
     for mode in modes:
         t0 = time.perf_counter()
         g = sw.capture(mode=mode, rtol=1e-8)  # runs one eager step first
@@ -174,6 +185,8 @@ def sweep_rates(n_var, log, steps=2, modes=('direct', 'fp64', 'mixed')):
                          max_relres=float(rel.max()), converged=int((rel <= 1e-8).sum()),
                          pcg_iters_total=int(its.sum()), pcg_iters_max=int(its.max()),
                          first_step_s=first)
+        if mode == 'direct' and tuned:
+            out[mode]['autotune'] = tuned
         log('sweep %s: %.1f ms/sweep, %.1f solves/s, max relres %.2e, iters %d'
             % (mode, ms, n_var / (ms * 1e-3), rel.max(), its.sum()))
         del g

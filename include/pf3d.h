@@ -437,17 +437,22 @@ int pf3d_band_max_width(void);
  * permuted (global) node index, bw[v] the bandwidth in scalars and
  * boff[v] the offset of the system's band (dof count x (bw+1) doubles) in
  * band (band_size doubles in all).  *bad (device int) counts systems with
- * a non-positive pivot. */
+ * a non-positive pivot or a bandwidth above max_b.  Launch configuration:
+ * threads per workgroup (256, 512 or 1024; 0 = 1024) and the bandwidth the
+ * LDS panel is sized for, max_b (256, 448 or 640; 0 = 640): a narrower
+ * panel fits more workgroups per CU (chosen by pyfe3d_amd.autotune). */
 int pf3d_band_factor(int64_t n_var, int64_t n_nodes, const int32_t *order, const int64_t *dof0,
                      const int32_t *bw, const int64_t *boff, int64_t band_size, double *band,
                      const int32_t *row_ptr, const int32_t *col_idx, const double *val,
-                     const int32_t *var, const int64_t *pinv, int *bad, void *stream);
+                     const int32_t *var, const int64_t *pinv, int *bad, int threads, int max_b,
+                     void *stream);
 
 /* x = A^-1 f per system with the factors of pf3d_band_factor; pnode[p] is
- * the original node of permuted node p; work: one double per DOF. */
+ * the original node of permuted node p; work: one double per DOF; threads
+ * per workgroup 256, 512 or 1024 (0 = 1024). */
 int pf3d_band_solve(int64_t n_var, const int32_t *order, const int64_t *dof0, const int32_t *bw,
                     const int64_t *boff, const double *band, const int64_t *pnode,
-                    const double *f, double *x, double *work, void *stream);
+                    const double *f, double *x, double *work, int threads, void *stream);
 
 /* ------------------------------------------------------------------ */
 /* FP64 dense helpers of the eigen-solvers (Krylov bases, column-major  */

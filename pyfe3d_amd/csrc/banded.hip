@@ -30,10 +30,10 @@ int pf3d_hip_check(hipError_t e, const char *where);
 namespace {
 
 constexpr int NB = 16;       // panel width
-constexpr int BT = 1024;     // threads per workgroup
 constexpr int MAXB = 640;    // largest supported bandwidth (LDS panel rows b + NB)
-constexpr int PR = MAXB + NB;
-constexpr int QMAX = (PR + 63) / 64;  // panel rows per lane
+// launch configurations (threads per workgroup BT, largest bandwidth MB the
+// LDS panel is sized for): a narrower panel fits more workgroups per CU
+// (MB 640: 89 KB of LDS, one per CU; 448: 63 KB, two; 256: 37 KB, four)
 
 struct BandSys {
     int64_t n_var;
@@ -78,8 +78,11 @@ __global__ void band_fill(int64_t n_nodes, const int32_t *__restrict__ rp,
 }
 
 // ------------------------------------------------------------ Cholesky
+template <int BT, int MB>
 __global__ void __launch_bounds__(BT) band_chol(BandSys s, int *__restrict__ bad)
 {
+    constexpr int PR = MB + NB;
+    constexpr int QMAX = (PR + 63) / 64;  // panel rows per lane
     __shared__ double P[PR][NB + 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t v = s.order ? (int64_t)s.order[blockIdx.x] : (int64_t)blockIdx.x;
@@ -87,6 +90,10 @@ __global__ void __launch_bounds__(BT) band_chol(BandSys s, int *__restrict__ bad
     const int b = s.bw[v];
     double *L = s.band + s.boff[v];
     const int64_t ld = b + 1;
+    if (b > MB) {  // the panel would not fit: report, leave the band untouched
+        if (tid == 0) atomicAdd(bad, 1);
+        return;
+    }
     bool failed = false;
     for (int64_t j0 = 0; j0 < n; j0 += NB) {
         const int jn = (int)(n - j0 < NB ? n - j0 : NB);
@@ -181,13 +188,14 @@ __global__ void __launch_bounds__(BT) band_chol(BandSys s, int *__restrict__ bad
 
 // --------------------------------------------------------- triangular
 // x <- A^-1 f for system v: permute f into y, L y = g, L^T x = y, scatter.
+template <int BT>
 __global__ void __launch_bounds__(BT) band_solve(BandSys s, const int64_t *__restrict__ pnode,
                                                   const double *__restrict__ f,
                                                   double *__restrict__ x,
                                                   double *__restrict__ work)
 {
     __shared__ double yb[NB];
-    __shared__ double part[BT / 64][NB];
+    __shared__ double part[NB];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t v = s.order ? (int64_t)s.order[blockIdx.x] : (int64_t)blockIdx.x;
     const int64_t base = s.dof0[v];
@@ -238,21 +246,24 @@ __global__ void __launch_bounds__(BT) band_solve(BandSys s, const int64_t *__res
         const int64_t j0 = bk * NB;
         const int jn = (int)(n - j0 < NB ? n - j0 : NB);
         // contributions of the solved rows below: s_c = sum_i L[i][j0+c] y[i]
-        // for i in [j0+jn, j0+c+b]; 64 lanes per column, NB columns per pass
-        const int c = wave;  // BT/64 = 16 = NB waves, one per column
-        double sacc = 0.;
-        if (c < jn) {
-            const int64_t j = j0 + c;
-            const int64_t iend = (j + b < n - 1) ? j + b : n - 1;
-            for (int64_t i = j0 + jn + lane; i <= iend; i += 64) sacc += L[j * ld + (i - j)] * y[i];
-        }
+        // for i in [j0+jn, j0+c+b]; 64 lanes per column, one wave per
+        // column (columns c, c + BT/64, ... when there are fewer waves)
+        for (int c = wave; c < NB; c += BT / 64) {
+            double sacc = 0.;
+            if (c < jn) {
+                const int64_t j = j0 + c;
+                const int64_t iend = (j + b < n - 1) ? j + b : n - 1;
+                for (int64_t i = j0 + jn + lane; i <= iend; i += 64)
+                    sacc += L[j * ld + (i - j)] * y[i];
+            }
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) sacc += __shfl_xor(sacc, o, 64);
-        if (lane == 0) part[wave][0] = sacc;
+            for (int o = 32; o > 0; o >>= 1) sacc += __shfl_xor(sacc, o, 64);
+            if (lane == 0) part[c] = sacc;
+        }
         __syncthreads();
         if (wave == 0) {
             // back substitution in the diagonal triangle, lane c holds x[j0+c]
-            double rhs = lane < jn ? y[j0 + lane] - part[lane][0] : 0.;
+            double rhs = lane < jn ? y[j0 + lane] - part[lane] : 0.;
             double xc = 0.;
             for (int cc = jn - 1; cc >= 0; cc--) {
                 double piv = lane == cc ? rhs / L[(j0 + cc) * ld] : 0.;
@@ -279,11 +290,34 @@ extern "C" {
 
 int pf3d_band_max_width(void) { return MAXB; }
 
+// band_chol<BT, MB> for a (threads, max_b) configuration; false if none
+static bool launch_chol(int threads, int max_b, unsigned grid, hipStream_t s, BandSys sys,
+                        int *bad)
+{
+#define PF3D_CHOL(BTV, MBV)                                                                 \
+    if (threads == BTV && max_b == MBV) {                                                   \
+        hipLaunchKernelGGL((band_chol<BTV, MBV>), dim3(grid), dim3(BTV), 0, s, sys, bad); \
+        return true;                                                                        \
+    }
+    PF3D_CHOL(1024, 640) PF3D_CHOL(1024, 448) PF3D_CHOL(1024, 256)
+    PF3D_CHOL(512, 640) PF3D_CHOL(512, 448) PF3D_CHOL(512, 256)
+    PF3D_CHOL(256, 640) PF3D_CHOL(256, 448) PF3D_CHOL(256, 256)
+#undef PF3D_CHOL
+    return false;
+}
+
 int pf3d_band_factor(int64_t n_var, int64_t n_nodes, const int32_t *order, const int64_t *dof0,
                      const int32_t *bw, const int64_t *boff, int64_t band_size, double *band,
                      const int32_t *row_ptr, const int32_t *col_idx, const double *val,
-                     const int32_t *var, const int64_t *pinv, int *bad, void *stream)
+                     const int32_t *var, const int64_t *pinv, int *bad, int threads, int max_b,
+                     void *stream)
 {
+    if (threads == 0) threads = 1024;
+    if (max_b == 0) max_b = MAXB;
+    if ((threads != 256 && threads != 512 && threads != 1024) ||
+        (max_b != 256 && max_b != 448 && max_b != 640))
+        return pf3d_fail(PF3D_E_ARG, "pf3d_band_factor: threads must be 256/512/1024 and "
+                                     "max_b 256/448/640 (got %d, %d)", threads, max_b);
     if (n_var < 0 || n_nodes < 0) return pf3d_fail(PF3D_E_ARG, "pf3d_band_factor: negative size");
     if (n_var == 0) return PF3D_OK;
     if (!dof0 || !bw || !boff || !band || !row_ptr || !col_idx || !val || !var || !pinv || !bad)
@@ -295,21 +329,30 @@ int pf3d_band_factor(int64_t n_var, int64_t n_nodes, const int32_t *order, const
     hipLaunchKernelGGL(band_fill, dim3(nb256(n_nodes * 36)), dim3(256), 0, s, n_nodes, row_ptr,
                        col_idx, val, var, pinv, dof0, bw, boff, band);
     BandSys sys{n_var, order, dof0, bw, boff, band};
-    hipLaunchKernelGGL(band_chol, dim3((unsigned)n_var), dim3(BT), 0, s, sys, bad);
+    launch_chol(threads, max_b, (unsigned)n_var, s, sys, bad);
     return pf3d_hip_check(hipGetLastError(), "pf3d_band_factor");
 }
 
 int pf3d_band_solve(int64_t n_var, const int32_t *order, const int64_t *dof0, const int32_t *bw,
                     const int64_t *boff, const double *band, const int64_t *pnode,
-                    const double *f, double *x, double *work, void *stream)
+                    const double *f, double *x, double *work, int threads, void *stream)
 {
     if (n_var < 0) return pf3d_fail(PF3D_E_ARG, "pf3d_band_solve: negative size");
     if (n_var == 0) return PF3D_OK;
     if (!dof0 || !bw || !boff || !band || !pnode || !f || !x || !work)
         return pf3d_fail(PF3D_E_ARG, "pf3d_band_solve: null argument");
+    if (threads == 0) threads = 1024;
     BandSys sys{n_var, order, dof0, bw, boff, const_cast<double *>(band)};
-    hipLaunchKernelGGL(band_solve, dim3((unsigned)n_var), dim3(BT), 0, (hipStream_t)stream, sys,
-                       pnode, f, x, work);
+    const dim3 grid((unsigned)n_var);
+    hipStream_t s = (hipStream_t)stream;
+    if (threads == 1024)
+        hipLaunchKernelGGL(band_solve<1024>, grid, dim3(1024), 0, s, sys, pnode, f, x, work);
+    else if (threads == 512)
+        hipLaunchKernelGGL(band_solve<512>, grid, dim3(512), 0, s, sys, pnode, f, x, work);
+    else if (threads == 256)
+        hipLaunchKernelGGL(band_solve<256>, grid, dim3(256), 0, s, sys, pnode, f, x, work);
+    else
+        return pf3d_fail(PF3D_E_ARG, "pf3d_band_solve: threads must be 256, 512 or 1024");
     return pf3d_hip_check(hipGetLastError(), "pf3d_band_solve");
 }
 

@@ -60,7 +60,7 @@ class GeneralizedEigen:
         c = self.C
         _lib.call('pf3d_band_solve', 1, None, c._band_dof0.data_ptr(), c._band_bw.data_ptr(),
                   c._band_off.data_ptr(), c._band.data_ptr(), c._band_pnode.data_ptr(),
-                  f.data_ptr(), x.data_ptr(), c._band_work.data_ptr(), _s())
+                  f.data_ptr(), x.data_ptr(), c._band_work.data_ptr(), 0, _s())
 
     def _dots(self, V, j, x, out):
         """out[0:j] = V[:, 0:j]^T x"""

@@ -107,7 +107,7 @@ class StaticSystem:
                   self._band_bw.data_ptr(), self._band_off.data_ptr(), self._band_size,
                   self._band.data_ptr(), self.plan.row_ptr.data_ptr(),
                   self.plan.col_idx.data_ptr(), self.val.data_ptr(), self._band_var.data_ptr(),
-                  self._band_pinv.data_ptr(), self._band_bad.data_ptr(), _s(stream))
+                  self._band_pinv.data_ptr(), self._band_bad.data_ptr(), 0, 0, _s(stream))
         self._factored = True
 
     def setup_amg(self, xyz, log=None, precisions=('fp64',), graph=True, **kw):
@@ -199,7 +199,7 @@ class StaticSystem:
             _lib.call('pf3d_band_solve', 1, None, self._band_dof0.data_ptr(),
                       self._band_bw.data_ptr(), self._band_off.data_ptr(),
                       self._band.data_ptr(), self._band_pnode.data_ptr(), b.data_ptr(),
-                      x.data_ptr(), self._band_work.data_ptr(), _s(stream))
+                      x.data_ptr(), self._band_work.data_ptr(), 0, _s(stream))
             if int(self._band_bad.item()):
                 raise _lib.Pf3dError('band factorisation met a non-positive pivot')
             info = dict(iters=0, outer=0)

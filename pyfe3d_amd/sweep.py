@@ -205,23 +205,49 @@ class PlateSweep:
         self.band_work = torch.empty(6 * self.n_nodes, dtype=torch.float64, device=dev)
         self.band_bad = torch.zeros(1, dtype=torch.int32, device=dev)
 
-    def solve_direct(self, rtol=1e-8, polish_iter=2000, stream=None):
+    def solve_direct(self, rtol=1e-8, polish_iter=2000, config=None, stream=None):
         """Banded Cholesky of every variant (one workgroup each), forward and
         back substitution, then the FP64 true residual of each variant
-        (PCG polish from the direct solution if it misses rtol)."""
+        (PCG polish from the direct solution if it misses rtol).  config:
+        (factor threads, panel bandwidth, solve threads), default the one
+        chosen by :meth:`autotune_direct`, else (1024, 640, 1024)."""
         if not hasattr(self, 'band'):
             self._band_setup()
         s = _s(stream)
+        ft, mb, st = config or getattr(self, 'direct_config', None) or (1024, 640, 1024)
         _lib.call('pf3d_band_factor', self.n_var, self.n_nodes, self.order.data_ptr(),
                   self.band_dof0.data_ptr(), self.band_bw.data_ptr(), self.band_off.data_ptr(),
                   self.band_size, self.band.data_ptr(), self.plan.row_ptr.data_ptr(),
                   self.plan.col_idx.data_ptr(), self.val.data_ptr(), self.band_var.data_ptr(),
-                  self.band_pinv.data_ptr(), self.band_bad.data_ptr(), s)
+                  self.band_pinv.data_ptr(), self.band_bad.data_ptr(), int(ft), int(mb), s)
         _lib.call('pf3d_band_solve', self.n_var, self.order.data_ptr(),
                   self.band_dof0.data_ptr(), self.band_bw.data_ptr(), self.band_off.data_ptr(),
                   self.band.data_ptr(), self.band_pnode.data_ptr(), self.f.data_ptr(),
-                  self.u.data_ptr(), self.band_work.data_ptr(), s)
+                  self.u.data_ptr(), self.band_work.data_ptr(), int(st), s)
         self.solve(mode='fp64', rtol=rtol, max_iter=polish_iter, x0=True, stream=stream)
+
+    def autotune_direct(self, log=None, repeats=2):
+        """Time every launch configuration of the banded solver that fits this
+        sweep's largest bandwidth (factor threads x LDS panel width, solve
+        threads) on the current designs and keep the fastest for
+        solve_direct / step('direct') / capture('direct').  Returns the
+        table of timings."""
+        from .autotune import tune
+        if not hasattr(self, 'band'):
+            self._band_setup()
+        bmax = int(self.band_bw.max())
+        panels = [m for m in (256, 448, 640) if m >= bmax]
+
+        def run(c):
+            self.solve_direct(rtol=1.0, polish_iter=0, config=(c['ft'], c['mb'], c['st']))
+
+        # factorisation shape first (it dominates), then the solve's threads
+        cands = [dict(ft=ft, mb=mb, st=1024) for mb in panels for ft in (256, 512, 1024)]
+        best, table = tune(cands, run, repeats=repeats, log=log)
+        cands = [dict(best, st=st) for st in (256, 512)]
+        best2, table2 = tune(cands + [best], run, repeats=repeats, log=log)
+        self.direct_config = (best2['ft'], best2['mb'], best2['st'])
+        return table + table2
 
     def step(self, mode='mixed', rtol=1e-8, stream=None, **kw):
         self.assemble(stream=stream)

@@ -190,3 +190,53 @@ def test_sweep_direct_banded_matches_spsolve():
     torch.cuda.synchronize()
     assert torch.equal(ug, sw.u)
     assert (sw.relres.cpu().numpy() <= 1e-8).all()
+
+
+def test_direct_launch_configs_and_autotune():
+    """Every launch configuration of the banded solver (threads per
+    workgroup, LDS panel width) gives the same bits; the autotuner keeps
+    one of the candidates and the captured step uses it."""
+    sw, h, f = _sweep(seed=11)
+    sw.assemble()
+    sw.solve_direct(rtol=1e-10, polish_iter=0, config=(1024, 640, 1024))
+    torch.cuda.synchronize()
+    ref = sw.u.clone()
+    bmax = int(sw.band_bw.max())
+    for cfg in [(256, 256, 256), (512, 448, 512), (1024, 256, 256), (256, 640, 1024)]:
+        if cfg[1] < bmax:
+            continue
+        sw.u.zero_()
+        sw.solve_direct(rtol=1e-10, polish_iter=0, config=cfg)
+        torch.cuda.synchronize()
+        assert torch.equal(sw.u, ref), cfg
+    sw.check()
+    table = sw.autotune_direct(repeats=1)
+    assert sw.direct_config in [(c['ft'], c['mb'], c['st']) for c, _ in table]
+    assert all(ms > 0 for _, ms in table)
+    g = sw.capture(mode='direct', rtol=1e-10)
+    sw.u.zero_()
+    g.replay()
+    torch.cuda.synchronize()
+    assert torch.equal(sw.u, ref)
+
+def test_direct_panel_narrower_than_band_is_refused():
+    """A 60 x 50 plate (bandwidth ~312 > 256): the 256-wide panel refuses the
+    system (counted in band_bad, band untouched, kernel exits normally);
+    the 448-wide panel solves it."""
+    from pyfe3d_amd.sweep import PlateSweep
+    sw = PlateSweep([(60, 50)])
+    h, f = sw.random_designs(1)
+    sw.set_thickness(h)
+    sw.set_loads(f)
+    sw.assemble()
+    sw._band_setup()
+    assert int(sw.band_bw.max()) > 256
+    sw.band_bad.zero_()
+    sw.solve_direct(rtol=1.0, polish_iter=0, config=(1024, 256, 1024))
+    torch.cuda.synchronize()
+    assert int(sw.band_bad.item()) == 1
+    sw.band_bad.zero_()
+    sw.solve_direct(rtol=1e-8, polish_iter=0, config=(512, 448, 512))
+    torch.cuda.synchronize()
+    assert int(sw.band_bad.item()) == 0
+    assert float(sw.relres.max()) <= 1e-8
