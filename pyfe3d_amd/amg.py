@@ -282,18 +282,20 @@ class SAAMG:
         _axpby(1., self.g_out, 0., z)
         return z
 
-    def _cheb(self, L, b, x):
+    def _cheb(self, L, b, x, x_zero=False):
         """x += Chebyshev(D^-1 A) correction for A x = b (degree k, bounds
-        [lower, 1.1] lambda_max)."""
+        [lower, 1.1] lambda_max); x_zero: x is known to be 0 (r = b, no
+        product)."""
         lmax = 1.1 * L['lam']
         lmin = self.cheb_lower * L['lam']
         theta, delta = 0.5 * (lmax + lmin), 0.5 * (lmax - lmin)
         sigma = theta / delta
         rho_old = 1. / sigma
         r, d, t = L['r'], L['d'], L['t']
-        L['A'](x, t)
         _axpby(1., b, 0., r)
-        _axpby(-1., t, 1., r)                      # r = b - A x
+        if not x_zero:
+            L['A'](x, t)
+            _axpby(-1., t, 1., r)                  # r = b - A x
         L['Dinv'](r, d)
         _axpby(0., d, 1. / theta, d)               # d = D^-1 r / theta
         for k in range(self.cheb_degree):
@@ -322,7 +324,7 @@ class SAAMG:
             return x
         L = self.levels[lvl]
         _axpby(0., b, 0., x)                       # x = 0
-        self._cheb(L, b, x)
+        self._cheb(L, b, x, x_zero=True)
         r, t = L['r'], L['t']
         L['A'](x, t)
         _axpby(1., b, 0., r)

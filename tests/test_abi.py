@@ -64,3 +64,51 @@ def test_argument_errors_need_no_device():
     st = lib.pf3d_shell_kc0(0, 10, None, None, None, None, None, None, None, None, None,
                             None, None, None, 0, None)
     assert st == -1 and b'null' in lib.pf3d_last_error()
+
+
+def test_argument_errors_of_newer_entry_points():
+    """Mixed-type, rod, beam, piston and solver entry points reject bad
+    arguments before any device work (no GPU needed)."""
+    import ctypes
+    from pyfe3d_amd.assembly import _CGroup
+    lib = _lib.load()
+    call = lambda name, *a: getattr(lib, name)(*a)
+    fake = ctypes.c_void_p(16)  # never dereferenced: the checks fail first
+    # mixed: bad matrix, too many groups, repeated element type, BeamC without vxy
+    g = (_CGroup * 2)()
+    g[0] = _CGroup(0, 4, fake, 0, None, None, None, fake, None, None, None)
+    g[1] = _CGroup(0, 4, fake, 0, None, None, None, fake, None, None, None)
+    gp = ctypes.cast(g, ctypes.c_void_p)
+    assert call('pf3d_mixed_assemble', 2, 0, 1, gp, fake, None, fake, fake, fake, 0, None) == -1
+    assert b'KC0, KG or M' in lib.pf3d_last_error()
+    assert call('pf3d_mixed_assemble', 0, 0, 5, gp, fake, None, fake, fake, fake, 0, None) == -1
+    assert call('pf3d_mixed_assemble', 0, 0, 2, gp, fake, None, fake, fake, fake, 0, None) == -1
+    assert b'repeated' in lib.pf3d_last_error()
+    g[1] = _CGroup(3, 4, fake, 0, None, None, None, fake, None, None, None)
+    assert call('pf3d_mixed_assemble', 0, 0, 2, gp, fake, None, fake, fake, fake, 0, None) == -1
+    assert b'vxy' in lib.pf3d_last_error()
+    assert call('pf3d_mixed_assemble', 1, 0, 1, gp, fake, None, fake, fake, fake, 0, None) == -1
+    assert b'null u' in lib.pf3d_last_error()
+    # piston theory is Quad4/Quad4R only
+    assert call('pf3d_shell_piston', 2, 4, 4, fake, fake, None, fake, fake, fake, 0, None) == -1
+    assert call('pf3d_shell_piston', 0, 3, 4, fake, fake, None, fake, fake, fake, 0, None) == -1
+    # rods: springs need axes, Truss mass type range
+    assert call('pf3d_spring_kc0', 4, fake, fake, None, None, None, fake, fake, fake, fake, 0,
+                None) == -1
+    assert call('pf3d_truss_m', 4, fake, fake, None, None, fake, 2, fake, fake, fake, 0,
+                None) == -1
+    assert call('pf3d_rod_fe', 3, 4, fake, fake, None, None, fake, fake, fake, None) == -1
+    # BeamLR mass type range
+    assert call('pf3d_beamlr_m', 4, fake, fake, None, fake, None, fake, 2, fake, fake, fake, 0,
+                None) == -1
+    # banded solver launch configuration
+    assert call('pf3d_band_factor', 1, 1, None, fake, fake, fake, 10, fake, fake, fake, fake,
+                fake, fake, fake, 128, 0, None) == -1
+    assert b'threads' in lib.pf3d_last_error()
+    assert call('pf3d_band_factor', 1, 1, None, fake, fake, fake, 10, fake, fake, fake, fake,
+                fake, fake, fake, 1024, 300, None) == -1
+    assert call('pf3d_band_solve', 1, None, fake, fake, fake, fake, fake, fake, fake, fake, 100,
+                None) == -1
+    # mixed-precision PCG needs every operand and a positive replacement period
+    assert call('pf3d_pcg_mixed', 4, fake, fake, fake, fake, fake, fake, fake, fake, 1e-8, 10, 0,
+                0, fake, 1 << 30, fake, fake, None) == -1
