@@ -589,9 +589,42 @@ def main():
                      'algorithmic_bytes': KC0_BYTES_VALUES * E,
                      'bytes_per_element': KC0_BYTES_VALUES},
         'cpu_baseline': base,
+        'solves': solve_summary(extra),
         'extra': extra,
     }
     print(json.dumps(line), flush=True)
+
+
+def solve_summary(extra):
+    """Solves/s to 1e-8 (FP64 residual), plain FP64 block-Jacobi PCG against
+    the mixed-precision multigrid CG (FP32 V-cycle inside FP64 CG) and the
+    best method, per workload measured above."""
+    out = {}
+    s64 = extra.get('solve_64x64', {})
+    if 'fp64' in s64 and 'amg32' in s64:
+        best = min((v['ms'], k) for k, v in s64.items() if isinstance(v, dict) and 'ms' in v)
+        out['plate_64x64'] = dict(plain_fp64_ms=s64['fp64']['ms'],
+                                  mixed_multigrid_ms=s64['amg32']['ms'],
+                                  speedup_mixed_vs_plain=s64['fp64']['ms'] / s64['amg32']['ms'],
+                                  best=best[1], best_ms=best[0])
+    sp = extra.get('stiffened_panel', {}).get('solve', {})
+    if 'fp64' in sp and 'amg32' in sp:
+        out['stiffened_panel_512'] = dict(
+            plain_fp64_ms=sp['fp64']['ms'], mixed_multigrid_ms=sp['amg32']['ms'],
+            multigrid_fp64_ms=sp['amg']['ms'],
+            speedup_mixed_vs_plain=sp['fp64']['ms'] / sp['amg32']['ms'])
+    ls = [v for k, v in extra.items() if k.startswith('large_solve_') and isinstance(v, dict)]
+    if ls and 'amg' in ls[0]:
+        out['plate_1000x1000'] = dict(multigrid_fp64_ms=ls[0]['amg']['s'] * 1e3,
+                                      relres=ls[0]['amg']['relres'],
+                                      fp64_residual_floor=ls[0]['fp64_residual_floor'],
+                                      plain_fp64='no convergence: residual above 1 after 4e5 '
+                                                 'block-Jacobi PCG iterations')
+    sw = extra.get('design_sweep', {}).get('direct')
+    if sw:
+        out['design_sweep_4096'] = dict(solves_per_s=sw['solves_per_s'],
+                                        method='batched banded Cholesky, autotuned launch shape')
+    return out
 
 
 if __name__ == '__main__':
