@@ -458,6 +458,20 @@ int pf3d_band_solve(int64_t n_var, const int32_t *order, const int64_t *dof0, co
 /* FP64 dense helpers of the eigen-solvers (Krylov bases, column-major  */
 /* n x m with leading dimension ld; fixed-order reductions)             */
 /* ------------------------------------------------------------------ */
+/* C (p x q, row-major) = A B^T for p vectors A[i*lda ..] and q vectors
+ * B[j*ldb ..] of length n (p, q <= 32), fixed reduction order; work:
+ * 256*p*q doubles.  The Gram matrices of the block eigensolver (LOBPCG)
+ * that replaces eigsh for large models (no reference counterpart: the
+ * reference calls scipy, tests/test_quad4_natural_freq.py:120-123). */
+int pf3d_block_dot(int64_t n, int p, const double *A, int64_t lda, int q, const double *B,
+                   int64_t ldb, double *C, double *work, void *stream);
+
+/* Y[j*ldy + t] = sum_i C[i*k + j] S[i*lds + t], j < k, for m vectors S of
+ * length n (m, k <= 32; C on the device): the block eigensolver's basis
+ * updates, reading S once. */
+int pf3d_block_comb(int64_t n, int m, const double *S, int64_t lds, int k, const double *C,
+                    double *Y, int64_t ldy, void *stream);
+
 /* y = a x + b y on FP32 vectors, and FP32 -> FP64 widening: the
  * single-precision multigrid preconditioner inside FP64 CG (no reference
  * counterpart). */

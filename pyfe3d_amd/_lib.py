@@ -60,6 +60,8 @@ SIGNATURES = {
     'pf3d_gemv_n': [I64, I32, P, I64, P, P, F64, F64, P],
     'pf3d_axpby': [I64, F64, P, F64, P, P],
     'pf3d_dot': [I64, P, P, P, P, P],
+    'pf3d_block_dot': [I64, I32, P, I64, I32, P, I64, P, P, P],
+    'pf3d_block_comb': [I64, I32, P, I64, I32, P, P, I64, P],
     'pf3d_axpby_f32': [I64, F32, P, F32, P, P],
     'pf3d_to_f64': [I64, P, P, P],
     'pf3d_bjacobi': [I64, P, P, P, P, P, P, P],

@@ -102,9 +102,114 @@ __global__ void __launch_bounds__(256) dot_finish(int nparts, const double *__re
     if (threadIdx.x == 0) out[0] = sh[0];
 }
 
+// C = A B^T for p vectors A[i][0:n] (row stride lda) and q vectors
+// B[j][0:n] (row stride ldb), p, q <= 32: the Gram matrices of a block
+// eigensolver.  Stage 1: block (g, ti, tj) accumulates a 4 x 4 tile over
+// the g-th stride of t in registers and reduces it over its threads;
+// stage 2 sums the BD_G partials of each entry in index order.
+constexpr int BD_G = 256;
+
+__global__ void __launch_bounds__(256) block_dot_partial(int64_t n, int p, const double *A,
+                                                         int64_t lda, int q, const double *B,
+                                                         int64_t ldb, double *part)
+{
+    __shared__ double sh[16][256];
+    const int g = blockIdx.x, i0 = 4 * blockIdx.y, j0 = 4 * blockIdx.z;
+    double acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = 0.;
+    for (int64_t t = (int64_t)g * 256 + threadIdx.x; t < n; t += (int64_t)BD_G * 256) {
+        double x[4], y[4];
+#pragma unroll
+        for (int a = 0; a < 4; a++) x[a] = i0 + a < p ? A[(int64_t)(i0 + a) * lda + t] : 0.;
+#pragma unroll
+        for (int b = 0; b < 4; b++) y[b] = j0 + b < q ? B[(int64_t)(j0 + b) * ldb + t] : 0.;
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) acc[a][b] += x[a] * y[b];
+    }
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) sh[4 * a + b][threadIdx.x] = acc[a][b];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o)
+#pragma unroll
+            for (int e = 0; e < 16; e++) sh[e][threadIdx.x] += sh[e][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x < 16) {
+        const int a = threadIdx.x / 4, b = threadIdx.x % 4;
+        if (i0 + a < p && j0 + b < q)
+            part[((int64_t)g * p + i0 + a) * q + j0 + b] = sh[threadIdx.x][0];
+    }
+}
+
+__global__ void __launch_bounds__(1024) block_dot_finish(int p, int q, const double *part,
+                                                         double *C)
+{
+    const int e = threadIdx.x;
+    if (e >= p * q) return;
+    double s = 0.;
+    for (int g = 0; g < BD_G; g++) s += part[(int64_t)g * p * q + e];
+    C[e] = s;
+}
+
+// Y[j][t] = sum_i C[i][j] S[i][t]: k combinations of m vectors, C (m x k,
+// row-major) staged in LDS, one thread per t (S read once)
+__global__ void __launch_bounds__(256) block_comb(int64_t n, int m, const double *S, int64_t lds,
+                                                  int k, const double *C, double *Y, int64_t ldy)
+{
+    __shared__ double cs[32 * 32];
+    for (int e = threadIdx.x; e < m * k; e += 256) cs[e] = C[e];
+    __syncthreads();
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    double s[32];
+#pragma unroll
+    for (int i = 0; i < 32; i++) s[i] = i < m ? S[(int64_t)i * lds + t] : 0.;
+    for (int j = 0; j < k; j++) {
+        double y = 0.;
+#pragma unroll
+        for (int i = 0; i < 32; i++)
+            if (i < m) y += cs[i * k + j] * s[i];
+        Y[(int64_t)j * ldy + t] = y;
+    }
+}
+
 }  // namespace
 
 extern "C" {
+
+int pf3d_block_comb(int64_t n, int m, const double *S, int64_t lds, int k, const double *C,
+                    double *Y, int64_t ldy, void *stream)
+{
+    if (n < 0 || m < 1 || k < 1 || m > 32 || k > 32 || lds < n || ldy < n || !S || !C || !Y)
+        return pf3d_fail(PF3D_E_ARG, "pf3d_block_comb: bad arguments");
+    if (!n) return PF3D_OK;
+    hipLaunchKernelGGL(block_comb, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, n, m, S, lds, k, C, Y, ldy);
+    return pf3d_hip_check(hipGetLastError(), "pf3d_block_comb");
+}
+
+
+int pf3d_block_dot(int64_t n, int p, const double *A, int64_t lda, int q, const double *B,
+                   int64_t ldb, double *C, double *work, void *stream)
+{
+    if (n < 0 || p < 1 || q < 1 || p > 32 || q > 32 || lda < n || ldb < n || !A || !B || !C ||
+        !work)
+        return pf3d_fail(PF3D_E_ARG, "pf3d_block_dot: bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(block_dot_partial, dim3(BD_G, (p + 3) / 4, (q + 3) / 4), dim3(256), 0, s,
+                       n, p, A, lda, q, B, ldb, work);
+    hipLaunchKernelGGL(block_dot_finish, dim3(1), dim3(1024), 0, s, p, q, work, C);
+    return pf3d_hip_check(hipGetLastError(), "pf3d_block_dot");
+}
+
 
 int pf3d_axpby_f32(int64_t n, float a, const float *x, float b, float *y, void *stream)
 {
