@@ -334,6 +334,16 @@ int pf3d_block_apply(int64_t nnzb, const int64_t *src_ptr, const int64_t *src, i
 /* tests/test_quad4_static_point_load.py:84-102)                         */
 /* ------------------------------------------------------------------ */
 
+/* r = b - A x (or r = A x when b is NULL) with compensated accumulation
+ * (FMA-split products, TwoSum): accurate to ~eps of the result instead of
+ * ~eps |A| |x|, which matters where the row sums cancel by many digits
+ * (shell stiffness: membrane and shear against bending and drilling).
+ * The true residuals of the multigrid CG and the products of the block
+ * eigensolver use it (no reference counterpart). */
+int pf3d_bsr_residual_dd(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx,
+                         const double *val, const double *x, const double *b, double *r,
+                         void *stream);
+
 /* y = K x for a 6x6 BSR matrix with nb block rows (FP64 / FP32). */
 int pf3d_bsr_spmv(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx, const double *val,
                   const double *x, double *y, void *stream);

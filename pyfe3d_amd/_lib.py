@@ -53,6 +53,7 @@ SIGNATURES = {
     'pf3d_block_apply': [I64, P, P, I32, P, P, I32, P],
     'pf3d_bsr_spmv': [I64, P, P, P, P, P, P],
     'pf3d_bsr_spmv_f32': [I64, P, P, P, P, P, P],
+    'pf3d_bsr_residual_dd': [I64, P, P, P, P, P, P, P],
     'pf3d_bsr_dirichlet': [I64, P, P, P, P, P],
     'pf3d_to_f32': [I64, P, P, P],
     'pf3d_bsr_dirichlet_diag': [I64, P, P, P, P, F64, P],

@@ -63,6 +63,63 @@ __global__ void __launch_bounds__(256) bsr_spmv(int64_t nb, const int32_t *__res
     }
 }
 
+// ------------------------------------------------- compensated products
+// Row sums of shell stiffness matrices cancel by many digits (stiff
+// membrane and shear terms against soft bending and drilling terms), so
+// b - A x in plain FP64 carries an error of ~eps |A| |x| that can exceed
+// 1e-8 |b|.  These kernels accumulate with error-free transformations
+// (product split by FMA, TwoSum for the running sum, compensation terms
+// summed apart): the result is accurate to ~eps of itself.
+__device__ __forceinline__ void two_sum(double &s, double &c, double p)
+{
+#pragma clang fp contract(off)
+    const double t = s + p;
+    const double bp = t - s;
+    c += (s - (t - bp)) + (p - bp);
+    s = t;
+}
+
+// r = b - A x (b may be null: r = -A x; sign = +1 gives r = A x when b is null)
+__global__ void __launch_bounds__(256) bsr_residual_dd(int64_t nb, const int32_t *__restrict__ rp,
+                                                       const int32_t *__restrict__ ci,
+                                                       const double *__restrict__ val,
+                                                       const double *__restrict__ x,
+                                                       const double *__restrict__ b, double sign,
+                                                       double *__restrict__ y)
+{
+#pragma clang fp contract(off)
+    __shared__ double ps[4][64], pc[4][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * 4 + wave;
+    const int j = lane / 6, i = lane - 6 * (lane / 6);
+    double s = 0., c = 0.;
+    if (row < nb && j < 10) {
+        const int b0 = rp[row], b1 = rp[row + 1];
+        for (int k = b0 + j; k < b1; k += 10) {
+            const double *v = val + (int64_t)k * 36 + i * 6;
+            const double *xv = x + (int64_t)ci[k] * 6;
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                const double p = sign * v[q] * xv[q];
+                const double e = fma(sign * v[q], xv[q], -p);  // exact: sign is +-1
+                c += e;
+                two_sum(s, c, p);
+            }
+        }
+    }
+    ps[wave][lane] = s;
+    pc[wave][lane] = c;
+    __syncthreads();
+    if (row < nb && lane < 6) {
+        double S = b ? b[row * 6 + lane] : 0., C = 0.;
+        for (int jj = 0; jj < 10; jj++) {
+            two_sum(S, C, ps[wave][lane + 6 * jj]);
+            C += pc[wave][lane + 6 * jj];
+        }
+        y[row * 6 + lane] = S + C;
+    }
+}
+
 // ------------------------------------------------------ boundary conditions
 // K <- P K P + (I - P) with P = diag(1 - fixed)
 __global__ void bsr_bc(int64_t nb, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
@@ -474,6 +531,19 @@ int pf3d_bsr_spmv(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx, co
     if (!nb) return PF3D_OK;
     Mat A{nb, row_ptr, col_idx, val, nullptr};
     return pf3d_hip_check(spmv<double, double>(A, x, y, (hipStream_t)stream), "pf3d_bsr_spmv");
+}
+
+int pf3d_bsr_residual_dd(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx,
+                         const double *val, const double *x, const double *b, double *r,
+                         void *stream)
+{
+    if (nb < 0 || (nb && (!row_ptr || !col_idx || !val || !x || !r)))
+        return pf3d_fail(PF3D_E_ARG, "pf3d_bsr_residual_dd: bad arguments");
+    if (!nb) return PF3D_OK;
+    // b given: r = b - A x ; b null: r = A x
+    hipLaunchKernelGGL(bsr_residual_dd, dim3(nblk(nb, 4)), dim3(256), 0, (hipStream_t)stream, nb,
+                       row_ptr, col_idx, val, x, b, b ? -1.0 : 1.0, r);
+    return pf3d_hip_check(hipGetLastError(), "pf3d_bsr_residual_dd");
 }
 
 int pf3d_bsr_spmv_f32(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx,

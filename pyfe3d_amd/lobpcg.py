@@ -52,6 +52,14 @@ class LOBPCG:
         _lib.call('pf3d_bsr_spmv', self.plan.n_nodes, self.plan.row_ptr.data_ptr(),
                   self.plan.col_idx.data_ptr(), val.data_ptr(), x.data_ptr(), y.data_ptr(), _s())
 
+    def _kmul(self, x, y):
+        """y = K x with compensated row sums: for the lowest modes K x is
+        ~lambda M x, many digits below |K| |x|, and the Rayleigh quotients
+        need it accurate to ~eps of itself."""
+        _lib.call('pf3d_bsr_residual_dd', self.plan.n_nodes, self.plan.row_ptr.data_ptr(),
+                  self.plan.col_idx.data_ptr(), self.K.data_ptr(), x.data_ptr(), None,
+                  y.data_ptr(), _s())
+
     def _gram(self, A, B, out_host=True):
         p, q = A.shape[0], B.shape[0]
         C = torch.empty(p * q, dtype=torch.float64, device=A.device)
@@ -102,7 +110,7 @@ class LOBPCG:
         BS = torch.zeros_like(S)
         S[:kb] = X
         for i in range(kb):
-            self._spmv(self.K, S[i], AS[i])
+            self._kmul(S[i], AS[i])
             self._spmv(self.M, S[i], BS[i])
         m = kb          # active vectors in S
         R = torch.empty((kb, n), dtype=torch.float64, device=dev)
@@ -149,7 +157,7 @@ class LOBPCG:
                 # zero on fixed DOFs: R is, and the V-cycle keeps them
                 # decoupled (identity rows, zero prolongator rows)
                 self.precond.apply(R[i], S[kb + i])
-                self._spmv(self.K, S[kb + i], AS[kb + i])
+                self._kmul(S[kb + i], AS[kb + i])
                 self._spmv(self.M, S[kb + i], BS[kb + i])
             if m > kb:
                 S[2 * kb:], AS[2 * kb:], BS[2 * kb:] = Pn, APn, BPn

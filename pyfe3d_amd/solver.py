@@ -52,6 +52,25 @@ class StaticSystem:
         _lib.call('pf3d_ir_bytes', nb, ctypes.byref(nbytes))
         self._wir = torch.empty(nbytes.value, dtype=torch.uint8, device=dev)
 
+    def residual(self, b, x, r=None, stream=None):
+        """r = b - K x with compensated accumulation (pf3d_bsr_residual_dd):
+        accurate to ~eps of r itself, not ~eps |K| |x|."""
+        if r is None:
+            r = torch.empty_like(b)
+        _lib.call('pf3d_bsr_residual_dd', self.nb, self.plan.row_ptr.data_ptr(),
+                  self.plan.col_idx.data_ptr(), self.val.data_ptr(), x.data_ptr(), b.data_ptr(),
+                  r.data_ptr(), _s(stream))
+        return r
+
+    def kmul_accurate(self, x, y=None, stream=None):
+        """y = K x with compensated row sums (pf3d_bsr_residual_dd, b NULL)."""
+        if y is None:
+            y = torch.empty_like(x)
+        _lib.call('pf3d_bsr_residual_dd', self.nb, self.plan.row_ptr.data_ptr(),
+                  self.plan.col_idx.data_ptr(), self.val.data_ptr(), x.data_ptr(), None,
+                  y.data_ptr(), _s(stream))
+        return y
+
     def matvec(self, x, y=None, stream=None):
         if y is None:
             y = torch.empty_like(x)
@@ -205,7 +224,7 @@ class StaticSystem:
             info = dict(iters=0, outer=0)
         else:
             raise ValueError(method)
-        # true FP64 residual on the FP64 matrix
-        r = b - self.matvec(x, stream=stream)
+        # true residual on the FP64 matrix (compensated products)
+        r = self.residual(b, x, stream=stream)
         info['relres'] = float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(b))
         return x, info
