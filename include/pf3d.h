@@ -344,6 +344,46 @@ int pf3d_bsr_residual_dd(int64_t nb, const int32_t *row_ptr, const int32_t *col_
                          const double *val, const double *x, const double *b, double *r,
                          void *stream);
 
+/* Smoothed-aggregation multigrid setup on the GPU (no reference
+ * counterpart: pyfe3d's callers factorise with SuperLU).  6x6 BSR, int32
+ * row pointers and block columns ascending within each row, FP64 values;
+ * fixed summation order (repeatable bits).
+ *
+ * C = X Y in two steps: the symbolic step writes C's row pointer (nr + 1)
+ * and returns the block count through *nnz (synchronises the stream; at
+ * most 128 block columns per row of C, else PF3D_E_ARG); the numeric step
+ * writes C's columns and values (nnz blocks). */
+int pf3d_bsr_spgemm_bytes(int64_t nr, int64_t *bytes);
+int pf3d_bsr_spgemm_symbolic(int64_t nr, const int32_t *x_row_ptr, const int32_t *x_col_idx,
+                             const int32_t *y_row_ptr, const int32_t *y_col_idx,
+                             int32_t *c_row_ptr, void *work, int64_t work_bytes, int64_t *nnz,
+                             void *stream);
+int pf3d_bsr_spgemm_numeric(int64_t nr, const int32_t *x_row_ptr, const int32_t *x_col_idx,
+                            const double *x_val, const int32_t *y_row_ptr,
+                            const int32_t *y_col_idx, const double *y_val,
+                            const int32_t *c_row_ptr, int32_t *c_col_idx, double *c_val,
+                            int64_t nnz, void *stream);
+/* T = X^T for X with nr block rows, nc block columns and nnzb blocks. */
+int pf3d_bsr_transpose_bytes(int64_t nr, int64_t nc, int64_t nnzb, int64_t *bytes);
+int pf3d_bsr_transpose(int64_t nr, int64_t nc, int64_t nnzb, const int32_t *row_ptr,
+                       const int32_t *col_idx, const double *val, int32_t *t_row_ptr,
+                       int32_t *t_col_idx, double *t_val, void *work, int64_t work_bytes,
+                       void *stream);
+/* P = P0 - omega D^-1 (A P0) on the structure of T = A P0 (values tv),
+ * where block row i of the tentative P0 is q[i] (36) at column agg[i] and
+ * dinv holds the inverted 6x6 diagonal blocks of A.  *missing counts rows
+ * of T without column agg[i] (P0's block would be lost). */
+int pf3d_amg_smooth_prolongator(int64_t nr, const int32_t *t_row_ptr, const int32_t *t_col_idx,
+                                const double *tv, const int32_t *agg, const double *q,
+                                const double *dinv, double omega, double *pv, int *missing,
+                                void *stream);
+/* out = (V + V^T) / 2 for a square BSR matrix of symmetric structure,
+ * plus 1 on the diagonal of each DOF with dead[dof] != 0 (dead may be
+ * NULL); *unpaired counts blocks whose transpose partner is absent. */
+int pf3d_bsr_symmetrize(int64_t nr, int64_t nnzb, const int32_t *row_ptr, const int32_t *col_idx,
+                        const double *val, const uint8_t *dead, double *out, int *unpaired,
+                        void *stream);
+
 /* y = K x for a 6x6 BSR matrix with nb block rows (FP64 / FP32). */
 int pf3d_bsr_spmv(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx, const double *val,
                   const double *x, double *y, void *stream);

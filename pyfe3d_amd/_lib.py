@@ -54,6 +54,13 @@ SIGNATURES = {
     'pf3d_bsr_spmv': [I64, P, P, P, P, P, P],
     'pf3d_bsr_spmv_f32': [I64, P, P, P, P, P, P],
     'pf3d_bsr_residual_dd': [I64, P, P, P, P, P, P, P],
+    'pf3d_bsr_spgemm_bytes': [I64, P],
+    'pf3d_bsr_spgemm_symbolic': [I64, P, P, P, P, P, P, I64, P, P],
+    'pf3d_bsr_spgemm_numeric': [I64, P, P, P, P, P, P, P, P, P, I64, P],
+    'pf3d_bsr_transpose_bytes': [I64, I64, I64, P],
+    'pf3d_bsr_transpose': [I64, I64, I64, P, P, P, P, P, P, P, I64, P],
+    'pf3d_amg_smooth_prolongator': [I64, P, P, P, P, P, P, F64, P, P, P],
+    'pf3d_bsr_symmetrize': [I64, I64, P, P, P, P, P, P, P],
     'pf3d_bsr_dirichlet': [I64, P, P, P, P, P],
     'pf3d_to_f32': [I64, P, P, P],
     'pf3d_bsr_dirichlet_diag': [I64, P, P, P, P, F64, P],

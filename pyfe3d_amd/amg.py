@@ -3,16 +3,20 @@ the Dirichlet-treated 6-DOF-per-node stiffness (a stronger preconditioner
 than block Jacobi for the static solve that pyfe3d's callers run with
 ``spsolve``/``cg``, tests/test_quad4_static_point_load.py:84-102).
 
-Setup (host, numpy/scipy, once per matrix):
-  * geometric aggregation: nodes (then aggregates, by centroid) grouped in
-    boxes of ``box`` times the typical edge length;
-  * near-null space: the six rigid-body modes of each node (translations,
-    rotations about the centroid, with the rotational DOFs themselves),
-    zero on fixed DOFs;
-  * tentative prolongator per aggregate by a rank-revealing orthonormal
-    factorisation (SVD) of the aggregate's modes, coarse modes = its R;
-  * smoothed prolongator P = (I - w D^-1 A) P0 with w = 4/3 / lambda_max
-    (power iteration), Galerkin coarse operator P^T A P, all in 6x6 BSR.
+Setup (once per matrix):
+  * host, O(nodes): geometric aggregation (nodes, then aggregates by
+    centroid, grouped in boxes of ``box`` times the typical edge length);
+    the near-null space (the six rigid-body modes of each node:
+    translations, rotations about the centroid with the rotational DOFs
+    themselves, zero on fixed DOFs); the tentative prolongator's blocks, by
+    a rank-revealing orthonormal factorisation (SVD) of each aggregate's
+    modes (coarse modes = its R);
+  * GPU, 6x6-block sparse kernels: inverted diagonal blocks
+    (pf3d_bjacobi), lambda_max(D^-1 A) by power iteration, A P0
+    (pf3d_bsr_spgemm_*), the smoothed prolongator P = P0 - w D^-1 A P0 with
+    w = 4/3 / lambda_max (pf3d_amg_smooth_prolongator), R = P^T
+    (pf3d_bsr_transpose), the Galerkin operator R (A P), symmetrised with
+    the coarse DOFs that carry no mode set to identity (pf3d_bsr_symmetrize).
 Solve (GPU, this library's kernels only): PCG whose preconditioner is one
 symmetric V-cycle with Chebyshev(D^-1 A) pre/post smoothing, restriction
 P^T and prolongation P as 6x6-block sparse products (pf3d_bsr_spmv), and
@@ -21,6 +25,7 @@ pf3d_dot, vector updates with pf3d_axpby.  With precision 'fp32' the whole
 V-cycle runs in single precision (pf3d_bsr_spmv_f32, pf3d_axpby_f32) and
 the CG around it, in FP64, uses the flexible (Polak-Ribiere) beta.
 """
+import ctypes
 import time
 
 import numpy as np
@@ -32,11 +37,6 @@ from . import _lib
 
 def _s():
     return torch.cuda.current_stream().cuda_stream
-
-
-def _bsr_host(row_ptr, col_idx, val, n_rows, n_cols):
-    return ss.bsr_matrix((val.reshape(-1, 6, 6), col_idx, row_ptr),
-                         shape=(6 * n_rows, 6 * n_cols))
 
 
 def rigid_body_modes(xyz, fixed):
@@ -68,8 +68,10 @@ def box_aggregates(xyz, size):
 
 
 def tentative(agg, n_agg, B, tol=1e-10):
-    """P0 (BSR, fine nodes x aggregates) with orthonormal columns per
-    aggregate and the coarse near-null space Bc (6 n_agg x 6)."""
+    """Blocks of the tentative prolongator P0 (block row i = Q[i] at block
+    column agg[i]; orthonormal columns per aggregate), the coarse
+    near-null space Bc (6 n_agg x 6) and the coarse DOFs without a mode
+    (dead: bool, 6 n_agg)."""
     n = agg.size
     order = np.argsort(agg, kind='stable')
     counts = np.bincount(agg, minlength=n_agg)
@@ -85,63 +87,28 @@ def tentative(agg, n_agg, B, tol=1e-10):
     U = U * keep[:, None, :]
     R = (S * keep)[:, :, None] * Vt                 # (n_agg, 6, 6): U R = Bp
     Q = U.reshape(n_agg, m, 6, 6)
-    blocks = Q[agg, pos]                            # (n, 6, 6) block of node i
-    P0 = ss.bsr_matrix((blocks, agg, np.arange(n + 1)), shape=(6 * n, 6 * n_agg))
-    dead = ~keep.reshape(-1)                        # coarse DOFs without a mode
-    return P0, R.reshape(6 * n_agg, 6), dead
-
-
-def _block_diag_inv(A):
-    """Inverses of the 6x6 diagonal blocks of a square 6x6-BSR matrix."""
-    n = A.shape[0] // 6
-    A = A.tobsr(blocksize=(6, 6))
-    A.sort_indices()
-    D = np.zeros((n, 6, 6))
-    rows = np.repeat(np.arange(n), np.diff(A.indptr))
-    diag = rows == A.indices
-    D[rows[diag]] = A.data[diag]
-    bad = np.abs(np.linalg.det(D)) == 0
-    D[bad] = np.eye(6)
-    return np.linalg.inv(D)
-
-
-def _lambda_max(A, Dinv, iters=20, seed=0):
-    """Power iteration on D^-1 A (host)."""
-    n = A.shape[0]
-    x = np.random.default_rng(seed).uniform(-1, 1, n)
-    lam = 1.
-    for _ in range(iters):
-        y = A @ x
-        y = np.einsum('nij,nj->ni', Dinv, y.reshape(-1, 6)).ravel()
-        lam = np.linalg.norm(y) / np.linalg.norm(x)
-        x = y / np.linalg.norm(y)
-    return lam
+    return np.ascontiguousarray(Q[agg, pos]), R.reshape(6 * n_agg, 6), ~keep.reshape(-1)
 
 
 class _Dev:
-    """A 6x6-BSR operator on the GPU (rows x cols in nodes), values and
-    vectors in FP64 or FP32."""
+    """The operator y = M x of a _Bsr matrix M on the GPU, with values and
+    vectors in FP64 or FP32 (an FP32 copy of the values)."""
 
-    def __init__(self, M, dev, dtype=torch.float64):
-        M = M.tobsr(blocksize=(6, 6))
-        M.sort_indices()
-        self.nb = M.shape[0] // 6
-        self.row_ptr = torch.as_tensor(M.indptr.astype(np.int32), device=dev)
-        self.col_idx = torch.as_tensor(M.indices.astype(np.int32), device=dev)
-        self.val = torch.as_tensor(np.ascontiguousarray(M.data.reshape(-1)), device=dev,
-                                   dtype=dtype)
-        self.fn = 'pf3d_bsr_spmv' if dtype == torch.float64 else 'pf3d_bsr_spmv_f32'
+    def __init__(self, M, dtype=torch.float64):
+        self.nb = M.nr
+        self.row_ptr, self.col_idx = M.row_ptr, M.col_idx
+        if dtype == torch.float64:
+            self.val = M.val
+            self.fn = 'pf3d_bsr_spmv'
+        else:
+            self.val = torch.empty(M.val.numel(), dtype=torch.float32, device=M.val.device)
+            _lib.call('pf3d_to_f32', M.val.numel(), M.val.data_ptr(), self.val.data_ptr(), _s())
+            self.fn = 'pf3d_bsr_spmv_f32'
 
     def __call__(self, x, y):
         _lib.call(self.fn, self.nb, self.row_ptr.data_ptr(), self.col_idx.data_ptr(),
                   self.val.data_ptr(), x.data_ptr(), y.data_ptr(), _s())
         return y
-
-
-def _diag_dev(Dinv, dev, dtype=torch.float64):
-    n = Dinv.shape[0]
-    return _Dev(ss.bsr_matrix((Dinv, np.arange(n), np.arange(n + 1)), shape=(6 * n, 6 * n)), dev,
-                dtype)
 
 
 def _axpby(a, x, b, y):
@@ -152,48 +119,152 @@ def _axpby(a, x, b, y):
         _lib.call('pf3d_axpby', y.numel(), float(a), x.data_ptr(), float(b), y.data_ptr(), _s())
 
 
-def setup_levels(A, fixed, xyz, box=3.0, coarse_nodes=300, max_levels=8, log=None):
-    """Host setup: list of levels dict(A, Dinv, P, lam) (scipy CSR, block
-    inverses) and the coarsest matrix (dense)."""
+class _Bsr:
+    """6x6 BSR matrix on the GPU: nr x nc blocks, int32 row pointers and
+    ascending block columns, FP64 values (36 per block, row-major)."""
+
+    def __init__(self, nr, nc, row_ptr, col_idx, val):
+        self.nr, self.nc = int(nr), int(nc)
+        self.row_ptr, self.col_idx, self.val = row_ptr, col_idx, val
+
+    @property
+    def nnzb(self):
+        return self.col_idx.numel()
+
+    def to_scipy(self):
+        return ss.bsr_matrix((self.val.cpu().numpy().reshape(-1, 6, 6), self.col_idx.cpu().numpy(),
+                              self.row_ptr.cpu().numpy()), shape=(6 * self.nr, 6 * self.nc))
+
+
+def _work(name, dev, *args):
+    nbytes = ctypes.c_int64()
+    _lib.call(name, *args, ctypes.byref(nbytes))
+    return torch.empty(nbytes.value, dtype=torch.uint8, device=dev), nbytes.value
+
+
+def spgemm(X, Y):
+    """C = X Y (pf3d_bsr_spgemm_symbolic, then pf3d_bsr_spgemm_numeric)."""
+    if X.nc != Y.nr:
+        raise ValueError('spgemm: inner sizes %d and %d differ' % (X.nc, Y.nr))
+    dev = X.val.device
+    work, nbytes = _work('pf3d_bsr_spgemm_bytes', dev, X.nr)
+    cr = torch.empty(X.nr + 1, dtype=torch.int32, device=dev)
+    nnz = ctypes.c_int64()
+    _lib.call('pf3d_bsr_spgemm_symbolic', X.nr, X.row_ptr.data_ptr(), X.col_idx.data_ptr(),
+              Y.row_ptr.data_ptr(), Y.col_idx.data_ptr(), cr.data_ptr(), work.data_ptr(), nbytes,
+              ctypes.byref(nnz), _s())
+    del work
+    cc = torch.empty(nnz.value, dtype=torch.int32, device=dev)
+    cv = torch.empty(nnz.value * 36, dtype=torch.float64, device=dev)
+    _lib.call('pf3d_bsr_spgemm_numeric', X.nr, X.row_ptr.data_ptr(), X.col_idx.data_ptr(),
+              X.val.data_ptr(), Y.row_ptr.data_ptr(), Y.col_idx.data_ptr(), Y.val.data_ptr(),
+              cr.data_ptr(), cc.data_ptr(), cv.data_ptr(), nnz.value, _s())
+    return _Bsr(X.nr, Y.nc, cr, cc, cv)
+
+
+def transpose(X):
+    """X^T (pf3d_bsr_transpose)."""
+    dev = X.val.device
+    work, nbytes = _work('pf3d_bsr_transpose_bytes', dev, X.nr, X.nc, X.nnzb)
+    trp = torch.empty(X.nc + 1, dtype=torch.int32, device=dev)
+    tci = torch.empty(X.nnzb, dtype=torch.int32, device=dev)
+    tv = torch.empty_like(X.val)
+    _lib.call('pf3d_bsr_transpose', X.nr, X.nc, X.nnzb, X.row_ptr.data_ptr(), X.col_idx.data_ptr(),
+              X.val.data_ptr(), trp.data_ptr(), tci.data_ptr(), tv.data_ptr(), work.data_ptr(),
+              nbytes, _s())
+    return _Bsr(X.nc, X.nr, trp, tci, tv)
+
+
+def _block_jacobi(A):
+    """Inverted 6x6 diagonal blocks of A as a block-diagonal _Bsr."""
+    dev = A.val.device
+    dinv = torch.empty(A.nr * 36, dtype=torch.float64, device=dev)
+    bad = torch.zeros(1, dtype=torch.int32, device=dev)
+    _lib.call('pf3d_bjacobi', A.nr, A.row_ptr.data_ptr(), A.col_idx.data_ptr(), A.val.data_ptr(),
+              dinv.data_ptr(), None, bad.data_ptr(), _s())
+    idx = torch.arange(A.nr + 1, dtype=torch.int32, device=dev)
+    return _Bsr(A.nr, A.nr, idx, idx[:A.nr], dinv)
+
+
+def _lambda_max(A, D, iters=20, seed=0):
+    """Power iteration on D^-1 A from a seeded uniform(-1, 1) start."""
+    dev = A.val.device
+    x = torch.as_tensor(np.random.default_rng(seed).uniform(-1, 1, 6 * A.nr), device=dev)
+    t, y = torch.empty_like(x), torch.empty_like(x)
+    s = torch.empty(1025, dtype=torch.float64, device=dev)
+    opA, opD = _Dev(A), _Dev(D)
+    lam = 1.
+    for _ in range(iters):
+        opA(x, t)
+        opD(t, y)
+        ny = _dot(y, y, s) ** 0.5
+        lam = ny / _dot(x, x, s) ** 0.5
+        _axpby(1. / ny, y, 0., x)
+    return lam
+
+
+def setup_hierarchy(row_ptr, col_idx, val, n, fixed, xyz, box=3.0, coarse_nodes=300,
+                    max_levels=8, log=None):
+    """Multigrid hierarchy of the n-node matrix (row_ptr, col_idx, val)
+    (Dirichlet-treated; kept by reference as the finest level), built on
+    the GPU except for the aggregation and the tentative prolongator's
+    blocks.  Returns (levels, Ad): levels = list of dict(A, D, P, R, lam)
+    of _Bsr matrices (D: inverted diagonal blocks of A) and the coarsest
+    matrix, dense on the host."""
     t0 = time.perf_counter()
-    A = A.tocsr()
-    n = A.shape[0] // 6
+    dev = val.device
     xyz = np.asarray(xyz, dtype=np.float64).reshape(-1, 3)
     B = rigid_body_modes(xyz, fixed)
-    Ab = A.tobsr(blocksize=(6, 6))
-    rows = np.repeat(np.arange(n), np.diff(Ab.indptr))
-    off = rows != Ab.indices
-    h = np.median(np.linalg.norm(xyz[rows[off]] - xyz[Ab.indices[off]], axis=1))
-    size = box * h
+    rp, ci = row_ptr.cpu().numpy(), col_idx.cpu().numpy()
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    off = rows != ci
+    size = box * np.median(np.linalg.norm(xyz[rows[off]] - xyz[ci[off]], axis=1))
+    del rows, off, rp, ci
+    A = _Bsr(n, n, row_ptr, col_idx, val)
     levels = []
-    while A.shape[0] // 6 > coarse_nodes and len(levels) < max_levels - 1:
-        Dinv = _block_diag_inv(A)
-        lam = _lambda_max(A, Dinv)
+    flags = torch.zeros(2, dtype=torch.int32, device=dev)
+    while A.nr > coarse_nodes and len(levels) < max_levels - 1:
+        D = _block_jacobi(A)
+        lam = _lambda_max(A, D)
         agg, n_agg = box_aggregates(xyz, size)
-        if n_agg >= A.shape[0] // 6:
+        if n_agg >= A.nr:
             break
-        P0, Bc, dead = tentative(agg, n_agg, B)
-        nb = Dinv.shape[0]
-        Dm = ss.bsr_matrix((Dinv, np.arange(nb), np.arange(nb + 1)), shape=A.shape).tocsr()
-        P0 = P0.tocsr()
-        P = (P0 - (4. / 3. / lam) * (Dm @ (A @ P0))).tocsr()
-        P.eliminate_zeros()
-        Ac = (P.T @ (A @ P)).tocsr()
-        if dead.any():
-            d = np.zeros(Ac.shape[0]); d[dead] = 1.
-            Ac = Ac + ss.diags(d)
-        Ac = (0.5 * (Ac + Ac.T)).tocsr()
-        levels.append(dict(A=A, Dinv=Dinv, P=P, lam=lam))
-        cnt = np.bincount(agg, minlength=n_agg)[:, None]
+        Q, Bc, dead = tentative(agg, n_agg, B)
+        agg_d = torch.as_tensor(agg.astype(np.int32), device=dev)
+        P0 = _Bsr(A.nr, n_agg, torch.arange(A.nr + 1, dtype=torch.int32, device=dev), agg_d,
+                  torch.as_tensor(Q.reshape(-1), device=dev))
+        T = spgemm(A, P0)                       # A P0, then P = P0 - w D^-1 A P0
+        pv = torch.empty_like(T.val)
+        _lib.call('pf3d_amg_smooth_prolongator', A.nr, T.row_ptr.data_ptr(), T.col_idx.data_ptr(),
+                  T.val.data_ptr(), agg_d.data_ptr(), P0.val.data_ptr(), D.val.data_ptr(),
+                  4. / 3. / lam, pv.data_ptr(), flags.data_ptr(), _s())
+        P = _Bsr(A.nr, n_agg, T.row_ptr, T.col_idx, pv)
+        del T, P0
+        AP = spgemm(A, P)
+        R = transpose(P)
+        Ac = spgemm(R, AP)
+        del AP
+        acv = torch.empty_like(Ac.val)
+        dead_d = torch.as_tensor(dead.astype(np.uint8), device=dev)
+        _lib.call('pf3d_bsr_symmetrize', Ac.nr, Ac.nnzb, Ac.row_ptr.data_ptr(),
+                  Ac.col_idx.data_ptr(), Ac.val.data_ptr(), dead_d.data_ptr(), acv.data_ptr(),
+                  flags[1:].data_ptr(), _s())
+        missing, unpaired = flags.tolist()
+        if missing or unpaired:
+            raise _lib.Pf3dError('multigrid setup: %d prolongator rows without their aggregate, '
+                                 '%d coarse blocks without a transpose partner'
+                                 % (missing, unpaired))
+        Ac.val = acv
+        levels.append(dict(A=A, D=D, P=P, R=R, lam=lam))
         cen = np.zeros((n_agg, 3))
         np.add.at(cen, agg, xyz)
-        xyz = cen / cnt
+        xyz = cen / np.bincount(agg, minlength=n_agg)[:, None]
         A, B = Ac, Bc
         size *= box
         if log:
-            log('amg level %d: %d -> %d DOF (%.1f s)' % (len(levels) - 1, P.shape[0], P.shape[1],
+            log('amg level %d: %d -> %d DOF (%.1f s)' % (len(levels) - 1, 6 * P.nr, 6 * P.nc,
                                                            time.perf_counter() - t0))
-    return levels, A.toarray()
+    return levels, A.to_scipy().toarray()
 
 
 class SAAMG:
@@ -202,29 +273,27 @@ class SAAMG:
     DOFs); fixed: bool per DOF; xyz: (n_nodes, 3) node coordinates."""
 
     def __init__(self, plan, val, fixed, xyz, box=3.0, coarse_nodes=300, max_levels=8,
-                 cheb_degree=2, cheb_lower=1. / 30, precision='fp64', log=None, host=None):
+                 cheb_degree=2, cheb_lower=1. / 30, precision='fp64', log=None, hierarchy=None):
         """precision 'fp32': every level's matrices, smoother and transfer
         operators in single precision (half the bytes of the V-cycle); the
-        coarsest solve and the CG around it stay FP64.  host: the result of
-        setup_levels for this matrix, to share one setup between precisions."""
+        coarsest solve and the CG around it stay FP64.  hierarchy: the
+        result of setup_hierarchy for this matrix, to share one setup
+        between precisions."""
         t0 = time.perf_counter()
         dev = val.device
-        n = plan.n_nodes
-        if host is None:
-            A = _bsr_host(plan.row_ptr.cpu().numpy(), plan.col_idx.cpu().numpy(),
-                          val.cpu().numpy(), n, n)
-            host = setup_levels(A, fixed, xyz, box, coarse_nodes, max_levels, log)
-        host, Ad = host
+        if hierarchy is None:
+            hierarchy = setup_hierarchy(plan.row_ptr, plan.col_idx, val, plan.n_nodes, fixed, xyz,
+                                        box, coarse_nodes, max_levels, log)
+        hier, Ad = hierarchy
         self.cheb_degree, self.cheb_lower = cheb_degree, cheb_lower
         if precision not in ('fp64', 'fp32'):
             raise ValueError(precision)
         self.precision = precision
         dt = torch.float64 if precision == 'fp64' else torch.float32
         self.levels = []
-        for i, H in enumerate(host):
-            self.levels.append(dict(A=_Dev(H['A'], dev, dt), Dinv=_diag_dev(H['Dinv'], dev, dt),
-                                    P=_Dev(H['P'], dev, dt), R=_Dev(H['P'].T.tocsr(), dev, dt),
-                                    lam=H['lam'], n=H['A'].shape[0]))
+        for H in hier:
+            self.levels.append(dict(A=_Dev(H['A'], dt), Dinv=_Dev(H['D'], dt), P=_Dev(H['P'], dt),
+                                    R=_Dev(H['R'], dt), lam=H['lam'], n=6 * H['A'].nr))
         self.nc = Ad.shape[0]
         self.coarse_inv = torch.as_tensor(np.ascontiguousarray(np.linalg.inv(Ad).T), device=dev)
         zeros = lambda m, t=dt: torch.zeros(m, dtype=t, device=dev)

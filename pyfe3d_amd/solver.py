@@ -131,21 +131,21 @@ class StaticSystem:
 
     def setup_amg(self, xyz, log=None, precisions=('fp64',), graph=True, **kw):
         """Smoothed-aggregation multigrid for method='amg' (FP64 V-cycle)
-        and/or 'amg32' (FP32 V-cycle): one host setup from the node
-        coordinates xyz (n_nodes, 3) (see pyfe3d_amd.amg), the V-cycle
-        captured in a HIP graph unless graph=False.  Returns the last one
-        built."""
-        from .amg import SAAMG, _bsr_host, setup_levels
+        and/or 'amg32' (FP32 V-cycle): one setup from the node coordinates
+        xyz (n_nodes, 3) (see pyfe3d_amd.amg; kept in self.amg_hierarchy),
+        the V-cycle captured in a HIP graph unless graph=False.  Returns the
+        last one built."""
+        from .amg import SAAMG, setup_hierarchy
         fixed = self.fixed.cpu().numpy().astype(bool)
         box = kw.pop('box', 3.0)
         coarse_nodes = kw.pop('coarse_nodes', 300)
         max_levels = kw.pop('max_levels', 8)
-        A = _bsr_host(self.plan.row_ptr.cpu().numpy(), self.plan.col_idx.cpu().numpy(),
-                      self.val.cpu().numpy(), self.nb, self.nb)
-        host = setup_levels(A, fixed, xyz, box, coarse_nodes, max_levels, log)
+        hier = setup_hierarchy(self.plan.row_ptr, self.plan.col_idx, self.val, self.nb, fixed, xyz,
+                               box, coarse_nodes, max_levels, log)
+        self.amg_hierarchy = hier
         m = None
         for prec in precisions:
-            m = SAAMG(self.plan, self.val, fixed, xyz, precision=prec, host=host, **kw)
+            m = SAAMG(self.plan, self.val, fixed, xyz, precision=prec, hierarchy=hier, **kw)
             if graph:
                 m.capture()
             if prec == 'fp64':

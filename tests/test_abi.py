@@ -112,3 +112,17 @@ def test_argument_errors_of_newer_entry_points():
     # mixed-precision PCG needs every operand and a positive replacement period
     assert call('pf3d_pcg_mixed', 4, fake, fake, fake, fake, fake, fake, fake, fake, 1e-8, 10, 0,
                 0, fake, 1 << 30, fake, fake, None) == -1
+    # multigrid setup: sizes, null operands, in-place symmetrisation
+    nb = ctypes.c_int64()
+    assert call('pf3d_bsr_spgemm_bytes', 0, ctypes.byref(nb)) == -1
+    assert call('pf3d_bsr_spgemm_symbolic', 4, fake, None, fake, fake, fake, fake, 1 << 20,
+                ctypes.byref(nb), None) == -1
+    assert call('pf3d_bsr_spgemm_numeric', 4, fake, fake, fake, fake, fake, fake, fake, None,
+                None, 5, None) == -1
+    assert call('pf3d_bsr_transpose_bytes', 4, 4, 0, ctypes.byref(nb)) == -1
+    assert call('pf3d_bsr_transpose', 4, 4, 8, fake, fake, fake, fake, fake, fake, None, 0,
+                None) == -1
+    assert call('pf3d_amg_smooth_prolongator', 4, fake, fake, fake, None, fake, fake, 0.5, fake,
+                fake, None) == -1
+    assert call('pf3d_bsr_symmetrize', 4, 8, fake, fake, fake, None, fake, fake, None) == -1
+    assert b'bad arguments' in lib.pf3d_last_error()
