@@ -2,11 +2,19 @@
 preconditioned by the smoothed-aggregation multigrid V-cycle.
 
 Replaces the reference callers' ``eigsh(A=KC0uu, M=Muu, sigma=-1.,
-which='LM')`` (tests/test_quad4_natural_freq.py:120-123) where a banded
+which='LM')`` (tests/test_quad4_natural_freq.py:120-123) and
+``eigsh(A=KGuu, M=KC0uu, which='SM')`` for linear buckling
+(tests/test_quad4_linear_buckling_plate_mat_coord.py:184) where a banded
 factorisation does not fit (the shift-invert Lanczos of
-:mod:`pyfe3d_amd.eigen` factorises K on one workgroup): K x = lambda M x,
-lowest k pairs, on the Dirichlet-treated matrices (identity rows of K and
-zero rows of M on fixed DOFs; the basis is zero there).
+:mod:`pyfe3d_amd.eigen` factorises K on one workgroup), on the
+Dirichlet-treated matrices (identity rows of K and zero rows of M or KG on
+fixed DOFs; the basis is zero there):
+
+* modal: K x = lambda M x, the k lowest lambda;
+* buckling: K x = lambda (-KG) x, the k lowest positive lambda, solved as
+  (-KG) x = mu K x for the k largest mu (lambda = 1 / mu), so that the
+  B-matrix of the iteration is the positive definite K and the multigrid
+  V-cycle of K is the preconditioner in both problems.
 
 Each iteration (Knyazev's locally optimal block method): residuals
 R = K X - M X Lambda, preconditioned W = V-cycle(R), Rayleigh-Ritz on
@@ -29,35 +37,39 @@ def _s():
 
 
 class LOBPCG:
-    """plan: BSR structure shared by K and M; k_val, m_val: BSR values;
+    """plan: BSR structure shared by K and the second matrix; k_val: K's
+    BSR values; m_val: M's (problem 'modal') or KG's (problem 'buckling');
     fixed: bool per DOF; precond: object with apply(r, z) (e.g. an SAAMG
     built for the Dirichlet-treated K)."""
 
-    def __init__(self, plan, k_val, m_val, fixed, precond):
+    def __init__(self, plan, k_val, m_val, fixed, precond, problem='modal'):
+        if problem not in ('modal', 'buckling'):
+            raise ValueError("problem must be 'modal' or 'buckling'")
         self.plan = plan
+        self.problem = problem
         nb = plan.n_nodes
         self.n = 6 * nb
         dev = k_val.device
         fixed_t = torch.as_tensor(np.asarray(fixed, dtype=np.uint8), device=dev)
         pr, pc = plan.row_ptr.data_ptr(), plan.col_idx.data_ptr()
-        self.K = k_val.clone()
-        _lib.call('pf3d_bsr_dirichlet', nb, pr, pc, self.K.data_ptr(), fixed_t.data_ptr(), _s())
-        self.M = m_val.clone()
-        _lib.call('pf3d_bsr_dirichlet_diag', nb, pr, pc, self.M.data_ptr(), fixed_t.data_ptr(),
+        K = k_val.clone()
+        _lib.call('pf3d_bsr_dirichlet', nb, pr, pc, K.data_ptr(), fixed_t.data_ptr(), _s())
+        G = m_val.clone() if problem == 'modal' else -m_val
+        _lib.call('pf3d_bsr_dirichlet_diag', nb, pr, pc, G.data_ptr(), fixed_t.data_ptr(),
                   0.0, _s())
+        # A x = theta B x: modal A = K, B = M (lowest theta); buckling
+        # A = -KG, B = K (largest theta)
+        self.A, self.B = (K, G) if problem == 'modal' else (G, K)
+        self.largest = problem == 'buckling'
         self.free = (1 - fixed_t.to(torch.float64))
         self.precond = precond
 
-    def _spmv(self, val, x, y):
-        _lib.call('pf3d_bsr_spmv', self.plan.n_nodes, self.plan.row_ptr.data_ptr(),
-                  self.plan.col_idx.data_ptr(), val.data_ptr(), x.data_ptr(), y.data_ptr(), _s())
-
-    def _kmul(self, x, y):
-        """y = K x with compensated row sums: for the lowest modes K x is
+    def _mul(self, val, x, y):
+        """y = val x with compensated row sums: for the lowest modes K x is
         ~lambda M x, many digits below |K| |x|, and the Rayleigh quotients
         need it accurate to ~eps of itself."""
         _lib.call('pf3d_bsr_residual_dd', self.plan.n_nodes, self.plan.row_ptr.data_ptr(),
-                  self.plan.col_idx.data_ptr(), self.K.data_ptr(), x.data_ptr(), None,
+                  self.plan.col_idx.data_ptr(), val.data_ptr(), x.data_ptr(), None,
                   y.data_ptr(), _s())
 
     def _gram(self, A, B, out_host=True):
@@ -94,11 +106,12 @@ class LOBPCG:
             arr[kb:m] = tmp
 
     def solve(self, k=6, block=None, tol=1e-7, maxiter=300, seed=0, log=None):
-        """Lowest k eigenpairs.  Converged when every one of the k
-        residuals ||K x - lambda M x|| <= tol * lambda ||M x||.  Returns
-        (lambda (k,), X (n, k) on the GPU, info)."""
+        """The k lowest eigenvalues (modal) or k lowest positive buckling
+        load multipliers.  Converged when every one of the k residuals
+        ||A x - theta B x|| <= tol * |theta| ||B x||.  Returns (lambda (k,)
+        ascending, X (n, k) on the GPU, info)."""
         n = self.n
-        dev = self.K.device
+        dev = self.A.device
         kb = block or min(k + 2, 10)
         if 3 * kb > 32:
             raise ValueError('block size above 10')
@@ -110,8 +123,8 @@ class LOBPCG:
         BS = torch.zeros_like(S)
         S[:kb] = X
         for i in range(kb):
-            self._kmul(S[i], AS[i])
-            self._spmv(self.M, S[i], BS[i])
+            self._mul(self.A, S[i], AS[i])
+            self._mul(self.B, S[i], BS[i])
         m = kb          # active vectors in S
         R = torch.empty((kb, n), dtype=torch.float64, device=dev)
         lam = None
@@ -127,7 +140,9 @@ class LOBPCG:
             keep = d > 1e-13 * d.max()
             T = V[:, keep] / np.sqrt(d[keep])
             theta, Y = np.linalg.eigh(T.T @ GA @ T)
-            C = T @ Y[:, :kb]                      # (m, kb), M-orthonormal Ritz vectors
+            if self.largest:
+                theta, Y = theta[::-1], Y[:, ::-1]
+            C = T @ Y[:, :kb]                      # (m, kb), B-orthonormal Ritz vectors
             lam = theta[:kb]
             Xn = self._comb(S[:m], C, torch.empty((kb, n), dtype=torch.float64, device=dev))
             AXn = self._comb(AS[:m], C, torch.empty_like(Xn))
@@ -149,7 +164,7 @@ class LOBPCG:
             rel = rn / (np.abs(lam) * bn)
             hist.append(float(rel[:k].max()))
             if log and it % 10 == 0:
-                log('lobpcg %d: lambda %s, max rel residual %.2e' % (it, lam[:3], rel[:k].max()))
+                log('lobpcg %d: theta %s, max rel residual %.2e' % (it, lam[:3], rel[:k].max()))
             if rel[:k].max() <= tol or it == maxiter:
                 break
             # W = T(R), then its images
@@ -157,8 +172,8 @@ class LOBPCG:
                 # zero on fixed DOFs: R is, and the V-cycle keeps them
                 # decoupled (identity rows, zero prolongator rows)
                 self.precond.apply(R[i], S[kb + i])
-                self._kmul(S[kb + i], AS[kb + i])
-                self._spmv(self.M, S[kb + i], BS[kb + i])
+                self._mul(self.A, S[kb + i], AS[kb + i])
+                self._mul(self.B, S[kb + i], BS[kb + i])
             if m > kb:
                 S[2 * kb:], AS[2 * kb:], BS[2 * kb:] = Pn, APn, BPn
                 m = 3 * kb
@@ -168,7 +183,8 @@ class LOBPCG:
             # direction to unit M-norm: keeps the Gram matrices well
             # conditioned when W and P shrink with the residuals
             self._orth(S, AS, BS, kb, m)
-        order = np.argsort(lam[:k])
+        vals = lam[:k] if not self.largest else 1. / lam[:k]
+        order = np.argsort(vals)
         Xo = S[:kb][order.tolist()].T.contiguous()
-        return lam[:k][order], Xo, dict(iters=it, residual=rel[:k][order], history=hist,
-                                        block=kb)
+        return vals[order], Xo, dict(iters=it, residual=rel[:k][order], history=hist,
+                                     block=kb)

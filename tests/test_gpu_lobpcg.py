@@ -73,3 +73,44 @@ def test_lobpcg_stiffened_panel_vs_eigsh():
     ref = np.sort(sla.eigsh(A=Kc, M=Mc, sigma=-1., which='LM', k=6, tol=1e-12)[0])
     print('panel lobpcg', lam, ref, info['iters'], info['residual'])
     np.testing.assert_allclose(lam, ref, rtol=1e-8)
+
+
+def test_lobpcg_buckling_stiffened_panel_vs_eigsh():
+    """Linear buckling of the stiffened panel under end compression and
+    pressure: KG from the static solution in one mixed-type launch, the
+    three lowest load multipliers K x = lambda (-KG) x against eigsh on the
+    oracle's matrices (as the reference's buckling tests call it)."""
+    from pyfe3d_amd.lobpcg import LOBPCG
+    from pyfe3d_amd.panel import StiffenedPanel
+    from pyfe3d_amd.solver import StaticSystem
+    from pyfe3d_amd.sparse import BSRPlan
+    p = StiffenedPanel(n=32, a=0.6, b=0.5, spacing=16, n_cutouts=3, cutout=(3, 9), seed=9)
+    fixed = p.fixed()
+    f = p.loads(Nxx=1e3, p=2e2)
+    _, r, c, v = _mixed(p, 'KC0')
+    plan = BSRPlan(r, c, p.n_nodes)
+    K = plan.apply(v)
+    sysm = StaticSystem(plan, K, fixed)
+    amg = sysm.setup_amg(p.mesh.ncoords, coarse_nodes=80)
+    u, info = sysm.solve(f, method='direct', rtol=1e-10)
+    uh = u.cpu().numpy()
+    _, rg, cg, vg = _mixed(p, 'KG', u=uh)
+    union = BSRPlan(torch.cat([r, rg]), torch.cat([c, cg]), p.n_nodes)
+    assert torch.equal(union.row_ptr, plan.row_ptr) and torch.equal(union.col_idx, plan.col_idx)
+    KG = union.apply(torch.cat([torch.zeros_like(v), vg]))
+    lam, X, info = LOBPCG(plan, K, KG, fixed, amg, problem='buckling').solve(k=3, tol=1e-8,
+                                                                           maxiter=400)
+    bu = ~fixed
+    parts, _ = _oracle(p, 'KC0')
+    Kc = _coo(parts, p.n_dof)[bu, :][:, bu]
+    partsG, _ = _oracle(p, 'KG', u=uh)
+    KGc = _coo(partsG, p.n_dof)[bu, :][:, bu]
+    mu = sla.eigsh(A=KGc, M=Kc, k=6, which='SA', tol=1e-12)[0]
+    ref = np.sort(-1. / mu[mu < 0])[:3]
+    print('panel buckling lobpcg', lam, ref, info['iters'], info['residual'])
+    np.testing.assert_allclose(lam, ref, rtol=1e-8)
+    Xh = X.cpu().numpy()
+    for i in range(3):
+        xi = Xh[bu, i]
+        res = Kc @ xi + lam[i] * (KGc @ xi)
+        assert np.linalg.norm(res) <= 1e-6 * np.linalg.norm(Kc @ xi)
