@@ -20,7 +20,8 @@ Setup (once per matrix):
 Solve (GPU, this library's kernels only): PCG whose preconditioner is one
 symmetric V-cycle with Chebyshev(D^-1 A) pre/post smoothing, restriction
 P^T and prolongation P as 6x6-block sparse products (pf3d_bsr_spmv), and
-a dense inverse on the coarsest level (pf3d_gemv_n); dot products with
+a dense inverse on the coarsest level (pf3d_gemv_t: one workgroup per row of
+the inverse); dot products with
 pf3d_dot, vector updates with pf3d_axpby.  With precision 'fp32' the whole
 V-cycle runs in single precision (pf3d_bsr_spmv_f32, pf3d_axpby_f32) and
 the CG around it, in FP64, uses the flexible (Polak-Ribiere) beta.
@@ -295,7 +296,9 @@ class SAAMG:
             self.levels.append(dict(A=_Dev(H['A'], dt), Dinv=_Dev(H['D'], dt), P=_Dev(H['P'], dt),
                                     R=_Dev(H['R'], dt), lam=H['lam'], n=6 * H['A'].nr))
         self.nc = Ad.shape[0]
-        self.coarse_inv = torch.as_tensor(np.ascontiguousarray(np.linalg.inv(Ad).T), device=dev)
+        # row j of the inverse at coarse_inv[j * nc:]: pf3d_gemv_t forms each
+        # entry of inv(Ad) b as one workgroup's dot product
+        self.coarse_inv = torch.as_tensor(np.ascontiguousarray(np.linalg.inv(Ad)), device=dev)
         zeros = lambda m, t=dt: torch.zeros(m, dtype=t, device=dev)
         for i, L in enumerate(self.levels):
             m = L['n']
@@ -382,12 +385,12 @@ class SAAMG:
         """x = M^-1 b (x is overwritten)."""
         if lvl == len(self.levels):
             if b.dtype == torch.float64:
-                _lib.call('pf3d_gemv_n', self.nc, self.nc, self.coarse_inv.data_ptr(), self.nc,
+                _lib.call('pf3d_gemv_t', self.nc, self.nc, self.coarse_inv.data_ptr(), self.nc,
                           b.data_ptr(), x.data_ptr(), 1.0, 0.0, _s())
             else:
                 b64, x64 = self.c64
                 _lib.call('pf3d_to_f64', self.nc, b.data_ptr(), b64.data_ptr(), _s())
-                _lib.call('pf3d_gemv_n', self.nc, self.nc, self.coarse_inv.data_ptr(), self.nc,
+                _lib.call('pf3d_gemv_t', self.nc, self.nc, self.coarse_inv.data_ptr(), self.nc,
                           b64.data_ptr(), x64.data_ptr(), 1.0, 0.0, _s())
                 _lib.call('pf3d_to_f32', self.nc, x64.data_ptr(), x.data_ptr(), _s())
             return x

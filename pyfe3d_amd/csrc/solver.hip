@@ -28,38 +28,72 @@ constexpr int RED_BLOCKS = 1024;  // max partials of a reduction
 inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 // ---------------------------------------------------------------- BSR SpMV
-// One wavefront per block row: lane (j, i) multiplies row i of the j-th
-// block of the row (10 blocks per pass) with its 6-vector, then the wave
-// sums the partial rows through LDS.  MT is the storage type of the matrix
-// values (FP64, or FP32 to halve the dominant traffic), VT that of the
-// vectors; products and sums are FP64.
+// One wavefront per RPW block rows: lane (j, i) multiplies row i of the
+// j-th block of each of its rows (10 blocks per pass) with its 6-vector,
+// then the wave sums the partial rows through LDS.  MT is the storage type
+// of the matrix values (FP64, or FP32 to halve the dominant traffic), VT
+// that of the vectors; products and sums are FP64.  A wave takes four rows
+// in lockstep (one block of each per pass) to keep four loads per lane in
+// flight: measured on the 6M-DOF plate's matrix, 556 -> 503 us (FP64) and
+// 335 -> 301 us (FP32) against one row per wave, same bits.
+#ifndef PF3D_SPMV_RPW64
+#define PF3D_SPMV_RPW64 4
+#endif
+#ifndef PF3D_SPMV_RPW32
+#define PF3D_SPMV_RPW32 4
+#endif
+template <typename MT>
+constexpr int spmv_rpw() { return sizeof(MT) == 8 ? PF3D_SPMV_RPW64 : PF3D_SPMV_RPW32; }
+
 template <typename MT, typename VT>
 __global__ void __launch_bounds__(256) bsr_spmv(int64_t nb, const int32_t *__restrict__ rp,
                                                 const int32_t *__restrict__ ci,
                                                 const MT *__restrict__ val,
                                                 const VT *__restrict__ x, VT *__restrict__ y)
 {
-    __shared__ double part[4][64];
+    constexpr int RPW = spmv_rpw<MT>();
+    __shared__ double part[4][RPW][64];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t row = (int64_t)blockIdx.x * 4 + wave;
+    const int64_t row0 = ((int64_t)blockIdx.x * 4 + wave) * RPW;
     const int j = lane / 6, i = lane - 6 * (lane / 6);
-    double acc = 0.;
-    if (row < nb && j < 10) {
-        const int b0 = rp[row], b1 = rp[row + 1];
-        for (int b = b0 + j; b < b1; b += 10) {
-            const MT *v = val + (int64_t)b * 36 + i * 6;
-            const VT *xv = x + (int64_t)ci[b] * 6;
+    double acc[RPW];
+    int b[RPW], b1[RPW];
 #pragma unroll
-            for (int k = 0; k < 6; k++) acc += (double)v[k] * (double)xv[k];
+    for (int q = 0; q < RPW; q++) {
+        acc[q] = 0.;
+        const int64_t row = row0 + q;
+        const bool ok = row < nb && j < 10;
+        b[q] = ok ? rp[row] + j : 0;
+        b1[q] = ok ? rp[row + 1] : 0;
+    }
+    // the rows' blocks in lockstep: one load of each row per pass
+    bool more = true;
+    while (more) {
+        more = false;
+#pragma unroll
+        for (int q = 0; q < RPW; q++) {
+            if (b[q] < b1[q]) {
+                const MT *v = val + (int64_t)b[q] * 36 + i * 6;
+                const VT *xv = x + (int64_t)ci[b[q]] * 6;
+#pragma unroll
+                for (int k = 0; k < 6; k++) acc[q] += (double)v[k] * (double)xv[k];
+                b[q] += 10;
+                more |= b[q] < b1[q];
+            }
         }
     }
-    part[wave][lane] = acc;
-    __syncthreads();
-    if (row < nb && lane < 6) {
-        double s = 0.;
 #pragma unroll
-        for (int jj = 0; jj < 10; jj++) s += part[wave][lane + 6 * jj];
-        y[row * 6 + lane] = (VT)s;
+    for (int q = 0; q < RPW; q++) part[wave][q][lane] = acc[q];
+    __syncthreads();
+    if (lane < 6 * RPW) {
+        const int q = lane / 6, l = lane - 6 * q;
+        const int64_t row = row0 + q;
+        if (row < nb) {
+            double s = 0.;
+#pragma unroll
+            for (int jj = 0; jj < 10; jj++) s += part[wave][q][l + 6 * jj];
+            y[row * 6 + l] = (VT)s;
+        }
     }
 }
 
@@ -346,8 +380,8 @@ struct Mat {
 template <typename MT, typename VT>
 hipError_t spmv(const Mat &A, const VT *x, VT *y, hipStream_t s)
 {
-    hipLaunchKernelGGL((bsr_spmv<MT, VT>), dim3(nblk(A.nb, 4)), dim3(256), 0, s, A.nb, A.rp,
-                       A.ci, (const MT *)A.val, x, y);
+    hipLaunchKernelGGL((bsr_spmv<MT, VT>), dim3(nblk(A.nb, 4 * spmv_rpw<MT>())), dim3(256), 0, s,
+                       A.nb, A.rp, A.ci, (const MT *)A.val, x, y);
     return hipGetLastError();
 }
 
