@@ -384,6 +384,19 @@ int pf3d_bsr_symmetrize(int64_t nr, int64_t nnzb, const int32_t *row_ptr, const 
                         const double *val, const uint8_t *dead, double *out, int *unpaired,
                         void *stream);
 
+/* Conjugate-gradient vector updates with the scalars on the device (no
+ * reference counterpart), so that an iteration needs no host round trip
+ * and can be captured in a graph.  s holds r.z at s[0], p.q at s[1], the
+ * new r.z at s[3], beta at s[4] and r.z_old at s[6] (flexible CG):
+ *   pf3d_cg_update_xr: alpha = s[0] / s[1]; x += alpha p; r -= alpha q
+ *   pf3d_cg_beta:      s[4] = (s[3] - flexible * s[6]) / s[0]; s[0] = s[3]
+ *   pf3d_cg_update_p:  p = z + s[4] p
+ * (a zero denominator gives a zero step). */
+int pf3d_cg_update_xr(int64_t n, const double *s, const double *p, const double *q, double *x,
+                      double *r, void *stream);
+int pf3d_cg_beta(double *s, int flexible, void *stream);
+int pf3d_cg_update_p(int64_t n, const double *s, const double *z, double *p, void *stream);
+
 /* y = K x for a 6x6 BSR matrix with nb block rows (FP64 / FP32). */
 int pf3d_bsr_spmv(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx, const double *val,
                   const double *x, double *y, void *stream);

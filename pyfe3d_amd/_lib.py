@@ -54,6 +54,9 @@ SIGNATURES = {
     'pf3d_bsr_spmv': [I64, P, P, P, P, P, P],
     'pf3d_bsr_spmv_f32': [I64, P, P, P, P, P, P],
     'pf3d_bsr_residual_dd': [I64, P, P, P, P, P, P, P],
+    'pf3d_cg_update_xr': [I64, P, P, P, P, P, P],
+    'pf3d_cg_beta': [P, I32, P],
+    'pf3d_cg_update_p': [I64, P, P, P, P],
     'pf3d_bsr_spgemm_bytes': [I64, P],
     'pf3d_bsr_spgemm_symbolic': [I64, P, P, P, P, P, P, I64, P, P],
     'pf3d_bsr_spgemm_numeric': [I64, P, P, P, P, P, P, P, P, P, I64, P],

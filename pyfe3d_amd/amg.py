@@ -418,19 +418,93 @@ def _dot(a, b, out):
     return float(out[0].item())
 
 
-def amg_pcg(system, amg, f, rtol=1e-8, maxiter=2000, x0=None, log=None, stall=100):
+class _CGIteration:
+    """Buffers of one preconditioned-CG solve and one iteration captured in
+    a HIP graph: q = A p, p.q, x and r updated, z = V-cycle(r), r.z (and
+    r.z_old), r.r, beta, p updated -- scalars on the device
+    (pf3d_cg_update_xr, pf3d_cg_beta, pf3d_cg_update_p), so replays need no
+    host round trip.  Kept on the system per preconditioner and reused by
+    later solves."""
+
+    def __init__(self, system, amg):
+        n = 6 * system.nb
+        dev = system.val.device
+        zeros = lambda m: torch.zeros(m, dtype=torch.float64, device=dev)
+        self.system, self.amg, self.n = system, amg, n
+        self.flexible = amg.precision != 'fp64'
+        self.x, self.r, self.p, self.q, self.z = (zeros(n) for _ in range(5))
+        self.zold = zeros(n) if self.flexible else None
+        self.s = zeros(8)           # r.z, p.q, -, r.z new, beta, r.r, r.z_old
+        self.work = zeros(1024)
+        self.graph = None
+
+    def _dot_into(self, a, b, k):
+        _lib.call('pf3d_dot', self.n, a.data_ptr(), b.data_ptr(), self.s[k:].data_ptr(),
+                  self.work.data_ptr(), _s())
+
+    def step(self):
+        """One iteration's launches (eager, or recorded by capture())."""
+        self.system.matvec(self.p, self.q)
+        self._dot_into(self.p, self.q, 1)
+        _lib.call('pf3d_cg_update_xr', self.n, self.s.data_ptr(), self.p.data_ptr(),
+                  self.q.data_ptr(), self.x.data_ptr(), self.r.data_ptr(), _s())
+        if self.flexible:
+            _axpby(1., self.z, 0., self.zold)
+        self.amg._apply(self.r, self.z)
+        self._dot_into(self.r, self.z, 3)
+        if self.flexible:
+            # beta = r.(z_new - z_old) / r_old.z_old keeps CG's rate with a
+            # preconditioner that is linear only up to FP32 rounding
+            self._dot_into(self.r, self.zold, 6)
+        self._dot_into(self.r, self.r, 5)
+        _lib.call('pf3d_cg_beta', self.s.data_ptr(), int(self.flexible), _s())
+        _lib.call('pf3d_cg_update_p', self.n, self.s.data_ptr(), self.z.data_ptr(),
+                  self.p.data_ptr(), _s())
+
+    def capture(self):
+        s = torch.cuda.Stream(device=self.x.device)
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            self.step()                 # warm-up outside the graph (zero buffers: no-op)
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            self.step()
+        torch.cuda.synchronize()
+        self.graph = g
+
+    def restart(self):
+        """p = z = V-cycle(r), s[0] = r.z (r holds the true residual)."""
+        self.amg._apply(self.r, self.z)
+        _axpby(1., self.z, 0., self.p)
+        self._dot_into(self.r, self.z, 0)
+
+
+def amg_pcg(system, amg, f, rtol=1e-8, maxiter=2000, x0=None, log=None, stall=100, check=None,
+            graph=True):
     """PCG with one V-cycle per iteration on a StaticSystem's matrix.
-    Returns (u, info) like StaticSystem.solve (true FP64 residual).  A run
-    whose recursive residual has not halved in `stall` iterations restarts
-    from the true residual (and ends when a restart does not halve that)."""
+    Returns (u, info) like StaticSystem.solve (compensated true residual).
+    The iteration runs as one captured graph (graph=True) replayed `check`
+    times between reads of the recursive residual (default: 8 below 1e6
+    DOF, where a host round trip per iteration would cost more than the
+    iteration's kernels, else 1); a run whose recursive residual has not
+    halved in `stall` iterations restarts from the true residual (and ends
+    when a restart does not gain 10 %)."""
     b = system.rhs(f)
-    x = torch.zeros_like(b) if x0 is None else x0.clone()
-    r = torch.empty_like(b)
-    q = torch.empty_like(b)
-    z = torch.empty_like(b)
+    if check is None:
+        check = 8 if 6 * system.nb < 1000000 else 1
+    cache = system.__dict__.setdefault('_cg_iterations', {})
+    cg = cache.get(id(amg))
+    if cg is None or cg.amg is not amg:
+        cg = cache[id(amg)] = _CGIteration(system, amg)
+    if graph and cg.graph is None:
+        cg.capture()
+    if x0 is None:
+        cg.x.zero_()
+    else:
+        _axpby(1., x0, 0., cg.x)
     s = torch.empty(1025, dtype=torch.float64, device=b.device)
-    flexible = amg.precision != 'fp64'
-    zold = torch.empty_like(b) if flexible else None
     bb = _dot(b, b, s) or 1.
     it = 0
     restarts = 0
@@ -441,47 +515,34 @@ def amg_pcg(system, amg, f, rtol=1e-8, maxiter=2000, x0=None, log=None, stall=10
         # the recursive residual drifts on these ill-conditioned systems,
         # and each restart gains what the plain products of the inner CG
         # allow.  Stop when a restart gains less than 10 %.
-        system.residual(b, x, r)
-        rel = (_dot(r, r, s) / bb) ** 0.5
+        system.residual(b, cg.x, cg.r)
+        rel = (_dot(cg.r, cg.r, s) / bb) ** 0.5
         if rel <= rtol or it >= maxiter or restarts > 12 or rel > 0.9 * best:
             break
         best = rel
         restarts += 1
         if log:
             log('amg pcg: restart %d at %d iterations, true relres %.3e' % (restarts, it, rel))
-        amg.apply(r, z)
-        p = z.clone()
-        rz = _dot(r, z, s)
+        cg.restart()
         low, since = rel, 0
         while it < maxiter:
-            system.matvec(p, q)
-            alpha = rz / _dot(p, q, s)
-            _axpby(alpha, p, 1., x)
-            _axpby(-alpha, q, 1., r)
-            it += 1
-            rr = (_dot(r, r, s) / bb) ** 0.5
+            k = min(check, maxiter - it)
+            for _ in range(k):
+                if graph:
+                    cg.graph.replay()
+                else:
+                    cg.step()
+            it += k
+            rr = (float(cg.s[5].item()) / bb) ** 0.5
             if rr <= rtol:
                 break
             if rr <= 0.5 * low:
                 low, since = rr, 0
             else:
-                since += 1
+                since += k
                 if since >= stall:
                     break
-            if flexible:
-                # beta = r.(z_new - z_old) / r_old.z_old keeps CG's rate with
-                # a preconditioner that is linear only up to FP32 rounding
-                zold.copy_(z)
-                amg.apply(r, z)
-                rz_new = _dot(r, z, s)
-                beta = (rz_new - _dot(r, zold, s)) / rz
-            else:
-                amg.apply(r, z)
-                rz_new = _dot(r, z, s)
-                beta = rz_new / rz
-            _axpby(1., z, beta, p)
-            rz = rz_new
-    return x, dict(iters=it, outer=restarts, relres=rel)
+    return cg.x.clone(), dict(iters=it, outer=restarts, relres=rel)
 
 
 def residual_floor(system, x):

@@ -59,6 +59,37 @@ __global__ void axpby_f32(int64_t n, float a, const float *__restrict__ x, float
     if (i < n) y[i] = a * x[i] + (b != 0.f ? b * y[i] : 0.f);
 }
 
+// Conjugate-gradient updates with the scalars kept on the device (slots:
+// s[0] = r.z, s[1] = p.q, s[3] = new r.z, s[4] = beta, s[6] = r.z_old), so
+// a whole iteration runs without a host round trip and can be captured
+__global__ void cg_update_xr(int64_t n, const double *__restrict__ s,
+                             const double *__restrict__ p, const double *__restrict__ q,
+                             double *__restrict__ x, double *__restrict__ r)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const double alpha = s[1] != 0. ? s[0] / s[1] : 0.;
+    if (i < n) {
+        x[i] += alpha * p[i];
+        r[i] -= alpha * q[i];
+    }
+}
+
+__global__ void cg_beta(double *s, int flexible)
+{
+    if (threadIdx.x == 0) {
+        const double num = s[3] - (flexible ? s[6] : 0.);
+        s[4] = s[0] != 0. ? num / s[0] : 0.;
+        s[0] = s[3];
+    }
+}
+
+__global__ void cg_update_p(int64_t n, const double *__restrict__ s, const double *__restrict__ z,
+                            double *__restrict__ p)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = z[i] + s[4] * p[i];
+}
+
 __global__ void widen(int64_t n, const float *__restrict__ a, double *__restrict__ b)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -264,6 +295,31 @@ int pf3d_gemv_n(int64_t n, int m, const double *V, int64_t ld, const double *c, 
     hipLaunchKernelGGL(gemv_n, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                        (hipStream_t)stream, n, m, V, ld, c, y, alpha, beta);
     return pf3d_hip_check(hipGetLastError(), "pf3d_gemv_n");
+}
+
+int pf3d_cg_update_xr(int64_t n, const double *s, const double *p, const double *q, double *x,
+                      double *r, void *stream)
+{
+    if (n <= 0 || !s || !p || !q || !x || !r)
+        return pf3d_fail(PF3D_E_ARG, "pf3d_cg_update_xr: bad arguments");
+    hipLaunchKernelGGL(cg_update_xr, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, n, s, p, q, x, r);
+    return pf3d_hip_check(hipGetLastError(), "pf3d_cg_update_xr");
+}
+
+int pf3d_cg_beta(double *s, int flexible, void *stream)
+{
+    if (!s) return pf3d_fail(PF3D_E_ARG, "pf3d_cg_beta: bad arguments");
+    hipLaunchKernelGGL(cg_beta, dim3(1), dim3(64), 0, (hipStream_t)stream, s, flexible);
+    return pf3d_hip_check(hipGetLastError(), "pf3d_cg_beta");
+}
+
+int pf3d_cg_update_p(int64_t n, const double *s, const double *z, double *p, void *stream)
+{
+    if (n <= 0 || !s || !z || !p) return pf3d_fail(PF3D_E_ARG, "pf3d_cg_update_p: bad arguments");
+    hipLaunchKernelGGL(cg_update_p, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, n, s, z, p);
+    return pf3d_hip_check(hipGetLastError(), "pf3d_cg_update_p");
 }
 
 int pf3d_axpby(int64_t n, double a, const double *x, double b, double *y, void *stream)

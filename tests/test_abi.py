@@ -126,3 +126,7 @@ def test_argument_errors_of_newer_entry_points():
                 fake, None) == -1
     assert call('pf3d_bsr_symmetrize', 4, 8, fake, fake, fake, None, fake, fake, None) == -1
     assert b'bad arguments' in lib.pf3d_last_error()
+    # device-scalar CG updates
+    assert call('pf3d_cg_update_xr', 0, fake, fake, fake, fake, fake, None) == -1
+    assert call('pf3d_cg_beta', None, 0, None) == -1
+    assert call('pf3d_cg_update_p', 4, fake, None, fake, None) == -1

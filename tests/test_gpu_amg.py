@@ -136,3 +136,27 @@ def test_vcycle_graph_replays_match_eager():
                 torch.cuda.synchronize()
                 assert torch.equal(z, e), (prec, rep)
                 _ = (r * 2).sum().item()   # eager work between replays
+
+
+@pytest.mark.parametrize('method', ['amg', 'amg32'])
+def test_captured_cg_iteration_matches_eager(method):
+    """The multigrid CG with its iteration replayed from a graph gives the
+    eager iteration's solution bit for bit, and two solves in a row (the
+    second with another load) reuse the captured graph correctly."""
+    from pyfe3d_amd.amg import amg_pcg
+    from pyfe3d_amd.mesh import PlateMesh
+    from pyfe3d_amd.shellprop import isotropic_plate
+    m = PlateMesh(29, 23, 1.0, 0.7, jitter=0.2, seed=8)
+    rec = isotropic_plate(thickness=0.004, E=70e9, nu=0.33, calc_scf=True).record()
+    sysm = _system(m, m.quads, 'quad4', rec, m.clamped())
+    sysm.setup_amg(m.ncoords, coarse_nodes=40, precisions=('fp64', 'fp32'))
+    amg = sysm.amg if method == 'amg' else sysm.amg32
+    rng = np.random.default_rng(2)
+    for rep in range(2):
+        f = np.zeros(m.n_dof)
+        f[2::6] = rng.uniform(-1, 1, m.n_nodes) * 100.
+        ug, ig = amg_pcg(sysm, amg, f, rtol=1e-10, maxiter=2000, graph=True)
+        ue, ie = amg_pcg(sysm, amg, f, rtol=1e-10, maxiter=2000, graph=False)
+        print(method, rep, ig, ie)
+        assert ig == ie and torch.equal(ug, ue)
+        assert ig['relres'] <= 1e-10
