@@ -558,6 +558,22 @@ def main():
     del vm
     log('KG %.3f ms, M %.3f ms' % (ms_g, ms_m))
 
+    # per-element laminate records on the device (one 8-ply stack per element
+    # with seeded angles: the property tables of a laminate design sweep)
+    from pyfe3d_amd.shellprop import laminate_records_device, material_row
+    g = torch.Generator(device=dev).manual_seed(5)
+    angles = torch.tensor([0., 45., -45., 90.], dtype=torch.float64, device=dev)[
+        torch.randint(0, 4, (E, 8), device=dev, generator=g)]
+    plyt = torch.full((E, 8), 0.000125, dtype=torch.float64, device=dev)
+    mats = [material_row((140e9, 10e9, 0.3, 5e9, 5e9, 5e9), 1600.)]
+    recs = torch.empty((E, 32), dtype=torch.float64, device=dev)
+    ms_r, _ = timed(lambda: laminate_records_device(angles, plyt, mats, out=recs),
+                    max(3, args.steps // 4), 1, stream)
+    extra['laminate_records'] = dict(ms=ms_r, elements_per_s=E / (ms_r * 1e-3), plies=8,
+                                     gbs=(8 * 16 + 256) * E / (ms_r * 1e-3) / 1e9)
+    log('laminate records (8 plies) %.3f ms' % ms_r)
+    del angles, plyt, recs
+
     # internal forces (update_fint) of the same plate: element forces + ordered node gather
     u = torch.randn(mesh.n_dof, dtype=torch.float64, device=dev,
                     generator=torch.Generator(device=dev).manual_seed(3)) * 1e-4

@@ -218,6 +218,20 @@ int pf3d_shell_piston(int etype, int matrix, int64_t n_elem, const double *x,
                       const int64_t *conn, const int64_t *init_k, int64_t *r, int64_t *c,
                       double *v, int flags, void *stream);
 
+/* Shell property records (32 doubles each) of laminates, one per element,
+ * from per-element ply stacks: theta_deg and plyt (n x n_plies, device;
+ * plies of zero thickness are padding), ply_mat (n x n_plies indices into
+ * mats, or NULL for material 0), mats (n_mat x 8, device: e1 e2 nu12 g12
+ * g13 g23 rho, unused), offset (n, or NULL for 0); calc_scf != 0 computes
+ * the shear correction factors, else 5/6.  Replaces one
+ * laminated_plate(stack, plyts, laminaprops, rhos, offset, calc_scf) per
+ * element on the caller side (shellprop_utils.py:96-180:
+ * Lamina.rebuild shellprop.pyx:278-337, calc_constitutive_matrix
+ * shellprop.pyx:568-621, calc_scf shellprop.pyx:485-548). */
+int pf3d_laminate_records(int64_t n, int n_plies, const double *theta_deg, const double *plyt,
+                          const int32_t *ply_mat, int n_mat, const double *mats,
+                          const double *offset, int calc_scf, double *out, void *stream);
+
 /* Shell property records (32 doubles each) of isotropic plates, one per
  * element, from a thickness field h[n] (device) and the record of the same
  * material at unit thickness (host, 32 doubles, h field = 1, no offset):

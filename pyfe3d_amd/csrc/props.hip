@@ -53,7 +53,138 @@ __global__ void iso_records(int64_t n, const double *__restrict__ h, Unit u,
     out[t] = u.f[k] * s;
 }
 
+// Laminate record of one element from its ply stack: ply stiffness in
+// laminate axes (Lamina.rebuild, shellprop.pyx:278-337), through-thickness
+// integration of A, B, D, E and the mass moments
+// (ShellProp.calc_constitutive_matrix, shellprop.pyx:568-621) and the
+// Vlachoutsis shear correction factors (ShellProp.calc_scf,
+// shellprop.pyx:485-548), in the reference's operation order.  Material
+// rows: e1 e2 nu12 g12 g13 g23 rho (8 doubles, last unused).  Plies of zero
+// thickness are padding and skipped.
+__global__ void laminate_records(int64_t n, int np_, const double *__restrict__ theta,
+                                 const double *__restrict__ plyt,
+                                 const int32_t *__restrict__ ply_mat,
+                                 const double *__restrict__ mats,
+                                 const double *__restrict__ offset, int calc_scf,
+                                 double *__restrict__ out)
+{
+    // no FMA contraction: the host builder rounds every product (the scf
+    // polynomial cancels by several digits)
+#pragma clang fp contract(off)
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const double *th = theta + e * np_, *tk = plyt + e * np_;
+    const int32_t *pm = ply_mat ? ply_mat + e * np_ : nullptr;
+    const double off = offset ? offset[e] : 0.;
+    const double deg = 4. * atan(1.) / 180.;
+    double h = 0.;
+    for (int k = 0; k < np_; k++) h += tk[k];
+    double A[6] = {0, 0, 0, 0, 0, 0}, B[6] = {0, 0, 0, 0, 0, 0}, D[6] = {0, 0, 0, 0, 0, 0};
+    double E44 = 0., E45 = 0., E55 = 0., ir = 0., irz = 0., irz2 = 0.;
+    double h0 = -h / 2. + off;
+    for (int k = 0; k < np_; k++) {
+        if (tk[k] == 0.) continue;
+        const double *m = mats + 8 * (pm ? pm[k] : 0);
+        const double e1 = m[0], e2 = m[1], nu12 = m[2], g12 = m[3], g13 = m[4], g23 = m[5];
+        const double nu21 = nu12 * e2 / e1;
+        const double t = th[k] * deg;
+        const double c = cos(t), s = sin(t);
+        const double c2 = c * c, c3 = c2 * c, c4 = c2 * c2, s2 = s * s, s3 = s2 * s, s4 = s2 * s2;
+        const double d = 1 - nu12 * nu21;
+        const double q11 = e1 / d, q12 = nu12 * e2 / d, q22 = e2 / d;
+        const double q44 = g23, q55 = g13, q66 = g12;
+        double qL[6];
+        qL[0] = q11 * c4 + 2 * (q12 + 2 * q66) * s2 * c2 + q22 * s4;
+        qL[1] = (q11 + q22 - 4 * q66) * s2 * c2 + q12 * (s4 + c4);
+        qL[2] = (q11 - q12 - 2 * q66) * s * c3 + (q12 - q22 + 2 * q66) * s3 * c;
+        qL[3] = q11 * s4 + 2 * (q12 + 2 * q66) * s2 * c2 + q22 * c4;
+        qL[4] = (q11 - q12 - 2 * q66) * s3 * c + (q12 - q22 + 2 * q66) * s * c3;
+        qL[5] = (q11 + q22 - 2 * q12 - 2 * q66) * s2 * c2 + q66 * (s4 + c4);
+        const double q44L = q44 * c2 + q55 * s2, q45L = (q55 - q44) * s * c;
+        const double q55L = q55 * c2 + q44 * s2;
+        const double hk_1 = h0;
+        h0 += tk[k];
+        const double hk = h0, rho = m[6];
+        ir += rho * (hk - hk_1);
+        irz += rho * (hk * hk / 2. - hk_1 * hk_1 / 2.);
+        irz2 += rho * (hk * hk * hk / 3. - hk_1 * hk_1 * hk_1 / 3.);
+        const double d1 = hk - hk_1;
+        for (int q = 0; q < 6; q++) {
+            A[q] += qL[q] * d1;
+            B[q] += 1 / 2. * qL[q] * (hk * hk - hk_1 * hk_1);
+            D[q] += 1 / 3. * qL[q] * (hk * hk * hk - hk_1 * hk_1 * hk_1);
+        }
+        E44 += q44L * d1;
+        E45 += q45L * d1;
+        E55 += q55L * d1;
+    }
+    double k13 = 5 / 6., k23 = 5 / 6.;
+    if (calc_scf) {
+        double D1 = 0., R1 = 0., den1 = 0., D2 = 0., R2 = 0., den2 = 0.;
+        const double zb = -h / 2. + off;
+        double z1 = zb;
+        for (int k = 0; k < np_; k++) {
+            if (tk[k] == 0.) continue;
+            const double *m = mats + 8 * (pm ? pm[k] : 0);
+            const double me1 = m[0], me2 = m[1], mnu12 = m[2], g13 = m[4], g23 = m[5];
+            const double mnu21 = mnu12 * me2 / me1;
+            const double z2 = z1 + tk[k];
+            const double t = th[k] * deg;
+            const double ct = cos(t), st = sin(t);
+            const double e1 = me1 * ct + me2 * st, e2 = me2 * ct + me1 * st;
+            const double nu12 = mnu12 * ct + mnu21 * st, nu21 = mnu21 * ct + mnu12 * st;
+            // the polynomial cancels by several digits: integer powers by
+            // pow(), as the reference's ** (libm pow) forms them
+            const double zb2 = pow(zb, 2.), o2 = pow(off, 2.);
+            const double poly =
+                15 * off * pow(z1, 4.) + 30 * off * pow(z1, 2.) * zb * (2 * off - zb) -
+                15 * off * pow(z2, 4.) + 30 * off * pow(z2, 2.) * zb * (-2 * off + zb) -
+                3 * pow(z1, 5.) + 10 * pow(z1, 3.) * (-2 * o2 - 2 * off * zb + zb2) -
+                15 * z1 * zb2 * (4 * o2 - 4 * off * zb + zb2) + 3 * pow(z2, 5.) +
+                10 * pow(z2, 3.) * (2 * o2 + 2 * off * zb - zb2) +
+                15 * z2 * zb2 * (4 * o2 - 4 * off * zb + zb2);
+            const double r3 = pow(z2 - off, 3.) / 3. - pow(z1 - off, 3.) / 3.;
+            D1 += e1 / (1 - nu12 * nu21);
+            R1 += D1 * r3;
+            den1 += g13 * tk[k] * (h / tk[k]) * pow(D1, 2.) * poly / (60 * g13);
+            D2 += e2 / (1 - nu12 * nu21);
+            R2 += D2 * r3;
+            den2 += g23 * tk[k] * (h / tk[k]) * pow(D2, 2.) * poly / (60 * g23);
+            z1 = z2;
+        }
+        k13 = pow(R1, 2.) / den1;
+        k23 = pow(R2, 2.) / den2;
+    }
+    double *o = out + e * pf3d::SHELL_STRIDE;
+    for (int q = 0; q < 6; q++) {
+        o[q] = A[q];
+        o[6 + q] = B[q];
+        o[12 + q] = D[q];
+    }
+    o[18] = E44; o[19] = E45; o[20] = E55;
+    o[21] = k13; o[22] = k23;
+    o[23] = h; o[24] = off;
+    o[25] = ir; o[26] = irz; o[27] = irz2;
+    o[28] = o[29] = o[30] = o[31] = 0.;
+}
+
 }  // namespace
+
+extern "C" int pf3d_laminate_records(int64_t n, int n_plies, const double *theta_deg,
+                                     const double *plyt, const int32_t *ply_mat, int n_mat,
+                                     const double *mats, const double *offset, int calc_scf,
+                                     double *out, void *stream)
+{
+    if (n < 0 || n_plies <= 0 || n_mat <= 0)
+        return pf3d_fail(PF3D_E_ARG, "pf3d_laminate_records: bad sizes");
+    if (n == 0) return PF3D_OK;
+    if (!theta_deg || !plyt || !mats || !out)
+        return pf3d_fail(PF3D_E_ARG, "pf3d_laminate_records: null argument");
+    hipLaunchKernelGGL(laminate_records, dim3((unsigned)((n + 127) / 128)), dim3(128), 0,
+                       (hipStream_t)stream, n, n_plies, theta_deg, plyt, ply_mat, mats, offset,
+                       calc_scf, out);
+    return pf3d_hip_check(hipGetLastError(), "pf3d_laminate_records");
+}
 
 extern "C" int pf3d_isotropic_records(int64_t n, const double *h, const double *unit_record,
                                       double *out, void *stream)

@@ -286,3 +286,47 @@ def isotropic_records(h, E, nu, rho=0., calc_scf=True):
     rec[:, RECORD_FIELDS.index('intrho')] = rho * h
     rec[:, RECORD_FIELDS.index('intrhoz2')] = rho * h ** 3 / 12.
     return rec
+
+
+def material_row(laminaprop, rho=0.):
+    """Material row (e1 e2 nu12 g12 g13 g23 rho 0) of the device laminate
+    builder from a 2- or 6-tuple laminaprop (shellprop_utils.py:13-93)."""
+    m = read_laminaprop(laminaprop, rho)
+    return [m.e1, m.e2, m.nu12, m.g12, m.g13, m.g23, m.rho, 0.]
+
+
+def laminate_records_device(theta_deg, plyt, mats, ply_mat=None, offset=None, calc_scf=True,
+                            out=None, stream=None):
+    """Records (n, 32) on the GPU of one laminate per element
+    (pf3d_laminate_records): theta_deg, plyt (n, n_plies) ply angles and
+    thicknesses (zero-thickness plies are padding), mats (n_mat, 8) rows of
+    material_row, ply_mat (n, n_plies) material indices or None, offset (n)
+    or None.  Arrays are torch tensors on the device or host arrays."""
+    import torch
+    from . import _lib
+    dev = 'cuda'
+    th = torch.as_tensor(np.asarray(theta_deg, dtype=np.float64) if not torch.is_tensor(theta_deg)
+                         else theta_deg, dtype=torch.float64, device=dev).contiguous()
+    tk = torch.as_tensor(np.asarray(plyt, dtype=np.float64) if not torch.is_tensor(plyt)
+                         else plyt, dtype=torch.float64, device=dev).contiguous()
+    if th.dim() != 2 or th.shape != tk.shape:
+        raise ValueError('theta_deg and plyt must both be (n, n_plies)')
+    n, n_plies = th.shape
+    mt = torch.as_tensor(np.asarray(mats, dtype=np.float64), device=dev).reshape(-1, 8)
+    pm = None
+    if ply_mat is not None:
+        pm = torch.as_tensor(np.asarray(ply_mat), dtype=torch.int32, device=dev).contiguous()
+        if pm.shape != th.shape:
+            raise ValueError('ply_mat must be (n, n_plies)')
+        if int(pm.min()) < 0 or int(pm.max()) >= mt.shape[0]:
+            raise ValueError('ply_mat outside [0, %d)' % mt.shape[0])
+    off = None
+    if offset is not None:
+        off = torch.as_tensor(np.asarray(offset, dtype=np.float64), device=dev).reshape(n)
+    if out is None:
+        out = torch.empty((n, RECORD_STRIDE), dtype=torch.float64, device=dev)
+    s = (stream if stream is not None else torch.cuda.current_stream()).cuda_stream
+    _lib.call('pf3d_laminate_records', n, n_plies, th.data_ptr(), tk.data_ptr(),
+              None if pm is None else pm.data_ptr(), mt.shape[0], mt.data_ptr(),
+              None if off is None else off.data_ptr(), int(bool(calc_scf)), out.data_ptr(), s)
+    return out

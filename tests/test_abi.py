@@ -130,3 +130,8 @@ def test_argument_errors_of_newer_entry_points():
     assert call('pf3d_cg_update_xr', 0, fake, fake, fake, fake, fake, None) == -1
     assert call('pf3d_cg_beta', None, 0, None) == -1
     assert call('pf3d_cg_update_p', 4, fake, None, fake, None) == -1
+    # laminate records: sizes and null operands
+    assert call('pf3d_laminate_records', 4, 0, fake, fake, None, 1, fake, None, 1, fake,
+                None) == -1
+    assert call('pf3d_laminate_records', 4, 2, fake, None, None, 1, fake, None, 1, fake,
+                None) == -1
