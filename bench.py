@@ -34,8 +34,8 @@ HBM_PEAK_GBS = 8000.0
 # `python bench.py --profile-kc0` (KiB per dispatch), each scaled by the
 # reading of the same counter for a 1 GiB device copy in the same run
 # (FETCH_SIZE reads 0.5 GiB for it: factor 2.0; WRITE_SIZE 1 GiB: 1.0):
-# 60,070 KiB x 2 + 4,500,577 KiB = 4.732e9 B (algorithmic: 4.736e9 B).
-KC0_TRAFFIC_BYTES = (60070 * 2 + 4500577) * 1024
+# 60,069 KiB x 2 + 4,500,309 KiB = 4.731e9 B (algorithmic: 4.736e9 B).
+KC0_TRAFFIC_BYTES = (60069 * 2 + 4500309) * 1024
 # algorithmic bytes of one Quad4 KC0 element, values only: 576 doubles out,
 # 4 int64 DOF offsets and 12 coordinates in
 KC0_BYTES_VALUES = 576 * 8 + 4 * 8 + 12 * 8
@@ -409,9 +409,10 @@ def large_solve(n, log):
     return out
 
 
-def profile_kc0(n):
-    """Three values-only KC0 launches on the n x n plate, then three copies
-    of 1 GiB (the calibration of the HBM byte counters)."""
+def profile_kc0(n, launches=3):
+    """`launches` values-only KC0 launches on the n x n plate (the bench's
+    warm-up and timed steps for the kernel trace; 3 for the counter runs),
+    then three copies of 1 GiB (the calibration of the HBM byte counters)."""
     import torch
     from pyfe3d_amd import assembly as A
     mesh = build_plate(n)
@@ -421,7 +422,7 @@ def profile_kc0(n):
     conn = torch.as_tensor(6 * mesh.quads, device='cuda')
     props = torch.as_tensor(prop.record()[None], device='cuda')
     v = torch.empty(576 * E, dtype=torch.float64, device='cuda')
-    for _ in range(3):
+    for _ in range(launches):
         A.shell_kc0('quad4', x, conn, None, None, v, props, values_only=True, overwrite=True,
                     check=False)
     torch.cuda.synchronize()
@@ -430,7 +431,8 @@ def profile_kc0(n):
     for _ in range(3):
         dst.copy_(src)
     torch.cuda.synchronize()
-    print('profile-kc0: %d elements, 3 KC0 launches, 3 x 1 GiB copies' % E, flush=True)
+    print('profile-kc0: %d elements, %d KC0 launches, 3 x 1 GiB copies' % (E, launches),
+          flush=True)
 
 
 def warm_start_rates(n_plates, log):
@@ -446,8 +448,10 @@ def warm_start_rates(n_plates, log):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=20)
-    ap.add_argument('--warmup', type=int, default=3)
+    # the GPU reaches its sustained clocks after ~25 ms of back-to-back
+    # launches: the default warm-up (50 x 0.8 ms) covers that ramp
+    ap.add_argument('--steps', type=int, default=50)
+    ap.add_argument('--warmup', type=int, default=50)
     ap.add_argument('--n', type=int, default=1000, help='elements per side')
     ap.add_argument('--cpu-sample', type=int, default=200000)
     ap.add_argument('--no-solve', action='store_true')
@@ -466,6 +470,8 @@ def main():
                     help='run only the stiffened-panel lines (no headline)')
     ap.add_argument('--profile-kc0', action='store_true',
                     help='only: 3 KC0 launches and a 1 GiB device copy (counter runs)')
+    ap.add_argument('--profile-launches', type=int, default=3,
+                    help='KC0 launches of --profile-kc0 (the kernel trace uses warmup + steps)')
     ap.add_argument('--quiet', action='store_true')
     args = ap.parse_args()
     log = (lambda *a: None) if args.quiet else (lambda *a: print(*a, file=sys.stderr, flush=True))
@@ -479,7 +485,7 @@ def main():
     dev = 'cuda'
     stream = torch.cuda.current_stream()
     if args.profile_kc0:
-        profile_kc0(args.n)
+        profile_kc0(args.n, args.profile_launches)
         return
     if args.large_solve:
         print(json.dumps({'large_solve': large_solve(args.n, log)}), flush=True)

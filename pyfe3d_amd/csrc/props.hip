@@ -61,12 +61,11 @@ __global__ void iso_records(int64_t n, const double *__restrict__ h, Unit u,
 // shellprop.pyx:485-548), in the reference's operation order.  Material
 // rows: e1 e2 nu12 g12 g13 g23 rho (8 doubles, last unused).  Plies of zero
 // thickness are padding and skipped.
-__global__ void laminate_records(int64_t n, int np_, const double *__restrict__ theta,
-                                 const double *__restrict__ plyt,
-                                 const int32_t *__restrict__ ply_mat,
-                                 const double *__restrict__ mats,
-                                 const double *__restrict__ offset, int calc_scf,
-                                 double *__restrict__ out)
+__global__ void __launch_bounds__(128)
+    laminate_records(int64_t n, int np_, const double *__restrict__ theta,
+                     const double *__restrict__ plyt, const int32_t *__restrict__ ply_mat,
+                     const double *__restrict__ mats, const double *__restrict__ offset,
+                     int calc_scf, double *__restrict__ out)
 {
     // no FMA contraction: the host builder rounds every product (the scf
     // polynomial cancels by several digits)
@@ -81,7 +80,10 @@ __global__ void laminate_records(int64_t n, int np_, const double *__restrict__ 
     for (int k = 0; k < np_; k++) h += tk[k];
     double A[6] = {0, 0, 0, 0, 0, 0}, B[6] = {0, 0, 0, 0, 0, 0}, D[6] = {0, 0, 0, 0, 0, 0};
     double E44 = 0., E45 = 0., E55 = 0., ir = 0., irz = 0., irz2 = 0.;
-    double h0 = -h / 2. + off;
+    // shear correction sums (calc_scf), accumulated in the same ply loop
+    double D1 = 0., R1 = 0., den1 = 0., D2 = 0., R2 = 0., den2 = 0.;
+    const double zb = -h / 2. + off;
+    double h0 = zb;
     for (int k = 0; k < np_; k++) {
         if (tk[k] == 0.) continue;
         const double *m = mats + 8 * (pm ? pm[k] : 0);
@@ -117,24 +119,13 @@ __global__ void laminate_records(int64_t n, int np_, const double *__restrict__ 
         E44 += q44L * d1;
         E45 += q45L * d1;
         E55 += q55L * d1;
-    }
-    double k13 = 5 / 6., k23 = 5 / 6.;
-    if (calc_scf) {
-        double D1 = 0., R1 = 0., den1 = 0., D2 = 0., R2 = 0., den2 = 0.;
-        const double zb = -h / 2. + off;
-        double z1 = zb;
-        for (int k = 0; k < np_; k++) {
-            if (tk[k] == 0.) continue;
-            const double *m = mats + 8 * (pm ? pm[k] : 0);
-            const double me1 = m[0], me2 = m[1], mnu12 = m[2], g13 = m[4], g23 = m[5];
-            const double mnu21 = mnu12 * me2 / me1;
-            const double z2 = z1 + tk[k];
-            const double t = th[k] * deg;
-            const double ct = cos(t), st = sin(t);
-            const double e1 = me1 * ct + me2 * st, e2 = me2 * ct + me1 * st;
-            const double nu12 = mnu12 * ct + mnu21 * st, nu21 = mnu21 * ct + mnu12 * st;
-            // the polynomial cancels by several digits: integer powers by
-            // pow(), as the reference's ** (libm pow) forms them
+        if (calc_scf) {
+            // ply moduli along the laminate axes as calc_scf forms them; the
+            // polynomial cancels by several digits: integer powers by pow(),
+            // as the reference's ** (libm pow) forms them
+            const double z1 = hk_1, z2 = hk;
+            const double se1 = e1 * c + e2 * s, se2 = e2 * c + e1 * s;
+            const double snu12 = nu12 * c + nu21 * s, snu21 = nu21 * c + nu12 * s;
             const double zb2 = pow(zb, 2.), o2 = pow(off, 2.);
             const double poly =
                 15 * off * pow(z1, 4.) + 30 * off * pow(z1, 2.) * zb * (2 * off - zb) -
@@ -144,14 +135,16 @@ __global__ void laminate_records(int64_t n, int np_, const double *__restrict__ 
                 10 * pow(z2, 3.) * (2 * o2 + 2 * off * zb - zb2) +
                 15 * z2 * zb2 * (4 * o2 - 4 * off * zb + zb2);
             const double r3 = pow(z2 - off, 3.) / 3. - pow(z1 - off, 3.) / 3.;
-            D1 += e1 / (1 - nu12 * nu21);
+            D1 += se1 / (1 - snu12 * snu21);
             R1 += D1 * r3;
             den1 += g13 * tk[k] * (h / tk[k]) * pow(D1, 2.) * poly / (60 * g13);
-            D2 += e2 / (1 - nu12 * nu21);
+            D2 += se2 / (1 - snu12 * snu21);
             R2 += D2 * r3;
             den2 += g23 * tk[k] * (h / tk[k]) * pow(D2, 2.) * poly / (60 * g23);
-            z1 = z2;
         }
+    }
+    double k13 = 5 / 6., k23 = 5 / 6.;
+    if (calc_scf) {
         k13 = pow(R1, 2.) / den1;
         k23 = pow(R2, 2.) / den2;
     }

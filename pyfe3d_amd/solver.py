@@ -143,6 +143,8 @@ class StaticSystem:
         hier = setup_hierarchy(self.plan.row_ptr, self.plan.col_idx, self.val, self.nb, fixed, xyz,
                                box, coarse_nodes, max_levels, log)
         self.amg_hierarchy = hier
+        # captured CG iterations of an earlier preconditioner are stale
+        self.__dict__.pop('_cg_iterations', None)
         m = None
         for prec in precisions:
             m = SAAMG(self.plan, self.val, fixed, xyz, precision=prec, hierarchy=hier, **kw)
