@@ -59,6 +59,14 @@ extern "C" {
 int pf3d_version(void);
 const char *pf3d_last_error(void);
 
+/* Validation pass: adds to *count (device int64, caller-zeroed) the
+ * number of elements among n_elem whose `size` slots from init_k[e] (NULL:
+ * size*e) hold a value that is not finite.  The reference's elements have
+ * no such check (cpdef void, cdivision: a zero area writes inf/nan,
+ * quad4.pyx:1-7); run it after an assembly to flag degenerate elements. */
+int pf3d_nonfinite_elements(int64_t n_elem, int size, const int64_t *init_k, const double *v,
+                            int64_t *count, void *stream);
+
 /* Slots written per element: Quad4Data/Quad4RData/Tria3RData/BeamCData
  * KC0_SPARSE_SIZE, KG_SPARSE_SIZE, M_SPARSE_SIZE (quad4.pyx:174-180,
  * quad4r.pyx:140-146, tria3r.pyx:148-151, beamc.pyx:43-46,

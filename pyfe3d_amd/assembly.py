@@ -59,6 +59,27 @@ def slot_count(etype, matrix='KC0', mtype=0):
     return _lib.call('pf3d_slot_count', ETYPE[etype], MATRIX[matrix], mtype)
 
 
+def nonfinite_elements(v, n_elem, size, init_k=None, stream=None):
+    """Number of elements whose `size` slots (from init_k[e], or size*e)
+    in the value array v hold inf or nan (pf3d_nonfinite_elements): the
+    validation pass for degenerate elements, which the reference's
+    elements write without a check."""
+    if v.dtype != torch.float64 or not v.is_cuda:
+        raise TypeError('v must be a float64 device tensor')
+    if init_k is not None:
+        if init_k.dtype != torch.int64 or init_k.numel() != n_elem:
+            raise ValueError('init_k must hold one int64 per element')
+        if n_elem and (int(init_k.min()) < 0 or int(init_k.max()) + size > v.numel()):
+            raise ValueError('init_k slots outside v')
+    elif size * n_elem > v.numel():
+        raise ValueError('v holds fewer than size * n_elem slots')
+    count = torch.zeros(1, dtype=torch.int64, device=v.device)
+    s = (stream if stream is not None else torch.cuda.current_stream()).cuda_stream
+    _lib.call('pf3d_nonfinite_elements', int(n_elem), int(size),
+              None if init_k is None else init_k.data_ptr(), v.data_ptr(), count.data_ptr(), s)
+    return int(count.item())
+
+
 def _flags(values_only, overwrite):
     return (VALUES_ONLY if values_only else 0) | (OVERWRITE if overwrite else 0)
 

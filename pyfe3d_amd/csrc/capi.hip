@@ -19,6 +19,27 @@ hipError_t pf3d_launch_geometry(int etype, int64_t n, const double *x, const int
 
 static thread_local std::string g_err;
 
+namespace {
+// One wave per element: the lanes read its `size` slots (coalesced) and
+// the first lane counts the element once if any value is not finite
+// (the reference divides by the area without a check, quad4.pyx:1-7, so a
+// degenerate element writes inf/nan silently).
+__global__ void __launch_bounds__(256) nonfinite_elements(int64_t n_elem, int size,
+                                                          const int64_t *__restrict__ init_k,
+                                                          const double *__restrict__ v,
+                                                          unsigned long long *__restrict__ count)
+{
+    const int64_t e = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    bool bad = false;
+    if (e < n_elem) {
+        const int64_t k0 = init_k ? init_k[e] : (int64_t)size * e;
+        for (int t = lane; t < size; t += 64) bad |= !isfinite(v[k0 + t]);
+    }
+    if (__any(bad) && lane == 0 && e < n_elem) atomicAdd(count, 1ULL);
+}
+}  // namespace
+
 int pf3d_fail(int code, const char *fmt, ...)
 {
     char buf[512];
@@ -39,6 +60,18 @@ int pf3d_hip_check(hipError_t e, const char *where)
 extern "C" {
 
 int pf3d_version(void) { return 10000; }
+
+int pf3d_nonfinite_elements(int64_t n_elem, int size, const int64_t *init_k, const double *v,
+                            int64_t *count, void *stream)
+{
+    if (n_elem < 0 || size <= 0 || (n_elem && (!v || !count)))
+        return pf3d_fail(PF3D_E_ARG, "pf3d_nonfinite_elements: bad arguments");
+    if (!n_elem) return PF3D_OK;
+    hipLaunchKernelGGL(nonfinite_elements, dim3((unsigned)((n_elem + 3) / 4)), dim3(256), 0,
+                       (hipStream_t)stream, n_elem, size, init_k, v,
+                       (unsigned long long *)count);
+    return pf3d_hip_check(hipGetLastError(), "pf3d_nonfinite_elements");
+}
 
 const char *pf3d_last_error(void) { return g_err.c_str(); }
 

@@ -135,3 +135,6 @@ def test_argument_errors_of_newer_entry_points():
                 None) == -1
     assert call('pf3d_laminate_records', 4, 2, fake, None, None, 1, fake, None, 1, fake,
                 None) == -1
+    # validation pass
+    assert call('pf3d_nonfinite_elements', 4, 0, None, fake, fake, None) == -1
+    assert call('pf3d_nonfinite_elements', 4, 576, None, None, fake, None) == -1
