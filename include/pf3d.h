@@ -419,6 +419,22 @@ int pf3d_cg_update_xr(int64_t n, const double *s, const double *p, const double 
 int pf3d_cg_beta(double *s, int flexible, void *stream);
 int pf3d_cg_update_p(int64_t n, const double *s, const double *z, double *p, void *stream);
 
+/* Fused multigrid vector step over the nb block rows of a 6x6 BSR matrix
+ * A (no reference counterpart): q = A xin (xin NULL: q = 0),
+ * r = sa * (b ? b : r) + sb * q, then dnew = c1 * Dinv r + c2 * dold (when
+ * dinv, nb x 36 inverted diagonal blocks, is given; dold may be NULL) and
+ * xacc += dold (when xacc is given).  One pass for a Chebyshev smoothing
+ * step, a residual b - A x, or y += P x.  xin must not alias r, dnew or
+ * xacc, nor dold dnew. */
+int pf3d_bsr_fused(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx, const double *val,
+                   const double *xin, const double *b, double *r, const double *dinv,
+                   const double *dold, double *dnew, double *xacc, double sa, double sb,
+                   double c1, double c2, void *stream);
+int pf3d_bsr_fused_f32(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx,
+                       const float *val, const float *xin, const float *b, float *r,
+                       const float *dinv, const float *dold, float *dnew, float *xacc, double sa,
+                       double sb, double c1, double c2, void *stream);
+
 /* y = K x for a 6x6 BSR matrix with nb block rows (FP64 / FP32). */
 int pf3d_bsr_spmv(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx, const double *val,
                   const double *x, double *y, void *stream);

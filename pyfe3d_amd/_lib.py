@@ -56,6 +56,8 @@ SIGNATURES = {
     'pf3d_bsr_spmv': [I64, P, P, P, P, P, P],
     'pf3d_bsr_spmv_f32': [I64, P, P, P, P, P, P],
     'pf3d_bsr_residual_dd': [I64, P, P, P, P, P, P, P],
+    'pf3d_bsr_fused': [I64, P, P, P, P, P, P, P, P, P, P, F64, F64, F64, F64, P],
+    'pf3d_bsr_fused_f32': [I64, P, P, P, P, P, P, P, P, P, P, F64, F64, F64, F64, P],
     'pf3d_cg_update_xr': [I64, P, P, P, P, P, P],
     'pf3d_cg_beta': [P, I32, P],
     'pf3d_cg_update_p': [I64, P, P, P, P],

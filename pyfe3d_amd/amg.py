@@ -357,28 +357,27 @@ class SAAMG:
     def _cheb(self, L, b, x, x_zero=False):
         """x += Chebyshev(D^-1 A) correction for A x = b (degree k, bounds
         [lower, 1.1] lambda_max); x_zero: x is known to be 0 (r = b, no
-        product)."""
+        product).  Each step is one fused pass (pf3d_bsr_fused): x += d,
+        r -= A d and the next d = a D^-1 r + c d together."""
         lmax = 1.1 * L['lam']
         lmin = self.cheb_lower * L['lam']
         theta, delta = 0.5 * (lmax + lmin), 0.5 * (lmax - lmin)
         sigma = theta / delta
         rho_old = 1. / sigma
-        r, d, t = L['r'], L['d'], L['t']
-        _axpby(1., b, 0., r)
-        if not x_zero:
-            L['A'](x, t)
-            _axpby(-1., t, 1., r)                  # r = b - A x
-        L['Dinv'](r, d)
-        _axpby(0., d, 1. / theta, d)               # d = D^-1 r / theta
+        A, D = L['A'], L['Dinv'].val
+        r, cur, nxt = L['r'], L['d'], L['t']
+        # r = b - A x (r = b from a zero x); d = D^-1 r / theta
+        _fused(A, None if x_zero else x, b, r, dinv=D, dnew=cur, sa=1., sb=-1.,
+               c1=1. / theta)
         for k in range(self.cheb_degree):
-            _axpby(1., d, 1., x)                   # x += d
             if k == self.cheb_degree - 1:
+                _axpby(1., cur, 1., x)             # x += d
                 break
-            L['A'](d, t)
-            _axpby(-1., t, 1., r)                  # r -= A d
             rho = 1. / (2. * sigma - rho_old)
-            L['Dinv'](r, t)
-            _axpby(2. * rho / delta, t, rho * rho_old, d)
+            # x += d; r -= A d; d = 2 rho / delta D^-1 r + rho rho_old d
+            _fused(A, cur, None, r, dinv=D, dold=cur, dnew=nxt, xacc=x, sa=1., sb=-1.,
+                   c1=2. * rho / delta, c2=rho * rho_old)
+            cur, nxt = nxt, cur
             rho_old = rho
 
     def vcycle(self, b, x, lvl=0):
@@ -397,17 +396,28 @@ class SAAMG:
         L = self.levels[lvl]
         _axpby(0., b, 0., x)                       # x = 0
         self._cheb(L, b, x, x_zero=True)
-        r, t = L['r'], L['t']
-        L['A'](x, t)
-        _axpby(1., b, 0., r)
-        _axpby(-1., t, 1., r)                      # r = b - A x
+        r = L['r']
+        _fused(L['A'], x, b, r, sa=1., sb=-1.)     # r = b - A x
         bc, xc = L['bc'], L['xc']
         L['R'](r, bc)
         self.vcycle(bc, xc, lvl + 1)
-        L['P'](xc, t)
-        _axpby(1., t, 1., x)                       # x += P xc
+        _fused(L['P'], xc, None, x, sa=1., sb=1.)  # x += P xc
         self._cheb(L, b, x)
         return x
+
+
+def _ptr(t):
+    return None if t is None else t.data_ptr()
+
+
+def _fused(M, xin, b, r, dinv=None, dold=None, dnew=None, xacc=None, sa=1., sb=1., c1=0.,
+           c2=0.):
+    """One pf3d_bsr_fused(_f32) pass with the operator M (a _Dev):
+    r = sa (b or r) + sb M xin; dnew = c1 Dinv r + c2 dold; xacc += dold."""
+    fn = 'pf3d_bsr_fused' if M.fn == 'pf3d_bsr_spmv' else 'pf3d_bsr_fused_f32'
+    _lib.call(fn, M.nb, M.row_ptr.data_ptr(), M.col_idx.data_ptr(), M.val.data_ptr(), _ptr(xin),
+              _ptr(b), r.data_ptr(), _ptr(dinv), _ptr(dold), _ptr(dnew), _ptr(xacc), float(sa),
+              float(sb), float(c1), float(c2), _s())
 
 
 def _dot(a, b, out):

@@ -97,6 +97,101 @@ __global__ void __launch_bounds__(256) bsr_spmv(int64_t nb, const int32_t *__res
     }
 }
 
+// ------------------------------------------ fused multigrid vector steps
+// One pass over the block rows for the Chebyshev smoother, the residual
+// and the prolongation of the V-cycle (same lane mapping as bsr_spmv):
+//   q_i = (A xin)_i                      (xin may be NULL: q = 0)
+//   r_i = sa * (b ? b_i : r_i) + sb * q_i
+//   dnew_i = c1 * (Dinv r)_i + c2 * dold_i   (when dinv is given)
+//   xacc_i += dold_i                     (when xacc is given)
+// r, dnew and xacc are written by their own row only; xin must not alias
+// any output and dold must not alias dnew.
+template <typename MT, typename VT>
+__global__ void __launch_bounds__(256)
+    bsr_fused(int64_t nb, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+              const MT *__restrict__ val, const VT *__restrict__ xin, const VT *__restrict__ b,
+              VT *r, const MT *__restrict__ dinv, const VT *__restrict__ dold,
+              VT *__restrict__ dnew, VT *__restrict__ xacc, double sa, double sb, double c1,
+              double c2)
+{
+    constexpr int RPW = spmv_rpw<MT>();
+    __shared__ double part[4][RPW][64];
+    __shared__ double rrow[4][RPW][6];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t row0 = ((int64_t)blockIdx.x * 4 + wave) * RPW;
+    const int j = lane / 6, i = lane - 6 * (lane / 6);
+    double acc[RPW];
+    int bb[RPW], b1[RPW];
+#pragma unroll
+    for (int q = 0; q < RPW; q++) {
+        acc[q] = 0.;
+        const int64_t row = row0 + q;
+        const bool ok = xin && row < nb && j < 10;
+        bb[q] = ok ? rp[row] + j : 0;
+        b1[q] = ok ? rp[row + 1] : 0;
+    }
+    bool more = true;
+    while (more) {
+        more = false;
+#pragma unroll
+        for (int q = 0; q < RPW; q++) {
+            if (bb[q] < b1[q]) {
+                const MT *v = val + (int64_t)bb[q] * 36 + i * 6;
+                const VT *xv = xin + (int64_t)ci[bb[q]] * 6;
+#pragma unroll
+                for (int k = 0; k < 6; k++) acc[q] += (double)v[k] * (double)xv[k];
+                bb[q] += 10;
+                more |= bb[q] < b1[q];
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < RPW; q++) part[wave][q][lane] = acc[q];
+    __syncthreads();
+    const int q = lane / 6, l = lane - 6 * (lane / 6);
+    const int64_t row = row0 + q;
+    const bool own = lane < 6 * RPW && row < nb;
+    if (own) {
+        double s = 0.;
+#pragma unroll
+        for (int jj = 0; jj < 10; jj++) s += part[wave][q][l + 6 * jj];
+        const int64_t k = row * 6 + l;
+        const double rv = sa * (double)(b ? b[k] : r[k]) + sb * s;
+        r[k] = (VT)rv;
+        rrow[wave][q][l] = (double)(VT)rv;
+    }
+    if (!dinv && !xacc) return;
+    __syncthreads();
+    if (own) {
+        const int64_t k = row * 6 + l;
+        const double dv = dold ? (double)dold[k] : 0.;
+        if (dinv) {
+            const MT *D = dinv + row * 36 + l * 6;
+            double t = 0.;
+#pragma unroll
+            for (int m = 0; m < 6; m++) t += (double)D[m] * rrow[wave][q][m];
+            dnew[k] = (VT)(c1 * t + c2 * dv);
+        }
+        if (xacc) xacc[k] = (VT)((double)xacc[k] + dv);
+    }
+}
+
+template <typename MT, typename VT>
+int fused_launch(int64_t nb, const int32_t *rp, const int32_t *ci, const MT *val, const VT *xin,
+                 const VT *b, VT *r, const MT *dinv, const VT *dold, VT *dnew, VT *xacc,
+                 double sa, double sb, double c1, double c2, void *stream, const char *name)
+{
+    if (nb < 0 || (nb && (!r || (xin && (!rp || !ci || !val)) || (dinv && !dnew) ||
+                          (xacc && !dold) || (dold && dold == dnew) ||
+                          (xin && (xin == r || xin == dnew || xin == xacc)))))
+        return pf3d_fail(PF3D_E_ARG, "%s: bad arguments", name);
+    if (!nb) return PF3D_OK;
+    hipLaunchKernelGGL((bsr_fused<MT, VT>), dim3(nblk(nb, 4 * spmv_rpw<MT>())), dim3(256), 0,
+                       (hipStream_t)stream, nb, rp, ci, val, xin, b, r, dinv, dold, dnew, xacc,
+                       sa, sb, c1, c2);
+    return pf3d_hip_check(hipGetLastError(), name);
+}
+
 // ------------------------------------------------- compensated products
 // Row sums of shell stiffness matrices cancel by many digits (stiff
 // membrane and shear terms against soft bending and drilling terms), so
@@ -588,6 +683,24 @@ int pf3d_bsr_spmv_f32(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx
     if (!nb) return PF3D_OK;
     Mat A{nb, row_ptr, col_idx, val, nullptr};
     return pf3d_hip_check(spmv<float, float>(A, x, y, (hipStream_t)stream), "pf3d_bsr_spmv_f32");
+}
+
+int pf3d_bsr_fused(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx, const double *val,
+                   const double *xin, const double *b, double *r, const double *dinv,
+                   const double *dold, double *dnew, double *xacc, double sa, double sb,
+                   double c1, double c2, void *stream)
+{
+    return fused_launch<double, double>(nb, row_ptr, col_idx, val, xin, b, r, dinv, dold, dnew,
+                                        xacc, sa, sb, c1, c2, stream, "pf3d_bsr_fused");
+}
+
+int pf3d_bsr_fused_f32(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx,
+                       const float *val, const float *xin, const float *b, float *r,
+                       const float *dinv, const float *dold, float *dnew, float *xacc, double sa,
+                       double sb, double c1, double c2, void *stream)
+{
+    return fused_launch<float, float>(nb, row_ptr, col_idx, val, xin, b, r, dinv, dold, dnew,
+                                      xacc, sa, sb, c1, c2, stream, "pf3d_bsr_fused_f32");
 }
 
 int pf3d_bsr_dirichlet(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx, double *val,

@@ -138,3 +138,10 @@ def test_argument_errors_of_newer_entry_points():
     # validation pass
     assert call('pf3d_nonfinite_elements', 4, 0, None, fake, fake, None) == -1
     assert call('pf3d_nonfinite_elements', 4, 576, None, None, fake, None) == -1
+    # fused multigrid step: aliasing and missing operands are refused
+    assert call('pf3d_bsr_fused', 4, fake, fake, fake, fake, None, fake, None, None, None, None,
+                1., -1., 0., 0., None) == -1      # xin == r
+    assert call('pf3d_bsr_fused', 4, fake, fake, fake, None, None, fake, fake, None, None, None,
+                1., -1., 1., 0., None) == -1      # dinv without dnew
+    assert call('pf3d_bsr_fused_f32', 4, fake, fake, fake, None, None, None, None, None, None,
+                None, 1., 1., 0., 0., None) == -1  # no r
