@@ -1,0 +1,571 @@
+"""pyfe3d's element classes over the GPU kernels: a drop-in for code written
+against ``pyfe3d`` (``from pyfe3d_amd.compat import Quad4, Quad4Data,
+Quad4Probe, ...``).
+
+Same class names, public attributes, defaults and method signatures as the
+reference (``pyfe3d/__init__.py:11-27``; ``quad4.pyx:450-489``,
+``quad4r.pyx:256-285``, ``tria3r.pyx:257-292``, ``beamc.pyx:130-157``,
+``beamprop.pyx:15-93``), and the same call order: ``update_rotation_matrix``
+then ``update_probe_xe`` (sets ``area``/``length`` and ``probe.xe``), then
+``update_KC0`` / ``update_M`` / ``update_KG_given_stress``; ``update_KG``
+and ``update_fint`` after ``update_probe_ue(u)``.  Output arrays are the
+caller's numpy arrays, written in the reference's slot layout from
+``init_k_*``: values accumulate (``+=``), rows and columns are written
+unless ``update_*v_only``.
+
+Each call is one GPU launch for one element (the element's node
+coordinates and displacements are sent, its slots come back).  The
+reference's loop over element objects therefore works unchanged; for
+throughput, the module's ``update_KC0_batch`` / ``update_KG_batch`` /
+``update_M_batch`` take a list of elements of one type and issue one launch
+for all of them (the batched entry points of :mod:`pyfe3d_amd.assembly`).
+Geometry is recomputed on the device from the coordinates given to the
+latest ``update_probe_xe`` (or ``update_rotation_matrix``) and the material
+direction given to ``update_rotation_matrix``, as the element kernels
+always derive it from x.
+"""
+import numpy as np
+import torch
+
+from . import assembly as A
+from .shellprop import (RECORD_FIELDS, RECORD_STRIDE, ShellProp, isotropic_plate,  # noqa: F401
+                        laminated_plate, read_laminaprop)
+
+DOF = 6
+INT = np.int64
+DOUBLE = np.float64
+_DEV = 'cuda'
+
+
+def _t(a, dtype=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=_DEV)
+
+
+def _shell_record(prop):
+    """32-double record of a ShellProp (this package's or any object with
+    the reference's public fields)."""
+    if hasattr(prop, 'record'):
+        return np.asarray(prop.record(), dtype=np.float64)
+    rec = np.zeros(RECORD_STRIDE)
+    rec[:len(RECORD_FIELDS)] = [float(getattr(prop, f)) for f in RECORD_FIELDS]
+    return rec
+
+
+BEAM_FIELDS = ['A', 'E', 'G', 'Iyy', 'Izz', 'Iyz', 'J', 'Ay', 'Az', 'intrho', 'intrhoy',
+               'intrhoz', 'intrhoy2', 'intrhoz2', 'intrhoyz']
+
+
+class BeamProp:
+    """Beam section properties (beamprop.pyx:15-93): every field 0."""
+
+    def __init__(self):
+        for f in BEAM_FIELDS:
+            setattr(self, f, 0.)
+
+    def record(self):
+        rec = np.zeros(A.BEAM_STRIDE)
+        rec[:len(BEAM_FIELDS)] = [float(getattr(self, f)) for f in BEAM_FIELDS]
+        return rec
+
+
+def _beam_record(prop):
+    if hasattr(prop, 'record'):
+        return np.asarray(prop.record(), dtype=np.float64)
+    rec = np.zeros(A.BEAM_STRIDE)
+    rec[:len(BEAM_FIELDS)] = [float(getattr(prop, f)) for f in BEAM_FIELDS]
+    return rec
+
+
+class _Data:
+    ETYPE = None
+
+    def __init__(self):
+        e = self.ETYPE
+        self.KC0_SPARSE_SIZE = A.slot_count(e, 'KC0')
+        self.KG_SPARSE_SIZE = A.slot_count(e, 'KG')
+        self.M_SPARSE_SIZE = A.slot_count(e, 'M', 0)
+        if e in ('quad4', 'quad4r'):
+            self.KA_BETA_SPARSE_SIZE = self.KA_GAMMA_SPARSE_SIZE = self.CA_SPARSE_SIZE = 144
+
+
+class Quad4Data(_Data):
+    """Slot counts (quad4.pyx:142-180)."""
+    ETYPE = 'quad4'
+
+
+class Quad4RData(_Data):
+    """Slot counts (quad4r.pyx:108-146)."""
+    ETYPE = 'quad4r'
+
+
+class Tria3RData(_Data):
+    """Slot counts (tria3r.pyx:128-151)."""
+    ETYPE = 'tria3r'
+
+
+class BeamCData(_Data):
+    """Slot counts (beamc.pyx:23-46)."""
+    ETYPE = 'beamc'
+
+
+class _Probe:
+    NN = None
+
+    def __init__(self):
+        self.xe = np.zeros(self.NN * DOF // 2)
+        self.ue = np.zeros(self.NN * DOF)
+        self.finte = np.zeros(self.NN * DOF)
+
+
+class Quad4Probe(_Probe):
+    """Per-element scratch of the reference's Quad4 (quad4.pyx:236-271)."""
+    NN = 4
+
+
+class Quad4RProbe(_Probe):
+    NN = 4
+
+
+class Tria3RProbe(_Probe):
+    NN = 3
+
+
+class BeamCProbe(_Probe):
+    NN = 2
+
+
+_R_NAMES = ['r11', 'r12', 'r13', 'r21', 'r22', 'r23', 'r31', 'r32', 'r33']
+
+
+class _Element:
+    ETYPE = None
+    NN = None
+
+    def __init__(self, p):
+        self.probe = p
+        self.eid = -1
+        self.pid = -1
+        for i in range(1, self.NN + 1):
+            setattr(self, 'n%d' % i, -1)
+            setattr(self, 'c%d' % i, -1)
+        self.init_k_KC0 = 0
+        self.init_k_KG = 0
+        self.init_k_M = 0
+        for n in _R_NAMES:
+            setattr(self, n, 0.)
+        self._xl = None       # node coordinates (nn, 3), from the latest x given
+        self._ug = None       # node displacements in global axes (nn, 6)
+
+    # -- helpers --------------------------------------------------------
+    def _cs(self):
+        cs = np.array([getattr(self, 'c%d' % i) for i in range(1, self.NN + 1)], dtype=np.int64)
+        if (cs < 0).any():
+            raise ValueError('%s: c1..c%d must be set (6 * node position)'
+                             % (type(self).__name__, self.NN))
+        return cs
+
+    def _gather_x(self, x):
+        x = np.asarray(x, dtype=np.float64)
+        cs = self._cs()
+        self._xl = np.stack([x[c // 2:c // 2 + 3] for c in cs])
+
+    def _conn(self):
+        return _t(np.arange(self.NN, dtype=np.int64)[None] * DOF, torch.int64)
+
+    def _need_x(self):
+        if self._xl is None:
+            raise RuntimeError('call update_rotation_matrix / update_probe_xe first')
+
+    def _rot(self):
+        return np.array([getattr(self, n) for n in _R_NAMES]).reshape(3, 3)
+
+    def _set_geometry(self, geom):
+        for n, v in zip(_R_NAMES, geom[:9]):
+            setattr(self, n, float(v))
+
+    def _scatter(self, size, k0, rl, cl, vl, r, c, v, values_only):
+        """Accumulate the element's values at [k0, k0 + size) and write its
+        global rows/cols (local DOF 6 I + d -> c_I + d)."""
+        cs = self._cs()
+        v[k0:k0 + size] += vl
+        if not values_only:
+            r[k0:k0 + size] = cs[rl // DOF] + rl % DOF
+            c[k0:k0 + size] = cs[cl // DOF] + cl % DOF
+
+    def _buffers(self, size):
+        r = torch.zeros(size, dtype=torch.int64, device=_DEV)
+        return r, torch.zeros_like(r), torch.zeros(size, dtype=torch.float64, device=_DEV)
+
+    @staticmethod
+    def _host(*ts):
+        torch.cuda.synchronize()
+        return [t.cpu().numpy() for t in ts]
+
+    def update_probe_ue(self, u):
+        """probe.ue = element displacements in local axes (blockdiag(R^T,
+        R^T) u per node, quad4.pyx:627-680); kept in global axes too for
+        update_KG / update_fint."""
+        u = np.asarray(u, dtype=np.float64)
+        cs = self._cs()
+        self._ug = np.stack([u[c:c + DOF] for c in cs])
+        R = self._rot()
+        for i in range(self.NN):
+            self.probe.ue[6 * i:6 * i + 3] = R.T @ self._ug[i, :3]
+            self.probe.ue[6 * i + 3:6 * i + 6] = R.T @ self._ug[i, 3:]
+
+    def _need_u(self):
+        if self._ug is None:
+            raise RuntimeError('call update_probe_ue(u) first')
+
+
+class _Shell(_Element):
+    def __init__(self, p):
+        super().__init__(p)
+        self.area = 0.
+        self.K6ROT = 100.
+        self.m11, self.m12, self.m21, self.m22 = 1., 0., 0., 1.
+        self._xmat = np.zeros(3)
+
+    def _geometry(self):
+        geom, xe = A.geometry(self.ETYPE, _t(self._xl.ravel()), self._conn(),
+                              _t(self._xmat[None]))
+        geom, xe = self._host(geom, xe)
+        return geom[0], xe[0]
+
+    def update_rotation_matrix(self, x, xmati=0., xmatj=0., xmatk=0.):
+        """r11..r33 (local to global) and m11..m22 (material direction),
+        quad4.pyx:491-625."""
+        self._gather_x(x)
+        self._xmat = np.array([xmati, xmatj, xmatk], dtype=np.float64)
+        geom, _ = self._geometry()
+        self._set_geometry(geom)
+        self.m11, self.m12, self.m21, self.m22 = (float(v) for v in geom[9:13])
+
+    def update_probe_xe(self, x):
+        """probe.xe = node coordinates in element axes and the area
+        (quad4.pyx:682-752)."""
+        self._gather_x(x)
+        geom, xe = self._geometry()
+        self.probe.xe[:] = xe
+        self.area = float(geom[13])
+
+    def update_area(self):
+        self._need_x()
+        self.area = float(self._geometry()[0][13])
+
+    def _kw(self):
+        return dict(xmat=_t(self._xmat[None]))
+
+    def _kc0_kw(self):
+        return dict(k6rot=_t([self.K6ROT]))
+
+    def update_KC0(self, KC0r, KC0c, KC0v, prop, update_KC0v_only=0):
+        """Constitutive stiffness slots (quad4.pyx:1204-1313)."""
+        self._need_x()
+        size = A.slot_count(self.ETYPE, 'KC0')
+        r, c, v = self._buffers(size)
+        A.shell_kc0(self.ETYPE, _t(self._xl.ravel()), self._conn(), r, c, v,
+                    _t(_shell_record(prop)[None]), overwrite=True, **self._kw(),
+                    **self._kc0_kw())
+        self._scatter(size, self.init_k_KC0, *self._host(r, c, v), KC0r, KC0c, KC0v,
+                      update_KC0v_only)
+
+    def _kg(self, KGr, KGc, KGv, prop, values_only):
+        self._need_x()
+        self._need_u()
+        size = A.slot_count(self.ETYPE, 'KG')
+        r, c, v = self._buffers(size)
+        A.shell_kg(self.ETYPE, _t(self._xl.ravel()), self._conn(), _t(self._ug.ravel()), r, c,
+                   v, _t(_shell_record(prop)[None]), overwrite=True, **self._kw())
+        self._scatter(size, self.init_k_KG, *self._host(r, c, v), KGr, KGc, KGv, values_only)
+
+    def update_KG_given_stress(self, Nxx, Nyy, Nxy, KGr, KGc, KGv, update_KGv_only=0):
+        """Geometric stiffness of a given membrane stress state
+        (quad4.pyx:2259-3081)."""
+        self._need_x()
+        size = A.slot_count(self.ETYPE, 'KG')
+        r, c, v = self._buffers(size)
+        A.shell_kg_stress(self.ETYPE, _t(self._xl.ravel()), self._conn(),
+                          _t([Nxx, Nyy, Nxy]), r, c, v, overwrite=True, **self._kw())
+        self._scatter(size, self.init_k_KG, *self._host(r, c, v), KGr, KGc, KGv,
+                      update_KGv_only)
+
+    def update_M(self, Mr, Mc, Mv, prop, mtype=0):
+        """Mass slots, mtype 0 consistent, 1 reduced, 2 lumped
+        (quad4.pyx:3083-9489)."""
+        self._need_x()
+        size = A.slot_count(self.ETYPE, 'M', mtype)
+        r, c, v = self._buffers(size)
+        A.shell_m(self.ETYPE, _t(self._xl.ravel()), self._conn(), r, c, v,
+                  _t(_shell_record(prop)[None]), mtype=mtype, overwrite=True, **self._kw())
+        self._scatter(size, self.init_k_M, *self._host(r, c, v), Mr, Mc, Mv, False)
+
+    def _fe_kw(self):
+        return dict(k6rot=_t([self.K6ROT]))
+
+    def update_fint(self, fint, prop):
+        """fint += element internal forces in global axes; probe.finte =
+        the same in local axes (quad4.pyx:1174-1202, 1316-1363)."""
+        self._need_x()
+        self._need_u()
+        fe = A.shell_fe(self.ETYPE, _t(self._xl.ravel()), self._conn(), _t(self._ug.ravel()),
+                        _t(_shell_record(prop)[None]), xmat=_t(self._xmat[None]),
+                        **self._fe_kw())
+        fe = self._host(fe)[0][0]
+        cs = self._cs()
+        R = self._rot()
+        for i in range(self.NN):
+            fint[cs[i]:cs[i] + DOF] += fe[i]
+            self.probe.finte[6 * i:6 * i + 3] = R.T @ fe[i, :3]
+            self.probe.finte[6 * i + 3:6 * i + 6] = R.T @ fe[i, 3:]
+
+
+class Quad4(_Shell):
+    """Quad4 shell element (quad4.pyx:398-489)."""
+    ETYPE = 'quad4'
+    NN = 4
+
+    def __init__(self, p):
+        super().__init__(p)
+        self.init_k_KA_beta = 0
+        self.init_k_KA_gamma = 0
+        self.init_k_CA = 0
+
+    def _piston(self, matrix, k0, r_out, c_out, v_out):
+        self._need_x()
+        r, c, v = self._buffers(144)
+        A.shell_piston(self.ETYPE, matrix, _t(self._xl.ravel()), self._conn(), r, c, v,
+                       overwrite=True)
+        self._scatter(144, k0, *self._host(r, c, v), r_out, c_out, v_out, False)
+
+    def update_KA_beta(self, KA_betar, KA_betac, KA_betav):
+        """Piston-theory KA_beta (quad4.pyx:9491-10311)."""
+        self._piston('KA_beta', self.init_k_KA_beta, KA_betar, KA_betac, KA_betav)
+
+    def update_KA_gamma(self, KA_gammar, KA_gammac, KA_gammav):
+        """Piston-theory KA_gamma (quad4.pyx:10312-11114)."""
+        self._piston('KA_gamma', self.init_k_KA_gamma, KA_gammar, KA_gammac, KA_gammav)
+
+    def update_CA(self, CAr, CAc, CAv):
+        """Piston-theory aerodynamic damping CA (quad4.pyx:11115-11917)."""
+        self._piston('CA', self.init_k_CA, CAr, CAc, CAv)
+
+    def update_KG(self, KGr, KGc, KGv, prop):
+        """Geometric stiffness from probe.ue (quad4.pyx:1365-2257; Quad4
+        always writes the indices)."""
+        self._kg(KGr, KGc, KGv, prop, False)
+
+
+class Quad4R(Quad4):
+    """Quad4R shell element, reduced integration (quad4r.pyx:205-285)."""
+    ETYPE = 'quad4r'
+
+    def update_KC0(self, KC0r, KC0c, KC0v, prop, update_KC0v_only=0, hgfactor_u=1.,
+                   hgfactor_v=1., hgfactor_w=1., hgfactor_rx=1., hgfactor_ry=1.):
+        """Constitutive stiffness with hourglass control factors
+        (quad4r.pyx:1145-4619)."""
+        self._hg = [hgfactor_u, hgfactor_v, hgfactor_w, hgfactor_rx, hgfactor_ry]
+        try:
+            super().update_KC0(KC0r, KC0c, KC0v, prop, update_KC0v_only)
+        finally:
+            self._hg = None
+
+    def _kc0_kw(self):
+        hg = getattr(self, '_hg', None)
+        return dict(k6rot=_t([self.K6ROT]), hg=_t([hg if hg is not None else [1.] * 5]))
+
+    def update_KG(self, KGr, KGc, KGv, prop, update_KGv_only=0):
+        """Geometric stiffness from probe.ue (quad4r.pyx:4681-5573)."""
+        self._kg(KGr, KGc, KGv, prop, update_KGv_only)
+
+    def _fe_kw(self):
+        return self._kc0_kw()
+
+    def update_fint(self, fint, prop, hgfactor_u=1., hgfactor_v=1., hgfactor_w=1.,
+                    hgfactor_rx=1., hgfactor_ry=1.):
+        """fint += internal forces with hourglass control factors
+        (quad4r.pyx:549-719, 4620-4680)."""
+        self._hg = [hgfactor_u, hgfactor_v, hgfactor_w, hgfactor_rx, hgfactor_ry]
+        try:
+            super().update_fint(fint, prop)
+        finally:
+            self._hg = None
+
+
+class Tria3R(_Shell):
+    """Tria3R shell element (tria3r.pyx:205-292)."""
+    ETYPE = 'tria3r'
+    NN = 3
+
+    def __init__(self, p):
+        super().__init__(p)
+        self.alpha_shear_locking = 0.7
+
+    def _kc0_kw(self):
+        return dict(k6rot=_t([self.K6ROT]), alpha=_t([self.alpha_shear_locking]))
+
+    def _fe_kw(self):
+        return self._kc0_kw()
+
+    def update_KG(self, KGr, KGc, KGv, prop, update_KGv_only=0):
+        """Geometric stiffness from probe.ue (tria3r.pyx:3020-3575)."""
+        self._kg(KGr, KGc, KGv, prop, update_KGv_only)
+
+
+class BeamC(_Element):
+    """BeamC Timoshenko beam element (beamc.pyx:99-157)."""
+    ETYPE = 'beamc'
+    NN = 2
+
+    def __init__(self, p):
+        super().__init__(p)
+        self.length = 0.
+        self.vxyi = self.vxyj = self.vxyk = 0.
+
+    def _vxy(self):
+        return _t([[self.vxyi, self.vxyj, self.vxyk]])
+
+    def _geometry(self):
+        geom, xe = A.geometry('beamc', _t(self._xl.ravel()), self._conn(), self._vxy())
+        geom, xe = self._host(geom, xe)
+        return geom[0], xe[0]
+
+    def update_rotation_matrix(self, vxyi, vxyj, vxyk, x):
+        """r11..r33 from the axis and the XY-plane vector (beamc.pyx:158-232)."""
+        self.vxyi, self.vxyj, self.vxyk = float(vxyi), float(vxyj), float(vxyk)
+        self._gather_x(x)
+        self._set_geometry(self._geometry()[0])
+
+    def update_probe_xe(self, x):
+        """probe.xe and length (beamc.pyx:287-350)."""
+        self._gather_x(x)
+        geom, xe = self._geometry()
+        self.probe.xe[:] = xe
+        self.length = float(geom[13])
+
+    def update_length(self):
+        self._need_x()
+        self.length = float(self._geometry()[0][13])
+
+    def update_KC0(self, KC0r, KC0c, KC0v, prop, update_KC0v_only=0):
+        """Constitutive stiffness slots (beamc.pyx:489-1353)."""
+        self._need_x()
+        r, c, v = self._buffers(144)
+        A.beam_kc0(_t(self._xl.ravel()), self._conn(), self._vxy(), r, c, v,
+                   _t(_beam_record(prop)[None]), overwrite=True)
+        self._scatter(144, self.init_k_KC0, *self._host(r, c, v), KC0r, KC0c, KC0v,
+                      update_KC0v_only)
+
+    def update_KG(self, KGr, KGc, KGv, prop, update_KGv_only=0):
+        """Geometric stiffness from probe.ue (beamc.pyx:1392-2180)."""
+        self._need_x()
+        self._need_u()
+        r, c, v = self._buffers(144)
+        A.beam_kg(_t(self._xl.ravel()), self._conn(), self._vxy(), _t(self._ug.ravel()), r, c,
+                  v, _t(_beam_record(prop)[None]), overwrite=True)
+        self._scatter(144, self.init_k_KG, *self._host(r, c, v), KGr, KGc, KGv,
+                      update_KGv_only)
+
+    def update_M(self, Mr, Mc, Mv, prop, mtype=0):
+        """Mass slots, mtype 0 consistent (144), 1 lumped (36)
+        (beamc.pyx:2181-3152)."""
+        self._need_x()
+        size = A.slot_count('beamc', 'M', mtype)
+        r, c, v = self._buffers(size)
+        A.beam_m(_t(self._xl.ravel()), self._conn(), self._vxy(), r, c, v,
+                 _t(_beam_record(prop)[None]), mtype=mtype, overwrite=True)
+        self._scatter(size, self.init_k_M, *self._host(r, c, v), Mr, Mc, Mv, False)
+
+    def update_fint(self, fint, prop):
+        """fint += element internal forces in global axes
+        (beamc.pyx:352-488, 1354-1391)."""
+        self._need_x()
+        self._need_u()
+        fe = A.beam_fe(_t(self._xl.ravel()), self._conn(), self._vxy(), _t(self._ug.ravel()),
+                       _t(_beam_record(prop)[None]))
+        fe = self._host(fe)[0][0]
+        cs = self._cs()
+        R = self._rot()
+        for i in range(2):
+            fint[cs[i]:cs[i] + DOF] += fe[i]
+            self.probe.finte[6 * i:6 * i + 3] = R.T @ fe[i, :3]
+            self.probe.finte[6 * i + 3:6 * i + 6] = R.T @ fe[i, 3:]
+
+
+# ---------------------------------------------------------------------------
+# one launch for a list of elements of one type (the loop of the reference's
+# callers, e.g. tests/test_quad4_static_point_load.py:64-78)
+# ---------------------------------------------------------------------------
+
+def _batch(elems, x, prop, init_attr):
+    if not elems:
+        raise ValueError('empty element list')
+    cls = type(elems[0])
+    if any(type(e) is not cls for e in elems):
+        raise ValueError('one element type per batch')
+    conn = np.stack([e._cs() for e in elems])
+    init_k = np.array([getattr(e, init_attr) for e in elems], dtype=np.int64)
+    return cls, _t(np.asarray(x, dtype=np.float64)), _t(conn, torch.int64), \
+        _t(init_k, torch.int64)
+
+
+def _write_back(KCr, KCc, KCv, r, c, v, values_only):
+    rr, cc, vv = _Element._host(r, c, v)
+    KCv += vv
+    if not values_only:
+        touched = rr >= 0
+        KCr[touched] = rr[touched]
+        KCc[touched] = cc[touched]
+
+
+def _slots(KCv):
+    n = len(KCv)
+    r = torch.full((n,), -1, dtype=torch.int64, device=_DEV)
+    return r, torch.full_like(r, -1), torch.zeros(n, dtype=torch.float64, device=_DEV)
+
+
+def update_KC0_batch(elems, x, KC0r, KC0c, KC0v, prop, update_KC0v_only=0):
+    """``for e in elems: e.update_KC0(KC0r, KC0c, KC0v, prop,
+    update_KC0v_only)`` as one launch (elements of one type; shells use
+    their material direction and K6ROT / alpha_shear_locking, beams their
+    vxy).  x: node coordinates (3 per node)."""
+    cls, xt, conn, init_k = _batch(elems, x, prop, 'init_k_KC0')
+    r, c, v = _slots(KC0v)
+    if cls is BeamC:
+        A.beam_kc0(xt, conn, _t([[e.vxyi, e.vxyj, e.vxyk] for e in elems]), r, c, v,
+                   _t(_beam_record(prop)[None]), init_k=init_k)
+    else:
+        kw = dict(xmat=_t(np.stack([e._xmat for e in elems])),
+                  k6rot=_t([e.K6ROT for e in elems]))
+        if cls is Tria3R:
+            kw['alpha'] = _t([e.alpha_shear_locking for e in elems])
+        A.shell_kc0(cls.ETYPE, xt, conn, r, c, v, _t(_shell_record(prop)[None]),
+                    init_k=init_k, **kw)
+    _write_back(KC0r, KC0c, KC0v, r, c, v, update_KC0v_only)
+
+
+def update_KG_batch(elems, x, u, KGr, KGc, KGv, prop):
+    """``update_probe_ue(u)`` + ``update_KG`` of every element, one launch."""
+    cls, xt, conn, init_k = _batch(elems, x, prop, 'init_k_KG')
+    r, c, v = _slots(KGv)
+    ut = _t(np.asarray(u, dtype=np.float64))
+    if cls is BeamC:
+        A.beam_kg(xt, conn, _t([[e.vxyi, e.vxyj, e.vxyk] for e in elems]), ut, r, c, v,
+                  _t(_beam_record(prop)[None]), init_k=init_k)
+    else:
+        A.shell_kg(cls.ETYPE, xt, conn, ut, r, c, v, _t(_shell_record(prop)[None]),
+                   init_k=init_k, xmat=_t(np.stack([e._xmat for e in elems])))
+    _write_back(KGr, KGc, KGv, r, c, v, False)
+
+
+def update_M_batch(elems, x, Mr, Mc, Mv, prop, mtype=0):
+    """``update_M(Mr, Mc, Mv, prop, mtype)`` of every element, one launch."""
+    cls, xt, conn, init_k = _batch(elems, x, prop, 'init_k_M')
+    r, c, v = _slots(Mv)
+    if cls is BeamC:
+        A.beam_m(xt, conn, _t([[e.vxyi, e.vxyj, e.vxyk] for e in elems]), r, c, v,
+                 _t(_beam_record(prop)[None]), mtype=mtype, init_k=init_k)
+    else:
+        A.shell_m(cls.ETYPE, xt, conn, r, c, v, _t(_shell_record(prop)[None]), mtype=mtype,
+                  init_k=init_k, xmat=_t(np.stack([e._xmat for e in elems])))
+    _write_back(Mr, Mc, Mv, r, c, v, False)
