@@ -5,6 +5,7 @@ Quad4Probe, ...``).
 Same class names, public attributes, defaults and method signatures as the
 reference (``pyfe3d/__init__.py:11-27``; ``quad4.pyx:450-489``,
 ``quad4r.pyx:256-285``, ``tria3r.pyx:257-292``, ``beamc.pyx:130-157``,
+``beamlr.pyx:141-159``, ``truss.pyx:133-149``, ``spring.pyx:126-146``,
 ``beamprop.pyx:15-93``), and the same call order: ``update_rotation_matrix``
 then ``update_probe_xe`` (sets ``area``/``length`` and ``probe.xe``), then
 ``update_KC0`` / ``update_M`` / ``update_KG_given_stress``; ``update_KG``
@@ -82,8 +83,9 @@ class _Data:
     def __init__(self):
         e = self.ETYPE
         self.KC0_SPARSE_SIZE = A.slot_count(e, 'KC0')
-        self.KG_SPARSE_SIZE = A.slot_count(e, 'KG')
-        self.M_SPARSE_SIZE = A.slot_count(e, 'M', 0)
+        if e != 'truss':
+            self.KG_SPARSE_SIZE = A.slot_count(e, 'KG') if e != 'spring' else 0
+        self.M_SPARSE_SIZE = A.slot_count(e, 'M', 0) if e != 'spring' else 0
         if e in ('quad4', 'quad4r'):
             self.KA_BETA_SPARSE_SIZE = self.KA_GAMMA_SPARSE_SIZE = self.CA_SPARSE_SIZE = 144
 
@@ -106,6 +108,21 @@ class Tria3RData(_Data):
 class BeamCData(_Data):
     """Slot counts (beamc.pyx:23-46)."""
     ETYPE = 'beamc'
+
+
+class BeamLRData(_Data):
+    """Slot counts (beamlr.pyx:23-46)."""
+    ETYPE = 'beamlr'
+
+
+class TrussData(_Data):
+    """Slot counts (truss.pyx:25-47): KC0 and M only."""
+    ETYPE = 'truss'
+
+
+class SpringData(_Data):
+    """Slot counts (spring.pyx:20-43): KC0 only."""
+    ETYPE = 'spring'
 
 
 class _Probe:
@@ -131,6 +148,18 @@ class Tria3RProbe(_Probe):
 
 
 class BeamCProbe(_Probe):
+    NN = 2
+
+
+class BeamLRProbe(_Probe):
+    NN = 2
+
+
+class TrussProbe(_Probe):
+    NN = 2
+
+
+class SpringProbe(_Probe):
     NN = 2
 
 
@@ -416,6 +445,7 @@ class BeamC(_Element):
     """BeamC Timoshenko beam element (beamc.pyx:99-157)."""
     ETYPE = 'beamc'
     NN = 2
+    KG_SIZE = 144
 
     def __init__(self, p):
         super().__init__(p)
@@ -426,7 +456,7 @@ class BeamC(_Element):
         return _t([[self.vxyi, self.vxyj, self.vxyk]])
 
     def _geometry(self):
-        geom, xe = A.geometry('beamc', _t(self._xl.ravel()), self._conn(), self._vxy())
+        geom, xe = A.geometry(self.ETYPE, _t(self._xl.ravel()), self._conn(), self._vxy())
         geom, xe = self._host(geom, xe)
         return geom[0], xe[0]
 
@@ -452,7 +482,7 @@ class BeamC(_Element):
         self._need_x()
         r, c, v = self._buffers(144)
         A.beam_kc0(_t(self._xl.ravel()), self._conn(), self._vxy(), r, c, v,
-                   _t(_beam_record(prop)[None]), overwrite=True)
+                   _t(_beam_record(prop)[None]), overwrite=True, etype=self.ETYPE)
         self._scatter(144, self.init_k_KC0, *self._host(r, c, v), KC0r, KC0c, KC0v,
                       update_KC0v_only)
 
@@ -460,20 +490,20 @@ class BeamC(_Element):
         """Geometric stiffness from probe.ue (beamc.pyx:1392-2180)."""
         self._need_x()
         self._need_u()
-        r, c, v = self._buffers(144)
+        r, c, v = self._buffers(self.KG_SIZE)
         A.beam_kg(_t(self._xl.ravel()), self._conn(), self._vxy(), _t(self._ug.ravel()), r, c,
-                  v, _t(_beam_record(prop)[None]), overwrite=True)
-        self._scatter(144, self.init_k_KG, *self._host(r, c, v), KGr, KGc, KGv,
+                  v, _t(_beam_record(prop)[None]), overwrite=True, etype=self.ETYPE)
+        self._scatter(self.KG_SIZE, self.init_k_KG, *self._host(r, c, v), KGr, KGc, KGv,
                       update_KGv_only)
 
     def update_M(self, Mr, Mc, Mv, prop, mtype=0):
         """Mass slots, mtype 0 consistent (144), 1 lumped (36)
         (beamc.pyx:2181-3152)."""
         self._need_x()
-        size = A.slot_count('beamc', 'M', mtype)
+        size = A.slot_count(self.ETYPE, 'M', mtype)
         r, c, v = self._buffers(size)
         A.beam_m(_t(self._xl.ravel()), self._conn(), self._vxy(), r, c, v,
-                 _t(_beam_record(prop)[None]), mtype=mtype, overwrite=True)
+                 _t(_beam_record(prop)[None]), mtype=mtype, overwrite=True, etype=self.ETYPE)
         self._scatter(size, self.init_k_M, *self._host(r, c, v), Mr, Mc, Mv, False)
 
     def update_fint(self, fint, prop):
@@ -482,7 +512,7 @@ class BeamC(_Element):
         self._need_x()
         self._need_u()
         fe = A.beam_fe(_t(self._xl.ravel()), self._conn(), self._vxy(), _t(self._ug.ravel()),
-                       _t(_beam_record(prop)[None]))
+                       _t(_beam_record(prop)[None]), etype=self.ETYPE)
         fe = self._host(fe)[0][0]
         cs = self._cs()
         R = self._rot()
@@ -490,6 +520,124 @@ class BeamC(_Element):
             fint[cs[i]:cs[i] + DOF] += fe[i]
             self.probe.finte[6 * i:6 * i + 3] = R.T @ fe[i, :3]
             self.probe.finte[6 * i + 3:6 * i + 6] = R.T @ fe[i, 3:]
+
+
+class BeamLR(BeamC):
+    """BeamLR linear Timoshenko beam, reduced integration (beamlr.pyx:99-159):
+    BeamC's interface, KG over the rotations (36 slots)."""
+    ETYPE = 'beamlr'
+    KG_SIZE = 36
+
+
+class _Rod(_Element):
+    def _add_fe(self, fe, fint):
+        cs = self._cs()
+        R = self._rot()
+        for i in range(2):
+            fint[cs[i]:cs[i] + DOF] += fe[i]
+            self.probe.finte[6 * i:6 * i + 3] = R.T @ fe[i, :3]
+            self.probe.finte[6 * i + 3:6 * i + 6] = R.T @ fe[i, 3:]
+
+
+class Truss(_Rod):
+    """Truss element: axial and torsional stiffness (truss.pyx:99-149)."""
+    ETYPE = 'truss'
+    NN = 2
+
+    def __init__(self, p):
+        super().__init__(p)
+        self.length = 0.
+
+    def _geometry(self):
+        geom, xe = A.geometry('truss', _t(self._xl.ravel()), self._conn())
+        geom, xe = self._host(geom, xe)
+        return geom[0], xe[0]
+
+    def update_rotation_matrix(self, x):
+        """r11..r33 from the node positions (truss.pyx:203-273)."""
+        self._gather_x(x)
+        self._set_geometry(self._geometry()[0])
+
+    def update_probe_xe(self, x):
+        """probe.xe and length (truss.pyx:274-338)."""
+        self._gather_x(x)
+        geom, xe = self._geometry()
+        self.probe.xe[:] = xe
+        self.length = float(geom[13])
+
+    def update_length(self):
+        self._need_x()
+        self.length = float(self._geometry()[0][13])
+
+    def update_KC0(self, KC0r, KC0c, KC0v, prop, update_KC0v_only=0):
+        """Constitutive stiffness slots (truss.pyx:386-801)."""
+        self._need_x()
+        r, c, v = self._buffers(72)
+        A.truss_kc0(_t(self._xl.ravel()), self._conn(), r, c, v, _t(_beam_record(prop)[None]),
+                    overwrite=True)
+        self._scatter(72, self.init_k_KC0, *self._host(r, c, v), KC0r, KC0c, KC0v,
+                      update_KC0v_only)
+
+    def update_M(self, Mr, Mc, Mv, prop, mtype=0):
+        """Mass slots, mtype 0 consistent (144), 1 lumped (36)
+        (truss.pyx:838-1800)."""
+        self._need_x()
+        size = A.slot_count('truss', 'M', mtype)
+        r, c, v = self._buffers(size)
+        A.truss_m(_t(self._xl.ravel()), self._conn(), r, c, v, _t(_beam_record(prop)[None]),
+                  mtype=mtype, overwrite=True)
+        self._scatter(size, self.init_k_M, *self._host(r, c, v), Mr, Mc, Mv, False)
+
+    def update_fint(self, fint, prop):
+        """fint += internal forces (truss.pyx:339-385, 802-837)."""
+        self._need_x()
+        self._need_u()
+        fe = A.rod_fe('truss', _t(self._xl.ravel()), self._conn(), _t(self._ug.ravel()),
+                      _t(_beam_record(prop)[None]))
+        self._add_fe(self._host(fe)[0][0], fint)
+
+
+class Spring(_Rod):
+    """Spring element: six stiffnesses along element axes given by a
+    direction and an XY-plane vector (spring.pyx:99-146)."""
+    ETYPE = 'spring'
+    NN = 2
+
+    def __init__(self, p):
+        super().__init__(p)
+        self.kxe = self.kye = self.kze = 0.
+        self.krxe = self.krye = self.krze = 0.
+        self.vxyi = self.vxyj = self.vxyk = 0.
+        self.r11 = self.r22 = self.r33 = 1.
+        self._axes = np.array([1., 0., 0., 0., 1., 0.])
+        self._xl = np.zeros((2, 3))   # springs have no length: positions unused
+
+    def update_rotation_matrix(self, xi, xj, xk, vxyi, vxyj, vxyk):
+        """r11..r33 from the x direction and the XY-plane vector
+        (spring.pyx:147-206)."""
+        self.vxyi, self.vxyj, self.vxyk = float(vxyi), float(vxyj), float(vxyk)
+        self._axes = np.array([xi, xj, xk, vxyi, vxyj, vxyk], dtype=np.float64)
+        geom, _ = A.geometry('spring', _t(self._xl.ravel()), self._conn(),
+                             _t(self._axes[None]))
+        self._set_geometry(self._host(geom)[0][0])
+
+    def _k(self):
+        return _t([[self.kxe, self.kye, self.kze, self.krxe, self.krye, self.krze]])
+
+    def update_KC0(self, KC0r, KC0c, KC0v, update_KC0v_only=0):
+        """Stiffness slots from kxe..krze (spring.pyx:295-707)."""
+        r, c, v = self._buffers(72)
+        A.spring_kc0(_t(self._xl.ravel()), self._conn(), _t(self._axes[None]), r, c, v,
+                     self._k(), overwrite=True)
+        self._scatter(72, self.init_k_KC0, *self._host(r, c, v), KC0r, KC0c, KC0v,
+                      update_KC0v_only)
+
+    def update_fint(self, fint):
+        """fint += spring forces (spring.pyx:262-294, 708-739)."""
+        self._need_u()
+        fe = A.rod_fe('spring', _t(self._xl.ravel()), self._conn(), _t(self._ug.ravel()),
+                      self._k(), axes=_t(self._axes[None]))
+        self._add_fe(self._host(fe)[0][0], fint)
 
 
 # ---------------------------------------------------------------------------
@@ -503,6 +651,9 @@ def _batch(elems, x, prop, init_attr):
     cls = type(elems[0])
     if any(type(e) is not cls for e in elems):
         raise ValueError('one element type per batch')
+    if issubclass(cls, _Rod):
+        raise NotImplementedError('batch helpers take shells and beams; use assembly.truss_kc0 '
+                                  '/ truss_m / spring_kc0 for rods')
     conn = np.stack([e._cs() for e in elems])
     init_k = np.array([getattr(e, init_attr) for e in elems], dtype=np.int64)
     return cls, _t(np.asarray(x, dtype=np.float64)), _t(conn, torch.int64), \
@@ -531,9 +682,9 @@ def update_KC0_batch(elems, x, KC0r, KC0c, KC0v, prop, update_KC0v_only=0):
     vxy).  x: node coordinates (3 per node)."""
     cls, xt, conn, init_k = _batch(elems, x, prop, 'init_k_KC0')
     r, c, v = _slots(KC0v)
-    if cls is BeamC:
+    if issubclass(cls, BeamC):
         A.beam_kc0(xt, conn, _t([[e.vxyi, e.vxyj, e.vxyk] for e in elems]), r, c, v,
-                   _t(_beam_record(prop)[None]), init_k=init_k)
+                   _t(_beam_record(prop)[None]), init_k=init_k, etype=cls.ETYPE)
     else:
         kw = dict(xmat=_t(np.stack([e._xmat for e in elems])),
                   k6rot=_t([e.K6ROT for e in elems]))
@@ -549,9 +700,9 @@ def update_KG_batch(elems, x, u, KGr, KGc, KGv, prop):
     cls, xt, conn, init_k = _batch(elems, x, prop, 'init_k_KG')
     r, c, v = _slots(KGv)
     ut = _t(np.asarray(u, dtype=np.float64))
-    if cls is BeamC:
+    if issubclass(cls, BeamC):
         A.beam_kg(xt, conn, _t([[e.vxyi, e.vxyj, e.vxyk] for e in elems]), ut, r, c, v,
-                  _t(_beam_record(prop)[None]), init_k=init_k)
+                  _t(_beam_record(prop)[None]), init_k=init_k, etype=cls.ETYPE)
     else:
         A.shell_kg(cls.ETYPE, xt, conn, ut, r, c, v, _t(_shell_record(prop)[None]),
                    init_k=init_k, xmat=_t(np.stack([e._xmat for e in elems])))
@@ -562,9 +713,9 @@ def update_M_batch(elems, x, Mr, Mc, Mv, prop, mtype=0):
     """``update_M(Mr, Mc, Mv, prop, mtype)`` of every element, one launch."""
     cls, xt, conn, init_k = _batch(elems, x, prop, 'init_k_M')
     r, c, v = _slots(Mv)
-    if cls is BeamC:
+    if issubclass(cls, BeamC):
         A.beam_m(xt, conn, _t([[e.vxyi, e.vxyj, e.vxyk] for e in elems]), r, c, v,
-                 _t(_beam_record(prop)[None]), mtype=mtype, init_k=init_k)
+                 _t(_beam_record(prop)[None]), mtype=mtype, init_k=init_k, etype=cls.ETYPE)
     else:
         A.shell_m(cls.ETYPE, xt, conn, r, c, v, _t(_shell_record(prop)[None]), mtype=mtype,
                   init_k=init_k, xmat=_t(np.stack([e._xmat for e in elems])))

@@ -249,3 +249,86 @@ def test_beamc_and_piston_through_facade():
                    np.zeros(Eq, np.int64), np.zeros((1, 32)), rr, cc, vv)
         assert (arrs[0] == rr).all() and (arrs[1] == cc).all(), key
         assert np.abs(arrs[2] - vv).max() <= 1e-12 * np.abs(vv).max(), key
+
+
+@pytest.mark.parametrize('cls_name', ['BeamLR', 'Truss', 'Spring'])
+def test_beamlr_truss_spring_through_facade(cls_name):
+    """BeamLR (KC0, KG, M 0/1, fint), Truss (KC0, M 0/1, fint) and Spring
+    (KC0, fint) per element through the façade against the oracle on a
+    rotated 3-D polyline; BeamLR's batch helper gives the same bits."""
+    import pyfe3d_amd.compat as C
+    rng = np.random.default_rng(8)
+    n = 8
+    ncoords = np.cumsum(rng.uniform(0.1, 0.3, (n, 3)), axis=0)
+    xf = ncoords.flatten()
+    u = rng.uniform(-1, 1, 6 * n) * 1e-3
+    E = n - 1
+    etype = cls_name.lower()
+    conn = np.stack([np.arange(E), np.arange(1, n)], axis=1).astype(np.int64)
+    probe = getattr(C, cls_name + 'Probe')()
+    data = getattr(C, cls_name + 'Data')()
+    if cls_name == 'Spring':
+        k = np.array([1e5, 2e5, 3e5, 4e2, 5e2, 6e2])
+        xdir = rng.normal(size=(E, 3))
+        vxy = rng.normal(size=(E, 3))
+        ev = np.concatenate([xdir, vxy], axis=1)
+        props = k[None]
+    else:
+        prop = C.BeamProp()
+        prop.A, prop.E, prop.G, prop.Iyy, prop.Izz, prop.J = 2e-4, 70e9, 26e9, 3e-9, 5e-9, 6e-9
+        prop.intrho, prop.intrhoy2, prop.intrhoz2 = 2700 * 2e-4, 2700 * 5e-9, 2700 * 3e-9
+        props = O.props_table(prop.record()[None], etype)
+        ev = np.tile([0.2, 1., 0.1], (E, 1)) if cls_name == 'BeamLR' else np.zeros((E, 3))
+    keys = {'BeamLR': [('KC0', 0), ('KG', 0), ('M', 0), ('M', 1)],
+            'Truss': [('KC0', 0), ('M', 0), ('M', 1)], 'Spring': [('KC0', 0)]}[cls_name]
+    for key, mtype in keys:
+        size = O.slot_count(etype, key, mtype)
+        out = [np.zeros(size * E, np.int64), np.zeros(size * E, np.int64), np.zeros(size * E)]
+        fint = np.zeros(6 * n)
+        els = []
+        for i in range(E):
+            e = getattr(C, cls_name)(probe)
+            e.n1, e.n2, e.c1, e.c2 = i + 1, i + 2, 6 * i, 6 * (i + 1)
+            e.init_k_KC0 = e.init_k_KG = e.init_k_M = size * i
+            if cls_name == 'Spring':
+                e.kxe, e.kye, e.kze, e.krxe, e.krye, e.krze = k
+                e.update_rotation_matrix(*xdir[i], *vxy[i])
+            elif cls_name == 'BeamLR':
+                e.update_rotation_matrix(*ev[i], xf)
+                e.update_probe_xe(xf)
+            else:
+                e.update_rotation_matrix(xf)
+                e.update_probe_xe(xf)
+            e.update_probe_ue(u)
+            if key == 'KC0':
+                if cls_name == 'Spring':
+                    e.update_KC0(*out)
+                    e.update_fint(fint)
+                else:
+                    e.update_KC0(*out, prop)
+                    e.update_fint(fint, prop)
+            elif key == 'KG':
+                e.update_KG(*out, prop)
+            else:
+                e.update_M(*out, prop, mtype=mtype)
+            els.append(e)
+        rr = np.zeros(size * E, np.int64); cc = np.zeros_like(rr); vv = np.zeros(size * E)
+        O.assemble(etype, key, xf, conn, size * np.arange(E), ev, np.zeros(E, np.int64), props,
+                   rr, cc, vv, mtype=mtype, u=u)
+        assert (out[0] == rr).all() and (out[1] == cc).all(), key
+        assert np.abs(out[2] - vv).max() <= 1e-12 * np.abs(vv).max(), key
+        if key == 'KC0':
+            ref = O.fint(etype, xf, conn, ev, np.zeros(E, np.int64), props, u)
+            assert np.abs(fint - ref).max() <= 1e-12 * np.abs(ref).max()
+        if cls_name == 'BeamLR':
+            out2 = [np.zeros(size * E, np.int64), np.zeros(size * E, np.int64),
+                    np.zeros(size * E)]
+            if key == 'KC0':
+                C.update_KC0_batch(els, xf, *out2, prop)
+            elif key == 'KG':
+                C.update_KG_batch(els, xf, u, *out2, prop)
+            else:
+                C.update_M_batch(els, xf, *out2, prop, mtype=mtype)
+            for a1, a2 in zip(out, out2):
+                assert np.array_equal(a1, a2), key
+    assert data.KC0_SPARSE_SIZE == O.slot_count(etype, 'KC0')
