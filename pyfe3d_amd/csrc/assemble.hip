@@ -424,6 +424,28 @@ __device__ __forceinline__ void membrane_forces(const Shell &s, const double *Nx
 // reference layout, consecutive lanes computing consecutive slots from LDS,
 // so the stores are contiguous across lanes.
 // ------------------------------------------------------------------------
+// 16-byte vectors for the slot streaming; F_VEC (internal launch flag):
+// every element's first slot is even and v, r, c are 16-byte aligned
+using dbl2 = double __attribute__((ext_vector_type(2)));
+using lng2 = long long __attribute__((ext_vector_type(2)));
+constexpr int F_VEC = 4;
+
+// the COO output is written once and not read again by this kernel:
+// non-temporal stores keep it from evicting the inputs from the caches
+// (measured on MI355X: 1.12 -> 0.96 ms for 1e6 Quad4 KC0)
+#ifndef PF3D_NT_STORE
+#define PF3D_NT_STORE 1
+#endif
+template <typename T>
+__device__ __forceinline__ void store2(T *p, T v)
+{
+#if PF3D_NT_STORE
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+
 // LDS layout of one shell_stream block (also sizes the mixed-type kernels)
 template <int ET, int MAT>
 struct StreamLds {
@@ -607,11 +629,8 @@ __device__ __forceinline__ void shell_stream_body(const AsmArgs &a, int64_t blk,
     const int nel = (int)((a.n_elem - e0) < EPB ? (a.n_elem - e0) : EPB);
     const bool write_idx = !(a.flags & F_VALUES_ONLY);
     const bool overwrite = (a.flags & F_OVERWRITE);
-    // consecutive lanes write consecutive slots of the block's elements
-    const int total = nel * nslot;
-    for (int t = lane; t < total; t += 64) {
-        const int el = t / nslot, k = t - el * nslot;
-        int I, J, dr, dc;  // node pair and DOFs within the 6x6 block
+    // slot k of an element -> node pair (I, J) and DOFs (dr, dc) in its 6x6 block
+    auto slot_map = [&](int k, int &I, int &J, int &dr, int &dc) {
         if (kg_like(MAT)) {  // (I, a, J, b) over translations
             const int row = k / (3 * NN), col = k - row * 3 * NN;
             I = row / 3; dr = row - 3 * I;
@@ -629,10 +648,43 @@ __device__ __forceinline__ void shell_stream_body(const AsmArgs &a, int64_t blk,
             const int tt = col - 5 * J, skip = (dr + 3) % 6;
             dc = tt < skip ? tt : tt + 1;
         }
+    };
+    auto slot_val = [&](int el, int I, int J, int dr, int dc) {
         const double *d = sd[el];
-        double val;
-        if (kg_like(MAT)) val = (d[NP + dr] * d[I * NN + J]) * d[NP + dc];
-        else val = d[I * NN + J] * d[NP + 6 * dr + dc];
+        if (kg_like(MAT)) return (d[NP + dr] * d[I * NN + J]) * d[NP + dc];
+        return d[I * NN + J] * d[NP + 6 * dr + dc];
+    };
+    const int total = nel * nslot;
+    if (NN == 4 && (a.flags & F_VEC)) {
+        // pairs of consecutive slots (every row of these layouts has an even
+        // length, so a pair never crosses a row): one 16-byte store per pair;
+        // F_VEC: contiguous 16-byte aligned blocks of an even slot count
+        for (int t2 = lane; 2 * t2 < total; t2 += 64) {
+            const int t = 2 * t2;
+            const int el = t / nslot, k = t - el * nslot;
+            int I0, J0, dr0, dc0, I1, J1, dr1, dc1;
+            slot_map(k, I0, J0, dr0, dc0);
+            slot_map(k + 1, I1, J1, dr1, dc1);
+            const dbl2 val = {slot_val(el, I0, J0, dr0, dc0), slot_val(el, I1, J1, dr1, dc1)};
+            const int64_t slot = sk0[el] + k;
+            dbl2 *dst = reinterpret_cast<dbl2 *>(a.v + slot);
+            if (overwrite) store2(dst, val);
+            else *dst = *dst + val;
+            if (write_idx) {
+                store2(reinterpret_cast<lng2 *>(a.r + slot),
+                       lng2{sconn[el][I0] + dr0, sconn[el][I1] + dr1});
+                store2(reinterpret_cast<lng2 *>(a.c + slot),
+                       lng2{sconn[el][J0] + dc0, sconn[el][J1] + dc1});
+            }
+        }
+        return;
+    }
+    // consecutive lanes write consecutive slots of the block's elements
+    for (int t = lane; t < total; t += 64) {
+        const int el = t / nslot, k = t - el * nslot;
+        int I, J, dr, dc;
+        slot_map(k, I, J, dr, dc);
+        const double val = slot_val(el, I, J, dr, dc);
         const int64_t slot = sk0[el] + k;
         if (overwrite) __builtin_nontemporal_store(val, a.v + slot);
         else a.v[slot] += val;
@@ -662,27 +714,6 @@ __device__ __forceinline__ void upper_pair(int sub, int &I, int &J)
     J = I + sub - (I * NN - (I * (I - 1)) / 2);
 }
 
-// 16-byte vectors for the slot streaming; F_VEC (internal launch flag):
-// every element's first slot is even and v, r, c are 16-byte aligned
-using dbl2 = double __attribute__((ext_vector_type(2)));
-using lng2 = long long __attribute__((ext_vector_type(2)));
-constexpr int F_VEC = 4;
-
-// the COO output is written once and not read again by this kernel:
-// non-temporal stores keep it from evicting the inputs from the caches
-// (measured on MI355X: 1.12 -> 0.96 ms for 1e6 Quad4 KC0)
-#ifndef PF3D_NT_STORE
-#define PF3D_NT_STORE 1
-#endif
-template <typename T>
-__device__ __forceinline__ void store2(T *p, T v)
-{
-#if PF3D_NT_STORE
-    __builtin_nontemporal_store(v, p);
-#else
-    *p = v;
-#endif
-}
 
 template <int ET, int MAT>
 __global__ void __launch_bounds__(64) shell_stream(AsmArgs a)
@@ -1647,7 +1678,15 @@ static hipError_t launch_shell(const AsmArgs &a, hipStream_t s)
         constexpr int EPB = (MAT == MASS) ? 32 : 64;
         const int64_t blocks = (a.n_elem + EPB - 1) / EPB;
         if (blocks == 0) return hipSuccess;
-        hipLaunchKernelGGL((shell_stream<ET, MAT>), dim3((unsigned)blocks), dim3(64), 0, s, a);
+        // paired 16-byte stores: contiguous blocks (no init_k) of an even
+        // slot count (every Quad4/Quad4R layout) at 16-byte aligned arrays
+        AsmArgs b = a;
+        const bool vec = NN == 4 && !a.init_k && (reinterpret_cast<uintptr_t>(a.v) & 15) == 0 &&
+                         ((a.flags & F_VALUES_ONLY) ||
+                          ((reinterpret_cast<uintptr_t>(a.r) & 15) == 0 &&
+                           (reinterpret_cast<uintptr_t>(a.c) & 15) == 0));
+        if (vec) b.flags |= F_VEC;
+        hipLaunchKernelGGL((shell_stream<ET, MAT>), dim3((unsigned)blocks), dim3(64), 0, s, b);
         return hipGetLastError();
     }
 }

@@ -260,3 +260,42 @@ def test_nonfinite_elements_flags_degenerate_elements():
     expect = sorted(set(np.where(~np.isfinite(v))[0] // 576))
     assert sorted(perm[bad]) == expect
     assert count(x.ravel()) == len(bad)
+
+
+@pytest.mark.parametrize('etype', ['quad4', 'quad4r'])
+def test_stream_paired_stores_match_slot_path(etype):
+    """KG (from u and given stress), M (mtype 0, 1, 2) and piston theory
+    with contiguous slot blocks (init_k NULL: 16-byte paired stores) give
+    the same bits as with init_k = SIZE * e (one store per slot), in every
+    mode: accumulate, overwrite, values only."""
+    from pyfe3d_amd import assembly as A
+    x, conn, rng = _random_mesh(23, 17, 9)
+    E = conn.shape[0]
+    xt, ct = T(x, torch.float64), T(6 * conn, torch.int64)
+    rec = T(O.props_table(np.random.default_rng(2).uniform(1, 2, (1, 28)) * 1e6, etype),
+            torch.float64)
+    u = T(rng.uniform(-1, 1, x.size * 2) * 1e-3, torch.float64)
+    stress = T([-1e3, 300., 75.], torch.float64)
+    cases = [('KG', 0), ('KGs', 0), ('M', 0), ('M', 1), ('M', 2), ('KA_beta', 0),
+             ('KA_gamma', 0), ('CA', 0)]
+    for mat, mtype in cases:
+        size = A.slot_count(etype, 'KG' if mat == 'KGs' else mat, mtype)
+        for overwrite, values_only in ((False, False), (True, False), (True, True)):
+            outs = []
+            for init_k in (None, T(size * np.arange(E), torch.int64)):
+                r = torch.zeros(size * E, dtype=torch.int64, device=DEV)
+                c = torch.zeros_like(r)
+                v = torch.full((size * E,), 0.5, dtype=torch.float64, device=DEV)
+                kw = dict(init_k=init_k, overwrite=overwrite, values_only=values_only)
+                if mat == 'KG':
+                    A.shell_kg(etype, xt, ct, u, r, c, v, rec, **kw)
+                elif mat == 'KGs':
+                    A.shell_kg_stress(etype, xt, ct, stress, r, c, v, **kw)
+                elif mat == 'M':
+                    A.shell_m(etype, xt, ct, r, c, v, rec, mtype=mtype, **kw)
+                else:
+                    A.shell_piston(etype, mat, xt, ct, r, c, v, **kw)
+                outs.append((r, c, v))
+            (r0, c0, v0), (r1, c1, v1) = outs
+            assert torch.equal(v0, v1) and torch.equal(r0, r1) and torch.equal(c0, c1), \
+                (mat, mtype, overwrite, values_only)
