@@ -435,6 +435,22 @@ int pf3d_bsr_fused_f32(int64_t nb, const int32_t *row_ptr, const int32_t *col_id
                        const float *dinv, const float *dold, float *dnew, float *xacc, double sa,
                        double sb, double c1, double c2, void *stream);
 
+/* pf3d_bsr_fused for nv <= 8 vectors at once, vector v of xin at offset
+ * v * ldx and of every other vector operand at v * ld (ld >= 6 nb; the
+ * two differ for a rectangular operator such as a prolongator): each block
+ * of the matrix is read once for all of them (the block eigensolver's
+ * preconditioner).  sa == 0 reads neither b nor r. */
+int pf3d_bsr_fused_mv(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx,
+                      const double *val, int nv, int64_t ldx, const double *xin, int64_t ld,
+                      const double *b, double *r, const double *dinv, const double *dold,
+                      double *dnew, double *xacc, double sa, double sb, double c1, double c2,
+                      void *stream);
+int pf3d_bsr_fused_mv_f32(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx,
+                          const float *val, int nv, int64_t ldx, const float *xin, int64_t ld,
+                          const float *b, float *r, const float *dinv, const float *dold,
+                          float *dnew, float *xacc, double sa, double sb, double c1, double c2,
+                          void *stream);
+
 /* y = K x for a 6x6 BSR matrix with nb block rows (FP64 / FP32). */
 int pf3d_bsr_spmv(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx, const double *val,
                   const double *x, double *y, void *stream);

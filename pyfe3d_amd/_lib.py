@@ -58,6 +58,10 @@ SIGNATURES = {
     'pf3d_bsr_residual_dd': [I64, P, P, P, P, P, P, P],
     'pf3d_bsr_fused': [I64, P, P, P, P, P, P, P, P, P, P, F64, F64, F64, F64, P],
     'pf3d_bsr_fused_f32': [I64, P, P, P, P, P, P, P, P, P, P, F64, F64, F64, F64, P],
+    'pf3d_bsr_fused_mv': [I64, P, P, P, I32, I64, P, I64, P, P, P, P, P, P, F64, F64, F64, F64,
+                          P],
+    'pf3d_bsr_fused_mv_f32': [I64, P, P, P, I32, I64, P, I64, P, P, P, P, P, P, F64, F64, F64,
+                              F64, P],
     'pf3d_cg_update_xr': [I64, P, P, P, P, P, P],
     'pf3d_cg_beta': [P, I32, P],
     'pf3d_cg_update_p': [I64, P, P, P, P],

@@ -380,6 +380,97 @@ class SAAMG:
             cur, nxt = nxt, cur
             rho_old = rho
 
+    def _mv_buffers(self, nv):
+        cache = self.__dict__.setdefault('_mv', {})
+        if nv in cache:
+            return cache[nv]
+        dev = self.coarse_inv.device
+        dt = torch.float64 if self.precision == 'fp64' else torch.float32
+        zeros = lambda m, t=dt: torch.zeros(m, dtype=t, device=dev)
+        bufs = []
+        for i, L in enumerate(self.levels):
+            m = L['n']
+            mc = self.levels[i + 1]['n'] if i + 1 < len(self.levels) else self.nc
+            bufs.append(dict(r=zeros(nv * m), d=zeros(nv * m), t=zeros(nv * m),
+                             bc=zeros(nv * mc), xc=zeros(nv * mc)))
+        extra = dict(c64=(zeros(nv * self.nc, torch.float64), zeros(nv * self.nc, torch.float64)))
+        if dt == torch.float32:
+            n0 = self.levels[0]['n']
+            extra['f32'] = (zeros(nv * n0), zeros(nv * n0))
+        cache[nv] = (bufs, extra)
+        return cache[nv]
+
+    def _cheb_mv(self, L, B, nv, b, x, x_zero=False):
+        """_cheb on nv vectors at once (pf3d_bsr_fused_mv)."""
+        lmax = 1.1 * L['lam']
+        lmin = self.cheb_lower * L['lam']
+        theta, delta = 0.5 * (lmax + lmin), 0.5 * (lmax - lmin)
+        sigma = theta / delta
+        rho_old = 1. / sigma
+        A, D, n = L['A'], L['Dinv'].val, L['n']
+        r, cur, nxt = B['r'], B['d'], B['t']
+        _fused_mv(A, nv, n, None if x_zero else x, n, b, r, dinv=D, dnew=cur, sa=1., sb=-1.,
+                  c1=1. / theta)
+        for k in range(self.cheb_degree):
+            if k == self.cheb_degree - 1:
+                _axpby(1., cur, 1., x)
+                break
+            rho = 1. / (2. * sigma - rho_old)
+            _fused_mv(A, nv, n, cur, n, None, r, dinv=D, dold=cur, dnew=nxt, xacc=x, sa=1.,
+                      sb=-1., c1=2. * rho / delta, c2=rho * rho_old)
+            cur, nxt = nxt, cur
+            rho_old = rho
+
+    def _vcycle_mv(self, nv, bufs, extra, b, x, lvl=0):
+        if lvl == len(self.levels):
+            nc = self.nc
+            if b.dtype == torch.float64:
+                b64, x64 = b, x
+            else:
+                b64, x64 = extra['c64']
+                _lib.call('pf3d_to_f64', nv * nc, b.data_ptr(), b64.data_ptr(), _s())
+            for v in range(nv):
+                _lib.call('pf3d_gemv_t', nc, nc, self.coarse_inv.data_ptr(), nc,
+                          b64[v * nc:].data_ptr(), x64[v * nc:].data_ptr(), 1.0, 0.0, _s())
+            if b.dtype != torch.float64:
+                _lib.call('pf3d_to_f32', nv * nc, x64.data_ptr(), x.data_ptr(), _s())
+            return x
+        L, B = self.levels[lvl], bufs[lvl]
+        n = L['n']
+        mc = self.levels[lvl + 1]['n'] if lvl + 1 < len(self.levels) else self.nc
+        _axpby(0., b, 0., x)
+        self._cheb_mv(L, B, nv, b, x, x_zero=True)
+        _fused_mv(L['A'], nv, n, x, n, b, B['r'], sa=1., sb=-1.)           # r = b - A x
+        _fused_mv(L['R'], nv, n, B['r'], mc, None, B['bc'], sa=0., sb=1.)  # bc = R r
+        self._vcycle_mv(nv, bufs, extra, B['bc'], B['xc'], lvl + 1)
+        _fused_mv(L['P'], nv, mc, B['xc'], n, None, x, sa=1., sb=1.)       # x += P xc
+        self._cheb_mv(L, B, nv, b, x)
+        return x
+
+    def apply_block(self, R, Z):
+        """Z[v] = M^-1 R[v] for the rows of R (nv x n, FP64, contiguous):
+        one V-cycle for all of them, each matrix block read once per pass
+        (pf3d_bsr_fused_mv).  Z must be a contiguous nv x n FP64 tensor."""
+        nv = R.shape[0]
+        if not self.levels or nv == 1:
+            for v in range(nv):
+                self.apply(R[v], Z[v])
+            return Z
+        if nv > 8:
+            for v0 in range(0, nv, 8):
+                self.apply_block(R[v0:v0 + 8], Z[v0:v0 + 8])
+            return Z
+        bufs, extra = self._mv_buffers(nv)
+        rf, zf = R.reshape(-1), Z.reshape(-1)
+        if self.precision == 'fp64':
+            self._vcycle_mv(nv, bufs, extra, rf, zf)
+        else:
+            r32, z32 = extra['f32']
+            _lib.call('pf3d_to_f32', rf.numel(), rf.data_ptr(), r32.data_ptr(), _s())
+            self._vcycle_mv(nv, bufs, extra, r32, z32)
+            _lib.call('pf3d_to_f64', zf.numel(), z32.data_ptr(), zf.data_ptr(), _s())
+        return Z
+
     def vcycle(self, b, x, lvl=0):
         """x = M^-1 b (x is overwritten)."""
         if lvl == len(self.levels):
@@ -408,6 +499,16 @@ class SAAMG:
 
 def _ptr(t):
     return None if t is None else t.data_ptr()
+
+
+def _fused_mv(M, nv, ldx, xin, ld, b, r, dinv=None, dold=None, dnew=None, xacc=None, sa=1.,
+              sb=1., c1=0., c2=0.):
+    """pf3d_bsr_fused_mv(_f32) with the operator M (a _Dev) on nv vectors:
+    xin's vectors ldx apart, the other operands' ld apart."""
+    fn = 'pf3d_bsr_fused_mv' if M.fn == 'pf3d_bsr_spmv' else 'pf3d_bsr_fused_mv_f32'
+    _lib.call(fn, M.nb, M.row_ptr.data_ptr(), M.col_idx.data_ptr(), M.val.data_ptr(), nv, ldx,
+              _ptr(xin), ld, _ptr(b), r.data_ptr(), _ptr(dinv), _ptr(dold), _ptr(dnew),
+              _ptr(xacc), float(sa), float(sb), float(c1), float(c2), _s())
 
 
 def _fused(M, xin, b, r, dinv=None, dold=None, dnew=None, xacc=None, sa=1., sb=1., c1=0.,

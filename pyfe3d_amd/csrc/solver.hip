@@ -192,6 +192,95 @@ int fused_launch(int64_t nb, const int32_t *rp, const int32_t *ci, const MT *val
     return pf3d_hip_check(hipGetLastError(), name);
 }
 
+// Multi-vector form of bsr_fused for nv <= MAXV vectors stored one after
+// another (vector v of each operand at v * ld): each block of A is read
+// once for all of them.  One block row per wave; sa == 0 reads neither b
+// nor r.
+constexpr int MAXV = 8;
+
+template <typename MT, typename VT>
+__global__ void __launch_bounds__(256)
+    bsr_fused_mv(int64_t nb, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                 const MT *__restrict__ val, int nv, int64_t ldx, const VT *__restrict__ xin,
+                 int64_t ld, const VT *__restrict__ b, VT *r, const MT *__restrict__ dinv,
+                 const VT *__restrict__ dold, VT *__restrict__ dnew, VT *__restrict__ xacc,
+                 double sa, double sb, double c1, double c2)
+{
+    __shared__ double part[4][MAXV][64];
+    __shared__ double rrow[4][MAXV][6];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * 4 + wave;
+    const int j = lane / 6, i = lane - 6 * (lane / 6);
+    double acc[MAXV];
+#pragma unroll
+    for (int v = 0; v < MAXV; v++) acc[v] = 0.;
+    if (xin && row < nb && j < 10) {
+        for (int bb = rp[row] + j; bb < rp[row + 1]; bb += 10) {
+            const MT *m = val + (int64_t)bb * 36 + i * 6;
+            double mv[6];
+#pragma unroll
+            for (int k = 0; k < 6; k++) mv[k] = (double)m[k];
+            const int64_t c6 = (int64_t)ci[bb] * 6;
+#pragma unroll
+            for (int v = 0; v < MAXV; v++) {
+                if (v < nv) {
+                    const VT *xv = xin + v * ldx + c6;
+#pragma unroll
+                    for (int k = 0; k < 6; k++) acc[v] += mv[k] * (double)xv[k];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < MAXV; v++) part[wave][v][lane] = acc[v];
+    __syncthreads();
+    // lanes (v, l): vector v, component l of the row
+    const int v = lane / 6, l = lane - 6 * (lane / 6);
+    const bool own = v < nv && row < nb;
+    if (own) {
+        double s = 0.;
+#pragma unroll
+        for (int jj = 0; jj < 10; jj++) s += part[wave][v][l + 6 * jj];
+        const int64_t k = v * ld + row * 6 + l;
+        const double base = sa == 0. ? 0. : sa * (double)(b ? b[k] : r[k]);
+        const double rv = base + sb * s;
+        r[k] = (VT)rv;
+        rrow[wave][v][l] = (double)(VT)rv;
+    }
+    if (!dinv && !xacc) return;
+    __syncthreads();
+    if (own) {
+        const int64_t k = v * ld + row * 6 + l;
+        const double dv = dold ? (double)dold[k] : 0.;
+        if (dinv) {
+            const MT *D = dinv + row * 36 + l * 6;
+            double t = 0.;
+#pragma unroll
+            for (int m = 0; m < 6; m++) t += (double)D[m] * rrow[wave][v][m];
+            dnew[k] = (VT)(c1 * t + c2 * dv);
+        }
+        if (xacc) xacc[k] = (VT)((double)xacc[k] + dv);
+    }
+}
+
+template <typename MT, typename VT>
+int fused_mv_launch(int64_t nb, const int32_t *rp, const int32_t *ci, const MT *val, int nv,
+                    int64_t ldx, const VT *xin, int64_t ld, const VT *b, VT *r, const MT *dinv,
+                    const VT *dold, VT *dnew, VT *xacc, double sa, double sb, double c1,
+                    double c2, void *stream, const char *name)
+{
+    if (nb < 0 || nv < 1 || nv > MAXV || ld < 6 * nb || (xin && ldx < 6) ||
+        (nb && (!r || (xin && (!rp || !ci || !val)) || (dinv && !dnew) || (xacc && !dold) ||
+                (dold && dold == dnew) ||
+                (xin && (xin == r || xin == dnew || xin == xacc)))))
+        return pf3d_fail(PF3D_E_ARG, "%s: bad arguments", name);
+    if (!nb) return PF3D_OK;
+    hipLaunchKernelGGL((bsr_fused_mv<MT, VT>), dim3(nblk(nb, 4)), dim3(256), 0,
+                       (hipStream_t)stream, nb, rp, ci, val, nv, ldx, xin, ld, b, r, dinv, dold,
+                       dnew, xacc, sa, sb, c1, c2);
+    return pf3d_hip_check(hipGetLastError(), name);
+}
+
 // ------------------------------------------------- compensated products
 // Row sums of shell stiffness matrices cancel by many digits (stiff
 // membrane and shear terms against soft bending and drilling terms), so
@@ -701,6 +790,28 @@ int pf3d_bsr_fused_f32(int64_t nb, const int32_t *row_ptr, const int32_t *col_id
 {
     return fused_launch<float, float>(nb, row_ptr, col_idx, val, xin, b, r, dinv, dold, dnew,
                                       xacc, sa, sb, c1, c2, stream, "pf3d_bsr_fused_f32");
+}
+
+int pf3d_bsr_fused_mv(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx,
+                      const double *val, int nv, int64_t ldx, const double *xin, int64_t ld,
+                      const double *b, double *r, const double *dinv, const double *dold,
+                      double *dnew, double *xacc, double sa, double sb, double c1, double c2,
+                      void *stream)
+{
+    return fused_mv_launch<double, double>(nb, row_ptr, col_idx, val, nv, ldx, xin, ld, b, r,
+                                           dinv, dold, dnew, xacc, sa, sb, c1, c2, stream,
+                                           "pf3d_bsr_fused_mv");
+}
+
+int pf3d_bsr_fused_mv_f32(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx,
+                          const float *val, int nv, int64_t ldx, const float *xin, int64_t ld,
+                          const float *b, float *r, const float *dinv, const float *dold,
+                          float *dnew, float *xacc, double sa, double sb, double c1, double c2,
+                          void *stream)
+{
+    return fused_mv_launch<float, float>(nb, row_ptr, col_idx, val, nv, ldx, xin, ld, b, r,
+                                         dinv, dold, dnew, xacc, sa, sb, c1, c2, stream,
+                                         "pf3d_bsr_fused_mv_f32");
 }
 
 int pf3d_bsr_dirichlet(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx, double *val,

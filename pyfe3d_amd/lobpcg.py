@@ -168,10 +168,15 @@ class LOBPCG:
             if rel[:k].max() <= tol or it == maxiter:
                 break
             # W = T(R), then its images
+            # zero on fixed DOFs: R is, and the V-cycle keeps them decoupled
+            # (identity rows, zero prolongator rows); one multi-vector
+            # V-cycle when the preconditioner has it
+            if hasattr(self.precond, 'apply_block'):
+                self.precond.apply_block(R, S[kb:2 * kb])
+            else:
+                for i in range(kb):
+                    self.precond.apply(R[i], S[kb + i])
             for i in range(kb):
-                # zero on fixed DOFs: R is, and the V-cycle keeps them
-                # decoupled (identity rows, zero prolongator rows)
-                self.precond.apply(R[i], S[kb + i])
                 self._mul(self.A, S[kb + i], AS[kb + i])
                 self._mul(self.B, S[kb + i], BS[kb + i])
             if m > kb:

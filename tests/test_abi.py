@@ -145,3 +145,8 @@ def test_argument_errors_of_newer_entry_points():
                 1., -1., 1., 0., None) == -1      # dinv without dnew
     assert call('pf3d_bsr_fused_f32', 4, fake, fake, fake, None, None, None, None, None, None,
                 None, 1., 1., 0., 0., None) == -1  # no r
+    # multi-vector fused step: vector count and strides are checked
+    assert call('pf3d_bsr_fused_mv', 4, fake, fake, fake, 9, 24, fake, 24, None, fake, None,
+                None, None, None, 1., -1., 0., 0., None) == -1    # nv > 8
+    assert call('pf3d_bsr_fused_mv_f32', 4, fake, fake, fake, 2, 24, None, 12, None, fake,
+                None, None, None, None, 1., 1., 0., 0., None) == -1  # ld < 6 nb

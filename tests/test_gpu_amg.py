@@ -160,3 +160,28 @@ def test_captured_cg_iteration_matches_eager(method):
         print(method, rep, ig, ie)
         assert ig == ie and torch.equal(ug, ue)
         assert ig['relres'] <= 1e-10
+
+
+@pytest.mark.parametrize('prec', ['fp64', 'fp32'])
+def test_block_vcycle_matches_single(prec):
+    """One V-cycle on nv vectors at once (the multi-vector kernels, 3, 8
+    and 10 = 8 + 2 vectors) against the single-vector V-cycle per vector:
+    within 1e-12 (FP64 levels) or 1e-5 (FP32 levels) of each result."""
+    from pyfe3d_amd.mesh import PlateMesh
+    from pyfe3d_amd.shellprop import isotropic_plate
+    m = PlateMesh(31, 23, 1.0, 0.7, jitter=0.2, seed=9)
+    rec = isotropic_plate(thickness=0.004, E=70e9, nu=0.33, calc_scf=True).record()
+    sysm = _system(m, m.quads, 'quad4', rec, m.clamped())
+    amg = sysm.setup_amg(m.ncoords, coarse_nodes=40, precisions=(prec,), graph=False)
+    assert len(amg.levels) >= 2
+    keep = torch.as_tensor(~m.clamped(), device=DEV).double()
+    g = torch.Generator(device='cpu').manual_seed(4)
+    tol = 1e-12 if prec == 'fp64' else 1e-5
+    for nv in (3, 8, 10):
+        R = ((torch.rand((nv, m.n_dof), generator=g, dtype=torch.float64) - 0.5).to(DEV) * keep)
+        Z = torch.empty_like(R)
+        amg.apply_block(R, Z)
+        for v in range(nv):
+            z = amg.apply(R[v].clone(), torch.empty_like(R[v]))
+            err = float((Z[v] - z).abs().max() / z.abs().max())
+            assert err <= tol, (prec, nv, v, err)
