@@ -451,6 +451,13 @@ int pf3d_bsr_fused_mv_f32(int64_t nb, const int32_t *row_ptr, const int32_t *col
                           float *dnew, float *xacc, double sa, double sb, double c1, double c2,
                           void *stream);
 
+/* y[v] = A x[v] for nv <= 8 vectors (ld apart, ld >= 6 nb) with the
+ * compensated accumulation of pf3d_bsr_residual_dd (b = NULL), each
+ * matrix block read once for all of them; the same bits per vector. */
+int pf3d_bsr_product_dd_mv(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx,
+                           const double *val, int nv, int64_t ld, const double *x, double *y,
+                           void *stream);
+
 /* y = K x for a 6x6 BSR matrix with nb block rows (FP64 / FP32). */
 int pf3d_bsr_spmv(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx, const double *val,
                   const double *x, double *y, void *stream);

@@ -56,6 +56,7 @@ SIGNATURES = {
     'pf3d_bsr_spmv': [I64, P, P, P, P, P, P],
     'pf3d_bsr_spmv_f32': [I64, P, P, P, P, P, P],
     'pf3d_bsr_residual_dd': [I64, P, P, P, P, P, P, P],
+    'pf3d_bsr_product_dd_mv': [I64, P, P, P, I32, I64, P, P, P],
     'pf3d_bsr_fused': [I64, P, P, P, P, P, P, P, P, P, P, F64, F64, F64, F64, P],
     'pf3d_bsr_fused_f32': [I64, P, P, P, P, P, P, P, P, P, P, F64, F64, F64, F64, P],
     'pf3d_bsr_fused_mv': [I64, P, P, P, I32, I64, P, I64, P, P, P, P, P, P, F64, F64, F64, F64,

@@ -338,6 +338,64 @@ __global__ void __launch_bounds__(256) bsr_residual_dd(int64_t nb, const int32_t
     }
 }
 
+// y[v] = A x[v] for nv <= MAXV vectors (ld apart) with the compensated
+// accumulation of bsr_residual_dd: each block of A read once for all of
+// them; per vector the same operations in the same order (same bits).
+__global__ void __launch_bounds__(256) bsr_product_dd_mv(int64_t nb,
+                                                         const int32_t *__restrict__ rp,
+                                                         const int32_t *__restrict__ ci,
+                                                         const double *__restrict__ val, int nv,
+                                                         int64_t ld, const double *__restrict__ x,
+                                                         double *__restrict__ y)
+{
+#pragma clang fp contract(off)
+    __shared__ double ps[4][MAXV][64], pc[4][MAXV][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * 4 + wave;
+    const int j = lane / 6, i = lane - 6 * (lane / 6);
+    double s[MAXV], c[MAXV];
+#pragma unroll
+    for (int v = 0; v < MAXV; v++) { s[v] = 0.; c[v] = 0.; }
+    if (row < nb && j < 10) {
+        const int b0 = rp[row], b1 = rp[row + 1];
+        for (int k = b0 + j; k < b1; k += 10) {
+            const double *m = val + (int64_t)k * 36 + i * 6;
+            double mv[6];
+#pragma unroll
+            for (int q = 0; q < 6; q++) mv[q] = m[q];
+            const int64_t c6 = (int64_t)ci[k] * 6;
+#pragma unroll
+            for (int v = 0; v < MAXV; v++) {
+                if (v < nv) {
+                    const double *xv = x + v * ld + c6;
+#pragma unroll
+                    for (int q = 0; q < 6; q++) {
+                        const double p = mv[q] * xv[q];
+                        const double e = fma(mv[q], xv[q], -p);
+                        c[v] += e;
+                        two_sum(s[v], c[v], p);
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < MAXV; v++) {
+        ps[wave][v][lane] = s[v];
+        pc[wave][v][lane] = c[v];
+    }
+    __syncthreads();
+    const int v = lane / 6, l = lane - 6 * (lane / 6);
+    if (row < nb && v < nv) {
+        double S = 0., C = 0.;
+        for (int jj = 0; jj < 10; jj++) {
+            two_sum(S, C, ps[wave][v][l + 6 * jj]);
+            C += pc[wave][v][l + 6 * jj];
+        }
+        y[v * ld + row * 6 + l] = S + C;
+    }
+}
+
 // ------------------------------------------------------ boundary conditions
 // K <- P K P + (I - P) with P = diag(1 - fixed)
 __global__ void bsr_bc(int64_t nb, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
@@ -749,6 +807,19 @@ int pf3d_bsr_spmv(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx, co
     if (!nb) return PF3D_OK;
     Mat A{nb, row_ptr, col_idx, val, nullptr};
     return pf3d_hip_check(spmv<double, double>(A, x, y, (hipStream_t)stream), "pf3d_bsr_spmv");
+}
+
+int pf3d_bsr_product_dd_mv(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx,
+                           const double *val, int nv, int64_t ld, const double *x, double *y,
+                           void *stream)
+{
+    if (nb < 0 || nv < 1 || nv > MAXV || ld < 6 * nb ||
+        (nb && (!row_ptr || !col_idx || !val || !x || !y || x == y)))
+        return pf3d_fail(PF3D_E_ARG, "pf3d_bsr_product_dd_mv: bad arguments");
+    if (!nb) return PF3D_OK;
+    hipLaunchKernelGGL(bsr_product_dd_mv, dim3(nblk(nb, 4)), dim3(256), 0, (hipStream_t)stream,
+                       nb, row_ptr, col_idx, val, nv, ld, x, y);
+    return pf3d_hip_check(hipGetLastError(), "pf3d_bsr_product_dd_mv");
 }
 
 int pf3d_bsr_residual_dd(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx,

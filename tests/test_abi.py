@@ -150,3 +150,5 @@ def test_argument_errors_of_newer_entry_points():
                 None, None, None, 1., -1., 0., 0., None) == -1    # nv > 8
     assert call('pf3d_bsr_fused_mv_f32', 4, fake, fake, fake, 2, 24, None, 12, None, fake,
                 None, None, None, None, 1., 1., 0., 0., None) == -1  # ld < 6 nb
+    assert call('pf3d_bsr_product_dd_mv', 4, fake, fake, fake, 0, 24, fake, fake, None) == -1
+    assert call('pf3d_bsr_product_dd_mv', 4, fake, fake, fake, 2, 24, fake, fake, None) == -1

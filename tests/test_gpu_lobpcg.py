@@ -114,3 +114,31 @@ def test_lobpcg_buckling_stiffened_panel_vs_eigsh():
         xi = Xh[bu, i]
         res = Kc @ xi + lam[i] * (KGc @ xi)
         assert np.linalg.norm(res) <= 1e-6 * np.linalg.norm(Kc @ xi)
+
+
+def test_block_product_matches_single():
+    """The multi-vector compensated product (3, 8 and 11 = 8 + 3 vectors)
+    gives the single-vector compensated product's bits for every vector."""
+    from pyfe3d_amd import _lib
+    from pyfe3d_amd.shellprop import isotropic_plate
+    m = _plate(13, 11, 0.3, 0.5)
+    rec = isotropic_plate(thickness=0.01, E=203e9, nu=0.33, calc_scf=True, rho=7830.).record()
+    rK, cK, vK = _gpu_coo('KC0', m, rec)
+    plan, K = _bsr(rK, cK, vK, m.n_nodes)
+    n = m.n_dof
+    g = torch.Generator(device='cpu').manual_seed(7)
+    s = torch.cuda.current_stream().cuda_stream
+    for nv in (3, 8, 11):
+        X = (torch.rand((nv, n), generator=g, dtype=torch.float64) - 0.5).to('cuda')
+        Y = torch.empty_like(X)
+        for v0 in range(0, nv, 8):
+            k = min(8, nv - v0)
+            _lib.call('pf3d_bsr_product_dd_mv', m.n_nodes, plan.row_ptr.data_ptr(),
+                      plan.col_idx.data_ptr(), K.data_ptr(), k, n, X[v0].data_ptr(),
+                      Y[v0].data_ptr(), s)
+        for v in range(nv):
+            y = torch.empty(n, dtype=torch.float64, device='cuda')
+            _lib.call('pf3d_bsr_residual_dd', m.n_nodes, plan.row_ptr.data_ptr(),
+                      plan.col_idx.data_ptr(), K.data_ptr(), X[v].data_ptr(), None,
+                      y.data_ptr(), s)
+            assert torch.equal(Y[v], y), (nv, v)
