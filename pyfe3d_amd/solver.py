@@ -203,8 +203,12 @@ class StaticSystem:
             if getattr(self, name, None) is None:
                 raise ValueError("method=%r needs setup_amg(xyz%s) first"
                                  % (method, '' if method == 'amg' else ", precision='fp32'"))
+            # the FP32 V-cycle gives out near the FP64 residual floor: leave
+            # it after 25 iterations without halving the residual (the FP64
+            # V-cycle then finishes), not the 100 the FP64 one is allowed
             u, info = amg_pcg(self, getattr(self, name), f, rtol=rtol,
-                              maxiter=min(int(maxiter), 100000), x0=x0)
+                              maxiter=min(int(maxiter), 100000), x0=x0,
+                              stall=25 if method == 'amg32' else 100)
             if (method == 'amg32' and info['relres'] > rtol
                     and getattr(self, 'amg', None) is not None):
                 # the FP32 V-cycle stalled: finish with the FP64 one from u
