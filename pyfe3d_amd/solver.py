@@ -136,6 +136,9 @@ class StaticSystem:
         the V-cycle captured in a HIP graph unless graph=False.  Returns the
         last one built."""
         from .amg import SAAMG, setup_hierarchy
+        if 'precision' in kw:
+            raise TypeError("setup_amg takes precisions=('fp64',) / ('fp32',) / "
+                            "('fp64', 'fp32'), not precision=")
         fixed = self.fixed.cpu().numpy().astype(bool)
         box = kw.pop('box', 3.0)
         coarse_nodes = kw.pop('coarse_nodes', 300)
@@ -157,8 +160,11 @@ class StaticSystem:
         return m
 
     def solve(self, f, method='fp64', rtol=1e-8, x0=None, maxiter=200000, inner_rtol=1e-4,
-              max_outer=60, replace=50, refactor=False, stream=None):
-        """Returns (u, info) with info = dict(iters, outer, relres).
+              max_outer=60, replace=50, refactor=False, fp32_maxiter=None, stream=None):
+        """Returns (u, info) with info = dict(iters, outer, relres, converged);
+        converged is relres <= rtol for the compensated FP64 relative
+        residual ||f - K u|| / ||f|| (a solve that stops short of rtol is
+        returned, flagged, not raised).
 
         method: 'fp64' (block-Jacobi PCG), 'mixed' (PCG on the FP32-stored
         matrix with FP64 vectors and a true FP64 residual replacing the
@@ -168,7 +174,10 @@ class StaticSystem:
         factor kept for later right-hand sides unless refactor), 'amg' (CG
         preconditioned by one smoothed-aggregation V-cycle per iteration,
         after setup_amg) or 'amg32' (the same with the V-cycle in FP32, after
-        setup_amg(xyz, precision='fp32'))."""
+        setup_amg(xyz, precisions=('fp32',)); when it stalls short of rtol, or
+        after fp32_maxiter iterations if given, the FP64 V-cycle finishes from
+        its u if setup_amg also built it, info['fp64_iters'] counting those
+        iterations; maxiter bounds both phases together)."""
         b = self.rhs(f)
         x = torch.zeros_like(b) if x0 is None else x0.clone()
         pr, pc = self.plan.row_ptr.data_ptr(), self.plan.col_idx.data_ptr()
@@ -201,22 +210,28 @@ class StaticSystem:
             from .amg import amg_pcg
             name = 'amg' if method == 'amg' else 'amg32'
             if getattr(self, name, None) is None:
-                raise ValueError("method=%r needs setup_amg(xyz%s) first"
-                                 % (method, '' if method == 'amg' else ", precision='fp32'"))
+                raise ValueError("method=%r needs setup_amg(xyz, precisions=(%r,)) first"
+                                 % (method, 'fp64' if method == 'amg' else 'fp32'))
+            budget = min(int(maxiter), 100000)
+            first = budget if fp32_maxiter is None or method == 'amg' else \
+                min(budget, int(fp32_maxiter))
             # the FP32 V-cycle gives out near the FP64 residual floor: leave
             # it after 25 iterations without halving the residual (the FP64
             # V-cycle then finishes), not the 100 the FP64 one is allowed
-            u, info = amg_pcg(self, getattr(self, name), f, rtol=rtol,
-                              maxiter=min(int(maxiter), 100000), x0=x0,
+            u, info = amg_pcg(self, getattr(self, name), f, rtol=rtol, maxiter=first, x0=x0,
                               stall=25 if method == 'amg32' else 100)
-            if (method == 'amg32' and info['relres'] > rtol
-                    and getattr(self, 'amg', None) is not None):
-                # the FP32 V-cycle stalled: finish with the FP64 one from u
-                u, info64 = amg_pcg(self, self.amg, f, rtol=rtol,
-                                    maxiter=min(int(maxiter), 100000), x0=u)
-                info = dict(iters=info['iters'] + info64['iters'],
-                            outer=info['outer'] + info64['outer'], relres=info64['relres'],
-                            fp64_iters=info64['iters'])
+            if method == 'amg32':
+                info['fp64_iters'] = 0
+                if (info['relres'] > rtol and info['iters'] < budget
+                        and getattr(self, 'amg', None) is not None):
+                    # the FP32 phase stalled or used its share: finish with
+                    # the FP64 V-cycle from u, within what is left of maxiter
+                    u, info64 = amg_pcg(self, self.amg, f, rtol=rtol,
+                                        maxiter=budget - info['iters'], x0=u)
+                    info = dict(iters=info['iters'] + info64['iters'],
+                                outer=info['outer'] + info64['outer'],
+                                relres=info64['relres'], fp64_iters=info64['iters'])
+            info['converged'] = bool(info['relres'] <= rtol)
             return u, info
         elif method == 'direct':
             if refactor or not getattr(self, '_factored', False):
@@ -233,4 +248,56 @@ class StaticSystem:
         # true residual on the FP64 matrix (compensated products)
         r = self.residual(b, x, stream=stream)
         info['relres'] = float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(b))
+        info['converged'] = bool(info['relres'] <= rtol)
         return x, info
+
+    def solve_dd(self, f, method='amg', rtol=1e-8, u=None, inner_rtol=1e-3, max_steps=8,
+                 maxiter=4000, log=None, stream=None):
+        """Solution held as a double-double pair u = hi + lo (hi = fl(u)).
+
+        On thin shells the rounding of an FP64 vector alone moves K u by
+        ~eps |K| |u|, which can exceed rtol ||f||; hi + lo carries ~32
+        significant digits.  Refinement: r = (f - K hi) - K lo, both products
+        compensated (pf3d_bsr_residual_dd), K d = r solved to inner_rtol with
+        `method`, lo += d and (hi, lo) renormalised (Fast2Sum), until the
+        pair's relative residual is <= rtol.  ``u`` is a first FP64 solution
+        (solved with `method` to rtol if None).
+
+        Returns (hi, lo, info): info['relres'] is the pair's compensated
+        relative residual, info['relres_fp64'] that of hi alone (the pair
+        rounded to FP64), info['steps'] the refinement
+        steps, info['iters'] all CG iterations, info['converged'] relres <=
+        rtol."""
+        b = self.rhs(f)
+        nb_ = float(torch.linalg.vector_norm(b)) or 1.
+        iters = 0
+        if u is None:
+            u, info0 = self.solve(f, method=method, rtol=rtol, maxiter=maxiter, stream=stream)
+            iters += info0['iters']
+        hi = u.clone()
+        lo = torch.zeros_like(hi)
+        r1 = torch.empty_like(hi)
+        r = torch.empty_like(hi)
+        steps = 0
+        while True:
+            self.residual(b, hi, r1, stream=stream)
+            self.residual(r1, lo, r, stream=stream)
+            rel = float(torch.linalg.vector_norm(r)) / nb_
+            if log:
+                log('solve_dd: step %d, pair relres %.3e' % (steps, rel))
+            if rel <= rtol or steps >= max_steps:
+                break
+            d, info = self.solve(r, method=method, rtol=inner_rtol, maxiter=maxiter,
+                                 stream=stream)
+            iters += info['iters']
+            steps += 1
+            lo.add_(d)
+            # TwoSum (exact for any magnitudes): hi + lo = s + e, s = fl(hi + lo)
+            s = hi + lo
+            bb = s - hi
+            lo = (hi - (s - bb)) + (lo - bb)
+            hi = s
+        self.residual(b, hi, r1, stream=stream)
+        rel64 = float(torch.linalg.vector_norm(r1)) / nb_
+        return hi, lo, dict(relres=rel, relres_fp64=rel64, steps=steps, iters=iters,
+                            converged=bool(rel <= rtol))
