@@ -490,6 +490,21 @@ int pf3d_pcg(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx, const d
              const double *dinv, const double *b, double *x, double rtol, int maxiter, int x0,
              void *work, int64_t work_bytes, int64_t *iters, double *relres, void *stream);
 
+/* pf3d_pcg without a workspace argument and without any allocation: no
+ * counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  The PCG vectors live in a fixed device array of the
+ * library (one per GPU, ~24 MiB, held until the process ends), so nb is
+ * limited to PF3D_PCG_STATIC_MAX_ROWS block rows (786,432 DOF); larger
+ * systems get PF3D_E_ARG.  Calls that overlap on one GPU share that array
+ * and race: serialise them (one stream, or events).  Same kernels, same
+ * results (bit for bit) as pf3d_pcg. */
+#define PF3D_PCG_STATIC_MAX_ROWS 131072
+int pf3d_pcg_static(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx,
+                    const double *val, const double *dinv, const double *b, double *x,
+                    double rtol, int maxiter, int x0, int64_t *iters, double *relres,
+                    void *stream);
+
 /* FP64 iterative refinement around a block-Jacobi PCG on the FP32-stored
  * matrix and block inverses (FP64 vectors and arithmetic): the outer
  * residual b - K64 x and the update x += d are FP64; stops at

@@ -87,6 +87,7 @@ SIGNATURES = {
     'pf3d_bjacobi': [I64, P, P, P, P, P, P, P],
     'pf3d_pcg_bytes': [I64, I32, P],
     'pf3d_pcg': [I64, P, P, P, P, P, P, F64, I32, I32, P, I64, P, P, P],
+    'pf3d_pcg_static': [I64, P, P, P, P, P, P, F64, I32, I32, P, P, P],
     'pf3d_ir_bytes': [I64, P],
     'pf3d_ir_solve': [I64, P, P, P, P, P, P, P, F64, F64, I32, I32, I32, P, I64, P, P, P],
     'pf3d_pcg_mixed': [I64, P, P, P, P, P, P, P, P, F64, I64, I32, I32, P, I64, P, P, P],

@@ -14,6 +14,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "../../include/pf3d.h"
 #include "pf3d_device.hpp"
 
@@ -942,6 +944,70 @@ int pf3d_pcg(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx, const d
         return pf3d_fail(PF3D_E_ARG, "pf3d_pcg: workspace too small");
     Mat A{nb, row_ptr, col_idx, val, dinv};
     return pcg64_restarted(A, b, x, rtol, maxiter, x0, work, (hipStream_t)stream, iters, relres);
+}
+
+/* Allocation-free FP64 PCG (pf3d_pcg without the caller's workspace):
+ * the four PCG vectors, the reduction partials and the solver state live
+ * in `scratch`, a file-scope device array of fixed size (one copy per GPU,
+ * held until the process ends).  Its address on each device comes from
+ * hipGetSymbolAddress once, with that device current; host code uses that
+ * address only, never the symbol itself.  Every call on one GPU shares the
+ * array: calls that overlap on one GPU race. */
+}  // extern "C"
+
+namespace {
+constexpr size_t PCG_STATIC_BYTES =
+    ((4 * 6 * (size_t)PF3D_PCG_STATIC_MAX_ROWS * sizeof(double) + 255) / 256) * 256 +
+    2 * RED_BLOCKS * sizeof(double) + 256;
+}  // namespace
+
+static __device__ __align__(256) char scratch[PCG_STATIC_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+void *scratch_dev[MAX_DEVICES];
+
+// device address of `scratch` on the current device (looked up once per device)
+int scratch_ptr(void **ptr)
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return pf3d_hip_check(e, "pf3d_pcg_static: hipGetDevice");
+    if (dev < 0 || dev >= MAX_DEVICES)
+        return pf3d_fail(PF3D_E_ARG, "pf3d_pcg_static: device %d beyond %d", dev, MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return pf3d_hip_check(e, "pf3d_pcg_static: hipGetSymbolAddress");
+        if (!p) return pf3d_fail(PF3D_E_HIP, "pf3d_pcg_static: no address for the scratch array");
+        scratch_dev[dev] = p;
+    }
+    *ptr = scratch_dev[dev];
+    return PF3D_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int pf3d_pcg_static(int64_t nb, const int32_t *row_ptr, const int32_t *col_idx,
+                    const double *val, const double *dinv, const double *b, double *x,
+                    double rtol, int maxiter, int x0, int64_t *iters, double *relres,
+                    void *stream)
+{
+    if (nb <= 0 || !row_ptr || !col_idx || !val || !dinv || !b || !x || !iters || !relres)
+        return pf3d_fail(PF3D_E_ARG, "pf3d_pcg_static: bad arguments");
+    if (nb > PF3D_PCG_STATIC_MAX_ROWS)
+        return pf3d_fail(PF3D_E_ARG,
+                         "pf3d_pcg_static: %lld block rows exceed the static workspace "
+                         "(at most %d; pf3d_pcg takes a caller workspace of any size)",
+                         (long long)nb, PF3D_PCG_STATIC_MAX_ROWS);
+    void *w = nullptr;
+    int st = scratch_ptr(&w);
+    if (st) return st;
+    Mat A{nb, row_ptr, col_idx, val, dinv};
+    return pcg64_restarted(A, b, x, rtol, maxiter, x0, w, (hipStream_t)stream, iters, relres);
 }
 
 /* FP64 iterative refinement around a PCG on the FP32-stored matrix:

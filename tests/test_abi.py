@@ -152,3 +152,17 @@ def test_argument_errors_of_newer_entry_points():
                 None, None, None, None, 1., 1., 0., 0., None) == -1  # ld < 6 nb
     assert call('pf3d_bsr_product_dd_mv', 4, fake, fake, fake, 0, 24, fake, fake, None) == -1
     assert call('pf3d_bsr_product_dd_mv', 4, fake, fake, fake, 2, 24, fake, fake, None) == -1
+
+
+def test_static_workspace_pcg_limits_need_no_device():
+    """pf3d_pcg_static rejects null operands and systems beyond its fixed
+    workspace before any device work."""
+    import ctypes
+    lib = _lib.load()
+    fake = ctypes.c_void_p(16)  # never dereferenced: the checks fail first
+    it, rel = ctypes.c_int64(), ctypes.c_double()
+    args = (fake,) * 6 + (1e-8, 10, 0, ctypes.byref(it), ctypes.byref(rel), None)
+    assert lib.pf3d_pcg_static(0, *args) == -1
+    assert lib.pf3d_pcg_static(131073, *args) == -1
+    assert b'static workspace' in lib.pf3d_last_error()
+    assert lib.pf3d_pcg_static(10, None, *args[1:]) == -1
