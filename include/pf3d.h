@@ -240,6 +240,18 @@ int pf3d_laminate_records(int64_t n, int n_plies, const double *theta_deg, const
                           const int32_t *ply_mat, int n_mat, const double *mats,
                           const double *offset, int calc_scf, double *out, void *stream);
 
+/* Shell property records (32 doubles each) from per-element lamination
+ * parameters: thickness[n], xi[n][14] (xiA1..4, xiB1..4, xiD1..4, xiE1,
+ * xiE2), material mat_id[n] (NULL: material 0) among n_mat rows of
+ * mats[n_mat][8] = (U1..U7 Tsai-Pagano invariants, rho), all on the device.
+ * A = h (U0 + G xiA), B = h^2/4 G xiB, D = h^3/12 (U0 + G xiD),
+ * E = h (U6 + U7 xiE1, -U7 xiE2, U6 - U7 xiE1); shear factors 5/6, offset 0,
+ * intrho = rho h, intrhoz2 = rho h^3 / 12.  Replaces one
+ * shellprop_from_LaminationParameters(thickness, mat, lp) per element on the
+ * caller side (shellprop.pyx:767-865). */
+int pf3d_lp_records(int64_t n, const double *thickness, const double *xi, const int32_t *mat_id,
+                    int n_mat, const double *mats, double *out, void *stream);
+
 /* Shell property records (32 doubles each) of isotropic plates, one per
  * element, from a thickness field h[n] (device) and the record of the same
  * material at unit thickness (host, 32 doubles, h field = 1, no offset):

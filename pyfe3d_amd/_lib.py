@@ -30,6 +30,7 @@ SIGNATURES = {
     'pf3d_isotropic_records': [I64, P, P, P, P],
     'pf3d_nonfinite_elements': [I64, I32, P, P, P, P],
     'pf3d_laminate_records': [I64, I32, P, P, P, I32, P, P, I32, P, P],
+    'pf3d_lp_records': [I64, P, P, P, I32, P, P, P],
     'pf3d_shell_fe': [I32, I64, P, P, P, P, P, P, P, P, P, P, P],
     'pf3d_beam_fe': [I64, P, P, P, P, P, P, P, P],
     'pf3d_shell_piston': [I32, I32, I64, P, P, P, P, P, P, I32, P],

@@ -8,7 +8,8 @@
 // intrhoz are zero, D and intrhoz2 with h^3; the shear correction factors
 // do not depend on h.  The kernel scales the record of a unit-thickness
 // plate, so a new thickness field costs one streaming pass instead of a
-// host loop over ShellProp objects.
+// host loop over ShellProp objects.  Likewise per-element ply stacks
+// (laminate_records) and per-element lamination parameters (lp_records).
 #include <stdint.h>
 
 #include "../../include/pf3d.h"
@@ -58,7 +59,7 @@ __global__ void iso_records(int64_t n, const double *__restrict__ h, Unit u,
 // integration of A, B, D, E and the mass moments
 // (ShellProp.calc_constitutive_matrix, shellprop.pyx:568-621) and the
 // Vlachoutsis shear correction factors (ShellProp.calc_scf,
-// shellprop.pyx:485-548), in the reference's operation order.  Material
+// shellprop.pyx:485-548: k = R^2 / (h sum_k D_k^2 int (s^2 - sb^2)^2 ds / 4)).  Material
 // rows: e1 e2 nu12 g12 g13 g23 rho (8 doubles, last unused).  Plies of zero
 // thickness are padding and skipped.
 __global__ void __launch_bounds__(128)
@@ -67,8 +68,8 @@ __global__ void __launch_bounds__(128)
                      const double *__restrict__ mats, const double *__restrict__ offset,
                      int calc_scf, double *__restrict__ out)
 {
-    // no FMA contraction: the host builder rounds every product (the scf
-    // polynomial cancels by several digits)
+    // no FMA contraction: the stiffness sums round every product, as the
+    // reference's ply loop does
 #pragma clang fp contract(off)
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
@@ -120,33 +121,35 @@ __global__ void __launch_bounds__(128)
         E45 += q45L * d1;
         E55 += q55L * d1;
         if (calc_scf) {
-            // ply moduli along the laminate axes as calc_scf forms them; the
-            // polynomial cancels by several digits: integer powers by pow(),
-            // as the reference's ** (libm pow) forms them
-            const double z1 = hk_1, z2 = hk;
+            // ply moduli along the laminate axes as calc_scf defines them
+            // (linear in cos and sin), running sums D1, D2 over the plies;
+            // int (s^2 - sb^2)^2 ds over the ply (s = z - offset, sb the
+            // bottom surface) by 3-point Gauss-Legendre, exact for this
+            // degree-4 integrand and free of the cancellation of its
+            // expanded polynomial
             const double se1 = e1 * c + e2 * s, se2 = e2 * c + e1 * s;
             const double snu12 = nu12 * c + nu21 * s, snu21 = nu21 * c + nu12 * s;
-            const double zb2 = pow(zb, 2.), o2 = pow(off, 2.);
-            const double poly =
-                15 * off * pow(z1, 4.) + 30 * off * pow(z1, 2.) * zb * (2 * off - zb) -
-                15 * off * pow(z2, 4.) + 30 * off * pow(z2, 2.) * zb * (-2 * off + zb) -
-                3 * pow(z1, 5.) + 10 * pow(z1, 3.) * (-2 * o2 - 2 * off * zb + zb2) -
-                15 * z1 * zb2 * (4 * o2 - 4 * off * zb + zb2) + 3 * pow(z2, 5.) +
-                10 * pow(z2, 3.) * (2 * o2 + 2 * off * zb - zb2) +
-                15 * z2 * zb2 * (4 * o2 - 4 * off * zb + zb2);
-            const double r3 = pow(z2 - off, 3.) / 3. - pow(z1 - off, 3.) / 3.;
-            D1 += se1 / (1 - snu12 * snu21);
+            const double s1 = hk_1 - off, s2 = hk - off, sb = zb - off;
+            const double half = (s2 - s1) / 2., mid = (s2 + s1) / 2.;
+            const double xg = sqrt(0.6);
+            const double ga = (mid - half * xg - sb) * (mid - half * xg + sb);
+            const double gb = (mid - sb) * (mid + sb);
+            const double gc = (mid + half * xg - sb) * (mid + half * xg + sb);
+            const double F = half * (ga * ga * (5. / 9.) + gb * gb * (8. / 9.) + gc * gc * (5. / 9.));
+            const double r3 = (s2 * s2 * s2 - s1 * s1 * s1) / 3.;
+            const double dd = 1 - snu12 * snu21;
+            D1 += se1 / dd;
             R1 += D1 * r3;
-            den1 += g13 * tk[k] * (h / tk[k]) * pow(D1, 2.) * poly / (60 * g13);
-            D2 += se2 / (1 - snu12 * snu21);
+            den1 += D1 * D1 * F;
+            D2 += se2 / dd;
             R2 += D2 * r3;
-            den2 += g23 * tk[k] * (h / tk[k]) * pow(D2, 2.) * poly / (60 * g23);
+            den2 += D2 * D2 * F;
         }
     }
     double k13 = 5 / 6., k23 = 5 / 6.;
     if (calc_scf) {
-        k13 = pow(R1, 2.) / den1;
-        k23 = pow(R2, 2.) / den2;
+        k13 = R1 * R1 / (h * den1 / 4.);
+        k23 = R2 * R2 / (h * den2 / 4.);
     }
     double *o = out + e * pf3d::SHELL_STRIDE;
     for (int q = 0; q < 6; q++) {
@@ -159,6 +162,64 @@ __global__ void __launch_bounds__(128)
     o[23] = h; o[24] = off;
     o[25] = ir; o[26] = irz; o[27] = irz2;
     o[28] = o[29] = o[30] = o[31] = 0.;
+}
+
+// Record of one element from its thickness h and lamination parameters
+// xi = (xiA1..4, xiB1..4, xiD1..4, xiE1, xiE2) for one material given by its
+// invariants (U1..U7, rho) (shellprop_from_LaminationParameters,
+// shellprop.pyx:767-825): A = h (U0 + G xiA), B = h^2/4 G xiB,
+// D = h^3/12 (U0 + G xiD), E = h (U6 + U7 xiE1, -U7 xiE2, U6 - U7 xiE1) with
+// U0 = (U1, U4, 0, U1, 0, U5) over (11, 12, 16, 22, 26, 66) and G the
+// gradient rows below.  One thread per record field: the 32 threads of an
+// element read its 15 inputs from cache and write 256 contiguous bytes.
+// Shear factors 5/6, offset 0; mass moments of a homogeneous plate.
+__global__ void __launch_bounds__(256)
+    lp_records(int64_t n, const double *__restrict__ thick, const double *__restrict__ xi,
+               const int32_t *__restrict__ mat_id, const double *__restrict__ mats,
+               double *__restrict__ out)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * pf3d::SHELL_STRIDE) return;
+    const int64_t e = t / pf3d::SHELL_STRIDE;
+    const int k = (int)(t - e * pf3d::SHELL_STRIDE);
+    const double *u = mats + 8 * (mat_id ? mat_id[e] : 0);
+    const double h = thick[e];
+    const double *x = xi + 14 * e;
+    double v = 0.;
+    if (k < 18) {
+        const int q = k % 6, grp = k / 6;
+        const double *xg = x + 4 * grp;
+        const double u1 = u[0], u2 = u[1], u3 = u[2], u4 = u[3], u5 = u[4];
+        // U0[q] + sum_j G[q][j] xi_j, G rows: (u2 0 u3 0) (0 0 -u3 0)
+        // (0 u2/2 0 u3) (-u2 0 u3 0) (0 u2/2 0 -u3) (0 0 -u3 0)
+        double base, lin;
+        switch (q) {
+        case 0: base = u1; lin = u2 * xg[0] + u3 * xg[2]; break;
+        case 1: base = u4; lin = -u3 * xg[2]; break;
+        case 2: base = 0.; lin = 0.5 * u2 * xg[1] + u3 * xg[3]; break;
+        case 3: base = u1; lin = -u2 * xg[0] + u3 * xg[2]; break;
+        case 4: base = 0.; lin = 0.5 * u2 * xg[1] - u3 * xg[3]; break;
+        default: base = u5; lin = -u3 * xg[2];
+        }
+        if (grp == 0) v = h * (base + lin);
+        else if (grp == 1) v = h * h / 4. * lin;
+        else v = h * h * h / 12. * (base + lin);
+    } else if (k == 18) {
+        v = h * (u[5] + u[6] * x[12]);
+    } else if (k == 19) {
+        v = h * (-u[6] * x[13]);
+    } else if (k == 20) {
+        v = h * (u[5] - u[6] * x[12]);
+    } else if (k == 21 || k == 22) {
+        v = 5. / 6.;
+    } else if (k == 23) {
+        v = h;
+    } else if (k == 25) {
+        v = u[7] * h;
+    } else if (k == 27) {
+        v = u[7] * h * h * h / 12.;
+    }
+    out[t] = v;
 }
 
 }  // namespace
@@ -192,4 +253,18 @@ extern "C" int pf3d_isotropic_records(int64_t n, const double *h, const double *
     hipLaunchKernelGGL(iso_records, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0,
                        (hipStream_t)stream, n, h, u, out);
     return pf3d_hip_check(hipGetLastError(), "pf3d_isotropic_records");
+}
+
+extern "C" int pf3d_lp_records(int64_t n, const double *thickness, const double *xi,
+                               const int32_t *mat_id, int n_mat, const double *mats,
+                               double *out, void *stream)
+{
+    if (n < 0 || n_mat <= 0) return pf3d_fail(PF3D_E_ARG, "pf3d_lp_records: bad sizes");
+    if (n == 0) return PF3D_OK;
+    if (!thickness || !xi || !mats || !out)
+        return pf3d_fail(PF3D_E_ARG, "pf3d_lp_records: null argument");
+    const int64_t nt = n * pf3d::SHELL_STRIDE;
+    hipLaunchKernelGGL(lp_records, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, n, thickness, xi, mat_id, mats, out);
+    return pf3d_hip_check(hipGetLastError(), "pf3d_lp_records");
 }

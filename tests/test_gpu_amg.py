@@ -185,3 +185,28 @@ def test_block_vcycle_matches_single(prec):
             z = amg.apply(R[v].clone(), torch.empty_like(R[v]))
             err = float((Z[v] - z).abs().max() / z.abs().max())
             assert err <= tol, (prec, nv, v, err)
+
+
+def test_amg32_finishes_with_fp64_vcycle():
+    """The FP32 V-cycle phase bounded (fp32_maxiter), the FP64 V-cycle
+    finishes from its u within the remaining maxiter: fp64_iters > 0, the
+    total within maxiter, converged, and the solution that of spsolve
+    (20x20 plate of the reference's static test)."""
+    from pyfe3d_amd.mesh import PlateMesh
+    from pyfe3d_amd.shellprop import isotropic_plate
+    m = PlateMesh(21, 21, 1.0, 0.7, jitter=0.2, seed=6)
+    rec = isotropic_plate(thickness=0.005, E=70e9, nu=0.33, rho=2700., calc_scf=True).record()
+    fixed = m.clamped()
+    f = np.zeros(m.n_dof)
+    f[6 * m.centre_node() + 2] = -1000.
+    sysm = _system(m, m.quads, 'quad4', rec, fixed)
+    sysm.setup_amg(m.ncoords, coarse_nodes=50, precisions=('fp64', 'fp32'))
+    u, info = sysm.solve(f, method='amg32', rtol=1e-10, fp32_maxiter=16, maxiter=400)
+    uref = _spsolve(m, m.quads, 'quad4', rec, fixed, f)
+    err = np.linalg.norm(u.cpu().numpy() - uref) / np.linalg.norm(uref)
+    print(info, 'err %.2e' % err)
+    assert info['fp64_iters'] > 0 and info['iters'] <= 400
+    assert info['converged'] and info['relres'] <= 1e-10 and err <= 1e-8
+    # a budget too small to converge is reported, not hidden
+    _, short = sysm.solve(f, method='amg32', rtol=1e-10, fp32_maxiter=4, maxiter=10)
+    assert short['iters'] <= 10 and not short['converged']

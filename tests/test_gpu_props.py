@@ -104,3 +104,48 @@ def test_laminate_records_random_stacks_vs_host():
         _close(out[e], ref, e, _scf_cancellation(list(plyt[e, :k]), off[e]))
     out2 = laminate_records_device(theta, plyt, mats, ply_mat=pmat, offset=off, calc_scf=False)
     assert torch.all(out2[:, RECORD_FIELDS.index('scf_k13')] == 5 / 6.)
+
+
+def test_lp_records_match_host_builder():
+    """pf3d_lp_records against shellprop_from_LaminationParameters on 300
+    random designs (thickness, 14 lamination parameters, two materials, one
+    of them tri-axial): A, B, D within 1e-13 of their group's scale, E of
+    its own; and a stack's own lamination parameters give its A, B, D, E."""
+    from pyfe3d_amd.shellprop import (LaminationParameters, lp_material_row, lp_records_device,
+                                      read_laminaprop, shellprop_from_LaminationParameters)
+    rng = np.random.default_rng(5)
+    mats = [read_laminaprop((140e9, 10e9, 0.3, 5e9, 4e9, 3e9), 1600.),
+            read_laminaprop((71e9, 7e9, 0.28, 7e9, 7e9, 7e9, 7e9, 0.28, 0.28), 1500.)]
+    rows = [lp_material_row(m, m.rho) for m in mats]
+    n = 300
+    h = rng.uniform(5e-4, 5e-3, n)
+    xi = rng.uniform(-1, 1, (n, 14))
+    mid = rng.integers(0, 2, n)
+    out = lp_records_device(h, xi, rows, mat_id=mid).cpu().numpy()
+    ix = RECORD_FIELDS.index
+    for e in range(n):
+        lp = LaminationParameters()
+        for k, val in zip(LaminationParameters.NAMES, xi[e]):
+            setattr(lp, k, val)
+        ref = shellprop_from_LaminationParameters(h[e], mats[mid[e]], lp).record()
+        ref[ix('intrho')] = mats[mid[e]].rho * h[e]
+        ref[ix('intrhoz2')] = mats[mid[e]].rho * h[e] ** 3 / 12.
+        for X, p in (('A', 0), ('B', 1), ('D', 2)):
+            sl = slice(6 * ('ABD'.index(X)), 6 * ('ABD'.index(X)) + 6)
+            scale = np.abs(ref[0:6]).max() * h[e] ** p
+            assert np.abs(out[e, sl] - ref[sl]).max() <= 1e-13 * scale, (e, X)
+        es = slice(18, 21)
+        assert np.abs(out[e, es] - ref[es]).max() <= 1e-13 * np.abs(ref[es]).max(), e
+        assert np.allclose(out[e, 21:28], ref[21:28], rtol=1e-14, atol=0), e
+        assert not out[e, 28:].any()
+    # lamination parameters of real stacks through the device builder
+    stacks = [[0, 45, -45, 90], [30, -60, 15, 90, 0, 45], [0]]
+    for st in stacks:
+        p = laminated_plate(st, 0.000125, (140e9, 10e9, 0.3, 5e9, 4e9, 3e9))
+        lp = p.calc_lamination_parameters()
+        r = lp_records_device([p.h], lp.vector()[None], [rows[0]]).cpu().numpy()[0]
+        full = p.record()
+        scale = np.abs(full[:6]).max()
+        for X, pw in (('A', 0), ('B', 1), ('D', 2)):
+            sl = slice(6 * 'ABD'.index(X), 6 * 'ABD'.index(X) + 6)
+            assert np.abs(r[sl] - full[sl]).max() <= 1e-12 * scale * p.h ** pw, (st, X)
