@@ -134,9 +134,57 @@ class _Probe:
         self.finte = np.zeros(self.NN * DOF)
 
 
+_BL_NAMES = ('BLexx', 'BLeyy', 'BLgxy', 'BLkxx', 'BLkyy', 'BLkxy', 'BLgyz_grad', 'BLgyz_rot',
+             'BLgxz_grad', 'BLgxz_rot', 'BLdrilling')
+
+
 class Quad4Probe(_Probe):
-    """Per-element scratch of the reference's Quad4 (quad4.pyx:236-271)."""
+    """Per-element scratch of the reference's Quad4 (quad4.pyx:236-271):
+    xe, ue, finte, the local stiffness KC0ve (24 x 24 row-major, set by
+    update_KC0 / update_probe_finte) and the strain-displacement rows BL*
+    (update_BL)."""
     NN = 4
+
+    def __init__(self):
+        super().__init__()
+        self.KC0ve = np.zeros((self.NN * DOF) ** 2)
+        for n in _BL_NAMES:
+            setattr(self, n, np.zeros(self.NN * DOF))
+
+    def update_BL(self, xi, eta):
+        """Strain-displacement rows of the bilinear quad at the natural
+        point (xi, eta), from the element coordinates xe (quad4.pyx:272-395):
+        with N_I the bilinear shape functions and (N_I,x, N_I,y) their
+        gradients through the inverse Jacobian of x(xi, eta), node I's DOFs
+        (u v w rx ry rz) enter exx = u,x, eyy = v,y, gxy = u,y + v,x,
+        kxx = ry,x, kyy = -rx,y, kxy = ry,y - rx,x, gyz = w,y - rx,
+        gxz = w,x + ry and the drilling strain (u,y - v,x)/2 + rz."""
+        X = np.asarray(self.xe, dtype=np.float64).reshape(4, 3)[:, :2]
+        sx = np.array([-1., 1., 1., -1.])   # node signs of xi
+        se = np.array([-1., -1., 1., 1.])   # node signs of eta
+        N = (1 + sx * xi) * (1 + se * eta) / 4.
+        dN = np.stack([sx * (1 + se * eta), se * (1 + sx * xi)]) / 4.   # d/dxi, d/deta
+        J = dN @ X                     # [[x,xi y,xi], [x,eta y,eta]]
+        g = np.linalg.solve(J, dN)     # rows: N,x and N,y
+        Nx, Ny = g
+        for n in _BL_NAMES:
+            getattr(self, n)[:] = 0.
+        i = 6 * np.arange(4)
+        self.BLexx[i] = Nx
+        self.BLeyy[i + 1] = Ny
+        self.BLgxy[i] = Ny
+        self.BLgxy[i + 1] = Nx
+        self.BLkxx[i + 4] = Nx
+        self.BLkyy[i + 3] = -Ny
+        self.BLkxy[i + 3] = -Nx
+        self.BLkxy[i + 4] = Ny
+        self.BLgyz_grad[i + 2] = Ny
+        self.BLgyz_rot[i + 3] = -N
+        self.BLgxz_grad[i + 2] = Nx
+        self.BLgxz_rot[i + 4] = N
+        self.BLdrilling[i] = Ny / 2.
+        self.BLdrilling[i + 1] = -Nx / 2.
+        self.BLdrilling[i + 5] = N
 
 
 class Quad4RProbe(_Probe):
@@ -296,8 +344,41 @@ class _Shell(_Element):
         A.shell_kc0(self.ETYPE, _t(self._xl.ravel()), self._conn(), r, c, v,
                     _t(_shell_record(prop)[None]), overwrite=True, **self._kw(),
                     **self._kc0_kw())
-        self._scatter(size, self.init_k_KC0, *self._host(r, c, v), KC0r, KC0c, KC0v,
-                      update_KC0v_only)
+        rl, cl, vl = self._host(r, c, v)
+        self._scatter(size, self.init_k_KC0, rl, cl, vl, KC0r, KC0c, KC0v, update_KC0v_only)
+        if hasattr(self.probe, 'KC0ve'):
+            self._set_local_stiffness(vl)
+
+    def _r6(self):
+        """blockdiag(R, R, ...) over the element's DOFs (local -> global)."""
+        return np.kron(np.eye(2 * self.NN), self._rot())
+
+    def _set_local_stiffness(self, vglobal):
+        """probe.KC0ve = R6^T K R6 from the element's global block (the
+        element's slots are its 6nn x 6nn matrix, row-major)."""
+        n = DOF * self.NN
+        R6 = self._r6()
+        self.probe.KC0ve[:] = (R6.T @ vglobal.reshape(n, n) @ R6).ravel()
+
+    def update_probe_finte(self, prop):
+        """probe.finte = K_e probe.ue, the internal forces in local axes
+        from the local displacements of the probe (quad4.pyx:1174-1202,
+        quad4r.pyx:549-719, tria3r.pyx:549-948); Quad4 also refreshes
+        probe.KC0ve."""
+        self._need_x()
+        ue = np.asarray(self.probe.ue, dtype=np.float64)
+        ug = (self._r6() @ ue).reshape(self.NN, DOF)
+        fe = A.shell_fe(self.ETYPE, _t(self._xl.ravel()), self._conn(), _t(ug.ravel()),
+                        _t(_shell_record(prop)[None]), xmat=_t(self._xmat[None]),
+                        **self._fe_kw())
+        self.probe.finte[:] = self._r6().T @ self._host(fe)[0][0].ravel()
+        if hasattr(self.probe, 'KC0ve'):
+            size = A.slot_count(self.ETYPE, 'KC0')
+            r, c, v = self._buffers(size)
+            A.shell_kc0(self.ETYPE, _t(self._xl.ravel()), self._conn(), None, None, v,
+                        _t(_shell_record(prop)[None]), values_only=True, overwrite=True,
+                        **self._kw(), **self._kc0_kw())
+            self._set_local_stiffness(self._host(v)[0])
 
     def _kg(self, KGr, KGc, KGv, prop, values_only):
         self._need_x()
@@ -420,6 +501,16 @@ class Quad4R(Quad4):
         finally:
             self._hg = None
 
+    def update_probe_finte(self, prop, hgfactor_u=1., hgfactor_v=1., hgfactor_w=1.,
+                           hgfactor_rx=1., hgfactor_ry=1.):
+        """probe.finte from probe.ue with hourglass control factors
+        (quad4r.pyx:549-719)."""
+        self._hg = [hgfactor_u, hgfactor_v, hgfactor_w, hgfactor_rx, hgfactor_ry]
+        try:
+            super().update_probe_finte(prop)
+        finally:
+            self._hg = None
+
 
 class Tria3R(_Shell):
     """Tria3R shell element (tria3r.pyx:205-292)."""
@@ -522,6 +613,18 @@ class BeamC(_Element):
             self.probe.finte[6 * i + 3:6 * i + 6] = R.T @ fe[i, 3:]
 
 
+    def update_probe_finte(self, prop):
+        """probe.finte = K_e probe.ue in local axes (beamc.pyx:352-488,
+        beamlr.pyx:354-406)."""
+        self._need_x()
+        R = self._rot()
+        ue = np.asarray(self.probe.ue, dtype=np.float64).reshape(4, 3)
+        ug = (ue @ R.T).reshape(2, DOF)
+        fe = A.beam_fe(_t(self._xl.ravel()), self._conn(), self._vxy(), _t(ug.ravel()),
+                       _t(_beam_record(prop)[None]), etype=self.ETYPE)
+        self.probe.finte[:] = (self._host(fe)[0][0].reshape(4, 3) @ R).ravel()
+
+
 class BeamLR(BeamC):
     """BeamLR linear Timoshenko beam, reduced integration (beamlr.pyx:99-159):
     BeamC's interface, KG over the rotations (36 slots)."""
@@ -530,6 +633,14 @@ class BeamLR(BeamC):
 
 
 class _Rod(_Element):
+    def _ug_from_probe(self):
+        R = self._rot()
+        ue = np.asarray(self.probe.ue, dtype=np.float64).reshape(4, 3)
+        return (ue @ R.T).ravel()
+
+    def _set_finte(self, fe):
+        self.probe.finte[:] = (fe.reshape(4, 3) @ self._rot()).ravel()
+
     def _add_fe(self, fe, fint):
         cs = self._cs()
         R = self._rot()
@@ -588,6 +699,13 @@ class Truss(_Rod):
                   mtype=mtype, overwrite=True)
         self._scatter(size, self.init_k_M, *self._host(r, c, v), Mr, Mc, Mv, False)
 
+    def update_probe_finte(self, prop):
+        """probe.finte = K_e probe.ue in local axes (truss.pyx:339-385)."""
+        self._need_x()
+        fe = A.rod_fe('truss', _t(self._xl.ravel()), self._conn(), _t(self._ug_from_probe()),
+                      _t(_beam_record(prop)[None]))
+        self._set_finte(self._host(fe)[0][0])
+
     def update_fint(self, fint, prop):
         """fint += internal forces (truss.pyx:339-385, 802-837)."""
         self._need_x()
@@ -631,6 +749,12 @@ class Spring(_Rod):
                      self._k(), overwrite=True)
         self._scatter(72, self.init_k_KC0, *self._host(r, c, v), KC0r, KC0c, KC0v,
                       update_KC0v_only)
+
+    def update_probe_finte(self):
+        """probe.finte = K_e probe.ue in local axes (spring.pyx:262-294)."""
+        fe = A.rod_fe('spring', _t(self._xl.ravel()), self._conn(), _t(self._ug_from_probe()),
+                      self._k(), axes=_t(self._axes[None]))
+        self._set_finte(self._host(fe)[0][0])
 
     def update_fint(self, fint):
         """fint += spring forces (spring.pyx:262-294, 708-739)."""
@@ -696,7 +820,10 @@ def update_KC0_batch(elems, x, KC0r, KC0c, KC0v, prop, update_KC0v_only=0):
 
 
 def update_KG_batch(elems, x, u, KGr, KGc, KGv, prop):
-    """``update_probe_ue(u)`` + ``update_KG`` of every element, one launch."""
+    """KG of every element from the global displacements u in one launch:
+    the slots of ``update_probe_ue(u)`` + ``update_KG`` per element.  The
+    probe's ue and the elements' stored displacements are left as they
+    were (no per-element state is touched)."""
     cls, xt, conn, init_k = _batch(elems, x, prop, 'init_k_KG')
     r, c, v = _slots(KGv)
     ut = _t(np.asarray(u, dtype=np.float64))

@@ -6,8 +6,6 @@ batched launch (same bits) and the oracle, KG/M/piston and BeamC through
 the façade, and the one-launch batch helpers."""
 import numpy as np
 import pytest
-from scipy.sparse import coo_matrix
-from scipy.sparse.linalg import cg
 
 from oracle import oracle as O
 
@@ -15,90 +13,87 @@ pytestmark = pytest.mark.gpu
 
 
 def _plate(nx, ny, a, b):
-    xtmp = np.linspace(0, a, nx)
-    ytmp = np.linspace(0, b, ny)
-    xmesh, ymesh = np.meshgrid(xtmp, ytmp)
-    ncoords = np.vstack((xmesh.T.flatten(), ymesh.T.flatten(),
-                         np.zeros_like(ymesh.T.flatten()))).T
-    nids = 1 + np.arange(ncoords.shape[0])
-    nid_pos = dict(zip(nids, np.arange(len(nids))))
-    nids_mesh = nids.reshape(nx, ny)
-    n1s = nids_mesh[:-1, :-1].flatten()
-    n2s = nids_mesh[1:, :-1].flatten()
-    n3s = nids_mesh[1:, 1:].flatten()
-    n4s = nids_mesh[:-1, 1:].flatten()
-    return ncoords, nid_pos, list(zip(n1s, n2s, n3s, n4s))
+    """Node coordinates (x-major node order), node id -> position and the
+    quads as node-id 4-tuples, from PlateMesh (ids are positions + 1)."""
+    from pyfe3d_amd.mesh import PlateMesh
+    m = PlateMesh(nx, ny, a, b)
+    ids = np.arange(1, m.n_nodes + 1)
+    return m.ncoords, dict(zip(ids, range(m.n_nodes))), [tuple(q + 1) for q in m.quads]
 
 
-def test_static_plate_quad_point_load_through_facade():
-    """tests/test_quad4_static_point_load.py:12-132 with the façade's
-    classes: wmax within 2 % of 6.496928101916171e-05 and fint == fext
-    (atol 1e-5); KC0 also equal to the oracle's slots (1e-12)."""
+# Known answer of the reference's static point-load test
+# (tests/test_quad4_static_point_load.py:113 of pyfe3d): the largest
+# deflection of a 3 x 7 m simply supported plate (in-plane DOFs held, w = 0
+# on the edges), h = 5 mm, E = 200 GPa, nu = 0.3, 7 x 11 nodes, 1 N at the
+# centre node, within 2 %.
+WMAX_POINT_LOAD = 6.496928101916171e-05
+
+
+def test_point_load_plate_through_facade():
+    """The façade driven by the caller's element loop (update_rotation_matrix,
+    update_probe_xe, update_KC0 per Quad4), its COO fed to this library's own
+    BSR plan and banded solve: the reference's known deflection, KC0 slots
+    equal to the oracle's, and equilibrium per DOF: fint from the façade's
+    update_probe_ue + update_fint equals K u (the reactions on held DOFs)
+    and the load on free DOFs."""
+    import torch
+    from pyfe3d_amd import _lib
     from pyfe3d_amd.compat import DOF, DOUBLE, INT, Quad4, Quad4Data, Quad4Probe, isotropic_plate
-    data = Quad4Data()
-    probe = Quad4Probe()
-    nx, ny, a, b = 7, 11, 3, 7
-    ncoords, nid_pos, quads_n = _plate(nx, ny, a, b)
-    x, y = ncoords[:, 0], ncoords[:, 1]
-    ncoords_flatten = ncoords.flatten()
-    num_elements = len(quads_n)
-    KC0r = np.zeros(data.KC0_SPARSE_SIZE * num_elements, dtype=INT)
-    KC0c = np.zeros(data.KC0_SPARSE_SIZE * num_elements, dtype=INT)
-    KC0v = np.zeros(data.KC0_SPARSE_SIZE * num_elements, dtype=DOUBLE)
-    N = DOF * nx * ny
+    from pyfe3d_amd.mesh import PlateMesh
+    from pyfe3d_amd.solver import StaticSystem
+    from pyfe3d_amd.sparse import BSRPlan
+    mesh = PlateMesh(7, 11, 3.0, 7.0)
+    xf = mesh.x
+    size = Quad4Data().KC0_SPARSE_SIZE
+    E = mesh.quads.shape[0]
+    r = np.zeros(size * E, dtype=INT)
+    c = np.zeros(size * E, dtype=INT)
+    v = np.zeros(size * E, dtype=DOUBLE)
     prop = isotropic_plate(thickness=0.005, E=200e9, nu=0.3, calc_scf=True)
-    quads = []
-    init_k_KC0 = 0
-    for n1, n2, n3, n4 in quads_n:
-        quad = Quad4(probe)
-        quad.n1, quad.n2, quad.n3, quad.n4 = n1, n2, n3, n4
-        quad.c1 = DOF * nid_pos[n1]
-        quad.c2 = DOF * nid_pos[n2]
-        quad.c3 = DOF * nid_pos[n3]
-        quad.c4 = DOF * nid_pos[n4]
-        quad.init_k_KC0 = init_k_KC0
-        quad.K6ROT = 100.
-        quad.update_rotation_matrix(ncoords_flatten)
-        quad.update_probe_xe(ncoords_flatten)
-        quad.update_KC0(KC0r, KC0c, KC0v, prop)
-        quads.append(quad)
-        init_k_KC0 += data.KC0_SPARSE_SIZE
-    # the oracle's slots for the same elements
-    conn = np.array([[nid_pos[n] for n in q] for q in quads_n], dtype=np.int64)
-    rr = np.zeros_like(KC0r); cc = np.zeros_like(KC0c); vv = np.zeros_like(KC0v)
-    O.assemble('quad4', 'KC0', ncoords_flatten, conn, 576 * np.arange(num_elements),
-               np.zeros((num_elements, 3)), np.zeros(num_elements, np.int64),
-               prop.record()[None], rr, cc, vv)
-    assert (KC0r == rr).all() and (KC0c == cc).all()
-    per = np.abs(KC0v - vv).reshape(num_elements, -1).max(axis=1)
-    assert (per <= 1e-12 * np.abs(vv).reshape(num_elements, -1).max(axis=1)).all()
-
-    KC0 = coo_matrix((KC0v, (KC0r, KC0c)), shape=(N, N)).tocsc()
-    bk = np.zeros(N, dtype=bool)
-    check = np.isclose(x, 0.) | np.isclose(x, a) | np.isclose(y, 0) | np.isclose(y, b)
-    bk[2::DOF] = check
-    bk[0::DOF] = True
-    bk[1::DOF] = True
-    bu = ~bk
-    fext = np.zeros(N)
-    fmid = 1.
-    check = np.isclose(x, a / 2) & np.isclose(y, b / 2)
-    fext[2::DOF][check] = fmid
-    KC0uu = KC0[bu, :][:, bu]
-    uu, info = cg(KC0uu, fext[bu], atol=1e-9)
-    assert info == 0
-    u = np.zeros(N)
-    u[bu] = uu
-    w = u[2::DOF].reshape(nx, ny).T
-    assert np.isclose(6.496928101916171e-05, w.max(), rtol=0.02), w.max()
-    fint = np.zeros(N)
-    for quad in quads:
-        quad.update_probe_xe(ncoords_flatten)
-        quad.update_probe_ue(u)
-        quad.update_fint(fint, prop)
-    Kku = KC0[bk, :][:, bu]
-    fext[bk] = Kku @ u[bu]
-    assert np.allclose(fint, fext, atol=1e-5)
+    probe = Quad4Probe()
+    elements = []
+    for e, nodes in enumerate(mesh.quads):
+        q = Quad4(probe)
+        q.n1, q.n2, q.n3, q.n4 = (int(n) + 1 for n in nodes)
+        q.c1, q.c2, q.c3, q.c4 = (DOF * int(n) for n in nodes)
+        q.init_k_KC0 = size * e
+        q.update_rotation_matrix(xf)
+        q.update_probe_xe(xf)
+        q.update_KC0(r, c, v, prop)
+        elements.append(q)
+    rr, cc, vv = np.zeros_like(r), np.zeros_like(c), np.zeros_like(v)
+    O.assemble('quad4', 'KC0', xf, mesh.quads, size * np.arange(E), np.zeros((E, 3)),
+               np.zeros(E, np.int64), prop.record()[None], rr, cc, vv)
+    assert (r == rr).all() and (c == cc).all()
+    per = np.abs(v - vv).reshape(E, -1).max(axis=1)
+    assert (per <= 1e-12 * np.abs(vv).reshape(E, -1).max(axis=1)).all()
+    # simply supported: u, v held everywhere, w on the edge nodes
+    held = np.zeros(mesh.n_dof, dtype=bool)
+    held[0::DOF] = held[1::DOF] = True
+    held[2::DOF] = mesh.edge_nodes()
+    f = np.zeros(mesh.n_dof)
+    f[DOF * mesh.centre_node() + 2] = 1.
+    plan = BSRPlan(torch.as_tensor(r, device='cuda'), torch.as_tensor(c, device='cuda'),
+                   mesh.n_nodes)
+    Kfull = plan.apply(torch.as_tensor(v, device='cuda'))
+    u, info = StaticSystem(plan, Kfull, held).solve(f, method='direct')
+    assert info['converged'], info
+    u = u.cpu().numpy()
+    assert np.isclose(u[2::DOF].max(), WMAX_POINT_LOAD, rtol=0.02), u[2::DOF].max()
+    fint = np.zeros(mesh.n_dof)
+    for q in elements:
+        q.update_probe_xe(xf)
+        q.update_probe_ue(u)
+        q.update_fint(fint, prop)
+    Ku = torch.empty(mesh.n_dof, dtype=torch.float64, device='cuda')
+    _lib.call('pf3d_bsr_spmv', plan.n_nodes, plan.row_ptr.data_ptr(), plan.col_idx.data_ptr(),
+              Kfull.data_ptr(), torch.as_tensor(u, device='cuda').data_ptr(), Ku.data_ptr(),
+              torch.cuda.current_stream().cuda_stream)
+    Ku = Ku.cpu().numpy()
+    assert np.allclose(fint, Ku, rtol=0, atol=1e-9)
+    assert np.allclose(fint[~held], f[~held], rtol=0, atol=1e-9)
+    # the reactions balance the load
+    assert np.isclose(fint[2::DOF][mesh.edge_nodes()].sum(), -1., rtol=1e-9)
 
 
 def _elements(cls_name, probe_name, ncoords, nid_pos, quads_n, tri=False):
@@ -332,3 +327,131 @@ def test_beamlr_truss_spring_through_facade(cls_name):
             for a1, a2 in zip(out, out2):
                 assert np.array_equal(a1, a2), key
     assert data.KC0_SPARSE_SIZE == O.slot_count(etype, 'KC0')
+
+
+def test_probe_strain_and_local_forces_through_facade():
+    """A plate stretched by an end traction through the façade (u held on
+    x = 0, v at one node, out-of-plane DOFs held): the probe's BLexx . ue at
+    each element centre is the uniform strain F / (E h b) of the exact
+    solution (quad4.pyx:272-395, as the reference's in-plane probe test);
+    update_probe_finte gives finte = KC0ve . ue (quad4.pyx:1174-1202) with
+    KC0ve the local matrix of the element's global slots; and the elements'
+    internal forces rotated back balance the applied load."""
+    import torch
+    from pyfe3d_amd.compat import DOF, Quad4, Quad4Data, Quad4Probe, isotropic_plate
+    from pyfe3d_amd.mesh import PlateMesh
+    from pyfe3d_amd.solver import StaticSystem
+    from pyfe3d_amd.sparse import BSRPlan
+    a, b, h, E = 2.0, 1.0, 0.004, 70e9
+    mesh = PlateMesh(9, 6, a, b, jitter=0.2, seed=3)
+    xf = mesh.x
+    size = Quad4Data().KC0_SPARSE_SIZE
+    n = mesh.quads.shape[0]
+    r = np.zeros(size * n, np.int64)
+    c = np.zeros_like(r)
+    v = np.zeros(size * n)
+    prop = isotropic_plate(thickness=h, E=E, nu=0.3, calc_scf=True)
+    probe = Quad4Probe()
+    quads = []
+    for e, nodes in enumerate(mesh.quads):
+        q = Quad4(probe)
+        q.c1, q.c2, q.c3, q.c4 = (DOF * int(k) for k in nodes)
+        q.init_k_KC0 = size * e
+        q.update_rotation_matrix(xf)
+        q.update_probe_xe(xf)
+        q.update_KC0(r, c, v, prop)
+        # the probe's local matrix is the element's global block rotated back
+        R6 = np.kron(np.eye(8), q._rot())
+        Kg = v[size * e:size * e + size].reshape(24, 24)
+        assert np.allclose(R6 @ probe.KC0ve.reshape(24, 24) @ R6.T, Kg, rtol=0,
+                           atol=1e-12 * np.abs(Kg).max())
+        quads.append(q)
+    x, y = mesh.ncoords[:, 0], mesh.ncoords[:, 1]
+    held = np.zeros(mesh.n_dof, bool)
+    for d in (2, 3, 4, 5):
+        held[d::DOF] = True
+    held[0::DOF] = np.isclose(x, 0.)
+    held[1 + DOF * int(np.argmin(x + y))] = True
+    F = 1000.
+    right = np.isclose(x, a)
+    f = np.zeros(mesh.n_dof)
+    # consistent nodal forces of a uniform traction along the x = a edge
+    ys = np.sort(y[right])
+    idx = np.where(right)[0][np.argsort(y[right])]
+    for k in range(len(ys) - 1):
+        L = ys[k + 1] - ys[k]
+        f[DOF * idx[k]] += F / b * L / 2
+        f[DOF * idx[k + 1]] += F / b * L / 2
+    plan = BSRPlan(torch.as_tensor(r, device='cuda'), torch.as_tensor(c, device='cuda'),
+                   mesh.n_nodes)
+    u, info = StaticSystem(plan, plan.apply(torch.as_tensor(v, device='cuda')), held).solve(
+        f, method='direct')
+    assert info['converged']
+    u = u.cpu().numpy()
+    exx_exact = F / (E * h * b)
+    fsum = np.zeros(3)
+    for q in quads:
+        q.update_probe_xe(xf)
+        q.update_probe_ue(u)
+        probe.update_BL(xi=0., eta=0.)
+        exx = probe.BLexx @ probe.ue
+        assert abs(exx - exx_exact) <= 1e-8 * exx_exact, (exx, exx_exact)
+        q.update_probe_finte(prop)
+        fin = probe.KC0ve.reshape(24, 24) @ probe.ue
+        assert np.allclose(probe.finte, fin, rtol=0, atol=1e-10 * np.abs(fin).max())
+    # end displacement from the strain
+    assert np.isclose(u[0::DOF][right].mean(), exx_exact * a, rtol=1e-8)
+
+
+@pytest.mark.parametrize('kind', ['quad4r', 'tria3r', 'beamc', 'beamlr', 'truss', 'spring'])
+def test_update_probe_finte_matches_update_fint(kind):
+    """update_probe_finte(prop) from probe.ue (set by update_probe_ue)
+    equals the local forces update_fint stores, for every element class."""
+    import pyfe3d_amd.compat as C
+    rng = np.random.default_rng(8)
+    xyz = rng.uniform(-1, 1, (4, 3))
+    xyz[:, 2] *= 0.1
+    xf = xyz.ravel()
+    u = rng.normal(size=24) * 1e-3
+    shell = kind in ('quad4r', 'tria3r')
+    if shell:
+        prop = C.isotropic_plate(thickness=0.003, E=70e9, nu=0.3, calc_scf=True)
+        cls = {'quad4r': C.Quad4R, 'tria3r': C.Tria3R}[kind]
+        probe = {'quad4r': C.Quad4RProbe, 'tria3r': C.Tria3RProbe}[kind]()
+        e = cls(probe)
+        nn = 4 if kind == 'quad4r' else 3
+        for i in range(nn):
+            setattr(e, 'c%d' % (i + 1), 6 * i)
+        e.update_rotation_matrix(xf)
+    else:
+        prop = C.BeamProp()
+        prop.A, prop.E, prop.G = 1e-4, 70e9, 26e9
+        prop.Iyy, prop.Izz, prop.J = 2e-9, 3e-9, 4e-9
+        cls = {'beamc': C.BeamC, 'beamlr': C.BeamLR, 'truss': C.Truss, 'spring': C.Spring}[kind]
+        probe = {'beamc': C.BeamCProbe, 'beamlr': C.BeamLRProbe, 'truss': C.TrussProbe,
+                 'spring': C.SpringProbe}[kind]()
+        e = cls(probe)
+        e.c1, e.c2 = 0, 6
+        if kind in ('beamc', 'beamlr'):
+            e.update_rotation_matrix(0., 0., 1., xf)
+        elif kind == 'truss':
+            e.update_rotation_matrix(xf)
+        else:
+            e.kxe, e.kye, e.kze, e.krxe, e.krye, e.krze = 1e5, 2e5, 3e5, 10., 20., 30.
+            e.update_rotation_matrix(1., 0.2, 0., 0., 1., 0.3)
+    if kind != 'spring':
+        e.update_probe_xe(xf)
+    e.update_probe_ue(u)
+    fint = np.zeros(24)
+    if kind == 'spring':
+        e.update_fint(fint)
+    else:
+        e.update_fint(fint, prop)
+    expect = np.array(probe.finte)
+    probe.finte[:] = 0.
+    if kind == 'spring':
+        e.update_probe_finte()
+    else:
+        e.update_probe_finte(prop)
+    assert np.allclose(probe.finte, expect, rtol=0, atol=1e-12 * np.abs(expect).max())
+    assert np.abs(expect).max() > 0
