@@ -10,9 +10,12 @@ design-sweep re-assembly of the reference's tutorials), inputs resident in
 HBM.  Alongside: measured fill and copy bandwidths on the same buffer, the
 same assembly with int64 row/col indices and accumulating, KG (given
 stress), M, internal forces, and static solves per second at 1e-8 relative
-residual (plain FP64 block-Jacobi PCG against mixed-precision multigrid CG,
-banded direct solves of a design sweep, the stiffened panel, the 6M-DOF
-plate), with the panel's lowest natural frequencies.
+residual (plain FP64 block-Jacobi PCG against multigrid CG with FP64 and
+FP32 V-cycles, the double-double refinement where an FP64 vector cannot
+reach 1e-8, banded direct solves of a design sweep, the stiffened panel,
+the 6M-DOF plate, the learned warm start), with the panel's lowest natural
+frequencies; the CPU baseline times the reference-equivalent path on the
+host.
 
 Prints one JSON line.  ``--gpus N`` with N > 1 exits: this build measures
 one GPU.
@@ -31,11 +34,13 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the values-only Quad4 KC0 kernel on the 1000x1000
 # plate, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of
-# `python bench.py --profile-kc0` (KiB per dispatch), each scaled by the
-# reading of the same counter for a 1 GiB device copy in the same run
-# (FETCH_SIZE reads 0.5 GiB for it: factor 2.0; WRITE_SIZE 1 GiB: 1.0):
-# 60,069 KiB x 2 + 4,500,309 KiB = 4.731e9 B (algorithmic: 4.736e9 B).
-KC0_TRAFFIC_BYTES = (60069 * 2 + 4500309) * 1024
+# `python bench.py --profile-kc0` (KiB per dispatch, mean of 3 launches),
+# each scaled by the reading of the same counter for a 1 GiB device copy in
+# the same run (FETCH_SIZE reads 0.5 GiB for it: factor 2.0; WRITE_SIZE
+# 1 GiB: 1.0): 60,062 KiB x 2 + 4,500,740 KiB = 4.732e9 B (algorithmic:
+# 4.736e9 B); profiles/kc0_quad4_1000x1000_pmc.txt.  A counter run cannot
+# be nested in the timed run, so the figure is carried here.
+KC0_TRAFFIC_BYTES = (60062 * 2 + 4500740) * 1024
 # algorithmic bytes of one Quad4 KC0 element, values only: 576 doubles out,
 # 4 int64 DOF offsets and 12 coordinates in
 KC0_BYTES_VALUES = 576 * 8 + 4 * 8 + 12 * 8
@@ -73,23 +78,89 @@ def timed(fn, steps, warmup, stream):
     return ev0.elapsed_time(ev1) / steps, wall
 
 
-def cpu_baseline(mesh, prop, n_sample):
-    """The oracle (plain-C restatement of the reference's per-element loop,
-    with its index writes) on one host core over the first n_sample elements
-    of the same mesh."""
+def _oracle_kc0(mesh, prop, n_sample, threads, values_only):
     from oracle import oracle as O
     q = mesh.quads[:n_sample]
     E = q.shape[0]
     r = np.zeros(576 * E, np.int64)
     c = np.zeros_like(r)
     v = np.zeros(576 * E)
+    used = O.set_threads(threads)
     t0 = time.perf_counter()
     O.assemble('quad4', 'KC0', mesh.x, q, 576 * np.arange(E), np.zeros((E, 3)),
-               np.zeros(E, np.int64), prop.record()[None], r, c, v)
+               np.zeros(E, np.int64), prop.record()[None], r, c, v, values_only=values_only)
     dt = time.perf_counter() - t0
-    return dict(value=E / dt, unit='elements/s', cores=1, kind='port',
-                sample='KC0 of the first %d elements of the 1000x1000 laminate plate '
-                       '(oracle, indices + values, %.1f s)' % (E, dt))
+    O.set_threads(1)
+    return E / dt, dt, used
+
+
+def _reference_path(n, log):
+    """The reference's CPU path on the n x n laminate plate under pressure,
+    piece by piece (tests/test_quad4_static_point_load.py:52-102): the
+    per-element KC0 loop (oracle, indices + values, one core), coo_matrix
+    -> tocsc, the free-DOF slice and spsolve (SuperLU)."""
+    import scipy.sparse as ss
+    import scipy.sparse.linalg as sla
+    from oracle import oracle as O
+    mesh = build_plate(n)
+    prop, _ = laminate_record()
+    E = mesh.quads.shape[0]
+    r = np.zeros(576 * E, np.int64)
+    c = np.zeros_like(r)
+    v = np.zeros(576 * E)
+    O.set_threads(1)
+    t0 = time.perf_counter()
+    O.assemble('quad4', 'KC0', mesh.x, mesh.quads, 576 * np.arange(E), np.zeros((E, 3)),
+               np.zeros(E, np.int64), prop.record()[None], r, c, v)
+    t_kc0 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    K = ss.coo_matrix((v, (r, c)), shape=(mesh.n_dof, mesh.n_dof)).tocsc()
+    t_csc = time.perf_counter() - t0
+    bu = ~mesh.clamped()
+    f = mesh.pressure_load(1e3)
+    t0 = time.perf_counter()
+    Kuu = K[bu, :][:, bu]
+    t_slice = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    uu = sla.spsolve(Kuu, f[bu])
+    t_solve = time.perf_counter() - t0
+    rel = float(np.linalg.norm(f[bu] - Kuu @ uu) / np.linalg.norm(f[bu]))
+    out = dict(dof=mesh.n_dof, elements=E, kc0_s=t_kc0, tocsc_s=t_csc, slice_s=t_slice,
+               spsolve_s=t_solve, total_s=t_kc0 + t_csc + t_slice + t_solve,
+               spsolve_relres_fp64=rel, cores=1)
+    log('cpu reference path %dx%d: %s' % (n, n, out))
+    return out
+
+
+def cpu_baseline(mesh, prop, n_sample, log, ladder=(20, 64, 128)):
+    """The reference-equivalent CPU path timed on this host (the reference
+    itself cannot travel with the benchmark): the oracle (a plain-C
+    restatement of the per-element loop, pinned to the reference's golden
+    vectors) on the first n_sample elements of the headline plate, values
+    only (the GPU line's mode) on one core (the reference's loop is serial)
+    and on all host cores, with indices on one core; and the whole
+    reference path (KC0 loop, tocsc, slice, spsolve) on smaller plates."""
+    ncpu = min(os.cpu_count() or 1, 16)
+    v1, dt1, _ = _oracle_kc0(mesh, prop, n_sample, 1, True)
+    vn, dtn, used = _oracle_kc0(mesh, prop, n_sample, ncpu, True)
+    vi, dti, _ = _oracle_kc0(mesh, prop, n_sample // 2, 1, False)
+    log('cpu baseline: KC0 values-only %.0f el/s (1 core), %.0f el/s (%d threads); '
+        'with indices %.0f el/s' % (v1, vn, used, vi))
+    legs = {'kc0_values_all_cores': dict(value=vn, cores=used, s=dtn),
+            'kc0_indices_1_core': dict(value=vi, cores=1, s=dti, elements=n_sample // 2)}
+    for n in ladder:
+        try:
+            legs['reference_path_%dx%d' % (n, n)] = _reference_path(n, log)
+        except Exception as exc:  # reported, not hidden
+            legs['reference_path_%dx%d' % (n, n)] = dict(error=str(exc))
+    legs['reference_path_1000x1000'] = (
+        'not run (DNF at 6,012,006 DOF): the host COO alone is 13.8 GB and SuperLU of '
+        'the free-DOF system grows past the host memory cap; see the ladder above')
+    return dict(value=v1, unit='elements/s', cores=1, kind='port',
+                sample='KC0 values-only (overwrite) of the first %d elements of the '
+                       '%dx%d laminate plate by the oracle, one core, %.1f s'
+                       % (n_sample, mesh.nx - 1, mesh.ny - 1, dt1),
+                legs=legs)
 
 
 def solve_rates(n, repeats, log):
@@ -124,6 +195,14 @@ def solve_rates(n, repeats, log):
     t0 = time.perf_counter()
     sysm.setup_amg(mesh.ncoords, precisions=('fp64', 'fp32'))
     amg_setup = time.perf_counter() - t0
+    # the exact solution of the same system to ~1e-15: double-double
+    # refinement (its residual far below the FP64 rounding of u)
+    hi, lo, dd = sysm.solve_dd(f, method='amg', rtol=1e-13)
+    uex = hi.cpu().numpy() + lo.cpu().numpy()
+    out['exact_reference'] = dict(relres=dd['relres'], steps=dd['steps'])
+    out['spsolve_u_err_vs_exact'] = float(np.linalg.norm(uref - uex) / np.linalg.norm(uex))
+    out['spsolve_u_err_rz_vs_exact'] = float(np.linalg.norm((uref - uex)[~keep]) /
+                                             np.linalg.norm(uex[~keep]))
     for method in ('direct', 'fp64', 'mixed', 'ir', 'amg', 'amg32'):
         u, info = sysm.solve(f, method=method, rtol=1e-8)
         torch.cuda.synchronize()
@@ -135,12 +214,16 @@ def solve_rates(n, repeats, log):
         dt = (time.perf_counter() - t0) / repeats
         ug = u.cpu().numpy()
         err = np.linalg.norm((ug - uref)[keep]) / np.linalg.norm(uref[keep])
-        err_rz = np.linalg.norm((ug - uref)[~keep]) / np.linalg.norm(uref[~keep])
-        out[method] = dict(solves_per_s=1. / dt, ms=dt * 1e3, iters=info['iters'],
-                           outer=info['outer'], relres=info['relres'],
-                           u_err_vs_spsolve=err, u_err_rz_vs_spsolve=err_rz)
-        log('solve %dx%d %s: %.2f ms, %s, u err %.1e (rz %.1e)'
-            % (n, n, method, dt * 1e3, info, err, err_rz))
+        err_x = np.linalg.norm(ug - uex) / np.linalg.norm(uex)
+        err_rz = np.linalg.norm((ug - uex)[~keep]) / np.linalg.norm(uex[~keep])
+        out[method] = dict(ms=dt * 1e3, iters=info['iters'], outer=info['outer'],
+                           relres=info['relres'], converged=info['converged'],
+                           u_err_vs_spsolve_without_rz=err, u_err_vs_exact=err_x,
+                           u_err_rz_vs_exact=err_rz)
+        if info['converged']:
+            out[method]['solves_per_s'] = 1. / dt
+        log('solve %dx%d %s: %.2f ms, %s, u err vs exact %.2e (rz %.1e)'
+            % (n, n, method, dt * 1e3, info, err_x, err_rz))
     out['amg']['setup_s'] = amg_setup
     return out
 
@@ -305,8 +388,10 @@ def panel_rates(n, log, steps=10, maxiter=200000):
         u, info = sysm.solve(f, method=method, rtol=1e-8, maxiter=maxiter)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        sol[method] = dict(ms=dt * 1e3, solves_per_s=1. / dt, iters=info['iters'],
-                           outer=info['outer'], relres=info['relres'])
+        sol[method] = dict(ms=dt * 1e3, iters=info['iters'], outer=info['outer'],
+                           relres=info['relres'], converged=info['converged'])
+        if info['converged']:
+            sol[method]['solves_per_s'] = 1. / dt
         log('panel solve %s: %.1f ms, %s' % (method, dt * 1e3, info))
     sol['amg']['setup_s'] = amg_setup
     out['solve'] = sol
@@ -374,13 +459,18 @@ def panel_rates(n, log, steps=10, maxiter=200000):
     return out
 
 
-def large_solve(n, log):
+def large_solve(n, log, plain_maxiter=10000):
     """Static solve of the n x n laminate plate (the headline mesh: KC0 of
     the jittered 8-ply plate, clamped edges, uniform pressure 1e3 Pa) to
-    ||f - K u|| <= 1e-8 ||f||: CG preconditioned by the smoothed-aggregation
-    V-cycle (setup on the GPU, timed apart).  Block-Jacobi PCG does not
-    reach the tolerance on this system in 4e5 iterations (its residual is
-    still above 1 there)."""
+    ||f - K u|| <= 1e-8 ||f||, residuals compensated (pf3d_bsr_residual_dd):
+
+    * multigrid CG (smoothed-aggregation V-cycle, FP64 and FP32 V-cycles;
+      setup on the GPU, timed apart): an FP64 vector stops near 1e-7 here;
+    * the same solution refined as a double-double pair hi + lo
+      (StaticSystem.solve_dd): the pair meets 1e-8; the residual of hi, the
+      pair rounded to FP64, is reported (the FP64 rounding floor), with
+      the distance of the plain FP64 solution from the pair;
+    * plain FP64 block-Jacobi PCG, bounded at plain_maxiter iterations."""
     import torch
     from pyfe3d_amd import assembly as A
     from pyfe3d_amd.solver import StaticSystem
@@ -403,7 +493,8 @@ def large_solve(n, log):
     amg = sysm.setup_amg(mesh.ncoords, log=log, precisions=('fp64', 'fp32'))
     setup = time.perf_counter() - t0
     log('large solve %dx%d: amg setup %.1f s, %d levels' % (n, n, setup, len(amg.levels)))
-    out = dict(dof=mesh.n_dof, amg_setup_s=setup, levels=len(amg.levels))
+    out = dict(dof=mesh.n_dof, amg_setup_s=setup, levels=len(amg.levels), rtol=1e-8)
+    sols = {}
     for method in ('amg', 'amg32'):
         for rep in range(2):
             torch.cuda.synchronize()
@@ -412,15 +503,48 @@ def large_solve(n, log):
             torch.cuda.synchronize()
             dt = time.perf_counter() - t0
             log('large solve %dx%d %s: %.2f s, %s' % (n, n, method, dt, info))
-        out[method] = dict(s=dt, solves_per_s=1. / dt, iters=info['iters'],
-                           relres=info['relres'], restarts=info['outer'],
+        sols[method] = u
+        out[method] = dict(s=dt, iters=info['iters'], relres=info['relres'],
+                           converged=info['converged'], restarts=info['outer'],
                            ms_per_iter=dt * 1e3 / max(info['iters'], 1))
+        if 'fp64_iters' in info:
+            out[method]['fp64_iters'] = info['fp64_iters']
+        if info['converged']:
+            out[method]['solves_per_s'] = 1. / dt
+    # double-double refinement from the FP64 multigrid solution
+    for rep in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        hi, lo, dd = sysm.solve_dd(f, method='amg', rtol=1e-8, u=sols['amg'], maxiter=4000)
+        torch.cuda.synchronize()
+        dt_dd = time.perf_counter() - t0
+    udd = hi + lo
+    nrm = float(torch.linalg.vector_norm(udd))
+    total = out['amg']['s'] + dt_dd
+    out['amg_dd'] = dict(s=total, refine_s=dt_dd, steps=dd['steps'], refine_iters=dd['iters'],
+                         relres=dd['relres'], converged=dd['converged'],
+                         relres_fp64_rounding=dd['relres_fp64'])
+    if dd['converged']:
+        out['amg_dd']['solves_per_s'] = 1. / total
+    for method in ('amg', 'amg32'):
+        out[method]['u_err_vs_double_double'] = float(
+            torch.linalg.vector_norm((sols[method] - hi) - lo)) / nrm
+    log('large solve %dx%d double-double: %.2f s refine, %s, u err amg %.2e amg32 %.2e'
+        % (n, n, dt_dd, dd, out['amg']['u_err_vs_double_double'],
+           out['amg32']['u_err_vs_double_double']))
+    # plain FP64 block-Jacobi PCG, bounded
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, info = sysm.solve(f, method='fp64', rtol=1e-8, maxiter=plain_maxiter)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    out['plain_fp64_pcg'] = dict(s=dt, iters=info['iters'], relres=info['relres'],
+                                 converged=info['converged'],
+                                 ms_per_iter=dt * 1e3 / max(info['iters'], 1),
+                                 note='bounded at %d iterations' % plain_maxiter)
+    log('large solve %dx%d plain FP64 PCG: %.1f s, %s' % (n, n, dt, info))
     from pyfe3d_amd.amg import residual_floor
-    floor = residual_floor(sysm, u)
-    log('large solve: FP64 residual floor eps |A| |u| / |A u| = %.2e' % floor)
-    out.update(fp64_residual_floor=floor,
-               note='relres: compensated b - K u (plain FP64 row sums cancel to the floor '
-                    'given); the CG restarts from it until a restart gains < 10 %')
+    out['fp64_residual_bound'] = residual_floor(sysm, sols['amg'])
     return out
 
 
@@ -469,6 +593,8 @@ def main():
     ap.add_argument('--warmup', type=int, default=50)
     ap.add_argument('--n', type=int, default=1000, help='elements per side')
     ap.add_argument('--cpu-sample', type=int, default=200000)
+    ap.add_argument('--no-cpu-ladder', action='store_true',
+                    help='skip the reference CPU path (tocsc + spsolve) on smaller plates')
     ap.add_argument('--no-solve', action='store_true')
     ap.add_argument('--sweep-variants', type=int, default=4096,
                     help='plates of the design sweep solved with the banded direct solver')
@@ -666,7 +792,8 @@ def main():
             log('warm start failed: %s' % exc)
 
     n_sample = min(args.cpu_sample, E)
-    base = cpu_baseline(mesh, prop, n_sample)
+    base = cpu_baseline(mesh, prop, n_sample, log,
+                        ladder=() if args.no_cpu_ladder else (20, 64, 128))
     log('cpu baseline %.0f elements/s' % base['value'])
 
     line = {
@@ -702,44 +829,62 @@ def main():
 
 
 def solve_summary(extra):
-    """Solves/s to 1e-8 (FP64 residual), plain FP64 block-Jacobi PCG against
-    the mixed-precision multigrid CG (FP32 V-cycle inside FP64 CG) and the
-    best method, per workload measured above."""
+    """Solves/s to 1e-8 (compensated FP64 residual on FP64 K) per workload,
+    with the three effects apart: the preconditioner (plain FP64 block-
+    Jacobi PCG against FP64 multigrid CG), the precision (FP64 against FP32
+    V-cycle inside the same FP64 CG) and the learned warm start (zero start
+    against the network's guess in the same solver).  Solves that miss the
+    tolerance carry no rate."""
     out = {}
+
+    def rate(d, key='ms'):
+        return None if not d.get('converged', True) else 1e3 / d[key]
+
     s64 = extra.get('solve_64x64', {})
-    if 'fp64' in s64 and 'amg32' in s64:
-        best = min((v['ms'], k) for k, v in s64.items() if isinstance(v, dict) and 'ms' in v)
-        out['plate_64x64'] = dict(plain_fp64_ms=s64['fp64']['ms'],
-                                  mixed_multigrid_ms=s64['amg32']['ms'],
-                                  speedup_mixed_vs_plain=s64['fp64']['ms'] / s64['amg32']['ms'],
-                                  best=best[1], best_ms=best[0])
+    if 'fp64' in s64 and 'amg' in s64 and 'amg32' in s64:
+        out['plate_64x64'] = dict(
+            plain_fp64_pcg_ms=s64['fp64']['ms'], multigrid_fp64_ms=s64['amg']['ms'],
+            multigrid_fp32_vcycle_ms=s64['amg32']['ms'], direct_ms=s64['direct']['ms'],
+            preconditioner_speedup=s64['fp64']['ms'] / s64['amg']['ms'],
+            precision_speedup=s64['amg']['ms'] / s64['amg32']['ms'],
+            mixed_multigrid_vs_plain=s64['fp64']['ms'] / s64['amg32']['ms'],
+            all_converged=all(v.get('converged', False) for v in s64.values()
+                              if isinstance(v, dict) and 'ms' in v))
     sp = extra.get('stiffened_panel', {}).get('solve', {})
     if 'fp64' in sp and 'amg32' in sp:
         out['stiffened_panel_512'] = dict(
-            plain_fp64_ms=sp['fp64']['ms'], mixed_multigrid_ms=sp['amg32']['ms'],
-            multigrid_fp64_ms=sp['amg']['ms'],
-            speedup_mixed_vs_plain=sp['fp64']['ms'] / sp['amg32']['ms'])
+            plain_fp64_pcg_ms=sp['fp64']['ms'], multigrid_fp64_ms=sp['amg']['ms'],
+            multigrid_fp32_vcycle_ms=sp['amg32']['ms'],
+            preconditioner_speedup=sp['fp64']['ms'] / sp['amg']['ms'],
+            precision_speedup=sp['amg']['ms'] / sp['amg32']['ms'],
+            all_converged=all(sp[k].get('converged', False) for k in ('fp64', 'amg', 'amg32')))
     ls = [v for k, v in extra.items() if k.startswith('large_solve_') and isinstance(v, dict)]
     if ls and 'amg' in ls[0]:
-        out['plate_1000x1000'] = dict(multigrid_fp64_ms=ls[0]['amg']['s'] * 1e3,
-                                      multigrid_setup_s=ls[0]['amg_setup_s'],
-                                      relres=ls[0]['amg']['relres'],
-                                      fp64_residual_floor=ls[0]['fp64_residual_floor'],
-                                      plain_fp64='no convergence: residual above 1 after 4e5 '
-                                                 'block-Jacobi PCG iterations')
+        L = ls[0]
+        out['plate_1000x1000'] = dict(
+            multigrid_fp64=dict(ms=L['amg']['s'] * 1e3, relres=L['amg']['relres'],
+                                converged=L['amg']['converged']),
+            multigrid_fp32_vcycle=dict(ms=L['amg32']['s'] * 1e3, relres=L['amg32']['relres'],
+                                       converged=L['amg32']['converged']),
+            multigrid_double_double=dict(ms=L['amg_dd']['s'] * 1e3,
+                                         relres=L['amg_dd']['relres'],
+                                         converged=L['amg_dd']['converged'],
+                                         solves_per_s=L['amg_dd'].get('solves_per_s')),
+            fp64_rounding_of_solution_relres=L['amg_dd']['relres_fp64_rounding'],
+            u_err_fp64_multigrid_vs_double_double=L['amg']['u_err_vs_double_double'],
+            plain_fp64_pcg=dict(ms=L['plain_fp64_pcg']['s'] * 1e3,
+                                iters=L['plain_fp64_pcg']['iters'],
+                                relres=L['plain_fp64_pcg']['relres'],
+                                converged=L['plain_fp64_pcg']['converged']),
+            multigrid_setup_s=L['amg_setup_s'])
     sw = extra.get('design_sweep', {}).get('direct')
     if sw:
         out['design_sweep_4096'] = dict(solves_per_s=sw['solves_per_s'],
+                                        all_converged=sw['converged'] == sw.get('variants', 4096),
                                         method='batched banded Cholesky, autotuned launch shape')
     ws = extra.get('warm_start_64x64', {})
-    if 'fp64' in ws and 'mixed' in ws:
-        out['learned_warm_start_64x64'] = dict(
-            plates=ws['plates'], plain_fp64_zero_ms=ws['fp64']['zero_ms'],
-            plain_fp64_warm_ms=ws['fp64']['warm_ms'], mixed_warm_ms=ws['mixed']['warm_ms'],
-            iteration_ratio_fp64=ws['fp64']['iteration_ratio'],
-            note='the network guess leaves ~99.97 % of the zero start\'s energy-norm error '
-                 '(the norm CG minimises), so it does not shorten the solve; the multigrid '
-                 'lines above are the mixed-precision speed-up')
+    if 'summary' in ws:
+        out['learned_warm_start_64x64'] = ws['summary']
     return out
 
 

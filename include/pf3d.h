@@ -674,6 +674,21 @@ int pf3d_nn_up2_bwd(int B, int H, int W, int C, const float *dy, float *dlo, voi
 int pf3d_nn_node_avg(int B, int n, int C, const float *e, float *u, void *stream);
 int pf3d_nn_node_loss_grad(int B, int n, int C, const float *u, const float *target,
                            const float *wc, float scale, float *de, void *stream);
+/* Energy-norm loss of an initial guess (per plate b of B, per = 6*nodes
+ * DOFs each): pf3d_nn_to_dofs scales the network's node values per channel
+ * into an FP64 DOF vector x (0 on fixed DOFs); with kx = K x (the plates'
+ * BSR product), f the loads and s[b] = u*.f the solution's energy,
+ * pf3d_nn_energy_dots writes out[3b..] = (a = x.f, c = x.K x,
+ * L = 1 - a^2 / (c s)), the relative energy-norm error of the best multiple
+ * of x; pf3d_nn_energy_grad writes gscale * dL/d(element outputs) through
+ * the adjoint of pf3d_nn_node_avg. */
+int pf3d_nn_to_dofs(int64_t n, int C, const float *u, const float *scale, const uint8_t *fixed,
+                    double *x, void *stream);
+int pf3d_nn_energy_dots(int B, int64_t per, const double *x, const double *kx, const double *f,
+                        const double *s, double *out, void *stream);
+int pf3d_nn_energy_grad(int B, int n, int C, const double *kx, const double *f,
+                        const double *acl, const double *s, const float *scale,
+                        const uint8_t *fixed, float gscale, float *de, void *stream);
 /* Adam step `step` (1-based) with bias correction */
 int pf3d_nn_adam(int64_t n, float *p, const float *g, float *m, float *v, float lr, float b1,
                  float b2, float eps, int step, void *stream);

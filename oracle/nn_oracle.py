@@ -147,13 +147,42 @@ def loss(u, target, wc):
     return 0.5 * float((d * d * wc).sum()) / count
 
 
+def energy_loss(u, scale, fixed, Ks, fs, ss):
+    """Mean over the plates of the relative energy-norm error of the best
+    multiple of the guess: x_b = u_b * scale (0 on fixed DOFs), a = x.f,
+    c = x.K x, L_b = 1 - a^2 / (c s_b) with s_b = u*.f the solution's
+    energy.  Ks: per-plate stiffness (boundary conditions applied, scipy or
+    dense), fs: loads, ss: energies.  Returns (loss, du (B, nn, nn, C),
+    captured) with captured = mean of a^2 / (c s) = 1 - loss, accurate where
+    the loss is within rounding of 1."""
+    B, nn, _, C = u.shape
+    du = np.zeros_like(u)
+    cap = 0.
+    for b in range(B):
+        x = (u[b] * scale).reshape(-1)
+        x[fixed[b]] = 0.
+        kx = Ks[b] @ x
+        a, c = float(x @ fs[b]), float(x @ kx)
+        cap += a * a / (c * ss[b])
+        g = 2. * a / (c * ss[b]) * (a / c * kx - fs[b])
+        g[fixed[b]] = 0.
+        du[b] = g.reshape(nn, nn, C) * scale / B
+    return 1. - cap / B, du, cap / B
+
+
 def backward(p, cache, u, target, wc):
     """Gradients of loss() with respect to every parameter."""
-    spec = {n: (s, k, r) for n, _, _, s, k, r in LAYERS}
     B, nn, _, C = u.shape
     count = B * (nn - 2) ** 2
     du = np.zeros_like(u)
     du[:, 1:-1, 1:-1] = (u - target)[:, 1:-1, 1:-1] * wc / count
+    return backward_from(p, cache, du)
+
+
+def backward_from(p, cache, du):
+    """Parameter gradients given the gradient du of a loss with respect to
+    the node values (B, n+1, n+1, C) of forward()."""
+    spec = {n: (s, k, r) for n, _, _, s, k, r in LAYERS}
     g = {}
 
     def conv_bwd(name, dy):

@@ -21,6 +21,9 @@
 #include <math.h>
 #include <stdint.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define DOF 6
 
@@ -1169,6 +1172,10 @@ int orc_assemble(int etype, int mat, int mtype, int64_t E, const double *x,
     int stride = (IS_BEAM(etype) || etype == ET_TRUSS) ? BP_STRIDE
                  : (etype == ET_SPRING ? SPR_STRIDE : SP_STRIDE);
     static const double hg_default[5] = {1., 1., 1., 1., 1.};
+    int err = 0;
+    /* elements are independent and write disjoint slots: the loop may run
+     * on several host threads (orc_set_threads) without changing a bit */
+#pragma omp parallel for schedule(static) reduction(| : err)
     for (int64_t e = 0; e < E; e++) {
         const int64_t *cn = conn + e * nn;
         double g[14], xe[12], ue[24], K[576], Kg[576];
@@ -1177,7 +1184,7 @@ int orc_assemble(int etype, int mat, int mtype, int64_t E, const double *x,
         int st = element_local(etype, mat, mtype, g, xe, props + pid[e] * stride,
                                k6rot ? k6rot[e] : 100., alpha ? alpha[e] : 0.7,
                                hg ? hg + 5 * e : hg_default, ue, stress, K);
-        if (st) return st;
+        if (st) { err |= st; continue; }
         /* Kg = R6 K R6^T with R6 = blockdiag(R, R) per node */
         for (int i = 0; i < n; i++)
             for (int j = 0; j < n; j++) {
@@ -1197,7 +1204,7 @@ int orc_assemble(int etype, int mat, int mtype, int64_t E, const double *x,
             v[k0 + k] += Kg[rl[k] * n + cl[k]];
         }
     }
-    return 0;
+    return err;
 }
 
 /*
@@ -1237,4 +1244,17 @@ int orc_fint(int etype, int64_t E, const double *x, const int64_t *conn, const d
                 }
     }
     return 0;
+}
+
+/* threads of the element loop (1 = the reference's serial loop); returns
+ * the count in use */
+int orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
 }

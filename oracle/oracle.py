@@ -42,11 +42,19 @@ def lib():
         _lib.orc_assemble.restype = ctypes.c_int
         _lib.orc_geometry.argtypes = [ctypes.c_int, ctypes.c_int64, P, P, P, P, P]
         _lib.orc_slot_count.argtypes = [ctypes.c_int] * 3
+        _lib.orc_set_threads.argtypes = [ctypes.c_int]
+        _lib.orc_set_threads.restype = ctypes.c_int
+        _lib.orc_set_threads(1)  # the reference's serial element loop by default
     return _lib
 
 
 def _p(a):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def set_threads(n):
+    """Host threads of the element loop of assemble(); returns the count."""
+    return lib().orc_set_threads(int(n))
 
 
 def slot_count(etype, mat, mtype=0):

@@ -106,6 +106,9 @@ SIGNATURES = {
     'pf3d_nn_node_avg': [I32, I32, I32, P, P, P],
     'pf3d_nn_node_loss_grad': [I32, I32, I32, P, P, P, F32, P, P],
     'pf3d_nn_adam': [I64, P, P, P, P, F32, F32, F32, F32, I32, P],
+    'pf3d_nn_to_dofs': [I64, I32, P, P, P, P, P],
+    'pf3d_nn_energy_dots': [I32, I64, P, P, P, P, P, P],
+    'pf3d_nn_energy_grad': [I32, I32, I32, P, P, P, P, P, P, F32, P, P],
     'pf3d_batch_solve': [I64, I64, P, P, P, P, P, P, P, P, P, P, I32, F64, I32, I32, P, I64,
                          P, P, P, I32, I32, P],
 }

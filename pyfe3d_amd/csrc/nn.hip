@@ -309,6 +309,91 @@ __global__ void node_loss_grad(int B, int n, int C, const float *__restrict__ u,
     de[t] = 0.25f * scale * wc[c] * s;
 }
 
+// ---- energy-norm loss of the initial guess -----------------------------
+// With u the network's node values scaled per channel, K the plate's
+// stiffness (boundary conditions applied), f its loads and s = u*.f the
+// energy of the solution u*, the best multiple alpha u of the guess
+// (alpha = u.f / u.K u) leaves the relative energy-norm error
+//   L = ||u* - alpha u||_K^2 / ||u*||_K^2 = 1 - a^2 / (c s),  a = u.f,
+//   c = u.K u,
+// the norm in which CG converges.  dL/du = (2a / (c s)) ((a / c) K u - f).
+
+// x[t] = 0 on fixed DOFs, else u[t] * scale[t % C] (FP64 DOF vector)
+__global__ void nn_to_dofs(int64_t n, int C, const float *__restrict__ u,
+                           const float *__restrict__ scale, const uint8_t *__restrict__ fixed,
+                           double *__restrict__ x)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    x[t] = fixed[t] ? 0. : (double)u[t] * (double)scale[t % C];
+}
+
+// one workgroup per plate: out[3b..3b+2] = (a, c, L) in a fixed order
+__global__ void __launch_bounds__(256) energy_dots(int64_t per, const double *__restrict__ x,
+                                                   const double *__restrict__ kx,
+                                                   const double *__restrict__ f,
+                                                   const double *__restrict__ s,
+                                                   double *__restrict__ out)
+{
+    __shared__ double sa[256], sc[256];
+    const int b = blockIdx.x;
+    const double *xb = x + (int64_t)b * per, *kb = kx + (int64_t)b * per, *fb = f + (int64_t)b * per;
+    double a = 0., c = 0.;
+    for (int64_t t = threadIdx.x; t < per; t += 256) {
+        a += xb[t] * fb[t];
+        c += xb[t] * kb[t];
+    }
+    sa[threadIdx.x] = a;
+    sc[threadIdx.x] = c;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sa[threadIdx.x] += sa[threadIdx.x + w];
+            sc[threadIdx.x] += sc[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double A = sa[0], Cc = sc[0];
+        out[3 * b] = A;
+        out[3 * b + 1] = Cc;
+        out[3 * b + 2] = (Cc > 0. && s[b] > 0.) ? 1. - A * A / (Cc * s[b]) : 1.;
+    }
+}
+
+// de[b][ei][ej][c] = gscale * scale[c] * 0.25 * sum over the element's
+// interior corner nodes of dL_b/du (the adjoint of node_avg)
+__global__ void energy_grad(int B, int n, int C, const double *__restrict__ kx,
+                            const double *__restrict__ f, const double *__restrict__ acl,
+                            const double *__restrict__ s, const float *__restrict__ scale,
+                            const uint8_t *__restrict__ fixed, float gscale,
+                            float *__restrict__ de)
+{
+    const int nn = n + 1;
+    const int64_t total = (int64_t)B * n * n * C;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int c = (int)(t % C);
+    int64_t r = t / C;
+    const int ej = (int)(r % n); r /= n;
+    const int ei = (int)(r % n);
+    const int b = (int)(r / n);
+    const double A = acl[3 * b], Cc = acl[3 * b + 1], S = s[b];
+    double g = 0.;
+    if (Cc > 0. && S > 0. && isfinite(A) && isfinite(Cc)) {
+        const double k1 = 2. * A / (Cc * S), k2 = A / Cc;
+        for (int di = 0; di < 2; di++)
+            for (int dj = 0; dj < 2; dj++) {
+                const int i = ei + di, j = ej + dj;
+                if (i > 0 && j > 0 && i < n && j < n) {
+                    const int64_t k = (((int64_t)b * nn + i) * nn + j) * C + c;
+                    if (!fixed[k]) g += k1 * (k2 * kx[k] - f[k]);
+                }
+            }
+    }
+    de[t] = (float)(0.25 * (double)gscale * (double)scale[c] * g);
+}
+
 // Adam (Kingma & Ba 2015) with bias correction folded into the step size
 __global__ void adam_step(int64_t n, float *__restrict__ p, const float *__restrict__ g,
                           float *__restrict__ m, float *__restrict__ v, float lr, float b1,
@@ -432,6 +517,39 @@ int pf3d_nn_node_loss_grad(int B, int n, int C, const float *u, const float *tar
     hipLaunchKernelGGL(node_loss_grad, dim3(nb256(tot)), dim3(256), 0, (hipStream_t)stream, B, n,
                        C, u, target, wc, scale, de);
     return pf3d_hip_check(hipGetLastError(), "pf3d_nn_node_loss_grad");
+}
+
+int pf3d_nn_to_dofs(int64_t n, int C, const float *u, const float *scale, const uint8_t *fixed,
+                    double *x, void *stream)
+{
+    if (n < 0 || C <= 0 || (n && (!u || !scale || !fixed || !x)))
+        return pf3d_fail(PF3D_E_ARG, "pf3d_nn_to_dofs: bad arguments");
+    if (!n) return PF3D_OK;
+    hipLaunchKernelGGL(nn_to_dofs, dim3(nb256(n)), dim3(256), 0, (hipStream_t)stream, n, C, u,
+                       scale, fixed, x);
+    return pf3d_hip_check(hipGetLastError(), "pf3d_nn_to_dofs");
+}
+
+int pf3d_nn_energy_dots(int B, int64_t per, const double *x, const double *kx, const double *f,
+                        const double *s, double *out, void *stream)
+{
+    if (B <= 0 || per <= 0 || !x || !kx || !f || !s || !out)
+        return pf3d_fail(PF3D_E_ARG, "pf3d_nn_energy_dots: bad arguments");
+    hipLaunchKernelGGL(energy_dots, dim3(B), dim3(256), 0, (hipStream_t)stream, per, x, kx, f, s,
+                       out);
+    return pf3d_hip_check(hipGetLastError(), "pf3d_nn_energy_dots");
+}
+
+int pf3d_nn_energy_grad(int B, int n, int C, const double *kx, const double *f,
+                        const double *acl, const double *s, const float *scale,
+                        const uint8_t *fixed, float gscale, float *de, void *stream)
+{
+    if (B <= 0 || n <= 1 || C <= 0 || !kx || !f || !acl || !s || !scale || !fixed || !de)
+        return pf3d_fail(PF3D_E_ARG, "pf3d_nn_energy_grad: bad arguments");
+    const int64_t tot = (int64_t)B * n * n * C;
+    hipLaunchKernelGGL(energy_grad, dim3(nb256(tot)), dim3(256), 0, (hipStream_t)stream, B, n, C,
+                       kx, f, acl, s, scale, fixed, gscale, de);
+    return pf3d_hip_check(hipGetLastError(), "pf3d_nn_energy_grad");
 }
 
 int pf3d_nn_adam(int64_t n, float *p, const float *g, float *m, float *v, float lr, float b1,

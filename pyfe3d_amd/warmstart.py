@@ -188,6 +188,40 @@ class WarmStartNet:
         de = self.dact['head']
         _lib.call('pf3d_nn_node_loss_grad', B, n, 6, self.u.data_ptr(),
                   target.contiguous().data_ptr(), wc.data_ptr(), 1.0 / count, de.data_ptr(), st)
+        self.backward_from(stream)
+
+    def energy_loss(self, system, f, s_energy, scale, stream=None):
+        """Energy-norm loss of the guesses of the last forward() on the
+        plates of ``system`` (a PlateSweep of B plates of n x n elements,
+        assembled): the mean over the plates of 1 - (x.f)^2 / ((x.K x) s),
+        x the node values scaled per channel (0 on fixed DOFs), s = u*.f the
+        solutions' energies.  Leaves the gradient with respect to the
+        element outputs in place for backward_from(); returns the per-plate
+        (x.f, x.K x, loss) on the device."""
+        n, B, st = self.n, self.B, _s(stream)
+        ndof = B * (n + 1) * (n + 1) * 6
+        if not hasattr(self, '_x64') or self._x64.numel() != ndof:
+            self._x64 = torch.empty(ndof, dtype=torch.float64, device=self.dev)
+            self._kx = torch.empty_like(self._x64)
+            self._acl = torch.empty(3 * B, dtype=torch.float64, device=self.dev)
+        _lib.call('pf3d_nn_to_dofs', ndof, 6, self.u.data_ptr(), scale.data_ptr(),
+                  system.fixed.data_ptr(), self._x64.data_ptr(), st)
+        _lib.call('pf3d_bsr_spmv', system.n_nodes, system.plan.row_ptr.data_ptr(),
+                  system.plan.col_idx.data_ptr(), system.val.data_ptr(), self._x64.data_ptr(),
+                  self._kx.data_ptr(), st)
+        _lib.call('pf3d_nn_energy_dots', B, ndof // B, self._x64.data_ptr(), self._kx.data_ptr(),
+                  f.data_ptr(), s_energy.data_ptr(), self._acl.data_ptr(), st)
+        _lib.call('pf3d_nn_energy_grad', B, n, 6, self._kx.data_ptr(), f.data_ptr(),
+                  self._acl.data_ptr(), s_energy.data_ptr(), scale.data_ptr(),
+                  system.fixed.data_ptr(), 1.0 / B, self.dact['head'].data_ptr(), st)
+        return self._acl.view(B, 3)
+
+    def backward_from(self, stream=None):
+        """Parameter gradients from the gradient with respect to the element
+        outputs held in ``dact['head']`` (set by backward() or
+        energy_loss())."""
+        n, B, st = self.n, self.B, _s(stream)
+        de = self.dact['head']
         A, D = self.act, self.dact
         self._conv_bwd('head', A['u1'], de, D['u1'], stream=stream)
         self._conv_bwd('u1', self.z[1], D['u1'], self.dz[1], stream=stream)
@@ -266,19 +300,8 @@ def features(h, p, h0=0.002, dh=0.001):
 # dataset, training, use as the PCG's initial guess
 # ----------------------------------------------------------------------
 
-class Scales:
-    """Target normalisation.  The network learns the shape of the
-    deflection w only: each training target is w / rms(w) of its plate
-    (then divided by the dataset std), because the amplitude comes from the
-    Galerkin projection at solve time and the rotations from w (Kirchhoff:
-    for thin plates independent rotation guesses carry shear energy that
-    is (L/h)^2 times their bending energy)."""
-
-    def __init__(self, std=None):
-        self.std = np.ones(6) if std is None else np.asarray(std, dtype=np.float64)
-
-
-W_CHANNEL = 2  # w among (u, v, w, rx, ry, rz)
+# channel order of the node values: u, v, w, rx, ry, rz
+W_CHANNEL = 2
 
 
 def _plates(n_var, n, seed, a=1.0, b=1.0):
@@ -295,138 +318,168 @@ def _plates(n_var, n, seed, a=1.0, b=1.0):
     return h, p, f
 
 
-def solve_plates(h, f, mode='fp64', rtol=1e-8, sweep=None, x0=None):
-    """Solve every plate (clamped edges) with the batched GPU solver.
+def _dof_loads(f):
+    """(V, n+1, n+1) vertical nodal forces -> (V * (n+1)^2 * 6) DOF vector."""
+    V, n1, _ = f.shape
+    fd = np.zeros((V, n1 * n1, 6))
+    fd[:, :, 2] = f.reshape(V, -1)
+    return fd.reshape(-1)
+
+
+def solve_plates(h, f, mode='direct', rtol=1e-8, sweep=None, x0=None, max_iter=200000):
+    """Solve every plate (clamped edges) with the batched GPU solvers
+    (banded Cholesky for 'direct', block-Jacobi PCG for 'fp64' / 'mixed').
     Returns (u (V, n+1, n+1, 6) float64 tensor, sweep)."""
     from .sweep import PlateSweep
     V, n = h.shape[0], h.shape[1]
     if sweep is None or sweep.n_var != V:
         sweep = PlateSweep([(n, n)] * V, a=1.0, b=1.0)
     sweep.set_thickness(h.reshape(-1))
-    fd = np.zeros((V, (n + 1) * (n + 1), 6))
-    fd[:, :, 2] = f.reshape(V, -1)
-    sweep.set_loads(fd.reshape(-1))
+    sweep.set_loads(_dof_loads(f))
     if x0 is not None:
         sweep.u.copy_(torch.as_tensor(x0, dtype=torch.float64).reshape(-1))
-    sweep.step(mode=mode, rtol=rtol, x0=x0 is not None)
+    if mode == 'direct':
+        sweep.step(mode='direct', rtol=rtol)
+    else:
+        sweep.step(mode=mode, rtol=rtol, x0=x0 is not None, max_iter=max_iter)
     return sweep.u.view(V, n + 1, n + 1, 6), sweep
 
 
-def make_dataset(n_samples, seed, n=64, chunk=1024, rtol=1e-6, log=None):
-    """Plates, their GPU solutions (to rtol: the network's own error is
-    orders of magnitude above it) and the network inputs/targets (float32
-    tensors on the GPU; targets normalised with per-channel std)."""
-    X, U, PS = [], [], []
+class Scales:
+    """Per-channel scale of the network's node values: the standard
+    deviation over the training solutions, each normalised by the rms of
+    its deflection (the amplitude of a guess comes from the Galerkin
+    projection at solve time, so only shapes are learnt).  Channels that
+    are zero in every solution (u, v, rz of these isotropic plates) get
+    scale 0: the guess leaves them at zero."""
+
+    def __init__(self, std):
+        self.std = np.asarray(std, dtype=np.float64)
+
+    def tensor(self, device='cuda'):
+        return torch.as_tensor(self.std, dtype=torch.float32, device=device)
+
+
+class EnergyData:
+    """Training plates on the GPU: inputs X (S, n, n, 2), thicknesses H
+    (S, n*n), loads F (S, (n+1)^2 * 6) and solution energies s = u*.f (S)."""
+
+    def __init__(self, X, H, F, S, scales):
+        self.X, self.H, self.F, self.S, self.scales = X, H, F, S, scales
+
+    @property
+    def size(self):
+        return self.X.shape[0]
+
+
+def make_energy_dataset(n_samples, seed, n=64, chunk=256, log=None):
+    """Seeded plates solved by the batched banded Cholesky (every residual
+    checked at 1e-8); keeps what the energy loss needs (no nodal targets):
+    inputs, thicknesses, loads and u*.f, plus the channel scales."""
+    X, H, F, S = [], [], [], []
     sweep = None
+    ssum = np.zeros(6)
+    count = 0
     for c0 in range(0, n_samples, chunk):
         V = min(chunk, n_samples - c0)
         h, p, f = _plates(V, n, seed * 1000003 + c0)
-        u, sweep = solve_plates(h, f, rtol=rtol,
+        u, sweep = solve_plates(h, f, mode='direct',
                                 sweep=sweep if sweep is not None and sweep.n_var == V else None)
         torch.cuda.synchronize()
         sweep.check()
-        if float(sweep.relres.max()) > rtol:
+        if float(sweep.relres.max()) > 1e-8:
             raise RuntimeError('training solve missed the tolerance')
-        x, ps = features(h, p)
+        x, _ = features(h, p)
         X.append(torch.as_tensor(x, device='cuda'))
-        U.append(u.to(torch.float32).clone())
-        PS.append(ps)
+        H.append(torch.as_tensor(h.reshape(V, -1), device='cuda'))
+        F.append(sweep.f.view(V, -1).clone())
+        S.append((sweep.u.view(V, -1) * sweep.f.view(V, -1)).sum(dim=1))
+        w = u[:, 1:-1, 1:-1, W_CHANNEL]
+        rms = w.pow(2).mean(dim=(1, 2)).sqrt().clamp_min(1e-300)
+        T = (u / rms[:, None, None, None])[:, 1:-1, 1:-1].reshape(-1, 6)
+        ssum += T.pow(2).sum(dim=0).cpu().numpy()
+        count += T.shape[0]
         if log:
             log('dataset %d/%d' % (c0 + V, n_samples))
-    X = torch.cat(X)
-    U = torch.cat(U)
-    w = U[:, 1:-1, 1:-1, W_CHANNEL]
-    rms = w.pow(2).mean(dim=(1, 2)).sqrt().clamp_min(1e-30)
-    T = U / rms[:, None, None, None]
-    std = T[:, 1:-1, 1:-1].reshape(-1, 6).std(dim=0).double().cpu().numpy()
-    std = np.where(std > 0, std, 1.)
-    sc = Scales(std)
-    T = T / torch.as_tensor(std, dtype=torch.float32, device='cuda')
-    return X, T.contiguous(), sc
+    std = np.sqrt(ssum / count)
+    # channels that carry no displacement (relative to w) stay at zero
+    std = np.where(std > 1e-9 * std[W_CHANNEL], std, 0.)
+    return EnergyData(torch.cat(X), torch.cat(H), torch.cat(F), torch.cat(S), Scales(std))
 
 
-def train(net, X, T, epochs=10, batch=32, lr=1e-3, seed=0, log=None):
-    """Adam on the standardised MSE of the deflection channel; returns the
-    loss history per epoch."""
-    wc = torch.zeros(6, dtype=torch.float32, device='cuda')
-    wc[W_CHANNEL] = 1.
-    S = X.shape[0]
+def train_energy(net, data, epochs=12, batch=32, lr=1e-3, seed=0, log=None):
+    """Adam on the energy-norm loss (mean over the batch of the relative
+    energy error of the best multiple of the guess); each batch's
+    stiffness is assembled on the fly (one KC0 launch and one block-plan
+    apply for its plates).  Returns the mean loss per epoch."""
+    from .sweep import PlateSweep
+    n = net.n
+    sweep = PlateSweep([(n, n)] * batch, a=1.0, b=1.0)
+    scale = data.scales.tensor()
+    S = data.size
     g = torch.Generator(device='cpu').manual_seed(seed)
     hist = []
     for ep in range(epochs):
         perm = torch.randperm(S, generator=g)
-        tot, nb = 0., 0
+        tot = torch.zeros((), dtype=torch.float64, device='cuda')
+        nb = 0
+        lr_ep = lr * (0.5 ** (ep // max(1, epochs // 4)))
         for b0 in range(0, S - batch + 1, batch):
             idx = perm[b0:b0 + batch].to('cuda')
-            xb, tb = X[idx].contiguous(), T[idx].contiguous()
-            u = net.forward(xb)
-            net.backward(tb, wc)
-            net.adam(lr=lr * (0.5 ** (ep // max(1, epochs // 3))))
-            if b0 % (batch * 16) == 0:
-                d = (u - tb)[:, 1:-1, 1:-1, W_CHANNEL]
-                tot += 0.5 * float((d * d).sum()) / (batch * (net.n - 1) ** 2)
-                nb += 1
-        hist.append(tot / max(1, nb))
+            sweep.set_thickness(data.H[idx].reshape(-1))
+            sweep.assemble()
+            net.forward(data.X[idx].contiguous())
+            acl = net.energy_loss(sweep, data.F[idx].contiguous().view(-1),
+                                  data.S[idx].contiguous(), scale)
+            tot += acl[:, 2].mean()
+            nb += 1
+            net.backward_from()
+            net.adam(lr=lr_ep)
+        hist.append(float(tot) / max(1, nb))
         if log:
-            log('epoch %d loss %.4e' % (ep, hist[-1]))
+            log('epoch %d energy loss %.4e (lr %.1e)' % (ep, hist[-1], lr_ep))
     return hist
 
 
-def kirchhoff_rotations(w, a=1.0, b=1.0):
-    """(V, n+1, n+1) nodal deflections -> (V, n+1, n+1, 6) DOF vectors with
-    rx = w,y and ry = -w,x (zero transverse shear for the element's
-    gyz = w,y - rx, gxz = w,x + ry), central differences inside, one-sided
-    at the edges; u, v, rz = 0."""
-    V, n1 = w.shape[0], w.shape[1]
-    dx, dy = a / (n1 - 1), b / (n1 - 1)
-    wx = torch.empty_like(w)
-    wy = torch.empty_like(w)
-    wx[:, 1:-1] = (w[:, 2:] - w[:, :-2]) / (2 * dx)
-    wx[:, 0] = (w[:, 1] - w[:, 0]) / dx
-    wx[:, -1] = (w[:, -1] - w[:, -2]) / dx
-    wy[:, :, 1:-1] = (w[:, :, 2:] - w[:, :, :-2]) / (2 * dy)
-    wy[:, :, 0] = (w[:, :, 1] - w[:, :, 0]) / dy
-    wy[:, :, -1] = (w[:, :, -1] - w[:, :, -2]) / dy
-    u = torch.zeros((V, n1, n1, 6), dtype=w.dtype, device=w.device)
-    u[..., 2] = w
-    u[..., 3] = wy
-    u[..., 4] = -wx
-    return u
-
-
-def predict(net, h, p, sc, batch=64):
-    """Initial-guess shapes (V, n+1, n+1, 6) float64 on the GPU: the
-    network's deflection with Kirchhoff rotations, before the Galerkin
-    scaling (which sets the amplitude)."""
-    x, _ = features(h, p)
+def predict(net, x, scales, fixed, batch=64):
+    """FP64 DOF vectors (V * (n+1)^2 * 6) of the network's guesses for the
+    inputs x (V, n, n, 2) (pf3d_nn_to_dofs: node values times the channel
+    scales, fixed DOFs 0), before the Galerkin scaling."""
     V = x.shape[0]
-    w = torch.empty((V, net.n + 1, net.n + 1), dtype=torch.float64, device='cuda')
+    n1 = net.n + 1
+    per = n1 * n1 * 6
+    out = torch.empty(V * per, dtype=torch.float64, device='cuda')
+    scale = scales.tensor()
+    st = torch.cuda.current_stream().cuda_stream
     for b0 in range(0, V, batch):
         xb = torch.as_tensor(x[b0:b0 + batch], device='cuda')
+        k = xb.shape[0]
         u = net.forward(xb)
-        w[b0:b0 + xb.shape[0]] = u[:xb.shape[0], :, :, W_CHANNEL].double() * sc.std[W_CHANNEL]
-    return kirchhoff_rotations(w)
+        _lib.call('pf3d_nn_to_dofs', k * per, 6, u.data_ptr(), scale.data_ptr(),
+                  fixed[b0 * per:(b0 + k) * per].data_ptr(), out[b0 * per:].data_ptr(), st)
+    return out
 
 
-def galerkin_scale(sweep, u0, stats=None):
-    """Per plate alpha = u0.f / u0.K u0 (K after the Dirichlet rows and
-    columns, u0 with its fixed DOFs zeroed): the multiple of u0 closest to
-    the solution in the energy norm, whose error ||u - alpha u0||_K^2 =
+def galerkin_scale(system, f, u0, n_var, stats=None):
+    """Per plate alpha = u0.f / u0.K u0: the multiple of u0 closest to the
+    solution in the energy norm, whose error ||u - alpha u0||_K^2 =
     ||u||_K^2 - (u0.f)^2 / u0.K u0 never exceeds that of a zero start (the
-    norm in which CG converges).  Plates with alpha <= 0 (or no finite
-    alpha) start from zero.  Returns (scaled u0, number of zero starts);
-    ``stats`` (a dict), if given, receives the per-plate u0.f and u0.K u0."""
-    V = sweep.n_var
-    u = u0.reshape(-1).clone()
-    u.mul_(1 - sweep.fixed.to(torch.float64))
+    norm in which CG converges).  Plates without a finite nonzero alpha
+    start from zero (either sign is fine: the identity above does not
+    depend on it).  ``system``: anything with plan, val, n_nodes (the
+    plates' block-diagonal BSR, boundary conditions applied).  Returns
+    (scaled u0, number of zero starts); ``stats`` receives u0.f, u0.K u0."""
+    u = u0.clone()
     Ku = torch.empty_like(u)
-    _lib.call('pf3d_bsr_spmv', sweep.n_nodes, sweep.plan.row_ptr.data_ptr(),
-              sweep.plan.col_idx.data_ptr(), sweep.val.data_ptr(), u.data_ptr(), Ku.data_ptr(),
+    _lib.call('pf3d_bsr_spmv', system.n_nodes, system.plan.row_ptr.data_ptr(),
+              system.plan.col_idx.data_ptr(), system.val.data_ptr(), u.data_ptr(), Ku.data_ptr(),
               torch.cuda.current_stream().cuda_stream)
-    uv, kv, fv = u.view(V, -1), Ku.view(V, -1), sweep.f.view(V, -1)
+    uv, kv, fv = u.view(n_var, -1), Ku.view(n_var, -1), f.view(n_var, -1)
     uf = (uv * fv).sum(dim=1)
     uku = (uv * kv).sum(dim=1)
-    ok = (uku > 0) & (uf > 0) & torch.isfinite(uf) & torch.isfinite(uku)
+    # the energy-norm identity holds for either sign of alpha (the energy
+    # loss does not fix the sign of a shape)
+    ok = (uku > 0) & (uf != 0) & torch.isfinite(uf) & torch.isfinite(uku)
     alpha = torch.where(ok, uf / torch.where(ok, uku, torch.ones_like(uku)),
                         torch.zeros_like(uku))
     uv.mul_(alpha[:, None])
@@ -435,92 +488,161 @@ def galerkin_scale(sweep, u0, stats=None):
     return u, int((~ok).sum())
 
 
-def save(net, sc, path):
-    """Parameters and target scales as an .npz (no pickled objects)."""
+def save(net, sc, path, meta=None):
+    """Parameters, channel scales and evaluation metadata (JSON text) as an
+    .npz (no pickled objects)."""
+    import json
     arrs = {k: v.astype(np.float32) for k, v in net.params_numpy().items()}
     arrs['scales.std'] = np.asarray(sc.std)
+    arrs['meta.json'] = np.array(json.dumps(meta or {}))
     np.savez(path, **arrs)
 
 
 def load(path, batch=64, device='cuda'):
+    """(net, scales, meta) from save()."""
+    import json
     z = np.load(path, allow_pickle=False)
     net = WarmStartNet(n=64, batch=batch, device=device)
     for k in net.p:
         net.p[k].copy_(torch.as_tensor(z[k], dtype=torch.float32))
-    sc = Scales(std=z['scales.std'])
-    return net, sc
+    meta = json.loads(str(z['meta.json'])) if 'meta.json' in z.files else {}
+    return net, Scales(z['scales.std']), meta
 
 
-def evaluate(net, sc, n_eval, seed, max_iter=60000, log=None):
-    """Held-out plates: batched solves to 1e-8 from zero and from the
-    network's (Galerkin-scaled) guess, FP64 PCG and FP32/FP64 refinement.
-    Times are GPU event times of the whole batch (network inference and
-    scaling included in the warm-start figures)."""
+def _timed(fn):
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    r = fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1), r
+
+
+def evaluate_pcg(net, sc, h, p, f, mode='fp64', max_iter=60000, log=None):
+    """Held-out plates through the batched block-Jacobi PCG to 1e-8 (every
+    plate), from zero and from the network's Galerkin-scaled guess (network
+    and scaling inside the timed region).  Per-plate iteration counts."""
     from .sweep import PlateSweep
-    n = net.n
-    h, p, f = _plates(n_eval, n, seed)
-    sw = PlateSweep([(n, n)] * n_eval, a=1.0, b=1.0)
+    V, n = h.shape[0], h.shape[1]
+    sw = PlateSweep([(n, n)] * V, a=1.0, b=1.0)
     sw.set_thickness(h.reshape(-1))
-    fd = np.zeros((n_eval, (n + 1) * (n + 1), 6))
-    fd[:, :, 2] = f.reshape(n_eval, -1)
-    sw.set_loads(fd.reshape(-1))
+    sw.set_loads(_dof_loads(f))
     sw.assemble()
     torch.cuda.synchronize()
     sw.check()
-    out = dict(plates=n_eval)
+    x, _ = features(h, p)
+    ms0, _ = _timed(lambda: sw.solve(mode=mode, rtol=1e-8, max_iter=max_iter))
+    it0 = sw.iters.cpu().numpy().copy()
+    ok0 = int((sw.relres <= 1e-8).sum())
+    usf = (sw.u.view(V, -1) * sw.f.view(V, -1)).sum(dim=1)
+    gst = {}
 
-    def timed(fn):
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        torch.cuda.synchronize()
-        e0.record()
-        r = fn()
-        e1.record()
-        torch.cuda.synchronize()
-        return e0.elapsed_time(e1), r
+    def warm():
+        u0 = predict(net, x, sc, sw.fixed)
+        us, fb = galerkin_scale(sw, sw.f, u0, V, gst)
+        sw.u.copy_(us)
+        sw.solve(mode=mode, rtol=1e-8, x0=True, max_iter=max_iter)
+        return fb
 
+    ms1, fallback = _timed(warm)
+    it1 = sw.iters.cpu().numpy().copy()
+    ok1 = int((sw.relres <= 1e-8).sum())
+    e2 = (1 - gst['uf'] ** 2 / (gst['uku'] * usf)).clamp_min(0).cpu().numpy()
+    ms_net, _ = _timed(lambda: predict(net, x, sc, sw.fixed))
+    out = dict(plates=V, zero_ms=ms0, warm_ms=ms1, network_ms=ms_net,
+               zero_iters_mean=float(it0.mean()), warm_iters_mean=float(it1.mean()),
+               iteration_ratio=float(it0.mean() / max(1e-9, it1.mean())),
+               plates_fewer_iterations=int((it1 < it0).sum()),
+               zero_converged=ok0, warm_converged=ok1, fallback=fallback,
+               energy_error_median=float(np.median(np.sqrt(e2))),
+               captured_energy_median=float(np.median(1 - e2)))
     if log:
-        log('evaluate: %d plates assembled' % n_eval)
-    for mode in ('fp64', 'mixed'):
-        ms0, _ = timed(lambda: sw.solve(mode=mode, rtol=1e-8, max_iter=max_iter))
-        if log:
-            log('%s zero start: %.1f ms' % (mode, ms0))
-        it0 = sw.iters.cpu().numpy().copy()
-        rel0 = float(sw.relres.max())
-        conv0 = int((sw.relres <= 1e-8).sum())
+        log('pcg %s, %d plates: zero %.0f ms (%.0f its), warm %.0f ms (%.0f its), ratio %.4f, '
+            'energy error %.3f' % (mode, V, ms0, it0.mean(), ms1, it1.mean(),
+                                   out['iteration_ratio'], out['energy_error_median']))
+    return out, it0, it1
 
-        # energy norm of the solution, ||u*||_K^2 = u*.f, for the error of
-        # the warm start: ||u* - alpha u0||_K^2 / ||u*||_K^2 = 1 - (u0.f)^2 /
-        # (u0.K u0 * u*.f)
-        usf = (sw.u.view(n_eval, -1) * sw.f.view(n_eval, -1)).sum(dim=1)
 
-        gstats = {}
+def evaluate_multigrid(net, sc, h, p, f, log=None):
+    """The same plates as one block-diagonal system (plates side by side,
+    so no multigrid aggregate spans two) solved by multigrid CG (FP64
+    V-cycle) to 1e-8 on every plate, from zero and from the network's
+    Galerkin-scaled guesses; joint iteration counts."""
+    from .solver import StaticSystem
+    from .sweep import PlateSweep
+    V, n = h.shape[0], h.shape[1]
+    sw = PlateSweep([(n, n)] * V, a=1.0, b=1.0)
+    sw.set_thickness(h.reshape(-1))
+    sw.set_loads(_dof_loads(f))
+    sw.assemble()
+    torch.cuda.synchronize()
+    xyz = sw.x.view(-1, 3).cpu().numpy().copy()
+    for v in range(V):
+        xyz[sw.node0[v]:sw.node0[v + 1], 0] += 1.5 * v
+    sysm = StaticSystem(sw.plan, sw.val, sw.fixed)
+    sysm.setup_amg(xyz, precisions=('fp64',))
+    fv = sw.f.view(V, -1)
+    nf = torch.linalg.vector_norm(fv, dim=1)
+    rtol = 1e-8 * float(nf.min() / torch.linalg.vector_norm(sw.f))
+    x, _ = features(h, p)
+    sysm.solve(sw.f, method='amg', rtol=rtol, maxiter=2000)  # warm-up (graph capture)
+    ms0, (u0s, info0) = _timed(lambda: sysm.solve(sw.f, method='amg', rtol=rtol, maxiter=2000))
 
-        def warm():
-            u0 = predict(net, h, p, sc)
-            us, fallback = galerkin_scale(sw, u0, gstats)
-            sw.u.copy_(us)
-            sw.solve(mode=mode, rtol=1e-8, x0=True, max_iter=max_iter)
-            return fallback
-        ms1, fallback = timed(warm)
-        uf, uku = gstats['uf'], gstats['uku']
-        e2 = (1 - uf * uf / (uku * usf)).clamp_min(0).sqrt().cpu().numpy()
-        it1 = sw.iters.cpu().numpy().copy()
-        rel1 = float(sw.relres.max())
-        conv1 = int((sw.relres <= 1e-8).sum())
-        ms_net, _ = timed(lambda: predict(net, h, p, sc))
-        out[mode] = dict(zero_ms=ms0, zero_solves_per_s=n_eval / (ms0 * 1e-3),
-                         zero_iters_mean=float(it0.mean()), zero_max_relres=rel0,
-                         zero_converged=conv0, warm_converged=conv1,
-                         zero_iters_median=float(np.median(it0)),
-                         warm_iters_median=float(np.median(it1)),
-                         warm_ms=ms1, warm_solves_per_s=n_eval / (ms1 * 1e-3),
-                         warm_iters_mean=float(it1.mean()), warm_max_relres=rel1,
-                         network_ms=ms_net, fallback=fallback,
-                         warm_energy_error_median=float(np.median(e2)),
-                         warm_energy_error_mean=float(e2.mean()),
-                         iteration_ratio=float(it0.mean() / max(1e-9, it1.mean())))
-        if log:
-            log('%s: zero %.1f ms (%.0f its), warm %.1f ms (%.0f its, net %.1f ms, %d fallback)'
-                % (mode, ms0, it0.mean(), ms1, it1.mean(), ms_net, fallback))
-    out['mixed_warm_vs_fp64_zero'] = out['fp64']['zero_ms'] / out['mixed']['warm_ms']
+    def warm():
+        g = predict(net, x, sc, sw.fixed)
+        us, fb = galerkin_scale(sw, sw.f, g, V)
+        return sysm.solve(sw.f, method='amg', rtol=rtol, maxiter=2000, x0=us), fb
+
+    ms1, ((u1s, info1), fallback) = _timed(warm)
+
+    def per_plate(u):
+        r = sysm.residual(sysm.rhs(sw.f), u).view(V, -1)
+        return float((torch.linalg.vector_norm(r, dim=1) / nf).max())
+
+    out = dict(plates=V, zero_ms=ms0, warm_ms=ms1, zero_iters=info0['iters'],
+               warm_iters=info1['iters'],
+               iteration_ratio=info0['iters'] / max(1, info1['iters']),
+               zero_max_plate_relres=per_plate(u0s), warm_max_plate_relres=per_plate(u1s),
+               fallback=fallback)
+    if log:
+        log('multigrid, %d plates: zero %.0f ms (%d its), warm %.0f ms (%d its)'
+            % (V, ms0, info0['iters'], ms1, info1['iters']))
     return out
+
+
+def evaluate(net, sc, n_eval, seed, chunk=500, multigrid_chunk=250, max_iter=60000,
+             log=None):
+    """Held-out plates (seeds disjoint from training): block-Jacobi PCG in
+    chunks of ``chunk`` plates and multigrid CG on one ``multigrid_chunk``
+    block-diagonal system, zero start against the warm start."""
+    res = dict(plates=n_eval, pcg=[], multigrid=None)
+    it0s, it1s = [], []
+    for c0 in range(0, n_eval, chunk):
+        V = min(chunk, n_eval - c0)
+        h, p, f = _plates(V, 64, seed * 1000003 + c0)
+        r, it0, it1 = evaluate_pcg(net, sc, h, p, f, max_iter=max_iter, log=log)
+        res['pcg'].append(r)
+        it0s.append(it0)
+        it1s.append(it1)
+        if c0 == 0 and multigrid_chunk:
+            res['multigrid'] = evaluate_multigrid(net, sc, h[:multigrid_chunk],
+                                                  p[:multigrid_chunk], f[:multigrid_chunk],
+                                                  log=log)
+    it0, it1 = np.concatenate(it0s), np.concatenate(it1s)
+    pc = res['pcg']
+    res['summary'] = dict(
+        plates=int(it0.size),
+        pcg_iteration_ratio=float(it0.mean() / it1.mean()),
+        pcg_zero_ms=sum(r['zero_ms'] for r in pc), pcg_warm_ms=sum(r['warm_ms'] for r in pc),
+        pcg_time_ratio=sum(r['zero_ms'] for r in pc) / sum(r['warm_ms'] for r in pc),
+        pcg_plates_fewer_iterations=int((it1 < it0).sum()),
+        pcg_all_converged=all(r['zero_converged'] == r['warm_converged'] == r['plates']
+                              for r in pc),
+        fallback=sum(r['fallback'] for r in pc),
+        energy_error_median=float(np.median([r['energy_error_median'] for r in pc])),
+        multigrid_iteration_ratio=res['multigrid']['iteration_ratio']
+        if res['multigrid'] else None,
+        multigrid_time_ratio=res['multigrid']['zero_ms'] / res['multigrid']['warm_ms']
+        if res['multigrid'] else None)
+    return res
