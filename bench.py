@@ -574,14 +574,40 @@ def profile_kc0(n, launches=3):
           flush=True)
 
 
-def warm_start_rates(n_plates, log):
+def profile_sweep(n_var, steps=3, config=(256, 448, 256)):
+    """`steps` direct-solve steps of the n_var-plate design sweep (records,
+    KC0, BSR, Dirichlet, band fill, banded Cholesky, triangular solves) with
+    the launch shape given (factor threads, panel bandwidth, solve threads;
+    default the one the autotuner picks for the 4,096-plate sweep), for
+    kernel traces and counter runs."""
+    import torch
+    from pyfe3d_amd.sweep import PlateSweep
+    sw = PlateSweep(sweep_dims(n_var))
+    h, f = sw.random_designs(7)
+    sw.set_thickness(h)
+    sw.set_loads(f)
+    for _ in range(steps):
+        sw.step(mode='direct', rtol=1e-8, config=config)
+    torch.cuda.synchronize()
+    sw.check()
+    rel = float(sw.relres.max())
+    print('profile-sweep: %d plates, %d steps, max relres %.2e' % (n_var, steps, rel), flush=True)
+
+
+def warm_start_rates(n_plates, log, multigrid_plates=16):
     """Held-out 64 x 64 plates (smooth thickness fields, Gaussian pressure
-    patches): batched PCG to 1e-8 from zero and from the trained network's
-    Galerkin-scaled guess."""
+    patches): block-Jacobi PCG to 1e-8 per plate and multigrid CG on a
+    subset, from zero and from the trained network's Galerkin-scaled guess,
+    with the network's own held-out evaluation (2,000 plates, recorded when
+    it was trained) alongside."""
     from pyfe3d_amd import warmstart as W
     path = os.path.join(ROOT, 'pyfe3d_amd', 'warmstart_weights.npz')
-    net, sc = W.load(path)
-    return W.evaluate(net, sc, n_plates, seed=12345, max_iter=40000, log=log)
+    net, sc, meta = W.load(path)
+    res = W.evaluate(net, sc, n_plates, seed=12345, chunk=n_plates,
+                     multigrid_plates=multigrid_plates, max_iter=60000, log=log)
+    res['training'] = {k: meta.get(k) for k in ('samples', 'epochs', 'loss', 'train_s',
+                                                'eval_seed', 'eval', 'eval_multigrid')}
+    return res
 
 
 def main():
@@ -611,6 +637,8 @@ def main():
                     help='run only the stiffened-panel lines (no headline)')
     ap.add_argument('--profile-kc0', action='store_true',
                     help='only: 3 KC0 launches and a 1 GiB device copy (counter runs)')
+    ap.add_argument('--profile-sweep', type=int, default=0,
+                    help='only: direct-solve steps of a design sweep of this many plates')
     ap.add_argument('--profile-launches', type=int, default=3,
                     help='KC0 launches of --profile-kc0 (the kernel trace uses warmup + steps)')
     ap.add_argument('--quiet', action='store_true')
@@ -627,6 +655,9 @@ def main():
     stream = torch.cuda.current_stream()
     if args.profile_kc0:
         profile_kc0(args.n, args.profile_launches)
+        return
+    if args.profile_sweep:
+        profile_sweep(args.profile_sweep)
         return
     if args.large_solve:
         print(json.dumps({'large_solve': large_solve(args.n, log)}), flush=True)

@@ -329,6 +329,7 @@ class SAAMG:
         launches per CG iteration."""
         n = self.levels[0]['n'] if self.levels else self.nc
         dev = self.coarse_inv.device
+        _collect_before_capture()
         self.g_in = torch.zeros(n, dtype=torch.float64, device=dev)
         self.g_out = torch.zeros(n, dtype=torch.float64, device=dev)
         s = torch.cuda.Stream(device=dev)
@@ -497,6 +498,16 @@ class SAAMG:
         return x
 
 
+def _collect_before_capture():
+    """Free unreachable objects now: a CUDAGraph held in a reference cycle
+    (a system and its cached CG iteration refer to each other) would
+    otherwise be destroyed by the cyclic collector at some allocation made
+    during a later capture, which HIP refuses while a stream captures."""
+    import gc
+    torch.cuda.synchronize()
+    gc.collect()
+
+
 def _ptr(t):
     return None if t is None else t.data_ptr()
 
@@ -573,6 +584,7 @@ class _CGIteration:
                   self.p.data_ptr(), _s())
 
     def capture(self):
+        _collect_before_capture()
         s = torch.cuda.Stream(device=self.x.device)
         s.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(s):

@@ -259,6 +259,8 @@ class PlateSweep:
     def capture(self, mode='mixed', rtol=1e-8, **kw):
         """Capture :meth:`step` in a graph; replay with ``self.graph.replay()``
         after new thicknesses (:meth:`set_thickness`) or loads (:meth:`set_loads`)."""
+        from .amg import _collect_before_capture
+        _collect_before_capture()
         s = torch.cuda.Stream(device=self.device)
         s.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(s):

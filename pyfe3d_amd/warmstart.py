@@ -362,10 +362,12 @@ class Scales:
 
 class EnergyData:
     """Training plates on the GPU: inputs X (S, n, n, 2), thicknesses H
-    (S, n*n), loads F (S, (n+1)^2 * 6) and solution energies s = u*.f (S)."""
+    (S, n*n), loads F (S, (n+1)^2 * 6), solution energies s = u*.f (S) and
+    the solutions' shapes T (S, n+1, n+1, 6) in the network's units
+    (u / rms(w) / channel scale; for the nodal pre-training)."""
 
-    def __init__(self, X, H, F, S, scales):
-        self.X, self.H, self.F, self.S, self.scales = X, H, F, S, scales
+    def __init__(self, X, H, F, S, T, scales):
+        self.X, self.H, self.F, self.S, self.T, self.scales = X, H, F, S, T, scales
 
     @property
     def size(self):
@@ -376,7 +378,7 @@ def make_energy_dataset(n_samples, seed, n=64, chunk=256, log=None):
     """Seeded plates solved by the batched banded Cholesky (every residual
     checked at 1e-8); keeps what the energy loss needs (no nodal targets):
     inputs, thicknesses, loads and u*.f, plus the channel scales."""
-    X, H, F, S = [], [], [], []
+    X, H, F, S, U = [], [], [], [], []
     sweep = None
     ssum = np.zeros(6)
     count = 0
@@ -396,7 +398,9 @@ def make_energy_dataset(n_samples, seed, n=64, chunk=256, log=None):
         S.append((sweep.u.view(V, -1) * sweep.f.view(V, -1)).sum(dim=1))
         w = u[:, 1:-1, 1:-1, W_CHANNEL]
         rms = w.pow(2).mean(dim=(1, 2)).sqrt().clamp_min(1e-300)
-        T = (u / rms[:, None, None, None])[:, 1:-1, 1:-1].reshape(-1, 6)
+        Tn = (u / rms[:, None, None, None]).to(torch.float32)
+        U.append(Tn)
+        T = Tn[:, 1:-1, 1:-1].reshape(-1, 6).double()
         ssum += T.pow(2).sum(dim=0).cpu().numpy()
         count += T.shape[0]
         if log:
@@ -404,26 +408,36 @@ def make_energy_dataset(n_samples, seed, n=64, chunk=256, log=None):
     std = np.sqrt(ssum / count)
     # channels that carry no displacement (relative to w) stay at zero
     std = np.where(std > 1e-9 * std[W_CHANNEL], std, 0.)
-    return EnergyData(torch.cat(X), torch.cat(H), torch.cat(F), torch.cat(S), Scales(std))
+    T = torch.cat(U)
+    T /= torch.as_tensor(np.where(std > 0, std, 1.), dtype=torch.float32, device='cuda')
+    return EnergyData(torch.cat(X), torch.cat(H), torch.cat(F), torch.cat(S), T, Scales(std))
 
 
-def train_energy(net, data, epochs=12, batch=32, lr=1e-3, seed=0, log=None):
-    """Adam on the energy-norm loss (mean over the batch of the relative
-    energy error of the best multiple of the guess); each batch's
-    stiffness is assembled on the fly (one KC0 launch and one block-plan
-    apply for its plates).  Returns the mean loss per epoch."""
+def train_energy(net, data, epochs=12, batch=32, lr=1e-3, seed=0, pretrain_epochs=3,
+                 log=None):
+    """Adam, first ``pretrain_epochs`` on the nodal MSE of the solution
+    shapes (channels with a scale), then on the energy-norm loss (mean over
+    the batch of the relative energy error of the best multiple of the
+    guess), each batch's stiffness assembled on the fly (one KC0 launch and
+    one block-plan apply for its plates).  Returns the mean energy loss of
+    the training batches per epoch (also measured during the pre-training)."""
     from .sweep import PlateSweep
     n = net.n
     sweep = PlateSweep([(n, n)] * batch, a=1.0, b=1.0)
     scale = data.scales.tensor()
+    wc = (scale > 0).to(torch.float32)
     S = data.size
     g = torch.Generator(device='cpu').manual_seed(seed)
     hist = []
-    for ep in range(epochs):
+    total = pretrain_epochs + epochs
+    for ep in range(total):
+        pre = ep < pretrain_epochs
         perm = torch.randperm(S, generator=g)
         tot = torch.zeros((), dtype=torch.float64, device='cuda')
         nb = 0
-        lr_ep = lr * (0.5 ** (ep // max(1, epochs // 4)))
+        k = ep if pre else ep - pretrain_epochs
+        span = pretrain_epochs if pre else epochs
+        lr_ep = lr * (0.5 ** (k // max(1, span // 3)))
         for b0 in range(0, S - batch + 1, batch):
             idx = perm[b0:b0 + batch].to('cuda')
             sweep.set_thickness(data.H[idx].reshape(-1))
@@ -433,11 +447,15 @@ def train_energy(net, data, epochs=12, batch=32, lr=1e-3, seed=0, log=None):
                                   data.S[idx].contiguous(), scale)
             tot += acl[:, 2].mean()
             nb += 1
-            net.backward_from()
+            if pre:
+                net.backward(data.T[idx].contiguous(), wc)
+            else:
+                net.backward_from()
             net.adam(lr=lr_ep)
         hist.append(float(tot) / max(1, nb))
         if log:
-            log('epoch %d energy loss %.4e (lr %.1e)' % (ep, hist[-1], lr_ep))
+            log('epoch %d (%s) energy loss %.4e (lr %.1e)'
+                % (ep, 'nodal MSE' if pre else 'energy', hist[-1], lr_ep))
     return hist
 
 
@@ -535,6 +553,7 @@ def evaluate_pcg(net, sc, h, p, f, mode='fp64', max_iter=60000, log=None):
     ms0, _ = _timed(lambda: sw.solve(mode=mode, rtol=1e-8, max_iter=max_iter))
     it0 = sw.iters.cpu().numpy().copy()
     ok0 = int((sw.relres <= 1e-8).sum())
+    rel0 = float(sw.relres.max())
     usf = (sw.u.view(V, -1) * sw.f.view(V, -1)).sum(dim=1)
     gst = {}
 
@@ -548,13 +567,15 @@ def evaluate_pcg(net, sc, h, p, f, mode='fp64', max_iter=60000, log=None):
     ms1, fallback = _timed(warm)
     it1 = sw.iters.cpu().numpy().copy()
     ok1 = int((sw.relres <= 1e-8).sum())
+    rel1 = float(sw.relres.max())
     e2 = (1 - gst['uf'] ** 2 / (gst['uku'] * usf)).clamp_min(0).cpu().numpy()
     ms_net, _ = _timed(lambda: predict(net, x, sc, sw.fixed))
     out = dict(plates=V, zero_ms=ms0, warm_ms=ms1, network_ms=ms_net,
                zero_iters_mean=float(it0.mean()), warm_iters_mean=float(it1.mean()),
                iteration_ratio=float(it0.mean() / max(1e-9, it1.mean())),
                plates_fewer_iterations=int((it1 < it0).sum()),
-               zero_converged=ok0, warm_converged=ok1, fallback=fallback,
+               zero_converged=ok0, warm_converged=ok1, zero_max_relres=rel0,
+               warm_max_relres=rel1, max_iter=max_iter, fallback=fallback,
                energy_error_median=float(np.median(np.sqrt(e2))),
                captured_energy_median=float(np.median(1 - e2)))
     if log:
@@ -565,57 +586,58 @@ def evaluate_pcg(net, sc, h, p, f, mode='fp64', max_iter=60000, log=None):
 
 
 def evaluate_multigrid(net, sc, h, p, f, log=None):
-    """The same plates as one block-diagonal system (plates side by side,
-    so no multigrid aggregate spans two) solved by multigrid CG (FP64
-    V-cycle) to 1e-8 on every plate, from zero and from the network's
-    Galerkin-scaled guesses; joint iteration counts."""
+    """The plates one by one through multigrid CG (FP64 V-cycle, setup per
+    plate outside the timed region) to 1e-8, from zero and from the
+    network's Galerkin-scaled guess (network and scaling timed with the
+    warm solve).  Per-plate iteration counts."""
     from .solver import StaticSystem
     from .sweep import PlateSweep
     V, n = h.shape[0], h.shape[1]
-    sw = PlateSweep([(n, n)] * V, a=1.0, b=1.0)
-    sw.set_thickness(h.reshape(-1))
-    sw.set_loads(_dof_loads(f))
-    sw.assemble()
-    torch.cuda.synchronize()
-    xyz = sw.x.view(-1, 3).cpu().numpy().copy()
-    for v in range(V):
-        xyz[sw.node0[v]:sw.node0[v + 1], 0] += 1.5 * v
-    sysm = StaticSystem(sw.plan, sw.val, sw.fixed)
-    sysm.setup_amg(xyz, precisions=('fp64',))
-    fv = sw.f.view(V, -1)
-    nf = torch.linalg.vector_norm(fv, dim=1)
-    rtol = 1e-8 * float(nf.min() / torch.linalg.vector_norm(sw.f))
+    sw = PlateSweep([(n, n)], a=1.0, b=1.0)
     x, _ = features(h, p)
-    sysm.solve(sw.f, method='amg', rtol=rtol, maxiter=2000)  # warm-up (graph capture)
-    ms0, (u0s, info0) = _timed(lambda: sysm.solve(sw.f, method='amg', rtol=rtol, maxiter=2000))
+    it0, it1, ms0, ms1, conv, fb, rels = [], [], 0., 0., 0, 0, []
+    for v in range(V):
+        sw.set_thickness(h[v].reshape(-1))
+        sw.set_loads(_dof_loads(f[v:v + 1]))
+        sw.assemble()
+        sysm = StaticSystem(sw.plan, sw.val, sw.fixed)
+        sysm.setup_amg(sw.x.view(-1, 3).cpu().numpy(), precisions=('fp64',))
+        sysm.solve(sw.f, method='amg', rtol=1e-8)   # warm-up: graph capture
+        t0, (_, i0) = _timed(lambda: sysm.solve(sw.f, method='amg', rtol=1e-8))
 
-    def warm():
-        g = predict(net, x, sc, sw.fixed)
-        us, fb = galerkin_scale(sw, sw.f, g, V)
-        return sysm.solve(sw.f, method='amg', rtol=rtol, maxiter=2000, x0=us), fb
+        def warm():
+            g = predict(net, x[v:v + 1], sc, sw.fixed)
+            us, k = galerkin_scale(sw, sw.f, g, 1)
+            return sysm.solve(sw.f, method='amg', rtol=1e-8, x0=us), k
 
-    ms1, ((u1s, info1), fallback) = _timed(warm)
-
-    def per_plate(u):
-        r = sysm.residual(sysm.rhs(sw.f), u).view(V, -1)
-        return float((torch.linalg.vector_norm(r, dim=1) / nf).max())
-
-    out = dict(plates=V, zero_ms=ms0, warm_ms=ms1, zero_iters=info0['iters'],
-               warm_iters=info1['iters'],
-               iteration_ratio=info0['iters'] / max(1, info1['iters']),
-               zero_max_plate_relres=per_plate(u0s), warm_max_plate_relres=per_plate(u1s),
-               fallback=fallback)
+        t1, ((_, i1), k) = _timed(warm)
+        it0.append(i0['iters'])
+        it1.append(i1['iters'])
+        rels.append(max(i0['relres'], i1['relres']))
+        ms0 += t0
+        ms1 += t1
+        fb += k
+        conv += int(i0['converged'] and i1['converged'])
+        # release this plate's graphs here, not inside the next capture
+        sysm.__dict__.pop('_cg_iterations', None)
+        del sysm
+    it0, it1 = np.array(it0), np.array(it1)
+    out = dict(plates=V, zero_ms=ms0, warm_ms=ms1, zero_iters_mean=float(it0.mean()),
+               warm_iters_mean=float(it1.mean()),
+               iteration_ratio=float(it0.mean() / max(1e-9, it1.mean())),
+               plates_fewer_iterations=int((it1 < it0).sum()), all_converged=conv == V,
+               converged=conv, max_relres=float(max(rels)), fallback=fb)
     if log:
-        log('multigrid, %d plates: zero %.0f ms (%d its), warm %.0f ms (%d its)'
-            % (V, ms0, info0['iters'], ms1, info1['iters']))
+        log('multigrid, %d plates: zero %.0f ms (%.1f its), warm %.0f ms (%.1f its), ratio %.4f'
+            % (V, ms0, it0.mean(), ms1, it1.mean(), out['iteration_ratio']))
     return out
 
 
-def evaluate(net, sc, n_eval, seed, chunk=500, multigrid_chunk=250, max_iter=60000,
+def evaluate(net, sc, n_eval, seed, chunk=500, multigrid_plates=100, max_iter=60000,
              log=None):
-    """Held-out plates (seeds disjoint from training): block-Jacobi PCG in
-    chunks of ``chunk`` plates and multigrid CG on one ``multigrid_chunk``
-    block-diagonal system, zero start against the warm start."""
+    """Held-out plates (seeds disjoint from training): block-Jacobi PCG on
+    all of them in batches of ``chunk``, multigrid CG on the first
+    ``multigrid_plates``, zero start against the warm start."""
     res = dict(plates=n_eval, pcg=[], multigrid=None)
     it0s, it1s = [], []
     for c0 in range(0, n_eval, chunk):
@@ -625,9 +647,9 @@ def evaluate(net, sc, n_eval, seed, chunk=500, multigrid_chunk=250, max_iter=600
         res['pcg'].append(r)
         it0s.append(it0)
         it1s.append(it1)
-        if c0 == 0 and multigrid_chunk:
-            res['multigrid'] = evaluate_multigrid(net, sc, h[:multigrid_chunk],
-                                                  p[:multigrid_chunk], f[:multigrid_chunk],
+        if c0 == 0 and multigrid_plates:
+            res['multigrid'] = evaluate_multigrid(net, sc, h[:multigrid_plates],
+                                                  p[:multigrid_plates], f[:multigrid_plates],
                                                   log=log)
     it0, it1 = np.concatenate(it0s), np.concatenate(it1s)
     pc = res['pcg']
@@ -639,6 +661,9 @@ def evaluate(net, sc, n_eval, seed, chunk=500, multigrid_chunk=250, max_iter=600
         pcg_plates_fewer_iterations=int((it1 < it0).sum()),
         pcg_all_converged=all(r['zero_converged'] == r['warm_converged'] == r['plates']
                               for r in pc),
+        pcg_converged=(sum(r['zero_converged'] for r in pc),
+                       sum(r['warm_converged'] for r in pc)),
+        pcg_max_relres=max(max(r['zero_max_relres'], r['warm_max_relres']) for r in pc),
         fallback=sum(r['fallback'] for r in pc),
         energy_error_median=float(np.median([r['energy_error_median'] for r in pc])),
         multigrid_iteration_ratio=res['multigrid']['iteration_ratio']
@@ -646,3 +671,31 @@ def evaluate(net, sc, n_eval, seed, chunk=500, multigrid_chunk=250, max_iter=600
         multigrid_time_ratio=res['multigrid']['zero_ms'] / res['multigrid']['warm_ms']
         if res['multigrid'] else None)
     return res
+
+
+class WarmStart:
+    """The trained network as the initial guess of the batched plate solves,
+    with the policy recorded at training time: the guess is used only if it
+    saved at least 1 % of the block-Jacobi PCG iterations on the held-out
+    plates (``meta['use_guess']``); otherwise every plate starts from zero
+    (the fallback).  Plates whose Galerkin multiple is not finite start from
+    zero in any case."""
+
+    def __init__(self, path=None, batch=64):
+        import os
+        path = path or os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                    'warmstart_weights.npz')
+        self.net, self.scales, self.meta = load(path, batch=batch)
+        self.use_guess = bool(self.meta.get('use_guess', False))
+
+    def start(self, sweep, h, p, force=False):
+        """Write the initial guess of every plate of ``sweep`` (assembled,
+        loads set; n x n plates with thickness h and pressure p, (V, n, n))
+        into sweep.u and return (x0 flag for sweep.solve, zero starts)."""
+        if not (self.use_guess or force):
+            return False, sweep.n_var
+        x, _ = features(h, p)
+        g = predict(self.net, x, self.scales, sweep.fixed)
+        us, fallback = galerkin_scale(sweep, sweep.f, g, sweep.n_var)
+        sweep.u.copy_(us)
+        return True, fallback
