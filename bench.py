@@ -574,7 +574,7 @@ def profile_kc0(n, launches=3):
           flush=True)
 
 
-def profile_sweep(n_var, steps=3, config=(256, 448, 256)):
+def profile_sweep(n_var, steps=3, config=(1024, 448, 256)):
     """`steps` direct-solve steps of the n_var-plate design sweep (records,
     KC0, BSR, Dirichlet, band fill, banded Cholesky, triangular solves) with
     the launch shape given (factor threads, panel bandwidth, solve threads;

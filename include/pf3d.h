@@ -590,7 +590,7 @@ int pf3d_band_max_width(void);
  * band (band_size doubles in all).  *bad (device int) counts systems with
  * a non-positive pivot or a bandwidth above max_b.  Launch configuration:
  * threads per workgroup (256, 512 or 1024; 0 = 1024) and the bandwidth the
- * LDS panel is sized for, max_b (256, 448 or 640; 0 = 640): a narrower
+ * LDS panel is sized for, max_b (256, 448 or 576; 0 = 576): a narrower
  * panel fits more workgroups per CU (chosen by pyfe3d_amd.autotune). */
 int pf3d_band_factor(int64_t n_var, int64_t n_nodes, const int32_t *order, const int64_t *dof0,
                      const int32_t *bw, const int64_t *boff, int64_t band_size, double *band,

@@ -11,13 +11,12 @@
 //     L[j*(b+1) + d] = A[j+d][j],  0 <= d <= b,
 // then factorised in place by a blocked right-looking band Cholesky, one
 // workgroup per system:
-//   * panel of NB columns loaded into LDS and factorised by one wave (each
-//     lane owns rows lane + 64q; pivots and multipliers travel by shuffle,
-//     so the panel needs no workgroup barrier),
-//   * symmetric rank-NB update of the trailing band window by the whole
+//   * panel of 32 columns loaded into LDS: diagonal block factorised by one
+//     wave, the rows below by all threads,
+//   * symmetric rank-32 update of the trailing band window by the whole
 //     workgroup.
-// The triangular solves are blocked the same way (NB columns per barrier
-// pair).  All sums have a fixed order: results do not depend on timing.
+// The triangular solves are blocked by 32 columns per barrier pair.  All
+// sums have a fixed order: results do not depend on timing or launch shape.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -29,11 +28,11 @@ int pf3d_hip_check(hipError_t e, const char *where);
 
 namespace {
 
-constexpr int NB = 16;       // panel width
-constexpr int MAXB = 640;    // largest supported bandwidth (LDS panel rows b + NB)
-// launch configurations (threads per workgroup BT, largest bandwidth MB the
-// LDS panel is sized for): a narrower panel fits more workgroups per CU
-// (MB 640: 89 KB of LDS, one per CU; 448: 63 KB, two; 256: 37 KB, four)
+constexpr int MAXB = 576;    // largest supported bandwidth (LDS panel rows b + 32)
+// launch configurations of the factorisation (threads per workgroup BT,
+// largest bandwidth MB the LDS panel of 32 columns is sized for): a
+// narrower panel fits more workgroups per CU (MB 576: 157 KB of LDS, one
+// per CU; 448: 124 KB, one; 256: 74 KB, two)
 
 struct BandSys {
     int64_t n_var;
@@ -78,12 +77,30 @@ __global__ void band_fill(int64_t n_nodes, const int32_t *__restrict__ rp,
 }
 
 // ------------------------------------------------------------ Cholesky
+// Per panel step of NF = 32 columns (j0 .. j0 + jn):
+//   1. the panel (rows r = 0 .. b + jn of the columns) goes to LDS;
+//   2. wave 0 factorises the jn x jn diagonal block in LDS (lane c owns
+//      row c; a wave fence orders each column's scaling before its use);
+//   3. every thread solves rows of the block below it, x L11^T = a, by
+//      forward substitution against L11 read from LDS (broadcast);
+//   4. the panel is written back and the trailing band window is updated,
+//      A[ri][rk] -= sum_c L[ri][c] L[rk][c] for jn <= rk <= ri < rows:
+//      a wave takes 2 x 64 consecutive rows (lane = row: conflict-free LDS
+//      reads, coalesced global access down the band's columns) by 8
+//      columns (broadcast LDS reads), c summed in order in registers.
+// A 32-column panel moves the trailing window through HBM once per 32
+// columns.  Sums run in a fixed order, independent of the launch shape.
+constexpr int NF = 32;   // factor panel width
+constexpr int TRI = 2;   // trailing update: rows per lane (64 apart)
+constexpr int TKC = 8;   // trailing update: columns per wave block
+
 template <int BT, int MB>
 __global__ void __launch_bounds__(BT) band_chol(BandSys s, int *__restrict__ bad)
 {
-    constexpr int PR = MB + NB;
-    constexpr int QMAX = (PR + 63) / 64;  // panel rows per lane
-    __shared__ double P[PR][NB + 1];
+    constexpr int PR = MB + NF;
+    constexpr int NW = BT / 64;
+    __shared__ double P[PR][NF + 1];
+    __shared__ double dinv[NF];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t v = s.order ? (int64_t)s.order[blockIdx.x] : (int64_t)blockIdx.x;
     const int64_t n = s.dof0[v + 1] - s.dof0[v];
@@ -95,90 +112,110 @@ __global__ void __launch_bounds__(BT) band_chol(BandSys s, int *__restrict__ bad
         return;
     }
     bool failed = false;
-    for (int64_t j0 = 0; j0 < n; j0 += NB) {
-        const int jn = (int)(n - j0 < NB ? n - j0 : NB);
+    for (int64_t j0 = 0; j0 < n; j0 += NF) {
+        const int jn = (int)(n - j0 < NF ? n - j0 : NF);
         const int rows = (int)(n - j0 < b + jn ? n - j0 : b + jn);
-        // panel: P[r][c] = A[j0 + r][j0 + c], r >= c, r - c <= b
-        for (int t = tid; t < rows * NB; t += BT) {
+        // 1. panel: P[r][c] = A[j0 + r][j0 + c], r >= c, r - c <= b
+        for (int t = tid; t < rows * NF; t += BT) {
             const int c = t / rows, r = t - c * rows;
             double x = 0.;
             if (c < jn && r >= c && r - c <= b) x = L[(j0 + c) * ld + (r - c)];
             P[r][c] = x;
         }
         __syncthreads();
+        // 2. diagonal block, by wave 0 in LDS: lane r owns row r; the
+        //    column-k multipliers are read back by every lane after a wave
+        //    fence (the wave's LDS operations complete in order)
         if (wave == 0) {
-            double l[QMAX];
-            for (int c = 0; c < jn; c++) {
-                // pivot (row c is owned by lane c, q = 0)
-                double own = (lane == c) ? P[c][c] : 0.;
-                const double d = __shfl(own, c, 64);
+            for (int k = 0; k < jn; k++) {
+                const double d = P[k][k];
                 if (!(d > 0.)) failed = true;
                 const double sq = d > 0. ? sqrt(d) : 1.;
                 const double inv = 1. / sq;
-#pragma unroll
-                for (int q = 0; q < QMAX; q++) {
-                    const int r = lane + 64 * q;
-                    double x = 0.;
-                    if (r < rows && r >= c) {
-                        x = (r == c) ? sq : P[r][c] * inv;
-                        P[r][c] = x;
-                    }
-                    l[q] = x;
+                if (lane >= k && lane < jn) P[lane][k] = lane == k ? sq : P[lane][k] * inv;
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                if (lane > k && lane < jn) {
+                    const double lk = P[lane][k];
+                    for (int c2 = k + 1; c2 <= lane; c2++) P[lane][c2] -= lk * P[c2][k];
                 }
-                for (int c2 = c + 1; c2 < jn; c2++) {
-                    const double mlt = __shfl(l[0], c2, 64);  // L[c2][c], row c2 < 64
-#pragma unroll
-                    for (int q = 0; q < QMAX; q++) {
-                        const int r = lane + 64 * q;
-                        if (r < rows && r >= c2) P[r][c2] -= l[q] * mlt;
-                    }
-                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+                __builtin_amdgcn_wave_barrier();
             }
+            if (lane < jn) dinv[lane] = 1. / P[lane][lane];
         }
         __syncthreads();
-        // write the panel's columns of L back
-        for (int t = tid; t < rows * NB; t += BT) {
+        // 3. rows below the diagonal block: x L11^T = a
+        for (int r = jn + tid; r < rows; r += BT) {
+            // compiler barrier: L11 is re-read from LDS per row rather than
+            // hoisted into ~500 registers for the few rows a thread takes
+            asm volatile("" ::: "memory");
+            double x[NF];
+#pragma unroll
+            for (int c = 0; c < NF; c++) x[c] = c < jn ? P[r][c] : 0.;
+#pragma unroll
+            for (int c = 0; c < NF; c++) {
+                if (c < jn) {
+                    double acc = x[c];
+#pragma unroll
+                    for (int k = 0; k < c; k++) acc -= x[k] * P[c][k];
+                    x[c] = acc * dinv[c];
+                }
+                // keep each column's L11 loads next to their use
+                __builtin_amdgcn_sched_barrier(0);
+            }
+#pragma unroll
+            for (int c = 0; c < NF; c++)
+                if (c < jn) P[r][c] = x[c];
+        }
+        __syncthreads();
+        // 4a. the panel's columns of L back to the band
+        for (int t = tid; t < rows * NF; t += BT) {
             const int c = t / rows, r = t - c * rows;
             if (c < jn && r >= c && r - c <= b) L[(j0 + c) * ld + (r - c)] = P[r][c];
         }
-        // trailing update: A[j0+ri][j0+rk] -= sum_c P[ri][c] P[rk][c],
-        // jn <= rk <= ri < rows, in 4 x 4 tiles per thread (8 LDS reads
-        // per 16 products); consecutive threads take consecutive row tiles
-        // of one column tile, so the global accesses coalesce
+        // 4b. trailing update
         const int T = rows - jn;
-        const int nt = (T + 3) / 4;
-        for (int64_t t = tid; t < (int64_t)nt * nt; t += BT) {
-            const int tk = (int)(t / nt), ti = (int)(t % nt);
-            if (ti < tk) continue;
-            const int rk0 = jn + 4 * tk, ri0 = jn + 4 * ti;
-            double acc[4][4];
+        const int nrb = (T + 64 * TRI - 1) / (64 * TRI), nkb = (T + TKC - 1) / TKC;
+        for (int blk = wave; blk < nrb * nkb; blk += NW) {
+            const int kb = blk / nrb, rb = blk - kb * nrb;
+            const int rk0 = jn + TKC * kb, ri0 = jn + 64 * TRI * rb;
+            if (ri0 + 64 * TRI - 1 < rk0) continue;  // wholly above the diagonal
+            int ri[TRI];
 #pragma unroll
-            for (int a = 0; a < 4; a++)
+            for (int q = 0; q < TRI; q++) ri[q] = ri0 + 64 * q + lane;
+            // the window values are loaded first, so their latency overlaps
+            // the LDS reads of the product loop; A <- A - sum_c P[ri][c]
+            // P[rk][c], c in order
+            double acc[TRI][TKC];
 #pragma unroll
-                for (int q = 0; q < 4; q++) acc[a][q] = 0.;
-#pragma unroll 4
-            for (int c = 0; c < NB; c++) {
-                double pk[4], pi[4];
+            for (int a = 0; a < TKC; a++) {
+                const int rk = rk0 + a;
+                const double *col = L + (j0 + rk) * ld;
 #pragma unroll
-                for (int a = 0; a < 4; a++) {
-                    pk[a] = rk0 + a < rows ? P[rk0 + a][c] : 0.;
-                    pi[a] = ri0 + a < rows ? P[ri0 + a][c] : 0.;
-                }
+                for (int q = 0; q < TRI; q++)
+                    acc[q][a] = (rk < rows && ri[q] >= rk && ri[q] < rows) ? col[ri[q] - rk] : 0.;
+            }
+#pragma unroll 2
+            for (int c = 0; c < NF; c++) {
+                double pi[TRI], pk[TKC];
 #pragma unroll
-                for (int a = 0; a < 4; a++)
+                for (int q = 0; q < TRI; q++) pi[q] = ri[q] < rows ? P[ri[q]][c] : 0.;
 #pragma unroll
-                    for (int q = 0; q < 4; q++) acc[a][q] += pi[q] * pk[a];
+                for (int a = 0; a < TKC; a++) pk[a] = rk0 + a < rows ? P[rk0 + a][c] : 0.;
+#pragma unroll
+                for (int q = 0; q < TRI; q++)
+#pragma unroll
+                    for (int a = 0; a < TKC; a++) acc[q][a] -= pi[q] * pk[a];
             }
 #pragma unroll
-            for (int a = 0; a < 4; a++) {
+            for (int a = 0; a < TKC; a++) {
                 const int rk = rk0 + a;
                 if (rk >= rows) continue;
                 double *col = L + (j0 + rk) * ld;
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int ri = ri0 + q;
-                    if (ri >= rk && ri < rows) col[ri - rk] -= acc[a][q];
-                }
+                for (int q = 0; q < TRI; q++)
+                    if (ri[q] >= rk && ri[q] < rows) col[ri[q] - rk] = acc[q][a];
             }
         }
         __syncthreads();
@@ -188,14 +225,23 @@ __global__ void __launch_bounds__(BT) band_chol(BandSys s, int *__restrict__ bad
 
 // --------------------------------------------------------- triangular
 // x <- A^-1 f for system v: permute f into y, L y = g, L^T x = y, scatter.
+// Blocks of NS = 32 columns: the block's diagonal triangle of L is staged
+// in LDS by all threads, wave 0 substitutes through it (lane c holds
+// unknown c; values travel by shuffle), then all threads apply the block
+// to the rows below it (forward) or gather the rows below into it
+// (backward, one wave per column, fixed-order shuffle reduction).
+constexpr int NS = 32;
+
 template <int BT>
 __global__ void __launch_bounds__(BT) band_solve(BandSys s, const int64_t *__restrict__ pnode,
                                                   const double *__restrict__ f,
                                                   double *__restrict__ x,
                                                   double *__restrict__ work)
 {
-    __shared__ double yb[NB];
-    __shared__ double part[NB];
+    constexpr int NW = BT / 64;
+    __shared__ double T[NS][NS + 1];  // T[r][c] = L[j0 + r][j0 + c], r >= c (0 elsewhere)
+    __shared__ double yb[NS];
+    __shared__ double part[NS];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t v = s.order ? (int64_t)s.order[blockIdx.x] : (int64_t)blockIdx.x;
     const int64_t base = s.dof0[v];
@@ -209,18 +255,25 @@ __global__ void __launch_bounds__(BT) band_solve(BandSys s, const int64_t *__res
         const int64_t p = t / 6;
         y[t] = f[6 * pnode[base / 6 + p] + (t - 6 * p)];
     }
+    auto stage = [&](int64_t j0, int jn) {
+        for (int t = tid; t < NS * NS; t += BT) {
+            const int r = t / NS, c = t - r * NS;
+            T[r][c] = (c <= r && r < jn && r - c <= b) ? L[(j0 + c) * ld + (r - c)] : 0.;
+        }
+    };
     __syncthreads();
     // forward: L y = g
-    for (int64_t j0 = 0; j0 < n; j0 += NB) {
-        const int jn = (int)(n - j0 < NB ? n - j0 : NB);
+    for (int64_t j0 = 0; j0 < n; j0 += NS) {
+        const int jn = (int)(n - j0 < NS ? n - j0 : NS);
+        stage(j0, jn);
+        __syncthreads();
         if (wave == 0) {
-            // diagonal triangle: lane c holds y[j0 + c]
             double yc = lane < jn ? y[j0 + lane] : 0.;
             for (int c = 0; c < jn; c++) {
-                double piv = lane == c ? yc / L[(j0 + c) * ld] : 0.;
+                const double piv = lane == c ? yc / T[c][c] : 0.;
                 const double yv = __shfl(piv, c, 64);
                 if (lane == c) yc = yv;
-                if (lane > c && lane < jn && lane - c <= b) yc -= L[(j0 + c) * ld + (lane - c)] * yv;
+                if (lane > c && lane < jn) yc -= T[lane][c] * yv;
             }
             if (lane < jn) {
                 y[j0 + lane] = yc;
@@ -241,14 +294,13 @@ __global__ void __launch_bounds__(BT) band_solve(BandSys s, const int64_t *__res
         __syncthreads();
     }
     // backward: L^T x = y, blocks from the end
-    const int64_t nblk = (n + NB - 1) / NB;
+    const int64_t nblk = (n + NS - 1) / NS;
     for (int64_t bk = nblk - 1; bk >= 0; bk--) {
-        const int64_t j0 = bk * NB;
-        const int jn = (int)(n - j0 < NB ? n - j0 : NB);
+        const int64_t j0 = bk * NS;
+        const int jn = (int)(n - j0 < NS ? n - j0 : NS);
         // contributions of the solved rows below: s_c = sum_i L[i][j0+c] y[i]
-        // for i in [j0+jn, j0+c+b]; 64 lanes per column, one wave per
-        // column (columns c, c + BT/64, ... when there are fewer waves)
-        for (int c = wave; c < NB; c += BT / 64) {
+        // for i in [j0+jn, j0+c+b]
+        for (int c = wave; c < NS; c += NW) {
             double sacc = 0.;
             if (c < jn) {
                 const int64_t j = j0 + c;
@@ -260,16 +312,17 @@ __global__ void __launch_bounds__(BT) band_solve(BandSys s, const int64_t *__res
             for (int o = 32; o > 0; o >>= 1) sacc += __shfl_xor(sacc, o, 64);
             if (lane == 0) part[c] = sacc;
         }
+        stage(j0, jn);
         __syncthreads();
         if (wave == 0) {
             // back substitution in the diagonal triangle, lane c holds x[j0+c]
             double rhs = lane < jn ? y[j0 + lane] - part[lane] : 0.;
             double xc = 0.;
             for (int cc = jn - 1; cc >= 0; cc--) {
-                double piv = lane == cc ? rhs / L[(j0 + cc) * ld] : 0.;
+                const double piv = lane == cc ? rhs / T[cc][cc] : 0.;
                 const double xv = __shfl(piv, cc, 64);
                 if (lane == cc) xc = xv;
-                if (lane < cc && cc - lane <= b) rhs -= L[(j0 + lane) * ld + (cc - lane)] * xv;
+                if (lane < cc) rhs -= T[cc][lane] * xv;
             }
             if (lane < jn) y[j0 + lane] = xc;
         }
@@ -299,9 +352,9 @@ static bool launch_chol(int threads, int max_b, unsigned grid, hipStream_t s, Ba
         hipLaunchKernelGGL((band_chol<BTV, MBV>), dim3(grid), dim3(BTV), 0, s, sys, bad); \
         return true;                                                                        \
     }
-    PF3D_CHOL(1024, 640) PF3D_CHOL(1024, 448) PF3D_CHOL(1024, 256)
-    PF3D_CHOL(512, 640) PF3D_CHOL(512, 448) PF3D_CHOL(512, 256)
-    PF3D_CHOL(256, 640) PF3D_CHOL(256, 448) PF3D_CHOL(256, 256)
+    PF3D_CHOL(1024, 576) PF3D_CHOL(1024, 448) PF3D_CHOL(1024, 256)
+    PF3D_CHOL(512, 576) PF3D_CHOL(512, 448) PF3D_CHOL(512, 256)
+    PF3D_CHOL(256, 576) PF3D_CHOL(256, 448) PF3D_CHOL(256, 256)
 #undef PF3D_CHOL
     return false;
 }
@@ -315,9 +368,9 @@ int pf3d_band_factor(int64_t n_var, int64_t n_nodes, const int32_t *order, const
     if (threads == 0) threads = 1024;
     if (max_b == 0) max_b = MAXB;
     if ((threads != 256 && threads != 512 && threads != 1024) ||
-        (max_b != 256 && max_b != 448 && max_b != 640))
+        (max_b != 256 && max_b != 448 && max_b != 576))
         return pf3d_fail(PF3D_E_ARG, "pf3d_band_factor: threads must be 256/512/1024 and "
-                                     "max_b 256/448/640 (got %d, %d)", threads, max_b);
+                                     "max_b 256/448/576 (got %d, %d)", threads, max_b);
     if (n_var < 0 || n_nodes < 0) return pf3d_fail(PF3D_E_ARG, "pf3d_band_factor: negative size");
     if (n_var == 0) return PF3D_OK;
     if (!dof0 || !bw || !boff || !band || !row_ptr || !col_idx || !val || !var || !pinv || !bad)

@@ -210,11 +210,11 @@ class PlateSweep:
         back substitution, then the FP64 true residual of each variant
         (PCG polish from the direct solution if it misses rtol).  config:
         (factor threads, panel bandwidth, solve threads), default the one
-        chosen by :meth:`autotune_direct`, else (1024, 640, 1024)."""
+        chosen by :meth:`autotune_direct`, else (1024, 576, 1024)."""
         if not hasattr(self, 'band'):
             self._band_setup()
         s = _s(stream)
-        ft, mb, st = config or getattr(self, 'direct_config', None) or (1024, 640, 1024)
+        ft, mb, st = config or getattr(self, 'direct_config', None) or (1024, 576, 1024)
         _lib.call('pf3d_band_factor', self.n_var, self.n_nodes, self.order.data_ptr(),
                   self.band_dof0.data_ptr(), self.band_bw.data_ptr(), self.band_off.data_ptr(),
                   self.band_size, self.band.data_ptr(), self.plan.row_ptr.data_ptr(),
@@ -236,7 +236,7 @@ class PlateSweep:
         if not hasattr(self, 'band'):
             self._band_setup()
         bmax = int(self.band_bw.max())
-        panels = [m for m in (256, 448, 640) if m >= bmax]
+        panels = [m for m in (256, 448, 576) if m >= bmax]
 
         def run(c):
             self.solve_direct(rtol=1.0, polish_iter=0, config=(c['ft'], c['mb'], c['st']))

@@ -198,11 +198,11 @@ def test_direct_launch_configs_and_autotune():
     one of the candidates and the captured step uses it."""
     sw, h, f = _sweep(seed=11)
     sw.assemble()
-    sw.solve_direct(rtol=1e-10, polish_iter=0, config=(1024, 640, 1024))
+    sw.solve_direct(rtol=1e-10, polish_iter=0, config=(1024, 576, 1024))
     torch.cuda.synchronize()
     ref = sw.u.clone()
     bmax = int(sw.band_bw.max())
-    for cfg in [(256, 256, 256), (512, 448, 512), (1024, 256, 256), (256, 640, 1024)]:
+    for cfg in [(256, 256, 256), (512, 448, 512), (1024, 256, 256), (256, 576, 1024)]:
         if cfg[1] < bmax:
             continue
         sw.u.zero_()
