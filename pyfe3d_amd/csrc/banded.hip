@@ -76,11 +76,21 @@ __global__ void band_fill(int64_t n_nodes, const int32_t *__restrict__ rp,
     }
 }
 
+// value of x in lane l (l uniform across the wave), as a scalar broadcast
+__device__ __forceinline__ double readlane_d(double x, int l)
+{
+    const long long v = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)v, l);
+    const int hi = __builtin_amdgcn_readlane((int)(v >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // ------------------------------------------------------------ Cholesky
 // Per panel step of NF = 32 columns (j0 .. j0 + jn):
 //   1. the panel (rows r = 0 .. b + jn of the columns) goes to LDS;
-//   2. wave 0 factorises the jn x jn diagonal block in LDS (lane c owns
-//      row c; a wave fence orders each column's scaling before its use);
+//   2. wave 0 factorises the jn x jn diagonal block in registers (two
+//      lanes per row, 16 columns each; pivot by readlane, multipliers
+//      broadcast through LDS);
 //   3. every thread solves rows of the block below it, x L11^T = a, by
 //      forward substitution against L11 read from LDS (broadcast);
 //   4. the panel is written back and the trailing band window is updated,
@@ -91,6 +101,7 @@ __global__ void band_fill(int64_t n_nodes, const int32_t *__restrict__ rp,
 // A 32-column panel moves the trailing window through HBM once per 32
 // columns.  Sums run in a fixed order, independent of the launch shape.
 constexpr int NF = 32;   // factor panel width
+constexpr int NH = NF / 2;  // diagonal block: columns per lane
 constexpr int TRI = 2;   // trailing update: rows per lane (64 apart)
 constexpr int TKC = 8;   // trailing update: columns per wave block
 
@@ -101,6 +112,7 @@ __global__ void __launch_bounds__(BT) band_chol(BandSys s, int *__restrict__ bad
     constexpr int NW = BT / 64;
     __shared__ double P[PR][NF + 1];
     __shared__ double dinv[NF];
+    __shared__ double Lk[2][NF];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t v = s.order ? (int64_t)s.order[blockIdx.x] : (int64_t)blockIdx.x;
     const int64_t n = s.dof0[v + 1] - s.dof0[v];
@@ -123,26 +135,51 @@ __global__ void __launch_bounds__(BT) band_chol(BandSys s, int *__restrict__ bad
             P[r][c] = x;
         }
         __syncthreads();
-        // 2. diagonal block, by wave 0 in LDS: lane r owns row r; the
-        //    column-k multipliers are read back by every lane after a wave
-        //    fence (the wave's LDS operations complete in order)
+        // 2. diagonal block, by wave 0 in registers: lane (h, r) =
+        //    (lane / 32, lane % 32) holds columns 16 h .. 16 h + 15 of row r.
+        //    Step k takes the pivot by readlane and broadcasts the
+        //    multipliers L[r][k], k < r < jn, through LDS (zero elsewhere,
+        //    two buffers so step k + 1 never overwrites what step k reads);
+        //    the rank-1 update then runs on all 16 columns without masks:
+        //    the lower triangle gets exactly its terms, the upper triangle
+        //    and rows past jn hold values that are never broadcast.
         if (wave == 0) {
-            for (int k = 0; k < jn; k++) {
-                const double d = P[k][k];
-                if (!(d > 0.)) failed = true;
-                const double sq = d > 0. ? sqrt(d) : 1.;
-                const double inv = 1. / sq;
-                if (lane >= k && lane < jn) P[lane][k] = lane == k ? sq : P[lane][k] * inv;
-                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-                __builtin_amdgcn_wave_barrier();
-                if (lane > k && lane < jn) {
-                    const double lk = P[lane][k];
-                    for (int c2 = k + 1; c2 <= lane; c2++) P[lane][c2] -= lk * P[c2][k];
+            int r = lane & 31;
+            // opaque per panel, so the per-step lane masks below are formed
+            // where they are used instead of being kept live across panels
+            asm volatile("" : "+v"(r));
+            const int h = lane >> 5;
+            double l[NH];
+#pragma unroll
+            for (int i = 0; i < NH; i++) l[i] = P[r][NH * h + i];
+#pragma unroll
+            for (int k = 0; k < NF; k++) {
+                if (k < jn) {
+                    const int kh = k / NH, ki = k % NH;
+                    const double d = readlane_d(l[ki], 32 * kh + k);
+                    if (!(d > 0.)) failed = true;
+                    const double sq = d > 0. ? sqrt(d) : 1.;
+                    const double inv = 1. / sq;
+                    if (h == kh) {
+                        l[ki] = r == k ? sq : l[ki] * inv;
+                        Lk[k & 1][r] = (r > k && r < jn) ? l[ki] : 0.;
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+                    __builtin_amdgcn_wave_barrier();
+                    const double lr = Lk[k & 1][r];
+#pragma unroll
+                    for (int i = 0; i < NH; i++) l[i] -= lr * Lk[k & 1][NH * h + i];
                 }
-                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-                __builtin_amdgcn_wave_barrier();
             }
-            if (lane < jn) dinv[lane] = 1. / P[lane][lane];
+            // rows jn .. 31 exist in P only when jn = 32; the upper
+            // triangle of the block is never read
+#pragma unroll
+            for (int i = 0; i < NH; i++) P[r][NH * h + i] = l[i];
+            double dg = 1.;
+#pragma unroll
+            for (int i = 0; i < NH; i++)
+                if (NH * h + i == r) dg = l[i];
+            if (r < jn && (r >> 4) == h) dinv[r] = 1. / dg;
         }
         __syncthreads();
         // 3. rows below the diagonal block: x L11^T = a
