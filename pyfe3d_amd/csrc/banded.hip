@@ -94,16 +94,14 @@ __device__ __forceinline__ double readlane_d(double x, int l)
 //   3. every thread solves rows of the block below it, x L11^T = a, by
 //      forward substitution against L11 read from LDS (broadcast);
 //   4. the panel is written back and the trailing band window is updated,
-//      A[ri][rk] -= sum_c L[ri][c] L[rk][c] for jn <= rk <= ri < rows:
-//      a wave takes 2 x 64 consecutive rows (lane = row: conflict-free LDS
-//      reads, coalesced global access down the band's columns) by 8
-//      columns (broadcast LDS reads), c summed in order in registers.
+//      A[ri][rk] -= sum_c L[ri][c] L[rk][c] for jn <= rk <= ri < rows, on
+//      16 x 16 tiles by FP64 MFMA (one tile per wave at a time, operands
+//      from LDS, the tile read from and written back to the band).
 // A 32-column panel moves the trailing window through HBM once per 32
 // columns.  Sums run in a fixed order, independent of the launch shape.
 constexpr int NF = 32;   // factor panel width
 constexpr int NH = NF / 2;  // diagonal block: columns per lane
-constexpr int TRI = 2;   // trailing update: rows per lane (64 apart)
-constexpr int TKC = 8;   // trailing update: columns per wave block
+typedef double v4d __attribute__((ext_vector_type(4)));
 
 template <int BT, int MB>
 __global__ void __launch_bounds__(BT) band_chol(BandSys s, int *__restrict__ bad)
@@ -211,48 +209,41 @@ __global__ void __launch_bounds__(BT) band_chol(BandSys s, int *__restrict__ bad
             const int c = t / rows, r = t - c * rows;
             if (c < jn && r >= c && r - c <= b) L[(j0 + c) * ld + (r - c)] = P[r][c];
         }
-        // 4b. trailing update
+        // 4b. trailing update on 16 x 16 tiles (I, K), K <= I, of the
+        //     window on the FP64 matrix cores: rows ri = jn + 16 I + i,
+        //     columns rk = jn + 16 K + j, D = A - P_I P_K^T as 8 steps of
+        //     v_mfma_f64_16x16x4 over c = 0 .. 31 in order.  Lane l supplies
+        //     operand row l % 16 at column 4 s + l / 16 and holds
+        //     D[l / 16 + 4 v][l % 16], v = 0 .. 3 (band entries of column rk).
         const int T = rows - jn;
-        const int nrb = (T + 64 * TRI - 1) / (64 * TRI), nkb = (T + TKC - 1) / TKC;
-        for (int blk = wave; blk < nrb * nkb; blk += NW) {
-            const int kb = blk / nrb, rb = blk - kb * nrb;
-            const int rk0 = jn + TKC * kb, ri0 = jn + 64 * TRI * rb;
-            if (ri0 + 64 * TRI - 1 < rk0) continue;  // wholly above the diagonal
-            int ri[TRI];
+        const int nt = (T + 15) >> 4;
+        const int ntiles = nt * (nt + 1) / 2;
+        const int li = lane & 15, lk = lane >> 4;
+        for (int t = wave; t < ntiles; t += NW) {
+            int I = (int)((sqrtf(8.f * t + 1.f) - 1.f) * 0.5f);
+            while ((I + 1) * (I + 2) / 2 <= t) I++;
+            while (I * (I + 1) / 2 > t) I--;
+            const int K = t - I * (I + 1) / 2;
+            const int ra = jn + 16 * I + li, rb = jn + 16 * K + li;
+            const int ri0 = jn + 16 * I + lk;
+            double *col = L + (j0 + rb) * ld;
+            v4d acc;
 #pragma unroll
-            for (int q = 0; q < TRI; q++) ri[q] = ri0 + 64 * q + lane;
-            // the window values are loaded first, so their latency overlaps
-            // the LDS reads of the product loop; A <- A - sum_c P[ri][c]
-            // P[rk][c], c in order
-            double acc[TRI][TKC];
-#pragma unroll
-            for (int a = 0; a < TKC; a++) {
-                const int rk = rk0 + a;
-                const double *col = L + (j0 + rk) * ld;
-#pragma unroll
-                for (int q = 0; q < TRI; q++)
-                    acc[q][a] = (rk < rows && ri[q] >= rk && ri[q] < rows) ? col[ri[q] - rk] : 0.;
-            }
-#pragma unroll 2
-            for (int c = 0; c < NF; c++) {
-                double pi[TRI], pk[TKC];
-#pragma unroll
-                for (int q = 0; q < TRI; q++) pi[q] = ri[q] < rows ? P[ri[q]][c] : 0.;
-#pragma unroll
-                for (int a = 0; a < TKC; a++) pk[a] = rk0 + a < rows ? P[rk0 + a][c] : 0.;
-#pragma unroll
-                for (int q = 0; q < TRI; q++)
-#pragma unroll
-                    for (int a = 0; a < TKC; a++) acc[q][a] -= pi[q] * pk[a];
+            for (int v = 0; v < 4; v++) {
+                const int ri = ri0 + 4 * v;
+                acc[v] = (rb < rows && ri < rows && ri >= rb) ? col[ri - rb] : 0.;
             }
 #pragma unroll
-            for (int a = 0; a < TKC; a++) {
-                const int rk = rk0 + a;
-                if (rk >= rows) continue;
-                double *col = L + (j0 + rk) * ld;
+            for (int st = 0; st < NF / 4; st++) {
+                const int c = 4 * st + lk;
+                const double x = ra < rows ? -P[ra][c] : 0.;
+                const double y = rb < rows ? P[rb][c] : 0.;
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, acc, 0, 0, 0);
+            }
 #pragma unroll
-                for (int q = 0; q < TRI; q++)
-                    if (ri[q] >= rk && ri[q] < rows) col[ri[q] - rk] = acc[q][a];
+            for (int v = 0; v < 4; v++) {
+                const int ri = ri0 + 4 * v;
+                if (rb < rows && ri < rows && ri >= rb) col[ri - rb] = acc[v];
             }
         }
         __syncthreads();
