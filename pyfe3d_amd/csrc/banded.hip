@@ -103,10 +103,78 @@ constexpr int NF = 32;   // factor panel width
 constexpr int NH = NF / 2;  // diagonal block: columns per lane
 typedef double v4d __attribute__((ext_vector_type(4)));
 
+// trailing-update tile t of the lower triangle (row-major: (0,0), (1,0),
+// (1,1), (2,0) ...; t uniform across the wave) -> this lane's A-operand
+// row ra, B-operand row and window column rb, first window row ri0 (the
+// lane's rows are ri0 + 4 v)
+struct TileAt {
+    int ra, rb, ri0;
+};
+
+__device__ __forceinline__ TileAt tile_at(int t, int jn, int li, int lk)
+{
+    int I = (int)((sqrtf(8.f * t + 1.f) - 1.f) * 0.5f);
+    I = __builtin_amdgcn_readfirstlane(I);
+    while ((I + 1) * (I + 2) / 2 <= t) I++;
+    while (I * (I + 1) / 2 > t) I--;
+    const int K = t - I * (I + 1) / 2;
+    return TileAt{jn + 16 * I + li, jn + 16 * K + li, jn + 16 * I + lk};
+}
+
+// the tile's window entries A[ri0 + 4 v][rb] from the band (Lp: the
+// panel's first column).  Addresses are clamped into the window, so every
+// lane loads; entries outside the window's lower triangle are never stored
+// and only reach their own output through the update.
+__device__ __forceinline__ v4d tile_load(const TileAt &q, const double *Lp, int64_t ld,
+                                         int rows)
+{
+    const int rb = min(q.rb, rows - 1);
+    const double *col = Lp + rb * ld;
+    v4d a;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int ri = min(max(q.ri0 + 4 * v, rb), rows - 1);
+        a[v] = col[ri - rb];
+    }
+    return a;
+}
+
+__device__ __forceinline__ void tile_store(const TileAt &q, const v4d &a, double *Lp,
+                                           int64_t ld, int rows)
+{
+    if (q.rb >= rows) return;
+    double *col = Lp + q.rb * ld;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int ri = q.ri0 + 4 * v;
+        if (ri < rows && ri >= q.rb) col[ri - q.rb] = a[v];
+    }
+}
+
+// a -= P[ra][0..31] P[rb][0..31]^T on the tile; operand rows past the
+// panel only feed outputs that are not stored
+template <int PR>
+__device__ __forceinline__ v4d tile_update(const TileAt &q, v4d a, const double (*P)[NF + 1],
+                                           int lk)
+{
+    const int ra = min(q.ra, PR - 1), rb = min(q.rb, PR - 1);
+    double x[NF / 4], y[NF / 4];
+#pragma unroll
+    for (int st = 0; st < NF / 4; st++) {
+        x[st] = -P[ra][4 * st + lk];
+        y[st] = P[rb][4 * st + lk];
+    }
+#pragma unroll
+    for (int st = 0; st < NF / 4; st++)
+        a = __builtin_amdgcn_mfma_f64_16x16x4f64(x[st], y[st], a, 0, 0, 0);
+    return a;
+}
+
 template <int BT, int MB>
 __global__ void __launch_bounds__(BT) band_chol(BandSys s, int *__restrict__ bad)
 {
     constexpr int PR = MB + NF;
+    constexpr int RCH = (PR + 63) / 64;  // 64-row chunks of a panel column
     constexpr int NW = BT / 64;
     __shared__ double P[PR][NF + 1];
     __shared__ double dinv[NF];
@@ -125,12 +193,22 @@ __global__ void __launch_bounds__(BT) band_chol(BandSys s, int *__restrict__ bad
     for (int64_t j0 = 0; j0 < n; j0 += NF) {
         const int jn = (int)(n - j0 < NF ? n - j0 : NF);
         const int rows = (int)(n - j0 < b + jn ? n - j0 : b + jn);
-        // 1. panel: P[r][c] = A[j0 + r][j0 + c], r >= c, r - c <= b
-        for (int t = tid; t < rows * NF; t += BT) {
-            const int c = t / rows, r = t - c * rows;
-            double x = 0.;
-            if (c < jn && r >= c && r - c <= b) x = L[(j0 + c) * ld + (r - c)];
-            P[r][c] = x;
+        // 1. panel: P[r][c] = A[j0 + r][j0 + c], r >= c, r - c <= b (zero
+        //    elsewhere); wave w takes columns w, w + NW, ..., 64 rows per
+        //    load, every load of a column in flight before its LDS stores
+        for (int c = wave; c < NF; c += NW) {
+            const double *col = L + (j0 + c) * ld;
+            double x[RCH];
+#pragma unroll
+            for (int q = 0; q < RCH; q++) {
+                const int r = 64 * q + lane;
+                x[q] = (c < jn && r >= c && r < rows && r - c <= b) ? col[r - c] : 0.;
+            }
+#pragma unroll
+            for (int q = 0; q < RCH; q++) {
+                const int r = 64 * q + lane;
+                if (r < rows) P[r][c] = x[q];
+            }
         }
         __syncthreads();
         // 2. diagonal block, by wave 0 in registers: lane (h, r) =
@@ -180,34 +258,38 @@ __global__ void __launch_bounds__(BT) band_chol(BandSys s, int *__restrict__ bad
             if (r < jn && (r >> 4) == h) dinv[r] = 1. / dg;
         }
         __syncthreads();
-        // 3. rows below the diagonal block: x L11^T = a
+        // 3. rows below the diagonal block: x L11^T = a, by columns: x[c] is
+        //    final once scaled, then its term goes into every later x[c2].
+        //    Each x[c2] receives its terms in the order c = 0, 1, ... as in
+        //    forward substitution, while the terms of one column are
+        //    independent.  (Rows exist below the block only when jn = NF.)
         for (int r = jn + tid; r < rows; r += BT) {
             // compiler barrier: L11 is re-read from LDS per row rather than
             // hoisted into ~500 registers for the few rows a thread takes
             asm volatile("" ::: "memory");
             double x[NF];
 #pragma unroll
-            for (int c = 0; c < NF; c++) x[c] = c < jn ? P[r][c] : 0.;
+            for (int c = 0; c < NF; c++) x[c] = P[r][c];
 #pragma unroll
             for (int c = 0; c < NF; c++) {
-                if (c < jn) {
-                    double acc = x[c];
+                x[c] *= dinv[c];
 #pragma unroll
-                    for (int k = 0; k < c; k++) acc -= x[k] * P[c][k];
-                    x[c] = acc * dinv[c];
-                }
+                for (int c2 = c + 1; c2 < NF; c2++) x[c2] -= x[c] * P[c2][c];
                 // keep each column's L11 loads next to their use
                 __builtin_amdgcn_sched_barrier(0);
             }
 #pragma unroll
-            for (int c = 0; c < NF; c++)
-                if (c < jn) P[r][c] = x[c];
+            for (int c = 0; c < NF; c++) P[r][c] = x[c];
         }
         __syncthreads();
-        // 4a. the panel's columns of L back to the band
-        for (int t = tid; t < rows * NF; t += BT) {
-            const int c = t / rows, r = t - c * rows;
-            if (c < jn && r >= c && r - c <= b) L[(j0 + c) * ld + (r - c)] = P[r][c];
+        // 4a. the panel's columns of L back to the band (same mapping)
+        for (int c = wave; c < jn; c += NW) {
+            double *col = L + (j0 + c) * ld;
+#pragma unroll
+            for (int q = 0; q < RCH; q++) {
+                const int r = 64 * q + lane;
+                if (r >= c && r < rows && r - c <= b) col[r - c] = P[r][c];
+            }
         }
         // 4b. trailing update on 16 x 16 tiles (I, K), K <= I, of the
         //     window on the FP64 matrix cores: rows ri = jn + 16 I + i,
@@ -219,31 +301,32 @@ __global__ void __launch_bounds__(BT) band_chol(BandSys s, int *__restrict__ bad
         const int nt = (T + 15) >> 4;
         const int ntiles = nt * (nt + 1) / 2;
         const int li = lane & 15, lk = lane >> 4;
-        for (int t = wave; t < ntiles; t += NW) {
-            int I = (int)((sqrtf(8.f * t + 1.f) - 1.f) * 0.5f);
-            while ((I + 1) * (I + 2) / 2 <= t) I++;
-            while (I * (I + 1) / 2 > t) I--;
-            const int K = t - I * (I + 1) / 2;
-            const int ra = jn + 16 * I + li, rb = jn + 16 * K + li;
-            const int ri0 = jn + 16 * I + lk;
-            double *col = L + (j0 + rb) * ld;
-            v4d acc;
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const int ri = ri0 + 4 * v;
-                acc[v] = (rb < rows && ri < rows && ri >= rb) ? col[ri - rb] : 0.;
-            }
-#pragma unroll
-            for (int st = 0; st < NF / 4; st++) {
-                const int c = 4 * st + lk;
-                const double x = ra < rows ? -P[ra][c] : 0.;
-                const double y = rb < rows ? P[rb][c] : 0.;
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, acc, 0, 0, 0);
-            }
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const int ri = ri0 + 4 * v;
-                if (rb < rows && ri < rows && ri >= rb) col[ri - rb] = acc[v];
+        // each wave loads its next tile before it multiplies and stores the
+        // current one (tiles are disjoint); two register sets, so the
+        // prefetched tile is never copied, and a wave with no next tile
+        // reloads its last one, so the load counter's order is the same on
+        // every path and the multiply waits only for its own tile
+        double *Lp = L + j0 * ld;
+        int t = __builtin_amdgcn_readfirstlane(wave);
+        if (t < ntiles) {
+            TileAt q0 = tile_at(t, jn, li, lk), q1;
+            v4d a0 = tile_load(q0, Lp, ld, rows), a1;
+            for (;;) {
+                const int t1 = t + NW;
+                q1 = tile_at(min(t1, ntiles - 1), jn, li, lk);
+                a1 = tile_load(q1, Lp, ld, rows);
+                __builtin_amdgcn_sched_barrier(0);  // loads issue before the multiply
+                a0 = tile_update<PR>(q0, a0, P, lk);
+                tile_store(q0, a0, Lp, ld, rows);
+                if (t1 >= ntiles) break;
+                const int t2 = t1 + NW;
+                q0 = tile_at(min(t2, ntiles - 1), jn, li, lk);
+                a0 = tile_load(q0, Lp, ld, rows);
+                __builtin_amdgcn_sched_barrier(0);
+                a1 = tile_update<PR>(q1, a1, P, lk);
+                tile_store(q1, a1, Lp, ld, rows);
+                if (t2 >= ntiles) break;
+                t = t2;
             }
         }
         __syncthreads();
