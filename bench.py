@@ -32,6 +32,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+HBM_PEAK_BPS = HBM_PEAK_GBS * 1e9
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix peak
 # HBM bytes per launch of the values-only Quad4 KC0 kernel on the 1000x1000
 # plate, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of
 # `python bench.py --profile-kc0` (KiB per dispatch, mean of 3 launches),
@@ -290,6 +292,40 @@ def sweep_rates(n_var, log, steps=2, modes=('direct', 'fp64', 'mixed')):
             % (mode, ms, n_var / (ms * 1e-3), rel.max(), its.sum()))
         del g
         sw.graph = None
+    if 'direct' in modes:
+        out['direct']['factor'] = factor_roofline(sw, log)
+    return out
+
+
+def factor_roofline(sw, log, reps=3):
+    """The banded factorisation alone (band storage fill + Cholesky of all
+    variants, autotuned shape), timed with HIP events on its stream, against
+    the FP64 matrix-core peak (its bound: the band read and written once at
+    8 TB/s takes less time than its flops at 78.6 TFLOP/s)."""
+    import torch
+    st = torch.cuda.current_stream()
+    sw.factor_direct(stream=st)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(st)
+    for _ in range(reps):
+        sw.factor_direct(stream=st)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    flop = sw.band_flops()
+    band_bytes = 16 * int((sw.band_n_host * (sw.band_bw_host + 1)).sum())
+    window_bytes = sw.band_window_bytes()
+    tf = flop / (ms * 1e-3) / 1e12
+    out = dict(ms=ms, flop=flop, bound='mfma', achieved=tf, peak=FP64_PEAK_TFLOPS,
+               unit='TFLOP/s', frac=tf / FP64_PEAK_TFLOPS, band_bytes=band_bytes,
+               min_hbm_ms=band_bytes / HBM_PEAK_BPS * 1e3, panel_traffic_bytes=window_bytes,
+               panel_traffic_gbps=window_bytes / (ms * 1e-3) / 1e9,
+               note='band fill + banded Cholesky (band_zero, band_fill, band_chol); '
+                    'panel_traffic = 32-column panels read/written once + trailing '
+                    'window lower triangle read/written once per panel')
+    log('sweep factor: %.1f ms, %.3g flop -> %.2f TFLOP/s = %.3f of FP64 peak; '
+        'panel traffic %.0f GB/s' % (ms, flop, tf, out['frac'], out['panel_traffic_gbps']))
     return out
 
 
@@ -574,7 +610,7 @@ def profile_kc0(n, launches=3):
           flush=True)
 
 
-def profile_sweep(n_var, steps=3, config=(1024, 448, 256)):
+def profile_sweep(n_var, steps=3, config=(512, 448, 512)):
     """`steps` direct-solve steps of the n_var-plate design sweep (records,
     KC0, BSR, Dirichlet, band fill, banded Cholesky, triangular solves) with
     the launch shape given (factor threads, panel bandwidth, solve threads;

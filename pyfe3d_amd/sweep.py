@@ -197,6 +197,8 @@ class PlateSweep:
         self.band_pinv = torch.as_tensor(pinv, device=dev)
         self.band_pnode = torch.as_tensor(pnode, device=dev)
         self.band_bw = torch.as_tensor(bws, device=dev)
+        self.band_n_host = 6 * np.array([(nxe + 1) * (nye + 1) for nxe, nye in self.dims], np.int64)
+        self.band_bw_host = bws.astype(np.int64)
         boff = np.concatenate([[0], np.cumsum(sizes)])
         self.band_off = torch.as_tensor(boff[:-1].astype(np.int64), device=dev)
         self.band_size = int(boff[-1])
@@ -204,6 +206,48 @@ class PlateSweep:
         self.band_dof0 = torch.as_tensor(6 * self.node0, device=dev)
         self.band_work = torch.empty(6 * self.n_nodes, dtype=torch.float64, device=dev)
         self.band_bad = torch.zeros(1, dtype=torch.int32, device=dev)
+
+    def band_flops(self):
+        """Floating-point operations of the banded Cholesky of every
+        variant: column j updates the m_j = min(b, n - 1 - j) entries below
+        it with m_j (m_j + 1) / 2 multiply-adds (square roots and divisions
+        not counted)."""
+        if not hasattr(self, 'band'):
+            self._band_setup()
+        n, b = self.band_n_host, np.minimum(self.band_bw_host, self.band_n_host - 1)
+        full = np.maximum(n - b, 0) * b * (b + 1)
+        tail = (b - 1) * b * (b + 1) // 3
+        return int((full + tail).sum())
+
+    def band_window_bytes(self, nf=32):
+        """HBM bytes the 32-column panel factorisation moves: per panel, the
+        panel's band entries read and written once and the trailing window's
+        lower triangle (T (T + 1) / 2 entries, T = rows below the panel in
+        reach) read and written once."""
+        if not hasattr(self, 'band'):
+            self._band_setup()
+        total = 0
+        for n, b in zip(self.band_n_host.tolist(), self.band_bw_host.tolist()):
+            j0 = np.arange(0, n, nf)
+            jn = np.minimum(nf, n - j0)
+            rows = np.minimum(n - j0, b + jn)
+            t = rows - jn
+            total += int((t * (t + 1) // 2).sum()) * 16
+        return total + 16 * int((self.band_n_host * (self.band_bw_host + 1)).sum())
+
+    def factor_direct(self, config=None, stream=None):
+        """Band storage of every variant's stiffness matrix and its banded
+        Cholesky factorisation in place (the first half of
+        :meth:`solve_direct`)."""
+        if not hasattr(self, 'band'):
+            self._band_setup()
+        ft, mb, _ = config or getattr(self, 'direct_config', None) or (1024, 576, 1024)
+        _lib.call('pf3d_band_factor', self.n_var, self.n_nodes, self.order.data_ptr(),
+                  self.band_dof0.data_ptr(), self.band_bw.data_ptr(), self.band_off.data_ptr(),
+                  self.band_size, self.band.data_ptr(), self.plan.row_ptr.data_ptr(),
+                  self.plan.col_idx.data_ptr(), self.val.data_ptr(), self.band_var.data_ptr(),
+                  self.band_pinv.data_ptr(), self.band_bad.data_ptr(), int(ft), int(mb),
+                  _s(stream))
 
     def solve_direct(self, rtol=1e-8, polish_iter=2000, config=None, stream=None):
         """Banded Cholesky of every variant (one workgroup each), forward and
@@ -215,11 +259,7 @@ class PlateSweep:
             self._band_setup()
         s = _s(stream)
         ft, mb, st = config or getattr(self, 'direct_config', None) or (1024, 576, 1024)
-        _lib.call('pf3d_band_factor', self.n_var, self.n_nodes, self.order.data_ptr(),
-                  self.band_dof0.data_ptr(), self.band_bw.data_ptr(), self.band_off.data_ptr(),
-                  self.band_size, self.band.data_ptr(), self.plan.row_ptr.data_ptr(),
-                  self.plan.col_idx.data_ptr(), self.val.data_ptr(), self.band_var.data_ptr(),
-                  self.band_pinv.data_ptr(), self.band_bad.data_ptr(), int(ft), int(mb), s)
+        self.factor_direct(config=(ft, mb, st), stream=stream)
         _lib.call('pf3d_band_solve', self.n_var, self.order.data_ptr(),
                   self.band_dof0.data_ptr(), self.band_bw.data_ptr(), self.band_off.data_ptr(),
                   self.band.data_ptr(), self.band_pnode.data_ptr(), self.f.data_ptr(),
