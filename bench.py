@@ -262,6 +262,7 @@ def sweep_rates(n_var, log, steps=2, modes=('direct', 'fp64', 'mixed')):
         table = sw.autotune_direct(log=log, repeats=1)
         tuned = dict(config=dict(zip(('factor_threads', 'panel_bandwidth', 'solve_threads'),
                                      sw.direct_config)),
+                     source=sw.direct_config_source, key=sw.autotune_key(),
                      tuning_s=time.perf_counter() - t0,
                      candidates=[dict(c, ms=round(ms, 3)) for c, ms in table])
         log('sweep direct: autotuned config %s' % (sw.direct_config,))
@@ -630,6 +631,36 @@ def profile_sweep(n_var, steps=3, config=(512, 448, 512)):
     print('profile-sweep: %d plates, %d steps, max relres %.2e' % (n_var, steps, rel), flush=True)
 
 
+def profile_nn(steps, batch=64):
+    """`steps` training steps (forward, nodal-loss backward, Adam) of the
+    warm-start network on a batch of random 64 x 64 inputs, for kernel
+    traces and counter runs; prints the GEMM flops per step and the step
+    time from HIP events."""
+    import torch
+    from pyfe3d_amd import warmstart as W
+    net = W.WarmStartNet(n=64, batch=batch, seed=3)
+    g = torch.Generator(device='cuda').manual_seed(5)
+    x = torch.randn((batch, 64, 64, 2), device='cuda', generator=g)
+    target = torch.randn((batch, 65, 65, 6), device='cuda', generator=g)
+    wc = torch.ones(6, device='cuda')
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    net.forward(x, st)
+    net.backward(target, wc, st)
+    torch.cuda.synchronize()
+    e0.record(st)
+    for _ in range(steps):
+        net.forward(x, st)
+        net.backward(target, wc, st)
+        net.adam(stream=st)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    fwd, bwd = net.gemm_flops()
+    print('profile-nn: batch %d, %d steps, %.3f ms per step, GEMM flops per step %d '
+          '(forward %d, backward %d)' % (batch, steps, ms, fwd + bwd, fwd, bwd), flush=True)
+
+
 def warm_start_rates(n_plates, log, multigrid_plates=16):
     """Held-out 64 x 64 plates (smooth thickness fields, Gaussian pressure
     patches): block-Jacobi PCG to 1e-8 per plate and multigrid CG on a
@@ -677,6 +708,9 @@ def main():
                     help='only: direct-solve steps of a design sweep of this many plates')
     ap.add_argument('--profile-launches', type=int, default=3,
                     help='KC0 launches of --profile-kc0 (the kernel trace uses warmup + steps)')
+    ap.add_argument('--profile-nn', type=int, default=0,
+                    help='run only this many forward + backward + Adam steps of the warm-start '
+                         'network on a batch of 64 plates of 64 x 64 elements (for rocprof)')
     ap.add_argument('--quiet', action='store_true')
     args = ap.parse_args()
     log = (lambda *a: None) if args.quiet else (lambda *a: print(*a, file=sys.stderr, flush=True))
@@ -694,6 +728,9 @@ def main():
         return
     if args.profile_sweep:
         profile_sweep(args.profile_sweep)
+        return
+    if args.profile_nn:
+        profile_nn(args.profile_nn)
         return
     if args.large_solve:
         print(json.dumps({'large_solve': large_solve(args.n, log)}), flush=True)

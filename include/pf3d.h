@@ -664,7 +664,11 @@ int pf3d_nn_im2col(int B, int H, int W, int C, int stride, const float *x, float
 int pf3d_nn_col2im(int B, int H, int W, int C, int stride, const float *dcol, float *dx,
                    int accumulate, void *stream);
 int pf3d_nn_relu_bwd(int64_t n, const float *y, float *dy, void *stream);
-int pf3d_nn_bias_grad(int64_t rows, int C, const float *dy, float *db, void *stream);
+/* db[c] = sum over the rows of dy[r][c] (fixed order: chunks of rows summed
+ * into work, then the chunks in a fixed order); work holds work_floats >= C
+ * floats (1024 C is enough for the most parallel split). */
+int pf3d_nn_bias_grad(int64_t rows, int C, const float *dy, float *db, float *work,
+                      int64_t work_floats, void *stream);
 /* y = nearest-upsample(lo) + skip on an H x W grid, and d(lo) */
 int pf3d_nn_up2_add(int B, int H, int W, int C, const float *lo, const float *skip, float *y,
                     void *stream);

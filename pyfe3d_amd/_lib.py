@@ -100,7 +100,7 @@ SIGNATURES = {
     'pf3d_nn_im2col': [I32, I32, I32, I32, I32, P, P, P],
     'pf3d_nn_col2im': [I32, I32, I32, I32, I32, P, P, I32, P],
     'pf3d_nn_relu_bwd': [I64, P, P, P],
-    'pf3d_nn_bias_grad': [I64, I32, P, P, P],
+    'pf3d_nn_bias_grad': [I64, I32, P, P, P, I64, P],
     'pf3d_nn_up2_add': [I32, I32, I32, I32, P, P, P, P],
     'pf3d_nn_up2_bwd': [I32, I32, I32, I32, P, P, P],
     'pf3d_nn_node_avg': [I32, I32, I32, P, P, P],

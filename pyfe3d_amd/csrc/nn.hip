@@ -8,7 +8,8 @@
 // fastest) in FP32 as the optimiser's master copy.  Every convolution is an
 // im2col gather followed by one GEMM on the matrix cores: operands are
 // rounded to bfloat16 when they are staged into LDS, products accumulate in
-// FP32 (v_mfma_f32_16x16x16_bf16).  The backward pass uses the same GEMM
+// FP32 (gfx950's v_mfma_f32_16x16x32_bf16: the whole 32-deep LDS tile in
+// one instruction per 16x16 output tile).  The backward pass uses the same GEMM
 // with other strides (dW = dY^T col, dcol = dY W) and a gather col2im, so
 // every gradient is a fixed-order sum.
 #include <hip/hip_runtime.h>
@@ -22,7 +23,8 @@ int pf3d_hip_check(hipError_t e, const char *where);
 
 namespace {
 
-using bf16x4 = short __attribute__((ext_vector_type(4)));
+using s16x8 = short __attribute__((ext_vector_type(8)));
+using bf16x8 = __bf16 __attribute__((ext_vector_type(8)));
 using f32x4 = float __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ unsigned short to_bf16(float f)
@@ -33,7 +35,9 @@ __device__ __forceinline__ unsigned short to_bf16(float f)
     return (unsigned short)(u >> 16);
 }
 
-constexpr int TM = 64, TN = 64, TK = 32, KP = TK + 4;  // tile and padded LDS row
+// tile and LDS row (padded to 40 bf16 = 80 B: 16-byte aligned rows for the
+// 8-value operand reads, rows 20 banks apart)
+constexpr int TM = 64, TN = 64, TK = 32, KP = TK + 8;
 
 struct Gemm {
     int M, N, K;
@@ -54,8 +58,8 @@ struct Gemm {
 // owns a 32x32 block of the 64x64 tile (2x2 MFMA tiles of 16x16).
 __global__ void __launch_bounds__(256) gemm_bf16(Gemm g)
 {
-    __shared__ unsigned short As[TM][KP];
-    __shared__ unsigned short Bs[TN][KP];
+    __shared__ __align__(16) unsigned short As[TM][KP];
+    __shared__ __align__(16) unsigned short Bs[TN][KP];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
@@ -88,23 +92,23 @@ __global__ void __launch_bounds__(256) gemm_bf16(Gemm g)
             Bs[r][kk] = to_bf16(w);
         }
         __syncthreads();
-#pragma unroll
-        for (int ks = 0; ks < TK; ks += 16) {
-            const int kq = ks + 4 * (lane >> 4);
-            bf16x4 a[2], b[2];
+        // operands: lane l supplies row l % 16 at k = 8 (l / 16) .. + 7 of
+        // the tile, in A and in B alike, so each k meets its own partner
+        {
+            const int kq = 8 * (lane >> 4);
+            s16x8 a[2], b[2];
 #pragma unroll
             for (int i = 0; i < 2; i++) {
-                const unsigned short *pa = &As[wm + 16 * i + (lane & 15)][kq];
-                const unsigned short *pb = &Bs[wn + 16 * i + (lane & 15)][kq];
-                a[i] = bf16x4{(short)pa[0], (short)pa[1], (short)pa[2], (short)pa[3]};
-                b[i] = bf16x4{(short)pb[0], (short)pb[1], (short)pb[2], (short)pb[3]};
+                a[i] = *reinterpret_cast<const s16x8 *>(&As[wm + 16 * i + (lane & 15)][kq]);
+                b[i] = *reinterpret_cast<const s16x8 *>(&Bs[wn + 16 * i + (lane & 15)][kq]);
             }
 #pragma unroll
             for (int i = 0; i < 2; i++)
 #pragma unroll
                 for (int j = 0; j < 2; j++)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j],
-                                                                           0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(bf16x8, a[i]), __builtin_bit_cast(bf16x8, b[j]),
+                        acc[i][j], 0, 0, 0);
         }
         __syncthreads();
     }
@@ -205,19 +209,39 @@ __global__ void relu_bwd(int64_t n, const float *__restrict__ y, float *__restri
 }
 
 // db[c] = sum over rows of dy[row][c] (one workgroup per 64 channels, fixed order)
-__global__ void __launch_bounds__(256) bias_grad(int64_t rows, int C, const float *__restrict__ dy,
-                                                 float *__restrict__ db)
+// db[c] = sum_r dy[r][c] in a fixed order: block (k, channel tile) sums
+// rows [k chunk, (k + 1) chunk) (four interleaved partial sums, added in
+// order) into part[k][c]; bias_sum then adds the chunks in a fixed order.
+__global__ void __launch_bounds__(256) bias_part(int64_t rows, int C, int64_t chunk,
+                                                 const float *__restrict__ dy,
+                                                 float *__restrict__ part)
 {
-    __shared__ float part[4][64];
-    const int c = blockIdx.x * 64 + (threadIdx.x & 63), q = threadIdx.x >> 6;
+    __shared__ float red[4][64];
+    const int cl = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int c = blockIdx.y * 64 + cl;
+    const int64_t r0 = (int64_t)blockIdx.x * chunk;
+    const int64_t r1 = r0 + chunk < rows ? r0 + chunk : rows;
     float s = 0.f;
     if (c < C)
-        for (int64_t r = q; r < rows; r += 4) s += dy[r * C + c];
-    part[q][threadIdx.x & 63] = s;
+        for (int64_t r = r0 + q; r < r1; r += 4) s += dy[r * C + c];
+    red[q][cl] = s;
     __syncthreads();
     if (q == 0 && c < C)
-        db[c] = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] +
-                part[3][threadIdx.x];
+        part[(int64_t)blockIdx.x * C + c] = red[0][cl] + red[1][cl] + red[2][cl] + red[3][cl];
+}
+
+// one wave per channel: lane l adds chunks l, l + 64, ... in order, then a
+// fixed butterfly over the lanes
+__global__ void __launch_bounds__(256) bias_sum(int nchunk, int C, const float *__restrict__ part,
+                                                float *__restrict__ db)
+{
+    const int c = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (c >= C) return;
+    float s = 0.f;
+    for (int k = lane; k < nchunk; k += 64) s += part[(int64_t)k * C + c];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) db[c] = s;
 }
 
 // y[b][y][x][c] = lo[b][y/2][x/2][c] + skip[b][y][x][c]   (H, W: fine grid)
@@ -470,11 +494,23 @@ int pf3d_nn_relu_bwd(int64_t n, const float *y, float *dy, void *stream)
     return pf3d_hip_check(hipGetLastError(), "pf3d_nn_relu_bwd");
 }
 
-int pf3d_nn_bias_grad(int64_t rows, int C, const float *dy, float *db, void *stream)
+int pf3d_nn_bias_grad(int64_t rows, int C, const float *dy, float *db, float *work,
+                      int64_t work_floats, void *stream)
 {
-    if (rows <= 0 || C <= 0 || !dy || !db) return pf3d_fail(PF3D_E_ARG, "pf3d_nn_bias_grad");
-    hipLaunchKernelGGL(bias_grad, dim3((C + 63) / 64), dim3(256), 0, (hipStream_t)stream, rows, C,
-                       dy, db);
+    if (rows <= 0 || C <= 0 || !dy || !db || work_floats < 0 || (work_floats && !work))
+        return pf3d_fail(PF3D_E_ARG, "pf3d_nn_bias_grad");
+    if (work_floats < C)
+        return pf3d_fail(PF3D_E_ARG, "pf3d_nn_bias_grad: work must hold at least C floats");
+    // chunks of >= 256 rows, at most 1024 of them, as many as the work holds
+    int64_t nchunk = (rows + 255) / 256;
+    if (nchunk > 1024) nchunk = 1024;
+    if (nchunk > work_floats / C) nchunk = work_floats / C;
+    const int64_t chunk = (rows + nchunk - 1) / nchunk;
+    nchunk = (rows + chunk - 1) / chunk;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(bias_part, dim3((unsigned)nchunk, (C + 63) / 64), dim3(256), 0, s, rows, C,
+                       chunk, dy, work);
+    hipLaunchKernelGGL(bias_sum, dim3((C + 3) / 4), dim3(256), 0, s, (int)nchunk, C, work, db);
     return pf3d_hip_check(hipGetLastError(), "pf3d_nn_bias_grad");
 }
 

@@ -266,15 +266,40 @@ class PlateSweep:
                   self.u.data_ptr(), self.band_work.data_ptr(), int(st), s)
         self.solve(mode='fp64', rtol=rtol, max_iter=polish_iter, x0=True, stream=stream)
 
-    def autotune_direct(self, log=None, repeats=2):
-        """Time every launch configuration of the banded solver that fits this
-        sweep's largest bandwidth (factor threads x LDS panel width, solve
-        threads) on the current designs and keep the fastest for
-        solve_direct / step('direct') / capture('direct').  Returns the
-        table of timings."""
-        from .autotune import tune
+    def autotune_key(self):
+        """Autotune table key of this sweep's banded solver: Quad4 plates,
+        panel class of the widest band, power-of-two buckets of the number
+        of variants and of the mean DOF per variant, GPU architecture."""
+        from . import autotune as AT
         if not hasattr(self, 'band'):
             self._band_setup()
+        bmax = int(self.band_bw_host.max())
+        panel = min(m for m in (256, 448, 576) if m >= bmax)
+        bucket = (panel, AT.size_bucket(self.n_var),
+                  AT.size_bucket(int(self.band_n_host.mean())))
+        return AT.make_key('band_direct', 'quad4', bucket, AT.arch())
+
+    def autotune_direct(self, log=None, repeats=2, use_table=True):
+        """Launch configuration of the banded solver (factor threads x LDS
+        panel width, solve threads) for solve_direct / step('direct') /
+        capture('direct'): the stored choice for this sweep's key if there is
+        one (use_table), else every configuration that fits the sweep's
+        widest band timed on the current designs, the fastest kept and
+        recorded.  Returns the table of timings (stored or measured)."""
+        from . import autotune as AT
+        if not hasattr(self, 'band'):
+            self._band_setup()
+        key = self.autotune_key()
+        if use_table:
+            entry = AT.lookup(key)
+            if entry is not None:
+                c = entry['config']
+                self.direct_config = (int(c['ft']), int(c['mb']), int(c['st']))
+                self.direct_config_source = 'table'
+                if log:
+                    log('autotune %s: stored choice %s' % (key, self.direct_config))
+                return [({k: v for k, v in t.items() if k != 'ms'}, t['ms'])
+                        for t in entry.get('timings', [])]
         bmax = int(self.band_bw.max())
         panels = [m for m in (256, 448, 576) if m >= bmax]
 
@@ -283,10 +308,12 @@ class PlateSweep:
 
         # factorisation shape first (it dominates), then the solve's threads
         cands = [dict(ft=ft, mb=mb, st=1024) for mb in panels for ft in (256, 512, 1024)]
-        best, table = tune(cands, run, repeats=repeats, log=log)
+        best, table = AT.tune(cands, run, repeats=repeats, log=log)
         cands = [dict(best, st=st) for st in (256, 512)]
-        best2, table2 = tune(cands + [best], run, repeats=repeats, log=log)
+        best2, table2 = AT.tune(cands + [best], run, repeats=repeats, log=log)
         self.direct_config = (best2['ft'], best2['mb'], best2['st'])
+        self.direct_config_source = 'measured'
+        AT.record(key, dict(best2), table + table2)
         return table + table2
 
     def step(self, mode='mixed', rtol=1e-8, stream=None, **kw):

@@ -141,7 +141,8 @@ class WarmStartNet:
         # dW[co][Kd] = sum_m dy[m][co] col[m][Kd]  (split-K over the pixels)
         self._gemm(co, Kd, M, dy, 1, co, col, 1, Kd, self.g[name + '.w'], Kd,
                    splits=min(self.splits, max(1, M // 256)), stream=stream)
-        _lib.call('pf3d_nn_bias_grad', M, co, dy.data_ptr(), self.g[name + '.b'].data_ptr(), st)
+        _lib.call('pf3d_nn_bias_grad', M, co, dy.data_ptr(), self.g[name + '.b'].data_ptr(),
+                  self.partial.data_ptr(), self.partial.numel(), st)
         if dx is None:
             return
         if k == 3:
@@ -248,6 +249,18 @@ class WarmStartNet:
             _lib.call('pf3d_nn_adam', p.numel(), p.data_ptr(), self.g[k].data_ptr(),
                       self.m[k].data_ptr(), self.v[k].data_ptr(), lr, b1, b2, eps,
                       self.step_count, _s(stream))
+
+    def gemm_flops(self):
+        """(forward, backward) floating-point operations of the matrix-core
+        GEMMs for the current batch: 2 M N K per product; backward computes
+        dW for every layer and dX for every layer but the first."""
+        fwd = bwd = 0
+        for name, ci, co, _, k, _ in LAYERS:
+            _, ho = self._shape(name)
+            M, Kd = self.B * ho * ho, k * k * ci
+            fwd += 2 * M * co * Kd
+            bwd += 2 * M * co * Kd * (1 if name == 'c0' else 2)
+        return fwd, bwd
 
     def params_numpy(self):
         return {k: t.detach().cpu().numpy().astype(np.float64) for k, t in self.p.items()}

@@ -192,10 +192,13 @@ def test_sweep_direct_banded_matches_spsolve():
     assert (sw.relres.cpu().numpy() <= 1e-8).all()
 
 
-def test_direct_launch_configs_and_autotune():
+def test_direct_launch_configs_and_autotune(tmp_path, monkeypatch):
     """Every launch configuration of the banded solver (threads per
     workgroup, LDS panel width) gives the same bits; the autotuner keeps
-    one of the candidates and the captured step uses it."""
+    one of the candidates, records it under the sweep's key, a second sweep
+    of the same key takes the stored choice without timing, and the
+    captured step uses it."""
+    monkeypatch.setenv('PF3D_AUTOTUNE_CACHE', str(tmp_path / 'tune.json'))
     sw, h, f = _sweep(seed=11)
     sw.assemble()
     sw.solve_direct(rtol=1e-10, polish_iter=0, config=(1024, 576, 1024))
@@ -211,8 +214,14 @@ def test_direct_launch_configs_and_autotune():
         assert torch.equal(sw.u, ref), cfg
     sw.check()
     table = sw.autotune_direct(repeats=1)
+    assert sw.direct_config_source == 'measured'
     assert sw.direct_config in [(c['ft'], c['mb'], c['st']) for c, _ in table]
     assert all(ms > 0 for _, ms in table)
+    sw2, _, _ = _sweep(seed=12)
+    table2 = sw2.autotune_direct(repeats=1)
+    assert sw2.direct_config_source == 'table'
+    assert sw2.direct_config == sw.direct_config
+    assert [c for c, _ in table2] == [c for c, _ in table]
     g = sw.capture(mode='direct', rtol=1e-10)
     sw.u.zero_()
     g.replay()
