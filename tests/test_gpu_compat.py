@@ -333,7 +333,8 @@ def test_probe_strain_and_local_forces_through_facade():
     """A plate stretched by an end traction through the façade (u held on
     x = 0, v at one node, out-of-plane DOFs held): the probe's BLexx . ue at
     each element centre is the uniform strain F / (E h b) of the exact
-    solution (quad4.pyx:272-395, as the reference's in-plane probe test);
+    solution seen along the element's axis (quad4.pyx:272-395, as the
+    reference's in-plane probe test);
     update_probe_finte gives finte = KC0ve . ue (quad4.pyx:1174-1202) with
     KC0ve the local matrix of the element's global slots; and the elements'
     internal forces rotated back balance the applied load."""
@@ -395,7 +396,11 @@ def test_probe_strain_and_local_forces_through_facade():
         q.update_probe_ue(u)
         probe.update_BL(xi=0., eta=0.)
         exx = probe.BLexx @ probe.ue
-        assert abs(exx - exx_exact) <= 1e-8 * exx_exact, (exx, exx_exact)
+        # the probe's strain is along the element's own x axis e1 (the
+        # first column of its rotation): exx e1x^2 + eyy e1y^2, eyy = -nu exx
+        e1 = q._rot()[:, 0]
+        exx_local = exx_exact * (e1[0] ** 2 - 0.3 * e1[1] ** 2)
+        assert abs(exx - exx_local) <= 1e-8 * exx_exact, (exx, exx_local)
         q.update_probe_finte(prop)
         fin = probe.KC0ve.reshape(24, 24) @ probe.ue
         assert np.allclose(probe.finte, fin, rtol=0, atol=1e-10 * np.abs(fin).max())
