@@ -420,16 +420,24 @@ def panel_rates(n, log, steps=10, maxiter=200000):
     amg_setup = time.perf_counter() - t0
     log('panel amg setup %.1f s' % amg_setup)
     for method in ('amg', 'amg32', 'fp64'):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        u, info = sysm.solve(f, method=method, rtol=1e-8, maxiter=maxiter)
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        sol[method] = dict(ms=dt * 1e3, iters=info['iters'], outer=info['outer'],
-                           relres=info['relres'], converged=info['converged'])
+        # multigrid: the first solve also captures the CG iteration's graph
+        # and pays first-use costs; the second (same matrix, same load) is
+        # the repeated-solve time reported as ms
+        first = None
+        for rep in range(2 if method != 'fp64' else 1):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            u, info = sysm.solve(f, method=method, rtol=1e-8, maxiter=maxiter)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            log('panel solve %s: %.1f ms, %s' % (method, dt * 1e3, info))
+            if first is None:
+                first = dt
+        sol[method] = dict(ms=dt * 1e3, first_ms=first * 1e3, iters=info['iters'],
+                           outer=info['outer'], relres=info['relres'],
+                           converged=info['converged'])
         if info['converged']:
             sol[method]['solves_per_s'] = 1. / dt
-        log('panel solve %s: %.1f ms, %s' % (method, dt * 1e3, info))
     sol['amg']['setup_s'] = amg_setup
     out['solve'] = sol
     gk, totk = p.groups('KG')

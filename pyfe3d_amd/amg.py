@@ -28,6 +28,7 @@ the CG around it, in FP64, uses the flexible (Polak-Ribiere) beta.
 """
 import ctypes
 import time
+import weakref
 
 import numpy as np
 import scipy.sparse as ss
@@ -329,18 +330,18 @@ class SAAMG:
         launches per CG iteration."""
         n = self.levels[0]['n'] if self.levels else self.nc
         dev = self.coarse_inv.device
-        _collect_before_capture()
-        self.g_in = torch.zeros(n, dtype=torch.float64, device=dev)
-        self.g_out = torch.zeros(n, dtype=torch.float64, device=dev)
-        s = torch.cuda.Stream(device=dev)
-        s.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(s):
-            self._apply(self.g_in, self.g_out)      # warm-up outside the graph
-        torch.cuda.current_stream().wait_stream(s)
-        torch.cuda.synchronize()
-        g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g):
-            self._apply(self.g_in, self.g_out)
+        with _capture_guard():
+            self.g_in = torch.zeros(n, dtype=torch.float64, device=dev)
+            self.g_out = torch.zeros(n, dtype=torch.float64, device=dev)
+            s = torch.cuda.Stream(device=dev)
+            s.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(s):
+                self._apply(self.g_in, self.g_out)      # warm-up outside the graph
+            torch.cuda.current_stream().wait_stream(s)
+            torch.cuda.synchronize()
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self._apply(self.g_in, self.g_out)
         self.graph = g
         return g
 
@@ -498,14 +499,27 @@ class SAAMG:
         return x
 
 
-def _collect_before_capture():
-    """Free unreachable objects now: a CUDAGraph held in a reference cycle
-    (a system and its cached CG iteration refer to each other) would
-    otherwise be destroyed by the cyclic collector at some allocation made
-    during a later capture, which HIP refuses while a stream captures."""
-    import gc
-    torch.cuda.synchronize()
-    gc.collect()
+class _capture_guard:
+    """Context for a graph capture: the device synchronised first, and
+    Python's cyclic collector paused while the stream captures.  A CUDAGraph
+    that became garbage inside a reference cycle would otherwise be
+    destroyed by the collector at some allocation made during the capture,
+    which HIP refuses while a stream captures; paused, such garbage is
+    collected after the capture instead (and without the cost of a full
+    collection before every capture)."""
+
+    def __enter__(self):
+        import gc
+        torch.cuda.synchronize()
+        self._was = gc.isenabled()
+        gc.disable()
+        return self
+
+    def __exit__(self, *exc):
+        import gc
+        if self._was:
+            gc.enable()
+        return False
 
 
 def _ptr(t):
@@ -552,13 +566,21 @@ class _CGIteration:
         n = 6 * system.nb
         dev = system.val.device
         zeros = lambda m: torch.zeros(m, dtype=torch.float64, device=dev)
-        self.system, self.amg, self.n = system, amg, n
+        # the system caches this object: a weak reference back keeps the
+        # pair out of a reference cycle, so the captured graph is freed with
+        # the system
+        self._system = weakref.ref(system)
+        self.amg, self.n = amg, n
         self.flexible = amg.precision != 'fp64'
         self.x, self.r, self.p, self.q, self.z = (zeros(n) for _ in range(5))
         self.zold = zeros(n) if self.flexible else None
         self.s = zeros(8)           # r.z, p.q, -, r.z new, beta, r.r, r.z_old
         self.work = zeros(1024)
         self.graph = None
+
+    @property
+    def system(self):
+        return self._system()
 
     def _dot_into(self, a, b, k):
         _lib.call('pf3d_dot', self.n, a.data_ptr(), b.data_ptr(), self.s[k:].data_ptr(),
@@ -584,17 +606,17 @@ class _CGIteration:
                   self.p.data_ptr(), _s())
 
     def capture(self):
-        _collect_before_capture()
-        s = torch.cuda.Stream(device=self.x.device)
-        s.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(s):
-            self.step()                 # warm-up outside the graph (zero buffers: no-op)
-        torch.cuda.current_stream().wait_stream(s)
-        torch.cuda.synchronize()
-        g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g, stream=s):
-            self.step()
-        torch.cuda.synchronize()
+        with _capture_guard():
+            s = torch.cuda.Stream(device=self.x.device)
+            s.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(s):
+                self.step()                 # warm-up outside the graph (zero buffers: no-op)
+            torch.cuda.current_stream().wait_stream(s)
+            torch.cuda.synchronize()
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g, stream=s):
+                self.step()
+            torch.cuda.synchronize()
         self.graph = g
 
     def restart(self):
@@ -604,7 +626,7 @@ class _CGIteration:
         self._dot_into(self.r, self.z, 0)
 
 
-def amg_pcg(system, amg, f, rtol=1e-8, maxiter=2000, x0=None, log=None, stall=100, check=None,
+def amg_pcg(system, amg, f, rtol=1e-8, maxiter=2000, x0=None, log=None, stall=400, check=None,
             graph=True):
     """PCG with one V-cycle per iteration on a StaticSystem's matrix.
     Returns (u, info) like StaticSystem.solve (compensated true residual).
@@ -613,7 +635,11 @@ def amg_pcg(system, amg, f, rtol=1e-8, maxiter=2000, x0=None, log=None, stall=10
     DOF, where a host round trip per iteration would cost more than the
     iteration's kernels, else 1); a run whose recursive residual has not
     halved in `stall` iterations restarts from the true residual (and ends
-    when a restart does not gain 10 %)."""
+    when a restart does not gain 10 %).  The window is long because CG's
+    residual norm is not monotone: on plates with thickness fields it rises
+    up to ~10x and takes a few hundred iterations to come back (a 100-
+    iteration window stopped 9 of 16 such plates, which then converge in
+    240-760 iterations)."""
     b = system.rhs(f)
     if check is None:
         check = 8 if 6 * system.nb < 1000000 else 1

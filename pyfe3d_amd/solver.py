@@ -217,9 +217,9 @@ class StaticSystem:
                 min(budget, int(fp32_maxiter))
             # the FP32 V-cycle gives out near the FP64 residual floor: leave
             # it after 25 iterations without halving the residual (the FP64
-            # V-cycle then finishes), not the 100 the FP64 one is allowed
+            # V-cycle then finishes), not the 400 the FP64 one is allowed
             u, info = amg_pcg(self, getattr(self, name), f, rtol=rtol, maxiter=first, x0=x0,
-                              stall=25 if method == 'amg32' else 100)
+                              stall=25 if method == 'amg32' else 400)
             if method == 'amg32':
                 info['fp64_iters'] = 0
                 if (info['relres'] > rtol and info['iters'] < budget

@@ -326,17 +326,17 @@ class PlateSweep:
     def capture(self, mode='mixed', rtol=1e-8, **kw):
         """Capture :meth:`step` in a graph; replay with ``self.graph.replay()``
         after new thicknesses (:meth:`set_thickness`) or loads (:meth:`set_loads`)."""
-        from .amg import _collect_before_capture
-        _collect_before_capture()
-        s = torch.cuda.Stream(device=self.device)
-        s.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(s):
-            self.step(mode=mode, rtol=rtol, stream=s, **kw)  # warm-up outside the graph
-        torch.cuda.current_stream().wait_stream(s)
-        torch.cuda.synchronize()
-        g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g):
-            self.step(mode=mode, rtol=rtol, **kw)
+        from .amg import _capture_guard
+        with _capture_guard():
+            s = torch.cuda.Stream(device=self.device)
+            s.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(s):
+                self.step(mode=mode, rtol=rtol, stream=s, **kw)  # warm-up outside the graph
+            torch.cuda.current_stream().wait_stream(s)
+            torch.cuda.synchronize()
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self.step(mode=mode, rtol=rtol, **kw)
         self.graph = g
         return g
 

@@ -210,3 +210,26 @@ def test_amg32_finishes_with_fp64_vcycle():
     # a budget too small to converge is reported, not hidden
     _, short = sysm.solve(f, method='amg32', rtol=1e-10, fp32_maxiter=4, maxiter=10)
     assert short['iters'] <= 10 and not short['converged']
+
+
+def test_amg_cg_on_thickness_field_plates():
+    """Clamped 64 x 64 plates with smooth thickness fields (0.2-3 mm) under
+    pressure patches, the warm-start workload: CG's residual norm rises up
+    to ~10x for a few hundred iterations before it falls, so a stall window
+    of 100 iterations stopped three of these four plates short (relres
+    3-9); each converges to 1e-8 within the default window."""
+    from pyfe3d_amd import warmstart as W
+    from pyfe3d_amd.solver import StaticSystem
+    from pyfe3d_amd.sweep import PlateSweep
+    h, p, f = W._plates(4, 64, 12345 * 1000003)
+    sw = PlateSweep([(64, 64)], a=1.0, b=1.0)
+    for v in range(4):
+        sw.set_thickness(h[v].reshape(-1))
+        sw.set_loads(W._dof_loads(f[v:v + 1]))
+        sw.assemble()
+        sysm = StaticSystem(sw.plan, sw.val, sw.fixed)
+        sysm.setup_amg(sw.x.view(-1, 3).cpu().numpy(), precisions=('fp64',))
+        u, info = sysm.solve(sw.f, method='amg', rtol=1e-8)
+        assert info['converged'] and info['relres'] <= 1e-8, (v, info)
+        assert info['iters'] < 2000, (v, info)
+        del sysm
