@@ -170,6 +170,56 @@ __device__ __forceinline__ v4d tile_update(const TileAt &q, v4d a, const double 
     return a;
 }
 
+// The jn x jn diagonal block held in P rows 0 .. 31, factorised in place
+// by one wave (L11 and the reciprocal pivots dinv; *failed set on a
+// non-positive pivot).  Lane (h, r) = (lane / 32, lane % 32) holds columns
+// 16 h .. 16 h + 15 of row r.  Step k takes the pivot by readlane and
+// broadcasts the multipliers L[r][k], k < r < jn, through LDS (zero
+// elsewhere, two buffers so step k + 1 never overwrites what step k reads);
+// the rank-1 update then runs on all 16 columns without masks: the lower
+// triangle gets exactly its terms, the upper triangle and rows past jn hold
+// values that are never broadcast.
+__device__ __forceinline__ void diag_block(double (*P)[NF + 1], double *dinv, double (*Lk)[NF],
+                                           int jn, int lane, bool &failed)
+{
+    int r = lane & 31, h = lane >> 5;
+    // opaque per call, so the per-step lane masks and column offsets below
+    // are formed where they are used instead of being kept live across
+    // panels
+    asm volatile("" : "+v"(r), "+v"(h));
+    double l[NH];
+#pragma unroll
+    for (int i = 0; i < NH; i++) l[i] = P[r][NH * h + i];
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+        if (k < jn) {
+            const int kh = k / NH, ki = k % NH;
+            const double d = readlane_d(l[ki], 32 * kh + k);
+            if (!(d > 0.)) failed = true;
+            const double sq = d > 0. ? sqrt(d) : 1.;
+            const double inv = 1. / sq;
+            if (h == kh) {
+                l[ki] = r == k ? sq : l[ki] * inv;
+                Lk[k & 1][r] = (r > k && r < jn) ? l[ki] : 0.;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            const double lr = Lk[k & 1][r];
+#pragma unroll
+            for (int i = 0; i < NH; i++) l[i] -= lr * Lk[k & 1][NH * h + i];
+        }
+    }
+    // rows jn .. 31 exist in P only when jn = 32; the upper
+    // triangle of the block is never read
+#pragma unroll
+    for (int i = 0; i < NH; i++) P[r][NH * h + i] = l[i];
+    double dg = 1.;
+#pragma unroll
+    for (int i = 0; i < NH; i++)
+        if (NH * h + i == r) dg = l[i];
+    if (r < jn && (r >> 4) == h) dinv[r] = 1. / dg;
+}
+
 template <int BT, int MB>
 __global__ void __launch_bounds__(BT) band_chol(BandSys s, int *__restrict__ bad)
 {
@@ -190,9 +240,11 @@ __global__ void __launch_bounds__(BT) band_chol(BandSys s, int *__restrict__ bad
         return;
     }
     bool failed = false;
+    bool ahead = false;  // this panel's diagonal block was factorised ahead
     for (int64_t j0 = 0; j0 < n; j0 += NF) {
         const int jn = (int)(n - j0 < NF ? n - j0 : NF);
         const int rows = (int)(n - j0 < b + jn ? n - j0 : b + jn);
+        const int skip = ahead ? jn : 0;  // rows of P already holding L11
         // 1. panel: P[r][c] = A[j0 + r][j0 + c], r >= c, r - c <= b (zero
         //    elsewhere); wave w takes columns w, w + NW, ..., 64 rows per
         //    load, every load of a column in flight before its LDS stores
@@ -202,62 +254,21 @@ __global__ void __launch_bounds__(BT) band_chol(BandSys s, int *__restrict__ bad
 #pragma unroll
             for (int q = 0; q < RCH; q++) {
                 const int r = 64 * q + lane;
-                x[q] = (c < jn && r >= c && r < rows && r - c <= b) ? col[r - c] : 0.;
+                x[q] = (c < jn && r >= c && r >= skip && r < rows && r - c <= b) ? col[r - c] : 0.;
             }
 #pragma unroll
             for (int q = 0; q < RCH; q++) {
                 const int r = 64 * q + lane;
-                if (r < rows) P[r][c] = x[q];
+                if (r >= skip && r < rows) P[r][c] = x[q];
             }
         }
         __syncthreads();
-        // 2. diagonal block, by wave 0 in registers: lane (h, r) =
-        //    (lane / 32, lane % 32) holds columns 16 h .. 16 h + 15 of row r.
-        //    Step k takes the pivot by readlane and broadcasts the
-        //    multipliers L[r][k], k < r < jn, through LDS (zero elsewhere,
-        //    two buffers so step k + 1 never overwrites what step k reads);
-        //    the rank-1 update then runs on all 16 columns without masks:
-        //    the lower triangle gets exactly its terms, the upper triangle
-        //    and rows past jn hold values that are never broadcast.
-        if (wave == 0) {
-            int r = lane & 31;
-            // opaque per panel, so the per-step lane masks below are formed
-            // where they are used instead of being kept live across panels
-            asm volatile("" : "+v"(r));
-            const int h = lane >> 5;
-            double l[NH];
-#pragma unroll
-            for (int i = 0; i < NH; i++) l[i] = P[r][NH * h + i];
-#pragma unroll
-            for (int k = 0; k < NF; k++) {
-                if (k < jn) {
-                    const int kh = k / NH, ki = k % NH;
-                    const double d = readlane_d(l[ki], 32 * kh + k);
-                    if (!(d > 0.)) failed = true;
-                    const double sq = d > 0. ? sqrt(d) : 1.;
-                    const double inv = 1. / sq;
-                    if (h == kh) {
-                        l[ki] = r == k ? sq : l[ki] * inv;
-                        Lk[k & 1][r] = (r > k && r < jn) ? l[ki] : 0.;
-                    }
-                    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-                    __builtin_amdgcn_wave_barrier();
-                    const double lr = Lk[k & 1][r];
-#pragma unroll
-                    for (int i = 0; i < NH; i++) l[i] -= lr * Lk[k & 1][NH * h + i];
-                }
-            }
-            // rows jn .. 31 exist in P only when jn = 32; the upper
-            // triangle of the block is never read
-#pragma unroll
-            for (int i = 0; i < NH; i++) P[r][NH * h + i] = l[i];
-            double dg = 1.;
-#pragma unroll
-            for (int i = 0; i < NH; i++)
-                if (NH * h + i == r) dg = l[i];
-            if (r < jn && (r >> 4) == h) dinv[r] = 1. / dg;
+        // 2. the diagonal block by wave 0 (unless the previous panel step
+        //    factorised it ahead, see 4b)
+        if (!ahead) {
+            if (wave == 0) diag_block(P, dinv, Lk, jn, lane, failed);
+            __syncthreads();
         }
-        __syncthreads();
         // 3. rows below the diagonal block: x L11^T = a, by columns: x[c] is
         //    final once scaled, then its term goes into every later x[c2].
         //    Each x[c2] receives its terms in the order c = 0, 1, ... as in
@@ -297,29 +308,42 @@ __global__ void __launch_bounds__(BT) band_chol(BandSys s, int *__restrict__ bad
         //     v_mfma_f64_16x16x4 over c = 0 .. 31 in order.  Lane l supplies
         //     operand row l % 16 at column 4 s + l / 16 and holds
         //     D[l / 16 + 4 v][l % 16], v = 0 .. 3 (band entries of column rk).
+        // Look-ahead: the next panel's diagonal block is window rows and
+        // columns jn .. jn + 31, i.e. tiles 0, 1, 2.  Wave 0 takes those
+        // three tiles, keeps their values in P rows 0 .. 31 (free once the
+        // write-back above is done) and factorises the block there while
+        // the other waves update the rest of the window; the next step
+        // then keeps those rows and skips its diagonal step.
+        const int64_t j0n = j0 + jn;
+        const int jnn = (int)(n - j0n < NF ? n - j0n : NF);  // next panel's width
+        const bool la = jnn > 0;
+        if (la) __syncthreads();
         const int T = rows - jn;
         const int nt = (T + 15) >> 4;
         const int ntiles = nt * (nt + 1) / 2;
         const int li = lane & 15, lk = lane >> 4;
+        double *Lp = L + j0 * ld;
         // each wave loads its next tile before it multiplies and stores the
         // current one (tiles are disjoint); two register sets, so the
         // prefetched tile is never copied, and a wave with no next tile
         // reloads its last one, so the load counter's order is the same on
-        // every path and the multiply waits only for its own tile
-        double *Lp = L + j0 * ld;
-        int t = __builtin_amdgcn_readfirstlane(wave);
-        if (t < ntiles) {
+        // every path and the multiply waits only for its own tile.  Without
+        // look-ahead wave w takes tiles w, w + NW, ...; with it waves 1 ..
+        // NW - 1 take tiles 3, 4, ... round robin.
+        const int tstride = la ? NW - 1 : NW;
+        int t = __builtin_amdgcn_readfirstlane(la ? 2 + wave : wave);
+        if (!(la && wave == 0) && t < ntiles) {
             TileAt q0 = tile_at(t, jn, li, lk), q1;
             v4d a0 = tile_load(q0, Lp, ld, rows), a1;
             for (;;) {
-                const int t1 = t + NW;
+                const int t1 = t + tstride;
                 q1 = tile_at(min(t1, ntiles - 1), jn, li, lk);
                 a1 = tile_load(q1, Lp, ld, rows);
                 __builtin_amdgcn_sched_barrier(0);  // loads issue before the multiply
                 a0 = tile_update<PR>(q0, a0, P, lk);
                 tile_store(q0, a0, Lp, ld, rows);
                 if (t1 >= ntiles) break;
-                const int t2 = t1 + NW;
+                const int t2 = t1 + tstride;
                 q0 = tile_at(min(t2, ntiles - 1), jn, li, lk);
                 a0 = tile_load(q0, Lp, ld, rows);
                 __builtin_amdgcn_sched_barrier(0);
@@ -329,7 +353,39 @@ __global__ void __launch_bounds__(BT) band_chol(BandSys s, int *__restrict__ bad
                 t = t2;
             }
         }
+        if (la && wave == 0) {
+            // the next block from the band (entries outside this window are
+            // untouched by this step; the window's are replaced below)
+            const int rown = (int)(n - j0n < b + jnn ? n - j0n : b + jnn);
+            const double *Ln = L + j0n * ld;
+            int r = lane & 31, h = lane >> 5;
+            asm volatile("" : "+v"(r), "+v"(h));
+            double x[NF / 2];
+#pragma unroll
+            for (int q = 0; q < NF / 2; q++) {
+                const int c = 2 * q + h;
+                x[q] = (c < jnn && r >= c && r < rown && r - c <= b) ? Ln[c * ld + (r - c)] : 0.;
+            }
+#pragma unroll
+            for (int q = 0; q < NF / 2; q++) P[r][2 * q + h] = x[q];
+            // this step's update of the block: tiles 0 (0, 0), 1 (1, 0),
+            // 2 (1, 1), also written back to the band
+            const int nmine = ntiles < 3 ? ntiles : 3;
+            for (int tt = 0; tt < nmine; tt++) {
+                const TileAt q = tile_at(tt, jn, li, lk);
+                v4d a = tile_load(q, Lp, ld, rows);
+                a = tile_update<PR>(q, a, P, lk);
+                tile_store(q, a, Lp, ld, rows);
+#pragma unroll
+                for (int v4 = 0; v4 < 4; v4++) {
+                    const int ri = q.ri0 + 4 * v4;
+                    if (q.rb < rows && ri < rows && ri >= q.rb) P[ri - jn][q.rb - jn] = a[v4];
+                }
+            }
+            diag_block(P, dinv, Lk, jnn, lane, failed);
+        }
         __syncthreads();
+        ahead = la;
     }
     if (failed && lane == 0) atomicAdd(bad, 1);
 }
