@@ -89,8 +89,8 @@ class PlateSweep:
         self.v = torch.empty(576 * self.n_elem, dtype=torch.float64, device=dev)
         self.val = torch.empty(nnzb * 36, dtype=torch.float64, device=dev)
         self.val32 = torch.empty(nnzb * 36, dtype=torch.float32, device=dev)
-        self.dinv = torch.empty(self.n_nodes * 36, dtype=torch.float64, device=dev)
-        self.dinv32 = torch.empty(self.n_nodes * 36, dtype=torch.float32, device=dev)
+        self.dinv = torch.zeros(self.n_nodes * 36, dtype=torch.float64, device=dev)
+        self.dinv32 = torch.zeros(self.n_nodes * 36, dtype=torch.float32, device=dev)
         self.singular = torch.zeros(1, dtype=torch.int32, device=dev)
         self.props = torch.empty((self.n_elem, A.SHELL_STRIDE), dtype=torch.float64, device=dev)
         unit = isotropic_plate(1.0, E, nu, rho=rho).record()
@@ -137,9 +137,10 @@ class PlateSweep:
         self.f.mul_(1 - self.fixed.to(torch.float64))
 
     # ------------------------------------------------------------- the step
-    def assemble(self, stream=None):
+    def assemble(self, stream=None, precond=True):
         """Property records from h -> KC0 of every variant -> BSR ->
-        Dirichlet -> block Jacobi -> FP32 copies."""
+        Dirichlet -> (precond: block Jacobi -> FP32 copies, which only the
+        PCG modes read)."""
         s = _s(stream)
         _lib.call('pf3d_isotropic_records', self.n_elem, self.h.data_ptr(),
                   self._unit.ctypes.data, self.props.data_ptr(), s)
@@ -149,6 +150,14 @@ class PlateSweep:
         self.plan.apply(self.v, out=self.val, stream=stream)
         _lib.call('pf3d_bsr_dirichlet', self.n_nodes, pr, pc, self.val.data_ptr(),
                   self.fixed.data_ptr(), s)
+        if precond:
+            self.precondition(stream=stream)
+
+    def precondition(self, stream=None):
+        """Block-Jacobi inverses (FP64 and FP32) and the FP32 copy of the
+        assembled values, for the PCG modes."""
+        s = _s(stream)
+        pr, pc = self.plan.row_ptr.data_ptr(), self.plan.col_idx.data_ptr()
         _lib.call('pf3d_bjacobi', self.n_nodes, pr, pc, self.val.data_ptr(),
                   self.dinv.data_ptr(), self.dinv32.data_ptr(), self.singular.data_ptr(), s)
         _lib.call('pf3d_to_f32', self.val.numel(), self.val.data_ptr(), self.val32.data_ptr(), s)
@@ -249,10 +258,11 @@ class PlateSweep:
                   self.band_pinv.data_ptr(), self.band_bad.data_ptr(), int(ft), int(mb),
                   _s(stream))
 
-    def solve_direct(self, rtol=1e-8, polish_iter=2000, config=None, stream=None):
+    def solve_direct(self, rtol=1e-8, polish_iter=0, config=None, stream=None):
         """Banded Cholesky of every variant (one workgroup each), forward and
-        back substitution, then the FP64 true residual of each variant
-        (PCG polish from the direct solution if it misses rtol).  config:
+        back substitution, then the FP64 true residual of each variant into
+        self.relres (and, with polish_iter > 0, block-Jacobi PCG from the
+        direct solution for any variant that misses rtol).  config:
         (factor threads, panel bandwidth, solve threads), default the one
         chosen by :meth:`autotune_direct`, else (1024, 576, 1024)."""
         if not hasattr(self, 'band'):
@@ -260,6 +270,8 @@ class PlateSweep:
         s = _s(stream)
         ft, mb, st = config or getattr(self, 'direct_config', None) or (1024, 576, 1024)
         self.factor_direct(config=(ft, mb, st), stream=stream)
+        if polish_iter > 0:
+            self.precondition(stream=stream)
         _lib.call('pf3d_band_solve', self.n_var, self.order.data_ptr(),
                   self.band_dof0.data_ptr(), self.band_bw.data_ptr(), self.band_off.data_ptr(),
                   self.band.data_ptr(), self.band_pnode.data_ptr(), self.f.data_ptr(),
@@ -317,7 +329,7 @@ class PlateSweep:
         return table + table2
 
     def step(self, mode='mixed', rtol=1e-8, stream=None, **kw):
-        self.assemble(stream=stream)
+        self.assemble(stream=stream, precond=mode != 'direct')
         if mode == 'direct':
             self.solve_direct(rtol=rtol, stream=stream, **kw)
         else:
