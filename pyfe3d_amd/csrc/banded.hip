@@ -394,7 +394,8 @@ __global__ void __launch_bounds__(BT) band_chol(BandSys s, int *__restrict__ bad
 // x <- A^-1 f for system v: permute f into y, L y = g, L^T x = y, scatter.
 // Blocks of NS = 32 columns: the block's diagonal triangle of L is staged
 // in LDS by all threads, wave 0 substitutes through it (lane c holds
-// unknown c; values travel by shuffle), then all threads apply the block
+// unknown c and finishes it with its pivot's reciprocal; values travel by
+// readlane), then all threads apply the block
 // to the rows below it (forward) or gather the rows below into it
 // (backward, one wave per column, fixed-order shuffle reduction).
 constexpr int NS = 32;
@@ -435,11 +436,13 @@ __global__ void __launch_bounds__(BT) band_solve(BandSys s, const int64_t *__res
         stage(j0, jn);
         __syncthreads();
         if (wave == 0) {
+            // lane c finishes unknown c with the reciprocal of its pivot
+            // (formed by every lane at once), readlane hands it to all
+            const double rd = lane < jn ? 1. / T[lane & 31][lane & 31] : 0.;
             double yc = lane < jn ? y[j0 + lane] : 0.;
             for (int c = 0; c < jn; c++) {
-                const double piv = lane == c ? yc / T[c][c] : 0.;
-                const double yv = __shfl(piv, c, 64);
-                if (lane == c) yc = yv;
+                if (lane == c) yc *= rd;
+                const double yv = readlane_d(yc, c);
                 if (lane > c && lane < jn) yc -= T[lane][c] * yv;
             }
             if (lane < jn) {
@@ -483,15 +486,14 @@ __global__ void __launch_bounds__(BT) band_solve(BandSys s, const int64_t *__res
         __syncthreads();
         if (wave == 0) {
             // back substitution in the diagonal triangle, lane c holds x[j0+c]
+            const double rd = lane < jn ? 1. / T[lane & 31][lane & 31] : 0.;
             double rhs = lane < jn ? y[j0 + lane] - part[lane] : 0.;
-            double xc = 0.;
             for (int cc = jn - 1; cc >= 0; cc--) {
-                const double piv = lane == cc ? rhs / T[cc][cc] : 0.;
-                const double xv = __shfl(piv, cc, 64);
-                if (lane == cc) xc = xv;
+                if (lane == cc) rhs *= rd;
+                const double xv = readlane_d(rhs, cc);
                 if (lane < cc) rhs -= T[cc][lane] * xv;
             }
-            if (lane < jn) y[j0 + lane] = xc;
+            if (lane < jn) y[j0 + lane] = rhs;
         }
         __syncthreads();
     }
