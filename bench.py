@@ -260,8 +260,8 @@ def sweep_rates(n_var, log, steps=2, modes=('direct', 'fp64', 'mixed')):
         sw.assemble()
         t0 = time.perf_counter()
         table = sw.autotune_direct(log=log, repeats=1)
-        tuned = dict(config=dict(zip(('factor_threads', 'panel_bandwidth', 'solve_threads'),
-                                     sw.direct_config)),
+        tuned = dict(config=dict(zip(('factor_threads', 'panel_bandwidth', 'solve_threads',
+                                      'narrow_band_factor_threads'), sw.direct_config)),
                      source=sw.direct_config_source, key=sw.autotune_key(),
                      tuning_s=time.perf_counter() - t0,
                      candidates=[dict(c, ms=round(ms, 3)) for c, ms in table])
@@ -619,12 +619,13 @@ def profile_kc0(n, launches=3):
           flush=True)
 
 
-def profile_sweep(n_var, steps=3, config=(512, 448, 512)):
+def profile_sweep(n_var, steps=3, config=(512, 448, 512, 256)):
     """`steps` direct-solve steps of the n_var-plate design sweep (records,
     KC0, BSR, Dirichlet, band fill, banded Cholesky, triangular solves) with
-    the launch shape given (factor threads, panel bandwidth, solve threads;
-    default the one the autotuner picks for the 4,096-plate sweep), for
-    kernel traces and counter runs."""
+    the launch shape given (factor threads, panel bandwidth, solve threads,
+    factor threads of the plates whose band fits the 256-wide panel; default
+    the one the autotuner picks for the 4,096-plate sweep), for kernel
+    traces and counter runs."""
     import torch
     from pyfe3d_amd.sweep import PlateSweep
     sw = PlateSweep(sweep_dims(n_var))

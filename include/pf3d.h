@@ -598,6 +598,24 @@ int pf3d_band_factor(int64_t n_var, int64_t n_nodes, const int32_t *order, const
                      const int32_t *var, const int64_t *pinv, int *bad, int threads, int max_b,
                      void *stream);
 
+/* The two halves of pf3d_band_factor, for callers that factorise groups
+ * of systems with different launch configurations (a design sweep whose
+ * narrow-band plates fit a narrower LDS panel, and so more workgroups per
+ * CU, than its widest): pf3d_band_load zeroes band and copies every
+ * system's entries into it (arguments as pf3d_band_factor); pf3d_band_chol
+ * factorises the n_sys systems order[0 .. n_sys) in place (order null:
+ * systems 0 .. n_sys - 1), counting failures and refusals in *bad as
+ * pf3d_band_factor.
+ * Same replaced reference step: the factorisation inside spsolve
+ * (tests/test_quad4_static_point_load.py:102). */
+int pf3d_band_load(int64_t n_nodes, const int64_t *dof0, const int32_t *bw, const int64_t *boff,
+                   int64_t band_size, double *band, const int32_t *row_ptr,
+                   const int32_t *col_idx, const double *val, const int32_t *var,
+                   const int64_t *pinv, void *stream);
+int pf3d_band_chol(int64_t n_sys, const int32_t *order, const int64_t *dof0, const int32_t *bw,
+                   const int64_t *boff, double *band, int *bad, int threads, int max_b,
+                   void *stream);
+
 /* x = A^-1 f per system with the factors of pf3d_band_factor; pnode[p] is
  * the original node of permuted node p; work: one double per DOF; threads
  * per workgroup 256, 512 or 1024 (0 = 1024). */

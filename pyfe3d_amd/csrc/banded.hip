@@ -423,23 +423,37 @@ __global__ void __launch_bounds__(BT) band_solve(BandSys s, const int64_t *__res
         const int64_t p = t / 6;
         y[t] = f[6 * pnode[base / 6 + p] + (t - 6 * p)];
     }
+    // (loads unconditional, addresses clamped into the block's columns)
     auto stage = [&](int64_t j0, int jn) {
-        for (int t = tid; t < NS * NS; t += BT) {
-            const int r = t / NS, c = t - r * NS;
-            T[r][c] = (c <= r && r < jn && r - c <= b) ? L[(j0 + c) * ld + (r - c)] : 0.;
+        static_assert((NS * NS) % BT == 0, "the stage covers the block in whole passes");
+        constexpr int PER = NS * NS / BT;
+        const double *Lb = L + j0 * ld;
+        double sv[PER];
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int t = tid + q * BT, r = t / NS, c = t % NS;
+            const bool in = c <= r && r < jn && r - c <= b;
+            sv[q] = Lb[(c < jn ? c : 0) * (int)ld + (in ? r - c : 0)];
+        }
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int t = tid + q * BT, r = t / NS, c = t % NS;
+            T[r][c] = (c <= r && r < jn && r - c <= b) ? sv[q] : 0.;
         }
     };
     __syncthreads();
     // forward: L y = g
     for (int64_t j0 = 0; j0 < n; j0 += NS) {
         const int jn = (int)(n - j0 < NS ? n - j0 : NS);
+        // the block's right-hand side, read with the stage's loads
+        const double yt = y[j0 + (lane < jn ? lane : jn - 1)];
         stage(j0, jn);
         __syncthreads();
         if (wave == 0) {
             // lane c finishes unknown c with the reciprocal of its pivot
             // (formed by every lane at once), readlane hands it to all
             const double rd = lane < jn ? 1. / T[lane & 31][lane & 31] : 0.;
-            double yc = lane < jn ? y[j0 + lane] : 0.;
+            double yc = lane < jn ? yt : 0.;
             for (int c = 0; c < jn; c++) {
                 if (lane == c) yc *= rd;
                 const double yv = readlane_d(yc, c);
@@ -453,11 +467,19 @@ __global__ void __launch_bounds__(BT) band_solve(BandSys s, const int64_t *__res
         __syncthreads();
         // rows below the block within the band
         const int64_t rend = (j0 + jn - 1 + b < n - 1) ? j0 + jn - 1 + b : n - 1;
+        // (branch-free: addresses clamped into the block's columns, masked
+        // terms enter as fma(l, 0, acc) = acc; eight loads in flight per
+        // round trip keep the registers of four workgroups per CU)
+        const double *Lb = L + j0 * ld;
         for (int64_t i = j0 + jn + tid; i <= rend; i += BT) {
+            const int di = (int)(i - j0);
             double acc = 0.;
-            for (int c = 0; c < jn; c++) {
-                const int64_t d = i - (j0 + c);
-                if (d <= b) acc += L[(j0 + c) * ld + d] * yb[c];
+#pragma unroll 8
+            for (int c = 0; c < NS; c++) {
+                const int cc = c < jn ? c : jn - 1;
+                const double l = *(const double *)((const char *)Lb +
+                                                   (unsigned)(cc * (int)ld + min(di - cc, b)) * 8u);
+                acc = fma(l, (c < jn && di - c <= b) ? yb[c] : 0., acc);
             }
             y[i] -= acc;
         }
@@ -468,6 +490,8 @@ __global__ void __launch_bounds__(BT) band_solve(BandSys s, const int64_t *__res
     for (int64_t bk = nblk - 1; bk >= 0; bk--) {
         const int64_t j0 = bk * NS;
         const int jn = (int)(n - j0 < NS ? n - j0 : NS);
+        // the block's right-hand side (its forward result), read up front
+        const double yt = y[j0 + (lane < jn ? lane : jn - 1)];
         // contributions of the solved rows below: s_c = sum_i L[i][j0+c] y[i]
         // for i in [j0+jn, j0+c+b]
         for (int c = wave; c < NS; c += NW) {
@@ -475,8 +499,16 @@ __global__ void __launch_bounds__(BT) band_solve(BandSys s, const int64_t *__res
             if (c < jn) {
                 const int64_t j = j0 + c;
                 const int64_t iend = (j + b < n - 1) ? j + b : n - 1;
-                for (int64_t i = j0 + jn + lane; i <= iend; i += 64)
-                    sacc += L[j * ld + (i - j)] * y[i];
+                const double *Lj = L + j * ld;
+                // two 64-row chunks in flight, addresses clamped, terms masked
+                for (int64_t ib = j0 + jn; ib <= iend; ib += 128) {
+                    const int64_t i1 = ib + lane, i2 = ib + 64 + lane;
+                    const int64_t c1 = i1 <= iend ? i1 : iend, c2 = i2 <= iend ? i2 : iend;
+                    const double l1 = Lj[(int)(c1 - j)], l2 = Lj[(int)(c2 - j)];
+                    const double y1 = y[c1], y2 = y[c2];
+                    sacc = fma(l1, i1 <= iend ? y1 : 0., sacc);
+                    sacc = fma(l2, i2 <= iend ? y2 : 0., sacc);
+                }
             }
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) sacc += __shfl_xor(sacc, o, 64);
@@ -487,7 +519,7 @@ __global__ void __launch_bounds__(BT) band_solve(BandSys s, const int64_t *__res
         if (wave == 0) {
             // back substitution in the diagonal triangle, lane c holds x[j0+c]
             const double rd = lane < jn ? 1. / T[lane & 31][lane & 31] : 0.;
-            double rhs = lane < jn ? y[j0 + lane] - part[lane] : 0.;
+            double rhs = lane < jn ? yt - part[lane & 31] : 0.;
             for (int cc = jn - 1; cc >= 0; cc--) {
                 if (lane == cc) rhs *= rd;
                 const double xv = readlane_d(rhs, cc);
@@ -528,6 +560,43 @@ static bool launch_chol(int threads, int max_b, unsigned grid, hipStream_t s, Ba
     return false;
 }
 
+int pf3d_band_load(int64_t n_nodes, const int64_t *dof0, const int32_t *bw, const int64_t *boff,
+                   int64_t band_size, double *band, const int32_t *row_ptr,
+                   const int32_t *col_idx, const double *val, const int32_t *var,
+                   const int64_t *pinv, void *stream)
+{
+    if (n_nodes < 0 || band_size < 0) return pf3d_fail(PF3D_E_ARG, "pf3d_band_load: negative size");
+    if (n_nodes == 0) return PF3D_OK;
+    if (!dof0 || !bw || !boff || !band || !row_ptr || !col_idx || !val || !var || !pinv)
+        return pf3d_fail(PF3D_E_ARG, "pf3d_band_load: null argument");
+    hipStream_t s = (hipStream_t)stream;
+    const unsigned zb = nb256(band_size) < 1u << 20 ? nb256(band_size) : 1u << 20;
+    hipLaunchKernelGGL(band_zero, dim3(zb), dim3(256), 0, s, band_size, band);
+    hipLaunchKernelGGL(band_fill, dim3(nb256(n_nodes * 36)), dim3(256), 0, s, n_nodes, row_ptr,
+                       col_idx, val, var, pinv, dof0, bw, boff, band);
+    return pf3d_hip_check(hipGetLastError(), "pf3d_band_load");
+}
+
+int pf3d_band_chol(int64_t n_sys, const int32_t *order, const int64_t *dof0, const int32_t *bw,
+                   const int64_t *boff, double *band, int *bad, int threads, int max_b,
+                   void *stream)
+{
+    if (threads == 0) threads = 1024;
+    if (max_b == 0) max_b = MAXB;
+    if ((threads != 256 && threads != 512 && threads != 1024) ||
+        (max_b != 256 && max_b != 448 && max_b != 576))
+        return pf3d_fail(PF3D_E_ARG, "pf3d_band_chol: threads must be 256/512/1024 and "
+                                     "max_b 256/448/576 (got %d, %d)", threads, max_b);
+    if (n_sys < 0) return pf3d_fail(PF3D_E_ARG, "pf3d_band_chol: negative size");
+    if (n_sys == 0) return PF3D_OK;
+    if (!dof0 || !bw || !boff || !band || !bad)
+        return pf3d_fail(PF3D_E_ARG, "pf3d_band_chol: null argument");
+    if (n_sys > 2147483647) return pf3d_fail(PF3D_E_ARG, "too many systems");
+    BandSys sys{n_sys, order, dof0, bw, boff, band};
+    launch_chol(threads, max_b, (unsigned)n_sys, (hipStream_t)stream, sys, bad);
+    return pf3d_hip_check(hipGetLastError(), "pf3d_band_chol");
+}
+
 int pf3d_band_factor(int64_t n_var, int64_t n_nodes, const int32_t *order, const int64_t *dof0,
                      const int32_t *bw, const int64_t *boff, int64_t band_size, double *band,
                      const int32_t *row_ptr, const int32_t *col_idx, const double *val,
@@ -545,14 +614,10 @@ int pf3d_band_factor(int64_t n_var, int64_t n_nodes, const int32_t *order, const
     if (!dof0 || !bw || !boff || !band || !row_ptr || !col_idx || !val || !var || !pinv || !bad)
         return pf3d_fail(PF3D_E_ARG, "pf3d_band_factor: null argument");
     if (n_var > 2147483647) return pf3d_fail(PF3D_E_ARG, "too many systems");
-    hipStream_t s = (hipStream_t)stream;
-    const unsigned zb = nb256(band_size) < 1u << 20 ? nb256(band_size) : 1u << 20;
-    hipLaunchKernelGGL(band_zero, dim3(zb), dim3(256), 0, s, band_size, band);
-    hipLaunchKernelGGL(band_fill, dim3(nb256(n_nodes * 36)), dim3(256), 0, s, n_nodes, row_ptr,
-                       col_idx, val, var, pinv, dof0, bw, boff, band);
-    BandSys sys{n_var, order, dof0, bw, boff, band};
-    launch_chol(threads, max_b, (unsigned)n_var, s, sys, bad);
-    return pf3d_hip_check(hipGetLastError(), "pf3d_band_factor");
+    int rc = pf3d_band_load(n_nodes, dof0, bw, boff, band_size, band, row_ptr, col_idx, val, var,
+                            pinv, stream);
+    if (rc != PF3D_OK) return rc;
+    return pf3d_band_chol(n_var, order, dof0, bw, boff, band, bad, threads, max_b, stream);
 }
 
 int pf3d_band_solve(int64_t n_var, const int32_t *order, const int64_t *dof0, const int32_t *bw,

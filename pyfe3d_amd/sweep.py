@@ -29,6 +29,10 @@ from .mesh import PlateMesh
 from .shellprop import isotropic_plate
 from .sparse import BlockPlan
 
+# LDS panel width of the narrow factorisation shape: 74 KB of LDS, so two
+# workgroups share a CU (the 448-wide panel takes one CU each)
+NARROW_PANEL = 256
+
 
 def _s(stream):
     return (stream if stream is not None else torch.cuda.current_stream()).cuda_stream
@@ -215,6 +219,14 @@ class PlateSweep:
         self.band_dof0 = torch.as_tensor(6 * self.node0, device=dev)
         self.band_work = torch.empty(6 * self.n_nodes, dtype=torch.float64, device=dev)
         self.band_bad = torch.zeros(1, dtype=torch.int32, device=dev)
+        # processing order split in two groups, largest plates first in each:
+        # bands too wide for the narrow LDS panel, then those that fit it
+        narrow = bws <= NARROW_PANEL
+        nodes = np.diff(self.node0)
+        by_size = np.argsort(-nodes, kind='stable')
+        split = np.concatenate([by_size[~narrow[by_size]], by_size[narrow[by_size]]])
+        self.band_order_split = torch.as_tensor(split.astype(np.int32), device=dev)
+        self.band_n_wide = int((~narrow).sum())
 
     def band_flops(self):
         """Floating-point operations of the banded Cholesky of every
@@ -244,19 +256,57 @@ class PlateSweep:
             total += int((t * (t + 1) // 2).sum()) * 16
         return total + 16 * int((self.band_n_host * (self.band_bw_host + 1)).sum())
 
+    def _direct_config(self, config=None):
+        """(factor threads, LDS panel width, solve threads, factor threads of
+        the plates whose band fits the narrow panel or 0) of config, else
+        of the chosen configuration, else the default."""
+        c = tuple(config or getattr(self, 'direct_config', None) or (1024, 576, 1024))
+        return c if len(c) == 4 else c + (0,)
+
     def factor_direct(self, config=None, stream=None):
         """Band storage of every variant's stiffness matrix and its banded
         Cholesky factorisation in place (the first half of
-        :meth:`solve_direct`)."""
+        :meth:`solve_direct`).  With a narrow-band thread count in the
+        configuration, the plates whose band fits the narrow LDS panel are
+        factorised by a second launch of that shape (more workgroups per CU)
+        after the wide ones."""
         if not hasattr(self, 'band'):
             self._band_setup()
-        ft, mb, _ = config or getattr(self, 'direct_config', None) or (1024, 576, 1024)
-        _lib.call('pf3d_band_factor', self.n_var, self.n_nodes, self.order.data_ptr(),
-                  self.band_dof0.data_ptr(), self.band_bw.data_ptr(), self.band_off.data_ptr(),
-                  self.band_size, self.band.data_ptr(), self.plan.row_ptr.data_ptr(),
+        ft, mb, _, sft = self._direct_config(config)
+        s = _s(stream)
+        if not sft or mb <= NARROW_PANEL or self.band_n_wide == self.n_var:
+            _lib.call('pf3d_band_factor', self.n_var, self.n_nodes, self.order.data_ptr(),
+                      self.band_dof0.data_ptr(), self.band_bw.data_ptr(),
+                      self.band_off.data_ptr(), self.band_size, self.band.data_ptr(),
+                      self.plan.row_ptr.data_ptr(), self.plan.col_idx.data_ptr(),
+                      self.val.data_ptr(), self.band_var.data_ptr(), self.band_pinv.data_ptr(),
+                      self.band_bad.data_ptr(), int(ft), int(mb), s)
+            return
+        _lib.call('pf3d_band_load', self.n_nodes, self.band_dof0.data_ptr(),
+                  self.band_bw.data_ptr(), self.band_off.data_ptr(), self.band_size,
+                  self.band.data_ptr(), self.plan.row_ptr.data_ptr(),
                   self.plan.col_idx.data_ptr(), self.val.data_ptr(), self.band_var.data_ptr(),
-                  self.band_pinv.data_ptr(), self.band_bad.data_ptr(), int(ft), int(mb),
-                  _s(stream))
+                  self.band_pinv.data_ptr(), s)
+        nw = self.band_n_wide
+        order = self.band_order_split
+
+        def chol(n_sys, ptr, threads, panel, st):
+            _lib.call('pf3d_band_chol', n_sys, ptr, self.band_dof0.data_ptr(),
+                      self.band_bw.data_ptr(), self.band_off.data_ptr(), self.band.data_ptr(),
+                      self.band_bad.data_ptr(), int(threads), int(panel), st)
+
+        # the two groups run concurrently (the narrow one on a side stream,
+        # forked from and joined back to the caller's), so the narrow-band
+        # plates fill the CUs the wide launch's tail leaves idle
+        main = stream if stream is not None else torch.cuda.current_stream()
+        if getattr(self, '_side', None) is None:
+            self._side = torch.cuda.Stream(device=self.device)
+        side = self._side
+        side.wait_stream(main)
+        chol(nw, order.data_ptr(), ft, mb, s)
+        with torch.cuda.stream(side):
+            chol(self.n_var - nw, order[nw:].data_ptr(), sft, NARROW_PANEL, side.cuda_stream)
+        main.wait_stream(side)
 
     def solve_direct(self, rtol=1e-8, polish_iter=0, config=None, stream=None):
         """Banded Cholesky of every variant (one workgroup each), forward and
@@ -268,8 +318,9 @@ class PlateSweep:
         if not hasattr(self, 'band'):
             self._band_setup()
         s = _s(stream)
-        ft, mb, st = config or getattr(self, 'direct_config', None) or (1024, 576, 1024)
-        self.factor_direct(config=(ft, mb, st), stream=stream)
+        cfg = self._direct_config(config)
+        st = cfg[2]
+        self.factor_direct(config=cfg, stream=stream)
         if polish_iter > 0:
             self.precondition(stream=stream)
         _lib.call('pf3d_band_solve', self.n_var, self.order.data_ptr(),
@@ -306,7 +357,8 @@ class PlateSweep:
             entry = AT.lookup(key)
             if entry is not None:
                 c = entry['config']
-                self.direct_config = (int(c['ft']), int(c['mb']), int(c['st']))
+                self.direct_config = (int(c['ft']), int(c['mb']), int(c['st']),
+                                      int(c.get('sft', 0)))
                 self.direct_config_source = 'table'
                 if log:
                     log('autotune %s: stored choice %s' % (key, self.direct_config))
@@ -316,17 +368,25 @@ class PlateSweep:
         panels = [m for m in (256, 448, 576) if m >= bmax]
 
         def run(c):
-            self.solve_direct(rtol=1.0, polish_iter=0, config=(c['ft'], c['mb'], c['st']))
+            self.solve_direct(rtol=1.0, polish_iter=0,
+                              config=(c['ft'], c['mb'], c['st'], c['sft']))
 
-        # factorisation shape first (it dominates), then the solve's threads
-        cands = [dict(ft=ft, mb=mb, st=1024) for mb in panels for ft in (256, 512, 1024)]
+        # factorisation shape first (it dominates), then the solve's threads,
+        # then (with plates on both sides of the narrow panel) a second
+        # factorisation launch for the narrow-band plates
+        cands = [dict(ft=ft, mb=mb, st=1024, sft=0) for mb in panels for ft in (256, 512, 1024)]
         best, table = AT.tune(cands, run, repeats=repeats, log=log)
         cands = [dict(best, st=st) for st in (256, 512)]
-        best2, table2 = AT.tune(cands + [best], run, repeats=repeats, log=log)
-        self.direct_config = (best2['ft'], best2['mb'], best2['st'])
+        best, table2 = AT.tune(cands + [best], run, repeats=repeats, log=log)
+        table += table2
+        if best['mb'] > NARROW_PANEL and 0 < self.band_n_wide < self.n_var:
+            cands = [dict(best, sft=t) for t in (256, 512)]
+            best, table3 = AT.tune(cands + [best], run, repeats=repeats, log=log)
+            table += table3
+        self.direct_config = (best['ft'], best['mb'], best['st'], best['sft'])
         self.direct_config_source = 'measured'
-        AT.record(key, dict(best2), table + table2)
-        return table + table2
+        AT.record(key, dict(best), table)
+        return table
 
     def step(self, mode='mixed', rtol=1e-8, stream=None, **kw):
         self.assemble(stream=stream, precond=mode != 'direct')

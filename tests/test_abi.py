@@ -109,6 +109,11 @@ def test_argument_errors_of_newer_entry_points():
                 fake, fake, fake, 1024, 300, None) == -1
     assert call('pf3d_band_solve', 1, None, fake, fake, fake, fake, fake, fake, fake, fake, 100,
                 None) == -1
+    assert call('pf3d_band_chol', 1, None, fake, fake, fake, fake, fake, 512, 300, None) == -1
+    assert b'max_b' in lib.pf3d_last_error()
+    assert call('pf3d_band_chol', 1, None, fake, fake, fake, fake, None, 512, 448, None) == -1
+    assert call('pf3d_band_load', 4, fake, fake, fake, -1, fake, fake, fake, fake, fake, fake,
+                None) == -1
     # mixed-precision PCG needs every operand and a positive replacement period
     assert call('pf3d_pcg_mixed', 4, fake, fake, fake, fake, fake, fake, fake, fake, 1e-8, 10, 0,
                 0, fake, 1 << 30, fake, fake, None) == -1

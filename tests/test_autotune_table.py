@@ -42,4 +42,5 @@ def test_shipped_table_entry_for_the_design_sweep():
     assert c['ft'] in (256, 512, 1024) and c['mb'] in (256, 448, 576)
     assert c['st'] in (256, 512, 1024)
     best = min(e['timings'], key=lambda t: t['ms'])
-    assert {k: best[k] for k in ('ft', 'mb', 'st')} == c
+    assert c.get('sft', 0) in (0, 256, 512)
+    assert {k: best.get(k, 0) for k in c} == c

@@ -215,7 +215,7 @@ def test_direct_launch_configs_and_autotune(tmp_path, monkeypatch):
     sw.check()
     table = sw.autotune_direct(repeats=1)
     assert sw.direct_config_source == 'measured'
-    assert sw.direct_config in [(c['ft'], c['mb'], c['st']) for c, _ in table]
+    assert sw.direct_config in [(c['ft'], c['mb'], c['st'], c['sft']) for c, _ in table]
     assert all(ms > 0 for _, ms in table)
     sw2, _, _ = _sweep(seed=12)
     table2 = sw2.autotune_direct(repeats=1)
@@ -227,6 +227,38 @@ def test_direct_launch_configs_and_autotune(tmp_path, monkeypatch):
     g.replay()
     torch.cuda.synchronize()
     assert torch.equal(sw.u, ref)
+
+def test_direct_split_factorisation_matches_single_launch():
+    """Plates on both sides of the narrow LDS panel (bandwidths 263 and
+    275 against 77, 137 and 65): factorising the narrow-band ones in a second
+    launch of the narrow shape (256 or 512 threads) gives the single
+    launch's bits, every system solved to 1e-8, none refused."""
+    from pyfe3d_amd.sweep import PlateSweep, NARROW_PANEL
+    sw = PlateSweep([(41, 50), (10, 12), (45, 43), (20, 30), (8, 64)])
+    h, f = sw.random_designs(5)
+    sw.set_thickness(h)
+    sw.set_loads(f)
+    sw.assemble()
+    sw._band_setup()
+    bw = sw.band_bw.cpu().numpy()
+    assert (bw > NARROW_PANEL).sum() == 2 and 0 < sw.band_n_wide < sw.n_var
+    sw.solve_direct(rtol=1e-8, polish_iter=0, config=(512, 448, 512, 0))
+    torch.cuda.synchronize()
+    ref = sw.u.clone()
+    for sft in (256, 512):
+        sw.u.zero_()
+        sw.band_bad.zero_()
+        sw.solve_direct(rtol=1e-8, polish_iter=0, config=(512, 448, 512, sft))
+        torch.cuda.synchronize()
+        assert int(sw.band_bad.item()) == 0
+        assert torch.equal(sw.u, ref), sft
+        assert float(sw.relres.max()) <= 1e-8
+    g = sw.capture(mode='direct', rtol=1e-8, config=(512, 448, 512, 256))
+    sw.u.zero_()
+    g.replay()
+    torch.cuda.synchronize()
+    assert torch.equal(sw.u, ref)
+
 
 def test_direct_panel_narrower_than_band_is_refused():
     """A 60 x 50 plate (bandwidth ~312 > 256): the 256-wide panel refuses the
