@@ -121,6 +121,42 @@ __device__ __forceinline__ TileAt tile_at(int t, int jn, int li, int lk)
     return TileAt{jn + 16 * I + li, jn + 16 * K + li, jn + 16 * I + lk};
 }
 
+// tile u of the trailing update's column-major walk of the lower triangle
+// of nt x nt tiles: down tile column K = 0 (rows I = K .. nt - 1), then
+// K = 1, ... , so that the waves working at once stream down the same
+// band columns (each column's entries are contiguous in the band) rather
+// than across 16 columns a band column apart.  With look-ahead (la) the
+// next diagonal block's tiles (0, 0), (1, 0), (1, 1) are left out (wave 0
+// takes them): columns 0 and 1 start at row 2, then the walk continues on
+// the triangle of columns and rows >= 2.  u uniform across the wave.
+__device__ __forceinline__ TileAt tile_cm(int u, int nt, bool la, int jn, int li, int lk)
+{
+    int off = 0, m = nt, v = u;
+    int I, K;
+    if (la) {
+        const int h = nt - 2;  // rows 2 .. nt - 1 of columns 0 and 1
+        off = 2;
+        m = h;
+        if (u < 2 * h) {
+            K = u < h ? 0 : 1;
+            I = 2 + (u < h ? u : u - h);
+            return TileAt{jn + 16 * I + li, jn + 16 * K + li, jn + 16 * I + lk};
+        }
+        v = u - 2 * h;
+    }
+    // column k of an m x m lower triangle starts at s_k = k m - k (k - 1) / 2
+    const float q = (float)(2 * m + 1);
+    int k = (int)((q - sqrtf(q * q - 8.f * v)) * 0.5f);
+    k = __builtin_amdgcn_readfirstlane(k);
+    k = k < 0 ? 0 : k;
+    while (k > 0 && k * m - k * (k - 1) / 2 > v) k--;
+    while ((k + 1) * m - (k + 1) * k / 2 <= v) k++;
+    const int i = k + (v - (k * m - k * (k - 1) / 2));
+    I = off + i;
+    K = off + k;
+    return TileAt{jn + 16 * I + li, jn + 16 * K + li, jn + 16 * I + lk};
+}
+
 // the tile's window entries A[ri0 + 4 v][rb] from the band (Lp: the
 // panel's first column).  Addresses are clamped into the window, so every
 // lane loads; entries outside the window's lower triangle are never stored
@@ -328,28 +364,30 @@ __global__ void __launch_bounds__(BT) band_chol(BandSys s, int *__restrict__ bad
         // prefetched tile is never copied, and a wave with no next tile
         // reloads its last one, so the load counter's order is the same on
         // every path and the multiply waits only for its own tile.  Without
-        // look-ahead wave w takes tiles w, w + NW, ...; with it waves 1 ..
-        // NW - 1 take tiles 3, 4, ... round robin.
+        // look-ahead wave w takes walk positions w, w + NW, ...; with it
+        // waves 1 .. NW - 1 take positions 0, 1, ... round robin.
+        // the tiles of this loop, walked column by column (tile_cm)
+        const int nwalk = la ? ntiles - 3 : ntiles;
         const int tstride = la ? NW - 1 : NW;
-        int t = __builtin_amdgcn_readfirstlane(la ? 2 + wave : wave);
-        if (!(la && wave == 0) && t < ntiles) {
-            TileAt q0 = tile_at(t, jn, li, lk), q1;
+        int t = __builtin_amdgcn_readfirstlane(la ? wave - 1 : wave);
+        if (!(la && wave == 0) && t < nwalk) {
+            TileAt q0 = tile_cm(t, nt, la, jn, li, lk), q1;
             v4d a0 = tile_load(q0, Lp, ld, rows), a1;
             for (;;) {
                 const int t1 = t + tstride;
-                q1 = tile_at(min(t1, ntiles - 1), jn, li, lk);
+                q1 = tile_cm(min(t1, nwalk - 1), nt, la, jn, li, lk);
                 a1 = tile_load(q1, Lp, ld, rows);
                 __builtin_amdgcn_sched_barrier(0);  // loads issue before the multiply
                 a0 = tile_update<PR>(q0, a0, P, lk);
                 tile_store(q0, a0, Lp, ld, rows);
-                if (t1 >= ntiles) break;
+                if (t1 >= nwalk) break;
                 const int t2 = t1 + tstride;
-                q0 = tile_at(min(t2, ntiles - 1), jn, li, lk);
+                q0 = tile_cm(min(t2, nwalk - 1), nt, la, jn, li, lk);
                 a0 = tile_load(q0, Lp, ld, rows);
                 __builtin_amdgcn_sched_barrier(0);
                 a1 = tile_update<PR>(q1, a1, P, lk);
                 tile_store(q1, a1, Lp, ld, rows);
-                if (t2 >= ntiles) break;
+                if (t2 >= nwalk) break;
                 t = t2;
             }
         }
