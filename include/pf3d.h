@@ -605,7 +605,10 @@ int pf3d_band_factor(int64_t n_var, int64_t n_nodes, const int32_t *order, const
  * system's entries into it (arguments as pf3d_band_factor); pf3d_band_chol
  * factorises the n_sys systems order[0 .. n_sys) in place (order null:
  * systems 0 .. n_sys - 1), counting failures and refusals in *bad as
- * pf3d_band_factor.
+ * pf3d_band_factor.  With work non-null, pf3d_band_chol also runs the
+ * forward substitution L y = f of each system from the panels it holds in
+ * LDS (pnode and f as in pf3d_band_solve; y left in work), so that
+ * pf3d_band_backsolve below finishes the solve reading the factor once.
  * Same replaced reference step: the factorisation inside spsolve
  * (tests/test_quad4_static_point_load.py:102). */
 int pf3d_band_load(int64_t n_nodes, const int64_t *dof0, const int32_t *bw, const int64_t *boff,
@@ -614,7 +617,7 @@ int pf3d_band_load(int64_t n_nodes, const int64_t *dof0, const int32_t *bw, cons
                    const int64_t *pinv, void *stream);
 int pf3d_band_chol(int64_t n_sys, const int32_t *order, const int64_t *dof0, const int32_t *bw,
                    const int64_t *boff, double *band, int *bad, int threads, int max_b,
-                   void *stream);
+                   const int64_t *pnode, const double *f, double *work, void *stream);
 
 /* x = A^-1 f per system with the factors of pf3d_band_factor; pnode[p] is
  * the original node of permuted node p; work: one double per DOF; threads
@@ -622,6 +625,13 @@ int pf3d_band_chol(int64_t n_sys, const int32_t *order, const int64_t *dof0, con
 int pf3d_band_solve(int64_t n_var, const int32_t *order, const int64_t *dof0, const int32_t *bw,
                     const int64_t *boff, const double *band, const int64_t *pnode,
                     const double *f, double *x, double *work, int threads, void *stream);
+
+/* The backward half of pf3d_band_solve (L^T x = y, then x scattered to the
+ * original numbering) for systems whose forward result y is already in
+ * work (pf3d_band_chol with work).  Same replaced step as pf3d_band_solve. */
+int pf3d_band_backsolve(int64_t n_var, const int32_t *order, const int64_t *dof0,
+                        const int32_t *bw, const int64_t *boff, const double *band,
+                        const int64_t *pnode, double *work, double *x, int threads, void *stream);
 
 /* ------------------------------------------------------------------ */
 /* FP64 dense helpers of the eigen-solvers (Krylov bases, column-major  */

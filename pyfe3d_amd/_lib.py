@@ -96,7 +96,8 @@ SIGNATURES = {
     'pf3d_band_max_width': [],
     'pf3d_band_factor': [I64, I64, P, P, P, P, I64, P, P, P, P, P, P, P, I32, I32, P],
     'pf3d_band_load': [I64, P, P, P, I64, P, P, P, P, P, P, P],
-    'pf3d_band_chol': [I64, P, P, P, P, P, P, I32, I32, P],
+    'pf3d_band_chol': [I64, P, P, P, P, P, P, I32, I32, P, P, P, P],
+    'pf3d_band_backsolve': [I64, P, P, P, P, P, P, P, P, I32, P],
     'pf3d_band_solve': [I64, P, P, P, P, P, P, P, P, P, I32, P],
     'pf3d_nn_gemm': [I32, I32, I32, P, I64, I64, P, I64, I64, P, I64, I32, P, I32, I32, P, P],
     'pf3d_nn_im2col': [I32, I32, I32, I32, I32, P, P, P],

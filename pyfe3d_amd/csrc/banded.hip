@@ -256,8 +256,18 @@ __device__ __forceinline__ void diag_block(double (*P)[NF + 1], double *dinv, do
     if (r < jn && (r >> 4) == h) dinv[r] = 1. / dg;
 }
 
+// forward substitution fused into the factorisation (y null: factor only):
+// y[6 p + d] = f[6 pnode[p] + d] per system, then L y = y panel by panel
+// from the panel in LDS, so the triangular solve afterwards only runs the
+// backward half and reads the factor once
+struct FwdArgs {
+    const int64_t *pnode;
+    const double *f;
+    double *y;  // one double per DOF (permuted numbering)
+};
+
 template <int BT, int MB>
-__global__ void __launch_bounds__(BT) band_chol(BandSys s, int *__restrict__ bad)
+__global__ void __launch_bounds__(BT) band_chol(BandSys s, FwdArgs fw, int *__restrict__ bad)
 {
     constexpr int PR = MB + NF;
     constexpr int RCH = (PR + 63) / 64;  // 64-row chunks of a panel column
@@ -265,6 +275,7 @@ __global__ void __launch_bounds__(BT) band_chol(BandSys s, int *__restrict__ bad
     __shared__ double P[PR][NF + 1];
     __shared__ double dinv[NF];
     __shared__ double Lk[2][NF];
+    __shared__ double yb[2][NF];  // solved forward unknowns of a block (by panel parity)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t v = s.order ? (int64_t)s.order[blockIdx.x] : (int64_t)blockIdx.x;
     const int64_t n = s.dof0[v + 1] - s.dof0[v];
@@ -275,12 +286,39 @@ __global__ void __launch_bounds__(BT) band_chol(BandSys s, int *__restrict__ bad
         if (tid == 0) atomicAdd(bad, 1);
         return;
     }
+    const bool fwd = fw.y != nullptr;
+    double *y = fwd ? fw.y + s.dof0[v] : nullptr;
+    if (fwd) {
+        // (visible to the first panel's substitution after its load barrier)
+        const int64_t base = s.dof0[v];
+        for (int64_t t = tid; t < n; t += BT) {
+            const int64_t p = t / 6;
+            y[t] = fw.f[6 * fw.pnode[base / 6 + p] + (t - 6 * p)];
+        }
+    }
     bool failed = false;
     bool ahead = false;  // this panel's diagonal block was factorised ahead
     for (int64_t j0 = 0; j0 < n; j0 += NF) {
         const int jn = (int)(n - j0 < NF ? n - j0 : NF);
         const int rows = (int)(n - j0 < b + jn ? n - j0 : b + jn);
         const int skip = ahead ? jn : 0;  // rows of P already holding L11
+        // Forward substitution.  With a next panel, wave 0 finishes the next
+        // block's rows (rows jn .. jn + 31 here) and solves through the next
+        // diagonal block during its look-ahead (4b), and waves 1 .. NW - 1
+        // update the rows below those; otherwise the block is solved in 3b
+        // and all threads update the rows below.  The unknowns this step
+        // reads (final since the last step) are loaded now, so the panel
+        // load covers their latency.
+        const bool la_step = n - (j0 + jn) > 0;
+        const int fr0 = la_step ? jn + NF : jn;  // first row of 4a'
+        const int ft0 = la_step ? tid - 64 : tid, fnt = la_step ? BT - 64 : BT;
+        const int par = (int)((j0 / NF) & 1);
+        double yblk = 0., yrow = 0., ynx = 0.;
+        if (fwd) {
+            if (wave == 0 && lane < jn && !ahead) yblk = y[j0 + lane];
+            if (la_step && wave == 0 && lane < NF && jn + lane < n - j0) ynx = y[j0 + jn + lane];
+            if (ft0 >= 0 && fr0 + ft0 < rows) yrow = y[j0 + fr0 + ft0];
+        }
         // 1. panel: P[r][c] = A[j0 + r][j0 + c], r >= c, r - c <= b (zero
         //    elsewhere); wave w takes columns w, w + NW, ..., 64 rows per
         //    load, every load of a column in flight before its LDS stores
@@ -329,6 +367,23 @@ __global__ void __launch_bounds__(BT) band_chol(BandSys s, int *__restrict__ bad
             for (int c = 0; c < NF; c++) P[r][c] = x[c];
         }
         __syncthreads();
+        // 3b. forward substitution through the panel's diagonal block by
+        //     wave 0 (lane c finishes unknown c with its pivot's reciprocal,
+        //     readlane hands it on), the same operations in the same order
+        //     as band_solve's forward step
+        if (fwd && wave == 0 && !ahead) {
+            const int rr = lane & 31;
+            double yv = yblk;
+            for (int c = 0; c < jn; c++) {
+                if (lane == c) yv *= dinv[c];
+                const double yc = readlane_d(yv, c);
+                if (lane > c && lane < jn) yv -= P[rr][c] * yc;
+            }
+            if (lane < jn) {
+                y[j0 + lane] = yv;
+                yb[par][lane] = yv;
+            }
+        }
         // 4a. the panel's columns of L back to the band (same mapping)
         for (int c = wave; c < jn; c += NW) {
             double *col = L + (j0 + c) * ld;
@@ -338,6 +393,7 @@ __global__ void __launch_bounds__(BT) band_chol(BandSys s, int *__restrict__ bad
                 if (r >= c && r < rows && r - c <= b) col[r - c] = P[r][c];
             }
         }
+
         // 4b. trailing update on 16 x 16 tiles (I, K), K <= I, of the
         //     window on the FP64 matrix cores: rows ri = jn + 16 I + i,
         //     columns rk = jn + 16 K + j, D = A - P_I P_K^T as 8 steps of
@@ -353,7 +409,20 @@ __global__ void __launch_bounds__(BT) band_chol(BandSys s, int *__restrict__ bad
         const int64_t j0n = j0 + jn;
         const int jnn = (int)(n - j0n < NF ? n - j0n : NF);  // next panel's width
         const bool la = jnn > 0;
-        if (la) __syncthreads();
+        if (la || fwd) __syncthreads();
+        // 4a'. the block's unknowns into the rows below it (one thread per
+        //      row, terms in column order; entries outside the band are 0;
+        //      P rows >= jn are not touched by 4b), by the waves that do not
+        //      run the look-ahead
+        if (fwd && ft0 >= 0) {
+            for (int r = fr0 + ft0; r < rows; r += fnt) {
+                asm volatile("" ::: "memory");  // yb read per row, not hoisted
+                double acc = 0.;
+#pragma unroll
+                for (int c = 0; c < NF; c++) acc = fma(P[r][c], yb[par][c], acc);
+                y[j0 + r] = (r == fr0 + ft0 ? yrow : y[j0 + r]) - acc;
+            }
+        }
         const int T = rows - jn;
         const int nt = (T + 15) >> 4;
         const int ntiles = nt * (nt + 1) / 2;
@@ -421,6 +490,32 @@ __global__ void __launch_bounds__(BT) band_chol(BandSys s, int *__restrict__ bad
                 }
             }
             diag_block(P, dinv, Lk, jnn, lane, failed);
+            if (fwd) {
+                // the next block's rows: this block's terms (4a' for rows
+                // jn .. jn + jnn - 1, from P rows jn .., untouched by the
+                // look-ahead), then the substitution through the next
+                // diagonal block now in P rows 0 .. 31 (3b of the next step)
+                const int rr = lane & 31;
+                double yv = 0.;
+                if (lane < jnn) {
+                    // (rows past this panel's band get no terms from it)
+                    double acc = 0.;
+                    if (jn + lane < rows) {
+#pragma unroll
+                        for (int c = 0; c < NF; c++) acc = fma(P[jn + rr][c], yb[par][c], acc);
+                    }
+                    yv = jn + lane < rows ? ynx - acc : ynx;
+                }
+                for (int c = 0; c < jnn; c++) {
+                    if (lane == c) yv *= dinv[c];
+                    const double yc = readlane_d(yv, c);
+                    if (lane > c && lane < jnn) yv -= P[rr][c] * yc;
+                }
+                if (lane < jnn) {
+                    y[j0n + lane] = yv;
+                    yb[par ^ 1][lane] = yv;
+                }
+            }
         }
         __syncthreads();
         ahead = la;
@@ -442,7 +537,7 @@ template <int BT>
 __global__ void __launch_bounds__(BT) band_solve(BandSys s, const int64_t *__restrict__ pnode,
                                                   const double *__restrict__ f,
                                                   double *__restrict__ x,
-                                                  double *__restrict__ work)
+                                                  double *__restrict__ work, int fwd_done)
 {
     constexpr int NW = BT / 64;
     __shared__ double T[NS][NS + 1];  // T[r][c] = L[j0 + r][j0 + c], r >= c (0 elsewhere)
@@ -456,10 +551,14 @@ __global__ void __launch_bounds__(BT) band_solve(BandSys s, const int64_t *__res
     const double *L = s.band + s.boff[v];
     const int64_t ld = b + 1;
     double *y = work + base;  // local vector in permuted numbering
-    // gather: y[6 p + d] = f[6 pnode[p] + d]
-    for (int64_t t = tid; t < n; t += BT) {
-        const int64_t p = t / 6;
-        y[t] = f[6 * pnode[base / 6 + p] + (t - 6 * p)];
+    // (fwd_done: work already holds the forward result for every system,
+    // from band_chol's fused substitution)
+    if (!fwd_done) {
+        // gather: y[6 p + d] = f[6 pnode[p] + d]
+        for (int64_t t = tid; t < n; t += BT) {
+            const int64_t p = t / 6;
+            y[t] = f[6 * pnode[base / 6 + p] + (t - 6 * p)];
+        }
     }
     // (loads unconditional, addresses clamped into the block's columns)
     auto stage = [&](int64_t j0, int jn) {
@@ -480,48 +579,50 @@ __global__ void __launch_bounds__(BT) band_solve(BandSys s, const int64_t *__res
         }
     };
     __syncthreads();
-    // forward: L y = g
-    for (int64_t j0 = 0; j0 < n; j0 += NS) {
-        const int jn = (int)(n - j0 < NS ? n - j0 : NS);
-        // the block's right-hand side, read with the stage's loads
-        const double yt = y[j0 + (lane < jn ? lane : jn - 1)];
-        stage(j0, jn);
-        __syncthreads();
-        if (wave == 0) {
-            // lane c finishes unknown c with the reciprocal of its pivot
-            // (formed by every lane at once), readlane hands it to all
-            const double rd = lane < jn ? 1. / T[lane & 31][lane & 31] : 0.;
-            double yc = lane < jn ? yt : 0.;
-            for (int c = 0; c < jn; c++) {
-                if (lane == c) yc *= rd;
-                const double yv = readlane_d(yc, c);
-                if (lane > c && lane < jn) yc -= T[lane][c] * yv;
+    if (!fwd_done) {
+        // forward: L y = g
+        for (int64_t j0 = 0; j0 < n; j0 += NS) {
+            const int jn = (int)(n - j0 < NS ? n - j0 : NS);
+            // the block's right-hand side, read with the stage's loads
+            const double yt = y[j0 + (lane < jn ? lane : jn - 1)];
+            stage(j0, jn);
+            __syncthreads();
+            if (wave == 0) {
+                // lane c finishes unknown c with the reciprocal of its pivot
+                // (formed by every lane at once), readlane hands it to all
+                const double rd = lane < jn ? 1. / T[lane & 31][lane & 31] : 0.;
+                double yc = lane < jn ? yt : 0.;
+                for (int c = 0; c < jn; c++) {
+                    if (lane == c) yc *= rd;
+                    const double yv = readlane_d(yc, c);
+                    if (lane > c && lane < jn) yc -= T[lane][c] * yv;
+                }
+                if (lane < jn) {
+                    y[j0 + lane] = yc;
+                    yb[lane] = yc;
+                }
             }
-            if (lane < jn) {
-                y[j0 + lane] = yc;
-                yb[lane] = yc;
+            __syncthreads();
+            // rows below the block within the band
+            const int64_t rend = (j0 + jn - 1 + b < n - 1) ? j0 + jn - 1 + b : n - 1;
+            // (branch-free: addresses clamped into the block's columns, masked
+            // terms enter as fma(l, 0, acc) = acc; eight loads in flight per
+            // round trip keep the registers of four workgroups per CU)
+            const double *Lb = L + j0 * ld;
+            for (int64_t i = j0 + jn + tid; i <= rend; i += BT) {
+                const int di = (int)(i - j0);
+                double acc = 0.;
+    #pragma unroll 8
+                for (int c = 0; c < NS; c++) {
+                    const int cc = c < jn ? c : jn - 1;
+                    const double l = *(const double *)((const char *)Lb +
+                                                       (unsigned)(cc * (int)ld + min(di - cc, b)) * 8u);
+                    acc = fma(l, (c < jn && di - c <= b) ? yb[c] : 0., acc);
+                }
+                y[i] -= acc;
             }
+            __syncthreads();
         }
-        __syncthreads();
-        // rows below the block within the band
-        const int64_t rend = (j0 + jn - 1 + b < n - 1) ? j0 + jn - 1 + b : n - 1;
-        // (branch-free: addresses clamped into the block's columns, masked
-        // terms enter as fma(l, 0, acc) = acc; eight loads in flight per
-        // round trip keep the registers of four workgroups per CU)
-        const double *Lb = L + j0 * ld;
-        for (int64_t i = j0 + jn + tid; i <= rend; i += BT) {
-            const int di = (int)(i - j0);
-            double acc = 0.;
-#pragma unroll 8
-            for (int c = 0; c < NS; c++) {
-                const int cc = c < jn ? c : jn - 1;
-                const double l = *(const double *)((const char *)Lb +
-                                                   (unsigned)(cc * (int)ld + min(di - cc, b)) * 8u);
-                acc = fma(l, (c < jn && di - c <= b) ? yb[c] : 0., acc);
-            }
-            y[i] -= acc;
-        }
-        __syncthreads();
     }
     // backward: L^T x = y, blocks from the end
     const int64_t nblk = (n + NS - 1) / NS;
@@ -584,12 +685,12 @@ int pf3d_band_max_width(void) { return MAXB; }
 
 // band_chol<BT, MB> for a (threads, max_b) configuration; false if none
 static bool launch_chol(int threads, int max_b, unsigned grid, hipStream_t s, BandSys sys,
-                        int *bad)
+                        FwdArgs fw, int *bad)
 {
-#define PF3D_CHOL(BTV, MBV)                                                                 \
-    if (threads == BTV && max_b == MBV) {                                                   \
-        hipLaunchKernelGGL((band_chol<BTV, MBV>), dim3(grid), dim3(BTV), 0, s, sys, bad); \
-        return true;                                                                        \
+#define PF3D_CHOL(BTV, MBV)                                                                     \
+    if (threads == BTV && max_b == MBV) {                                                       \
+        hipLaunchKernelGGL((band_chol<BTV, MBV>), dim3(grid), dim3(BTV), 0, s, sys, fw, bad); \
+        return true;                                                                            \
     }
     PF3D_CHOL(1024, 576) PF3D_CHOL(1024, 448) PF3D_CHOL(1024, 256)
     PF3D_CHOL(512, 576) PF3D_CHOL(512, 448) PF3D_CHOL(512, 256)
@@ -617,7 +718,7 @@ int pf3d_band_load(int64_t n_nodes, const int64_t *dof0, const int32_t *bw, cons
 
 int pf3d_band_chol(int64_t n_sys, const int32_t *order, const int64_t *dof0, const int32_t *bw,
                    const int64_t *boff, double *band, int *bad, int threads, int max_b,
-                   void *stream)
+                   const int64_t *pnode, const double *f, double *work, void *stream)
 {
     if (threads == 0) threads = 1024;
     if (max_b == 0) max_b = MAXB;
@@ -629,9 +730,12 @@ int pf3d_band_chol(int64_t n_sys, const int32_t *order, const int64_t *dof0, con
     if (n_sys == 0) return PF3D_OK;
     if (!dof0 || !bw || !boff || !band || !bad)
         return pf3d_fail(PF3D_E_ARG, "pf3d_band_chol: null argument");
+    if (work && (!pnode || !f))
+        return pf3d_fail(PF3D_E_ARG, "pf3d_band_chol: the forward substitution needs pnode and f");
     if (n_sys > 2147483647) return pf3d_fail(PF3D_E_ARG, "too many systems");
     BandSys sys{n_sys, order, dof0, bw, boff, band};
-    launch_chol(threads, max_b, (unsigned)n_sys, (hipStream_t)stream, sys, bad);
+    FwdArgs fw{pnode, f, work};
+    launch_chol(threads, max_b, (unsigned)n_sys, (hipStream_t)stream, sys, fw, bad);
     return pf3d_hip_check(hipGetLastError(), "pf3d_band_chol");
 }
 
@@ -655,7 +759,32 @@ int pf3d_band_factor(int64_t n_var, int64_t n_nodes, const int32_t *order, const
     int rc = pf3d_band_load(n_nodes, dof0, bw, boff, band_size, band, row_ptr, col_idx, val, var,
                             pinv, stream);
     if (rc != PF3D_OK) return rc;
-    return pf3d_band_chol(n_var, order, dof0, bw, boff, band, bad, threads, max_b, stream);
+    return pf3d_band_chol(n_var, order, dof0, bw, boff, band, bad, threads, max_b, nullptr,
+                          nullptr, nullptr, stream);
+}
+
+// band_solve<BT> over n_var systems; fwd_done: work holds the forward result
+static int launch_solve(const char *who, int64_t n_var, const int32_t *order, const int64_t *dof0,
+                        const int32_t *bw, const int64_t *boff, const double *band,
+                        const int64_t *pnode, const double *f, double *x, double *work,
+                        int threads, int fwd_done, void *stream)
+{
+    if (threads == 0) threads = 1024;
+    BandSys sys{n_var, order, dof0, bw, boff, const_cast<double *>(band)};
+    const dim3 grid((unsigned)n_var);
+    hipStream_t s = (hipStream_t)stream;
+    if (threads == 1024)
+        hipLaunchKernelGGL(band_solve<1024>, grid, dim3(1024), 0, s, sys, pnode, f, x, work,
+                           fwd_done);
+    else if (threads == 512)
+        hipLaunchKernelGGL(band_solve<512>, grid, dim3(512), 0, s, sys, pnode, f, x, work,
+                           fwd_done);
+    else if (threads == 256)
+        hipLaunchKernelGGL(band_solve<256>, grid, dim3(256), 0, s, sys, pnode, f, x, work,
+                           fwd_done);
+    else
+        return pf3d_fail(PF3D_E_ARG, "%s: threads must be 256, 512 or 1024", who);
+    return pf3d_hip_check(hipGetLastError(), who);
 }
 
 int pf3d_band_solve(int64_t n_var, const int32_t *order, const int64_t *dof0, const int32_t *bw,
@@ -666,19 +795,20 @@ int pf3d_band_solve(int64_t n_var, const int32_t *order, const int64_t *dof0, co
     if (n_var == 0) return PF3D_OK;
     if (!dof0 || !bw || !boff || !band || !pnode || !f || !x || !work)
         return pf3d_fail(PF3D_E_ARG, "pf3d_band_solve: null argument");
-    if (threads == 0) threads = 1024;
-    BandSys sys{n_var, order, dof0, bw, boff, const_cast<double *>(band)};
-    const dim3 grid((unsigned)n_var);
-    hipStream_t s = (hipStream_t)stream;
-    if (threads == 1024)
-        hipLaunchKernelGGL(band_solve<1024>, grid, dim3(1024), 0, s, sys, pnode, f, x, work);
-    else if (threads == 512)
-        hipLaunchKernelGGL(band_solve<512>, grid, dim3(512), 0, s, sys, pnode, f, x, work);
-    else if (threads == 256)
-        hipLaunchKernelGGL(band_solve<256>, grid, dim3(256), 0, s, sys, pnode, f, x, work);
-    else
-        return pf3d_fail(PF3D_E_ARG, "pf3d_band_solve: threads must be 256, 512 or 1024");
-    return pf3d_hip_check(hipGetLastError(), "pf3d_band_solve");
+    return launch_solve("pf3d_band_solve", n_var, order, dof0, bw, boff, band, pnode, f, x, work,
+                        threads, 0, stream);
+}
+
+int pf3d_band_backsolve(int64_t n_var, const int32_t *order, const int64_t *dof0,
+                        const int32_t *bw, const int64_t *boff, const double *band,
+                        const int64_t *pnode, double *work, double *x, int threads, void *stream)
+{
+    if (n_var < 0) return pf3d_fail(PF3D_E_ARG, "pf3d_band_backsolve: negative size");
+    if (n_var == 0) return PF3D_OK;
+    if (!dof0 || !bw || !boff || !band || !pnode || !x || !work)
+        return pf3d_fail(PF3D_E_ARG, "pf3d_band_backsolve: null argument");
+    return launch_solve("pf3d_band_backsolve", n_var, order, dof0, bw, boff, band, pnode,
+                        nullptr, x, work, threads, 1, stream);
 }
 
 }  // extern "C"

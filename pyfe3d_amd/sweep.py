@@ -263,38 +263,38 @@ class PlateSweep:
         c = tuple(config or getattr(self, 'direct_config', None) or (1024, 576, 1024))
         return c if len(c) == 4 else c + (0,)
 
-    def factor_direct(self, config=None, stream=None):
+    def factor_direct(self, config=None, stream=None, forward=False):
         """Band storage of every variant's stiffness matrix and its banded
         Cholesky factorisation in place (the first half of
         :meth:`solve_direct`).  With a narrow-band thread count in the
         configuration, the plates whose band fits the narrow LDS panel are
         factorised by a second launch of that shape (more workgroups per CU)
-        after the wide ones."""
+        at the same time as the wide ones.  forward: the factorisation also
+        runs the forward substitution with the current loads (self.f),
+        leaving y in self.band_work for :meth:`solve_direct`'s backward
+        pass."""
         if not hasattr(self, 'band'):
             self._band_setup()
         ft, mb, _, sft = self._direct_config(config)
         s = _s(stream)
-        if not sft or mb <= NARROW_PANEL or self.band_n_wide == self.n_var:
-            _lib.call('pf3d_band_factor', self.n_var, self.n_nodes, self.order.data_ptr(),
-                      self.band_dof0.data_ptr(), self.band_bw.data_ptr(),
-                      self.band_off.data_ptr(), self.band_size, self.band.data_ptr(),
-                      self.plan.row_ptr.data_ptr(), self.plan.col_idx.data_ptr(),
-                      self.val.data_ptr(), self.band_var.data_ptr(), self.band_pinv.data_ptr(),
-                      self.band_bad.data_ptr(), int(ft), int(mb), s)
-            return
         _lib.call('pf3d_band_load', self.n_nodes, self.band_dof0.data_ptr(),
                   self.band_bw.data_ptr(), self.band_off.data_ptr(), self.band_size,
                   self.band.data_ptr(), self.plan.row_ptr.data_ptr(),
                   self.plan.col_idx.data_ptr(), self.val.data_ptr(), self.band_var.data_ptr(),
                   self.band_pinv.data_ptr(), s)
-        nw = self.band_n_wide
-        order = self.band_order_split
+        fwd = ((self.band_pnode.data_ptr(), self.f.data_ptr(), self.band_work.data_ptr())
+               if forward else (None, None, None))
 
         def chol(n_sys, ptr, threads, panel, st):
             _lib.call('pf3d_band_chol', n_sys, ptr, self.band_dof0.data_ptr(),
                       self.band_bw.data_ptr(), self.band_off.data_ptr(), self.band.data_ptr(),
-                      self.band_bad.data_ptr(), int(threads), int(panel), st)
+                      self.band_bad.data_ptr(), int(threads), int(panel), *fwd, st)
 
+        if not sft or mb <= NARROW_PANEL or self.band_n_wide == self.n_var:
+            chol(self.n_var, self.order.data_ptr(), ft, mb, s)
+            return
+        nw = self.band_n_wide
+        order = self.band_order_split
         # the two groups run concurrently (the narrow one on a side stream,
         # forked from and joined back to the caller's), so the narrow-band
         # plates fill the CUs the wide launch's tail leaves idle
@@ -309,24 +309,25 @@ class PlateSweep:
         main.wait_stream(side)
 
     def solve_direct(self, rtol=1e-8, polish_iter=0, config=None, stream=None):
-        """Banded Cholesky of every variant (one workgroup each), forward and
-        back substitution, then the FP64 true residual of each variant into
-        self.relres (and, with polish_iter > 0, block-Jacobi PCG from the
-        direct solution for any variant that misses rtol).  config:
-        (factor threads, panel bandwidth, solve threads), default the one
-        chosen by :meth:`autotune_direct`, else (1024, 576, 1024)."""
+        """Banded Cholesky of every variant (one workgroup each) with the
+        forward substitution fused into it, the backward substitution, then
+        the FP64 true residual of each variant into self.relres (and, with
+        polish_iter > 0, block-Jacobi PCG from the direct solution for any
+        variant that misses rtol).  config: (factor threads, panel
+        bandwidth, solve threads[, narrow-band factor threads]), default the
+        one chosen by :meth:`autotune_direct`, else (1024, 576, 1024)."""
         if not hasattr(self, 'band'):
             self._band_setup()
         s = _s(stream)
         cfg = self._direct_config(config)
         st = cfg[2]
-        self.factor_direct(config=cfg, stream=stream)
+        self.factor_direct(config=cfg, stream=stream, forward=True)
         if polish_iter > 0:
             self.precondition(stream=stream)
-        _lib.call('pf3d_band_solve', self.n_var, self.order.data_ptr(),
+        _lib.call('pf3d_band_backsolve', self.n_var, self.order.data_ptr(),
                   self.band_dof0.data_ptr(), self.band_bw.data_ptr(), self.band_off.data_ptr(),
-                  self.band.data_ptr(), self.band_pnode.data_ptr(), self.f.data_ptr(),
-                  self.u.data_ptr(), self.band_work.data_ptr(), int(st), s)
+                  self.band.data_ptr(), self.band_pnode.data_ptr(), self.band_work.data_ptr(),
+                  self.u.data_ptr(), int(st), s)
         self.solve(mode='fp64', rtol=rtol, max_iter=polish_iter, x0=True, stream=stream)
 
     def autotune_key(self):

@@ -109,9 +109,16 @@ def test_argument_errors_of_newer_entry_points():
                 fake, fake, fake, 1024, 300, None) == -1
     assert call('pf3d_band_solve', 1, None, fake, fake, fake, fake, fake, fake, fake, fake, 100,
                 None) == -1
-    assert call('pf3d_band_chol', 1, None, fake, fake, fake, fake, fake, 512, 300, None) == -1
+    assert call('pf3d_band_chol', 1, None, fake, fake, fake, fake, fake, 512, 300, None, None,
+                None, None) == -1
     assert b'max_b' in lib.pf3d_last_error()
-    assert call('pf3d_band_chol', 1, None, fake, fake, fake, fake, None, 512, 448, None) == -1
+    assert call('pf3d_band_chol', 1, None, fake, fake, fake, fake, None, 512, 448, None, None,
+                None, None) == -1
+    assert call('pf3d_band_chol', 1, None, fake, fake, fake, fake, fake, 512, 448, None, None,
+                fake, None) == -1
+    assert b'forward substitution' in lib.pf3d_last_error()
+    assert call('pf3d_band_backsolve', 1, None, fake, fake, fake, fake, fake, fake, fake, 128,
+                None) == -1
     assert call('pf3d_band_load', 4, fake, fake, fake, -1, fake, fake, fake, fake, fake, fake,
                 None) == -1
     # mixed-precision PCG needs every operand and a positive replacement period

@@ -260,6 +260,32 @@ def test_direct_split_factorisation_matches_single_launch():
     assert torch.equal(sw.u, ref)
 
 
+def test_direct_fused_forward_matches_separate_solve():
+    """The forward substitution fused into the factorisation (panel by
+    panel from LDS) plus the backward-only solve gives the bits of the
+    factorisation followed by the full two-pass triangular solve, for the
+    single-launch and the split factorisation."""
+    from pyfe3d_amd import _lib
+    from pyfe3d_amd.sweep import _s
+    sw, h, f = _sweep(seed=5)
+    sw.assemble()
+    for cfg in [(512, 448, 512, 0), (256, 576, 256, 256)]:
+        sw.u.zero_()
+        sw.solve_direct(rtol=1e-8, polish_iter=0, config=cfg)
+        torch.cuda.synchronize()
+        fused = sw.u.clone()
+        assert float(sw.relres.max()) <= 1e-8
+        sw.u.zero_()
+        sw.band_work.fill_(float('nan'))
+        sw.factor_direct(config=cfg)
+        _lib.call('pf3d_band_solve', sw.n_var, sw.order.data_ptr(), sw.band_dof0.data_ptr(),
+                  sw.band_bw.data_ptr(), sw.band_off.data_ptr(), sw.band.data_ptr(),
+                  sw.band_pnode.data_ptr(), sw.f.data_ptr(), sw.u.data_ptr(),
+                  sw.band_work.data_ptr(), cfg[2], _s(None))
+        torch.cuda.synchronize()
+        assert torch.equal(sw.u, fused), cfg
+
+
 def test_direct_panel_narrower_than_band_is_refused():
     """A 60 x 50 plate (bandwidth ~312 > 256): the 256-wide panel refuses the
     system (counted in band_bad, band untouched, kernel exits normally);
