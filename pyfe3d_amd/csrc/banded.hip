@@ -14,9 +14,12 @@
 //   * panel of 32 columns loaded into LDS: diagonal block factorised by one
 //     wave, the rows below by all threads,
 //   * symmetric rank-32 update of the trailing band window by the whole
-//     workgroup.
-// The triangular solves are blocked by 32 columns per barrier pair.  All
-// sums have a fixed order: results do not depend on timing or launch shape.
+//     workgroup;
+// optionally with the forward substitution L y = f run panel by panel from
+// the panel in LDS (the loads are known when the sweep factorises).  The
+// triangular solves are blocked by 32 columns per barrier pair (both passes,
+// or the backward one after a fused forward pass).  All sums have a fixed
+// order: results do not depend on timing or launch shape.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -524,7 +527,8 @@ __global__ void __launch_bounds__(BT) band_chol(BandSys s, FwdArgs fw, int *__re
 }
 
 // --------------------------------------------------------- triangular
-// x <- A^-1 f for system v: permute f into y, L y = g, L^T x = y, scatter.
+// x <- A^-1 f for system v: permute f into y, L y = g, L^T x = y, scatter
+// (fwd_done: y = L^-1 g is already in work, from band_chol's fused pass).
 // Blocks of NS = 32 columns: the block's diagonal triangle of L is staged
 // in LDS by all threads, wave 0 substitutes through it (lane c holds
 // unknown c and finishes it with its pivot's reciprocal; values travel by
