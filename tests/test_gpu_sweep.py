@@ -286,6 +286,36 @@ def test_direct_fused_forward_matches_separate_solve():
         assert torch.equal(sw.u, fused), cfg
 
 
+def test_direct_fused_forward_with_bands_narrower_than_a_panel():
+    """Strips 2 elements wide (bandwidth 29 < the 32-column panel: the next
+    block's rows reach past the current panel's band) and a 1-wide-panel
+    case next to ordinary plates: the fused forward substitution gives the
+    two-pass solve's bits and every system meets 1e-8."""
+    from pyfe3d_amd import _lib
+    from pyfe3d_amd.sweep import PlateSweep, _s
+    sw = PlateSweep([(2, 40), (40, 2), (3, 30), (12, 9), (2, 6)])
+    h, f = sw.random_designs(8)
+    sw.set_thickness(h)
+    sw.set_loads(f)
+    sw.assemble()
+    sw._band_setup()
+    assert int(sw.band_bw.min()) < 32
+    for cfg in [(512, 448, 512, 0), (256, 256, 256, 0)]:
+        sw.u.zero_()
+        sw.solve_direct(rtol=1e-8, polish_iter=0, config=cfg)
+        torch.cuda.synchronize()
+        fused = sw.u.clone()
+        assert float(sw.relres.max()) <= 1e-8, cfg
+        sw.u.zero_()
+        sw.factor_direct(config=cfg)
+        _lib.call('pf3d_band_solve', sw.n_var, sw.order.data_ptr(), sw.band_dof0.data_ptr(),
+                  sw.band_bw.data_ptr(), sw.band_off.data_ptr(), sw.band.data_ptr(),
+                  sw.band_pnode.data_ptr(), sw.f.data_ptr(), sw.u.data_ptr(),
+                  sw.band_work.data_ptr(), cfg[2], _s(None))
+        torch.cuda.synchronize()
+        assert torch.equal(sw.u, fused), cfg
+
+
 def test_direct_panel_narrower_than_band_is_refused():
     """A 60 x 50 plate (bandwidth ~312 > 256): the 256-wide panel refuses the
     system (counted in band_bad, band untouched, kernel exits normally);
