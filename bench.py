@@ -266,7 +266,19 @@ def sweep_rates(n_var, log, steps=2, modes=('direct', 'fp64', 'mixed')):
                      tuning_s=time.perf_counter() - t0,
                      candidates=[dict(c, ms=round(ms, 3)) for c, ms in table])
         log('sweep direct: autotuned config %s' % (sw.direct_config,))
+    pcg_tuned = {}
     for mode in modes:
+        if mode != 'direct':
+            # launch shape of the persistent batched PCG, chosen by timing
+            sw.assemble()
+            t0 = time.perf_counter()
+            table = sw.autotune_pcg(mode, log=log)
+            groups, threads = sw.pcg_config[mode]
+            pcg_tuned[mode] = dict(config=dict(workgroups=groups, threads=threads),
+                                   source=sw.pcg_config_source, key=sw.autotune_key_pcg(mode),
+                                   tuning_s=time.perf_counter() - t0,
+                                   candidates=[dict(c, ms=round(ms, 3)) for c, ms in table])
+            log('sweep %s: autotuned PCG launch %s' % (mode, sw.pcg_config[mode]))
         t0 = time.perf_counter()
         g = sw.capture(mode=mode, rtol=1e-8)  # runs one eager step first
         torch.cuda.synchronize()
@@ -289,6 +301,8 @@ def sweep_rates(n_var, log, steps=2, modes=('direct', 'fp64', 'mixed')):
                          first_step_s=first)
         if mode == 'direct' and tuned:
             out[mode]['autotune'] = tuned
+        if mode in pcg_tuned:
+            out[mode]['autotune'] = pcg_tuned[mode]
         log('sweep %s: %.1f ms/sweep, %.1f solves/s, max relres %.2e, iters %d'
             % (mode, ms, n_var / (ms * 1e-3), rel.max(), its.sum()))
         del g

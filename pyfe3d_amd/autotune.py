@@ -1,10 +1,12 @@
 """A small autotuner: time each candidate launch configuration of a GPU
 step on the real inputs and keep the fastest, and remember the choice.
 
-Used for the batched banded solver of the design sweep (threads per
-workgroup and the LDS panel width, threads of the triangular solves); the
-candidates and the timing callable come from the caller, so the same
-search serves any kernel with runtime launch parameters.
+Used for the design sweep's batched banded solver (threads per workgroup
+and the LDS panel width, threads of the triangular solves, the launch of
+the narrow-band plates) and its persistent batched PCG (threads per
+workgroup and the number of workgroups); the candidates and the timing
+callable come from the caller, so the same search serves any kernel with
+runtime launch parameters.
 Timing: HIP events on the current stream around `repeats` calls, after one
 untimed warm-up call per candidate.
 

@@ -170,10 +170,15 @@ class PlateSweep:
               n_groups=0, threads=0, stream=None):
         """Solve every variant into self.u (x0: start from the current u).
         mode 'fp64': FP64 operator; 'mixed': FP32 operator and inverses with
-        an FP64 true-residual replacement every ``replace`` iterations."""
+        an FP64 true-residual replacement every ``replace`` iterations.
+        Launch shape: n_groups persistent workgroups of ``threads`` threads;
+        0, 0 takes the shape :meth:`autotune_pcg` chose for this mode, else
+        the library's default (1024 threads, one workgroup per CU)."""
         mixed = mode == 'mixed'
         if mode not in ('mixed', 'fp64'):
             raise ValueError(mode)
+        if not n_groups and not threads:
+            n_groups, threads = getattr(self, 'pcg_config', {}).get(mode, (0, 0))
         _lib.call('pf3d_batch_solve', self.n_var, self.n_nodes, self.row0.data_ptr(),
                   self.order.data_ptr(), self.plan.row_ptr.data_ptr(),
                   self.plan.col_idx.data_ptr(), self.val.data_ptr(), self.dinv.data_ptr(),
@@ -342,6 +347,54 @@ class PlateSweep:
         bucket = (panel, AT.size_bucket(self.n_var),
                   AT.size_bucket(int(self.band_n_host.mean())))
         return AT.make_key('band_direct', 'quad4', bucket, AT.arch())
+
+    def autotune_key_pcg(self, mode):
+        """Autotune table key of this sweep's batched PCG in one mode:
+        Quad4 plates, the mode, power-of-two buckets of the number of
+        variants and of the mean DOF per variant, GPU architecture."""
+        from . import autotune as AT
+        bucket = (AT.size_bucket(self.n_var), AT.size_bucket(6 * self.n_nodes // self.n_var))
+        return AT.make_key('batch_pcg_' + mode, 'quad4', bucket, AT.arch())
+
+    def autotune_pcg(self, mode='fp64', log=None, repeats=1, use_table=True, probe_iters=200):
+        """Launch shape of the persistent batched PCG (threads per workgroup
+        x number of workgroups) for :meth:`solve` in this mode: the stored
+        choice for the sweep's key if there is one (use_table), else every
+        shape timed on probe_iters iterations of every variant (a residual
+        target no system meets, so all do the same work per candidate) from
+        the current designs, the fastest kept and recorded.  Needs the
+        operator assembled and preconditioned (:meth:`assemble`).  Returns
+        the table of timings (stored or measured)."""
+        from . import autotune as AT
+        key = self.autotune_key_pcg(mode)
+        cfgs = getattr(self, 'pcg_config', {})
+        self.pcg_config = cfgs
+        if use_table:
+            entry = AT.lookup(key)
+            if entry is not None:
+                c = entry['config']
+                cfgs[mode] = (int(c['groups']), int(c['threads']))
+                self.pcg_config_source = 'table'
+                if log:
+                    log('autotune %s: stored choice %s' % (key, cfgs[mode]))
+                return [({k: v for k, v in t.items() if k != 'ms'}, t['ms'])
+                        for t in entry.get('timings', [])]
+        u0 = self.u.clone()
+
+        def run(c):
+            self.solve(mode=mode, rtol=1e-300, max_iter=probe_iters, n_groups=c['groups'],
+                       threads=c['threads'])
+
+        # one and two workgroups per CU at each width (the library's default
+        # is one 1024-thread workgroup per CU)
+        cands = [dict(threads=t, groups=k * 256 * (1024 // t)) for t in (256, 512, 1024)
+                 for k in (1, 2)]
+        best, table = AT.tune(cands, run, repeats=repeats, log=log)
+        self.u.copy_(u0)
+        cfgs[mode] = (best['groups'], best['threads'])
+        self.pcg_config_source = 'measured'
+        AT.record(key, dict(best), table)
+        return table
 
     def autotune_direct(self, log=None, repeats=2, use_table=True):
         """Launch configuration of the banded solver (factor threads x LDS

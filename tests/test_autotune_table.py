@@ -44,3 +44,13 @@ def test_shipped_table_entry_for_the_design_sweep():
     best = min(e['timings'], key=lambda t: t['ms'])
     assert c.get('sft', 0) in (0, 256, 512)
     assert {k: best.get(k, 0) for k in c} == c
+
+
+def test_shipped_table_entries_for_the_batched_pcg():
+    data = json.load(open(AT.SHIPPED))
+    for mode in ('fp64', 'mixed'):
+        e = data['batch_pcg_%s|quad4|256x8192|gfx950' % mode]
+        c = e['config']
+        assert c['threads'] in (256, 512, 1024) and c['groups'] > 0
+        best = min(e['timings'], key=lambda t: t['ms'])
+        assert {k: best[k] for k in c} == c

@@ -120,6 +120,38 @@ def test_sweep_matches_spsolve_per_variant(mode):
         assert err <= 1e-8, (v, err)
 
 
+def test_pcg_launch_autotune(tmp_path, monkeypatch):
+    """The batched PCG's launch shape (threads x persistent workgroups) is
+    chosen by timing six candidates, recorded under the sweep's key and
+    taken from the table by a second sweep of the same key; solved to
+    1e-10 with the chosen shape, every variant's displacements are within
+    1e-8 of the default shape's (the reductions follow the workgroup
+    width, so the bits may differ)."""
+    monkeypatch.setenv('PF3D_AUTOTUNE_CACHE', str(tmp_path / 'tune.json'))
+    sw, h, f = _sweep(seed=21)
+    sw.assemble()
+    sw.solve(mode='fp64', rtol=1e-10, n_groups=256, threads=1024)
+    torch.cuda.synchronize()
+    ref = sw.u.cpu().numpy()
+    table = sw.autotune_pcg('fp64', probe_iters=50)
+    assert sw.pcg_config_source == 'measured' and len(table) == 6
+    groups, threads = sw.pcg_config['fp64']
+    assert dict(threads=threads, groups=groups) in [c for c, _ in table]
+    sw.solve(mode='fp64', rtol=1e-10)
+    torch.cuda.synchronize()
+    assert (sw.relres.cpu().numpy() <= 1e-10).all()
+    u = sw.u.cpu().numpy()
+    for v in range(sw.n_var):
+        nodes, _ = sw.variant(v)
+        a, b = ref[6 * nodes.start:6 * nodes.stop], u[6 * nodes.start:6 * nodes.stop]
+        assert np.linalg.norm(a - b) <= 1e-8 * np.linalg.norm(a), v
+    sw2, _, _ = _sweep(seed=22)
+    sw2.assemble()
+    sw2.autotune_pcg('fp64')
+    assert sw2.pcg_config_source == 'table'
+    assert sw2.pcg_config['fp64'] == sw.pcg_config['fp64']
+
+
 def test_sweep_rtol_1e8_and_warm_start():
     """At the bench tolerance every variant meets 1e-8 in both modes; a
     second solve started from the converged u needs no iteration."""
