@@ -34,6 +34,18 @@ from .sparse import BlockPlan
 NARROW_PANEL = 256
 
 
+def split_order(bandwidths, sizes, narrow=NARROW_PANEL):
+    """Processing order of the split banded factorisation: the systems
+    whose bandwidth exceeds ``narrow`` first, then the others, largest
+    (by ``sizes``) first within each group, ties in index order.  Returns
+    (order as int32, number of wide systems)."""
+    bw = np.asarray(bandwidths)
+    by_size = np.argsort(-np.asarray(sizes), kind='stable')
+    wide = bw[by_size] > narrow
+    order = np.concatenate([by_size[wide], by_size[~wide]]).astype(np.int32)
+    return order, int(wide.sum())
+
+
 def _s(stream):
     return (stream if stream is not None else torch.cuda.current_stream()).cuda_stream
 
@@ -226,12 +238,8 @@ class PlateSweep:
         self.band_bad = torch.zeros(1, dtype=torch.int32, device=dev)
         # processing order split in two groups, largest plates first in each:
         # bands too wide for the narrow LDS panel, then those that fit it
-        narrow = bws <= NARROW_PANEL
-        nodes = np.diff(self.node0)
-        by_size = np.argsort(-nodes, kind='stable')
-        split = np.concatenate([by_size[~narrow[by_size]], by_size[narrow[by_size]]])
-        self.band_order_split = torch.as_tensor(split.astype(np.int32), device=dev)
-        self.band_n_wide = int((~narrow).sum())
+        split, self.band_n_wide = split_order(bws, np.diff(self.node0))
+        self.band_order_split = torch.as_tensor(split, device=dev)
 
     def band_flops(self):
         """Floating-point operations of the banded Cholesky of every

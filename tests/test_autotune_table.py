@@ -1,5 +1,8 @@
-"""The autotuner's table of stored choices (host logic, no GPU)."""
+"""Host logic (no GPU): the autotuner's table of stored choices and the
+sweep's grouping of systems for the split banded factorisation."""
 import json
+
+import numpy as np
 
 from pyfe3d_amd import autotune as AT
 
@@ -54,3 +57,17 @@ def test_shipped_table_entries_for_the_batched_pcg():
         assert c['threads'] in (256, 512, 1024) and c['groups'] > 0
         best = min(e['timings'], key=lambda t: t['ms'])
         assert {k: best[k] for k in c} == c
+
+
+def test_split_order_of_the_banded_factorisation():
+    """Wide bands (> 256) first, then the narrow ones, largest systems
+    first in each group, ties kept in index order."""
+    from pyfe3d_amd.sweep import split_order
+    bw = [65, 263, 137, 275, 257, 65, 256]
+    sizes = [126, 900, 400, 900, 700, 126, 800]
+    order, n_wide = split_order(bw, sizes)
+    assert n_wide == 3
+    assert order.dtype == np.int32
+    assert order.tolist() == [1, 3, 4, 6, 2, 0, 5]
+    order, n_wide = split_order([10, 20], [5, 9])
+    assert n_wide == 0 and order.tolist() == [1, 0]
