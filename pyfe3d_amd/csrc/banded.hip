@@ -547,6 +547,12 @@ __global__ void __launch_bounds__(BT) band_solve(BandSys s, const int64_t *__res
     __shared__ double T[NS][NS + 1];  // T[r][c] = L[j0 + r][j0 + c], r >= c (0 elsewhere)
     __shared__ double yb[NS];
     __shared__ double part[NS];
+    // backward: the solved unknowns also in an LDS ring (x[i] at xr[i % RX]),
+    // so the column sums read them from LDS; a slot is reused only after
+    // every column that reads its row is done
+    constexpr int RX = 1024;
+    static_assert(RX >= MAXB + 2 * NS, "ring shorter than the rows a block reads");
+    __shared__ double xr[RX];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t v = s.order ? (int64_t)s.order[blockIdx.x] : (int64_t)blockIdx.x;
     const int64_t base = s.dof0[v];
@@ -648,7 +654,7 @@ __global__ void __launch_bounds__(BT) band_solve(BandSys s, const int64_t *__res
                     const int64_t i1 = ib + lane, i2 = ib + 64 + lane;
                     const int64_t c1 = i1 <= iend ? i1 : iend, c2 = i2 <= iend ? i2 : iend;
                     const double l1 = Lj[(int)(c1 - j)], l2 = Lj[(int)(c2 - j)];
-                    const double y1 = y[c1], y2 = y[c2];
+                    const double y1 = xr[c1 & (RX - 1)], y2 = xr[c2 & (RX - 1)];
                     sacc = fma(l1, i1 <= iend ? y1 : 0., sacc);
                     sacc = fma(l2, i2 <= iend ? y2 : 0., sacc);
                 }
@@ -668,7 +674,10 @@ __global__ void __launch_bounds__(BT) band_solve(BandSys s, const int64_t *__res
                 const double xv = readlane_d(rhs, cc);
                 if (lane < cc) rhs -= T[cc][lane] * xv;
             }
-            if (lane < jn) y[j0 + lane] = rhs;
+            if (lane < jn) {
+                y[j0 + lane] = rhs;
+                xr[(j0 + lane) & (RX - 1)] = rhs;
+            }
         }
         __syncthreads();
     }
