@@ -106,14 +106,26 @@ constexpr int NF = 32;   // factor panel width
 constexpr int NH = NF / 2;  // diagonal block: columns per lane
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-// trailing-update tile t of the lower triangle (row-major: (0,0), (1,0),
-// (1,1), (2,0) ...; t uniform across the wave) -> this lane's A-operand
-// row ra, B-operand row and window column rb, first window row ri0 (the
-// lane's rows are ri0 + 4 v)
+// A trailing-update tile (I, K) is computed transposed: the MFMA output
+// D'[i][j] = A[ri][rk] - sum_c P[ri][c] P[rk][c] with rk = column block
+// K's i-th window column and ri = row block I's j-th window row, so lane l
+// holds window row ri = jn + 16 I + l % 16 of the columns rk0 + 4 v,
+// rk0 = jn + 16 K + l / 16.  One load or store instruction then covers 16
+// consecutive rows (128 contiguous bytes of the band) of 4 columns rather
+// than 4 rows of 16 columns.  The A operand is P row jn + 16 K + l % 16
+// (ra), the B operand P row jn + 16 I + l % 16 (rb); each output element
+// receives the same products in the same order as untransposed.
 struct TileAt {
-    int ra, rb, ri0;
+    int ra, rb, ri, rk0;
 };
 
+__device__ __forceinline__ TileAt make_tile(int I, int K, int jn, int li, int lk)
+{
+    return TileAt{jn + 16 * K + li, jn + 16 * I + li, jn + 16 * I + li, jn + 16 * K + lk};
+}
+
+// tile t of the lower triangle in row-major order ((0,0), (1,0), (1,1),
+// (2,0) ...; t uniform across the wave)
 __device__ __forceinline__ TileAt tile_at(int t, int jn, int li, int lk)
 {
     int I = (int)((sqrtf(8.f * t + 1.f) - 1.f) * 0.5f);
@@ -121,7 +133,7 @@ __device__ __forceinline__ TileAt tile_at(int t, int jn, int li, int lk)
     while ((I + 1) * (I + 2) / 2 <= t) I++;
     while (I * (I + 1) / 2 > t) I--;
     const int K = t - I * (I + 1) / 2;
-    return TileAt{jn + 16 * I + li, jn + 16 * K + li, jn + 16 * I + lk};
+    return make_tile(I, K, jn, li, lk);
 }
 
 // tile u of the trailing update's column-major walk of the lower triangle
@@ -143,7 +155,7 @@ __device__ __forceinline__ TileAt tile_cm(int u, int nt, bool la, int jn, int li
         if (u < 2 * h) {
             K = u < h ? 0 : 1;
             I = 2 + (u < h ? u : u - h);
-            return TileAt{jn + 16 * I + li, jn + 16 * K + li, jn + 16 * I + lk};
+            return make_tile(I, K, jn, li, lk);
         }
         v = u - 2 * h;
     }
@@ -157,23 +169,22 @@ __device__ __forceinline__ TileAt tile_cm(int u, int nt, bool la, int jn, int li
     const int i = k + (v - (k * m - k * (k - 1) / 2));
     I = off + i;
     K = off + k;
-    return TileAt{jn + 16 * I + li, jn + 16 * K + li, jn + 16 * I + lk};
+    return make_tile(I, K, jn, li, lk);
 }
 
-// the tile's window entries A[ri0 + 4 v][rb] from the band (Lp: the
+// the tile's window entries A[ri][rk0 + 4 v] from the band (Lp: the
 // panel's first column).  Addresses are clamped into the window, so every
 // lane loads; entries outside the window's lower triangle are never stored
 // and only reach their own output through the update.
 __device__ __forceinline__ v4d tile_load(const TileAt &q, const double *Lp, int64_t ld,
                                          int rows)
 {
-    const int rb = min(q.rb, rows - 1);
-    const double *col = Lp + rb * ld;
     v4d a;
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-        const int ri = min(max(q.ri0 + 4 * v, rb), rows - 1);
-        a[v] = col[ri - rb];
+        const int rk = min(q.rk0 + 4 * v, rows - 1);
+        const int ri = min(max(q.ri, rk), rows - 1);
+        a[v] = Lp[rk * ld + (ri - rk)];
     }
     return a;
 }
@@ -181,17 +192,15 @@ __device__ __forceinline__ v4d tile_load(const TileAt &q, const double *Lp, int6
 __device__ __forceinline__ void tile_store(const TileAt &q, const v4d &a, double *Lp,
                                            int64_t ld, int rows)
 {
-    if (q.rb >= rows) return;
-    double *col = Lp + q.rb * ld;
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-        const int ri = q.ri0 + 4 * v;
-        if (ri < rows && ri >= q.rb) col[ri - q.rb] = a[v];
+        const int rk = q.rk0 + 4 * v;
+        if (rk < rows && q.ri < rows && q.ri >= rk) Lp[rk * ld + (q.ri - rk)] = a[v];
     }
 }
 
-// a -= P[ra][0..31] P[rb][0..31]^T on the tile; operand rows past the
-// panel only feed outputs that are not stored
+// a -= P[ra][0..31] P[rb][0..31]^T on the (transposed) tile; operand rows
+// past the panel only feed outputs that are not stored
 template <int PR>
 __device__ __forceinline__ v4d tile_update(const TileAt &q, v4d a, const double (*P)[NF + 1],
                                            int lk)
@@ -488,8 +497,8 @@ __global__ void __launch_bounds__(BT) band_chol(BandSys s, FwdArgs fw, int *__re
                 tile_store(q, a, Lp, ld, rows);
 #pragma unroll
                 for (int v4 = 0; v4 < 4; v4++) {
-                    const int ri = q.ri0 + 4 * v4;
-                    if (q.rb < rows && ri < rows && ri >= q.rb) P[ri - jn][q.rb - jn] = a[v4];
+                    const int rk = q.rk0 + 4 * v4;
+                    if (rk < rows && q.ri < rows && q.ri >= rk) P[q.ri - jn][rk - jn] = a[v4];
                 }
             }
             diag_block(P, dinv, Lk, jnn, lane, failed);
